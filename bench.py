@@ -1,0 +1,204 @@
+"""SOAP-GAP E+F+virial throughput on one MI355X.
+
+Workload: si4096 -- an 8x8x8 diamond Si supercell (a = 5.43 A), Gaussian
+displacements sigma = 0.15 A, 1% random vacancies (4055 atoms); SOAP cutoff
+5.0, cutoff_transition_width 1.0, atom_sigma 0.5, n_max 8, l_max 6 (d = 253),
+2000 sparse points, zeta 4, delta 1, alpha ~ N(0, 1e-3).  A step is one full
+evaluation of energy, forces and virial (neighbour list included) with the
+positions already resident in HBM, through gapx_calc_device.
+
+Reported beside the FP64 plain form: the bf16x3 and fp16x3 split-precision
+forms (throughput and error against FP64), a per-stage breakdown from HIP
+events on the library's stream, the roofline of the dominant kernel and the
+oracle's FP64 CPU throughput on this host (a reported baseline).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from quip_amd import _lib as L  # noqa: E402
+from quip_amd import configs  # noqa: E402
+from quip_amd.gap_xml import GAPModel, GPCoordinate, write_gap_xml  # noqa: E402
+from quip_amd.potential import Potential  # noqa: E402
+
+MFMA_FP64_PEAK = 78.6e12
+MFMA_BF16_PEAK = 2.5e15
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_descriptor_fn(tmpdir):
+    cache = {}
+
+    def fn(d, at):
+        key = d.string
+        if key not in cache:
+            m = GAPModel("D", 1527075646, np.zeros(119),
+                         [GPCoordinate("D1", d, 2, 1.0, 0.0, 1.0, np.zeros((1, d.dim)), np.zeros(1), np.ones(1))])
+            path = os.path.join(tmpdir, "desc%d.xml" % len(cache))
+            write_gap_xml(path, m)
+            cache[key] = Potential(param_filename=path)
+        return cache[key].descriptors(at)[0]
+    return fn
+
+
+class DeviceArrays:
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.ptrs = []
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        if self.hip.hipMalloc(C.byref(p), C.c_size_t(max(nbytes, 8))) != 0:
+            raise RuntimeError("hipMalloc failed")
+        self.ptrs.append(p)
+        return p
+
+    def h2d(self, dst, arr):
+        if self.hip.hipMemcpy(dst, L.ptr(arr), C.c_size_t(arr.nbytes), 1) != 0:
+            raise RuntimeError("hipMemcpy H2D failed")
+
+    def d2h(self, arr, src):
+        if self.hip.hipMemcpy(L.ptr(arr), src, C.c_size_t(arr.nbytes), 2) != 0:
+            raise RuntimeError("hipMemcpy D2H failed")
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
+def run_form(pot, at, dev, bufs, steps, warmup):
+    lib = L.lib()
+    N = len(at)
+    Z, lat, pbc = L.i32(at.numbers), L.f64(at.cell), L.i32(at.pbc)
+    want = L.WANT_E | L.WANT_F | L.WANT_V
+
+    def step():
+        L.check(lib.gapx_calc_device(pot.handle, N, bufs["pos"], L.ptr(Z), L.ptr(lat), L.ptr(pbc), 0.0, want,
+                                     bufs["e"], None, bufs["f"], bufs["v"], None), "gapx_calc_device")
+
+    for _ in range(warmup):
+        step()
+    L.check(lib.gapx_sync(pot.handle), "sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    L.check(lib.gapx_sync(pot.handle), "sync")
+    dt = time.perf_counter() - t0
+    e, f, v = np.zeros(1), np.zeros((N, 3)), np.zeros(9)
+    dev.d2h(e, bufs["e"])
+    dev.d2h(f, bufs["f"])
+    dev.d2h(v, bufs["v"])
+    # per-stage times (HIP events on the library stream), separate pass
+    pot.set_timing(True)
+    acc = None
+    nt = 5
+    for _ in range(nt):
+        step()
+        t = pot.last_timings()
+        acc = t if acc is None else {k: acc[k] + t[k] for k in t}
+    pot.set_timing(False)
+    stages = {k: acc[k] / nt for k in acc}
+    return dt, dict(energy=e[0], force=f, virial=v.reshape(3, 3).T), stages
+
+
+def cpu_baseline(model, at, budget_s=12.0):
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    O.gap_calc(model, at, n_threads=threads)          # warm-up
+    first = time.perf_counter() - t0
+    reps = max(1, min(20, int(budget_s / max(first, 1e-3))))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = O.gap_calc(model, at, n_threads=threads)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return dict(value=len(at) / med, unit="atoms/s", cores=threads, kind="port",
+                sample="si4096 (4055 atoms) full E+F+virial, median of %d runs after 1 warm-up, "
+                       "oracle/liboracle.so FP64 with OpenMP" % reps), r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: --gpus %d is not supported (the SOAP-GAP path shards over centre "
+              "atoms; a multi-GPU run is not part of this build)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    tmp = tempfile.mkdtemp()
+    log("building si4096 model from GPU descriptors")
+    model = configs.model_si4096(gpu_descriptor_fn(tmp))
+    path = os.path.join(tmp, "si4096.xml")
+    write_gap_xml(path, model)
+    at = configs.si4096()
+    N = len(at)
+    dev = DeviceArrays()
+    bufs = dict(pos=dev.alloc(N * 24), e=dev.alloc(8), f=dev.alloc(N * 24), v=dev.alloc(72))
+    dev.h2d(bufs["pos"], L.f64(at.positions))
+    forms = {}
+    results = {}
+    for prec in ("fp64", "bf16x3", "fp16x3"):
+        pot = Potential(param_filename=path, precision=prec)
+        log("timing", prec)
+        dt, res, stages = run_form(pot, at, dev, bufs, args.steps, args.warmup)
+        forms[prec] = dict(atoms_per_s=N * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
+                           stage_ms={k: round(v, 5) for k, v in stages.items()})
+        results[prec] = res
+        del pot
+    ref = results["fp64"]
+    for prec in ("bf16x3", "fp16x3"):
+        r = results[prec]
+        forms[prec]["dE_per_atom_vs_fp64"] = abs(r["energy"] - ref["energy"]) / N
+        forms[prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - ref["force"]) ** 2)))
+        forms[prec]["speedup_vs_fp64"] = forms[prec]["atoms_per_s"] / forms["fp64"]["atoms_per_s"]
+    st = forms["fp64"]["stage_ms"]
+    nc, M, d = N, model.coords[0].n_sparse, model.coords[0].descriptor.dim
+    gemm_flops = 2.0 * nc * M * d
+    dom = "gemm2" if st["gemm2"] >= st["gemm1"] else "gemm1"
+    achieved = gemm_flops / (st[dom] * 1e-3)
+    roofline = dict(bound="mfma", kernel="k_gemm_f64 (%s, FP64 MFMA 16x16x4)" % dom, achieved=achieved / 1e12,
+                    peak=MFMA_FP64_PEAK / 1e12, unit="TFLOP/s", frac=achieved / MFMA_FP64_PEAK, traffic=None,
+                    flops_per_launch=gemm_flops)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu, ro = cpu_baseline(model, at)
+        forms["fp64"]["dE_per_atom_vs_oracle"] = abs(ref["energy"] - ro["energy"]) / N
+        forms["fp64"]["max_dF_vs_oracle"] = float(np.abs(ref["force"] - ro["forces"]).max())
+    dev.free()
+    fp = forms["fp64"]
+    line = {
+        "metric": "SOAP-GAP atoms/s (E+F+virial), mixed vs FP64 GPU; p50/p99 served latency",
+        "value": fp["atoms_per_s"], "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": fp["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded si4096 structure, model written as GAP XML from the seed)",
+        "config": {"workload": "si4096: 8x8x8 diamond Si, sigma 0.15, 1% vacancies (4055 atoms); SOAP n_max 8 "
+                               "l_max 6 cutoff 5.0 (d=253), 2000 sparse points, zeta 4; FP64 plain form",
+                   "atoms": N, "sparse_points": M, "descriptor_dim": d},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+/*
+ * gapx.h -- C ABI of the MI355X SOAP-GAP engine (libgapx.so).
+ *
+ * Drop-in boundary for QUIP's IP GAP calculator.  Each entry point names the
+ * reference interface it stands in for.  Conventions follow QUIP's Fortran
+ * arrays so a Fortran host can pass its arrays straight through ISO_C_BINDING:
+ *
+ *   pos[3*i + k]              pos(k,i)                 (Atoms_types.f95:332)
+ *   lattice[3*m + k]          lattice(k,m), column m = lattice vector m
+ *   f[3*i + k]                f(k,i)                   (IPModel_GAP.f95:237)
+ *   virial[3*b + a]           virial(a,b)
+ *   local_virial[9*i + 3*b+a] reshape(virial_in(:,:,i),(/9/)) (IPModel_GAP.f95:528-532)
+ *   Atom indices are 0-based; Z are atomic numbers; pbc[3] are 0/1.
+ *
+ * Every call returns 0 on success and a non-zero GAPX_E_* code otherwise; the
+ * library never aborts the host.  gapx_last_error() returns the message of the
+ * calling thread's last failure (the role of QUIP's error stack, error.inc:58-71).
+ * All outputs are caller-allocated.  A model handle is used by one host thread
+ * at a time; it owns one HIP stream and its device workspace.
+ */
+#ifndef GAPX_H
+#define GAPX_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GAPX_API_VERSION 1
+
+/* status codes */
+#define GAPX_OK 0
+#define GAPX_E_ARG 1        /* bad argument (sizes, null where required)        */
+#define GAPX_E_MODEL 2      /* model XML / sparseX content not usable           */
+#define GAPX_E_IO 3         /* file not readable                                */
+#define GAPX_E_DEVICE 4     /* HIP runtime error                                */
+#define GAPX_E_UNSUPPORTED 5/* descriptor / option outside what is built       */
+#define GAPX_E_GEOMETRY 6   /* e.g. atoms on top of each other, cutoff <= 0    */
+#define GAPX_E_CAPACITY 7   /* request exceeds the caller-set memory budget    */
+
+/* what to compute (bit mask): the optional outputs of IPModel_GAP_Calc */
+#define GAPX_WANT_E 1u
+#define GAPX_WANT_F 2u
+#define GAPX_WANT_V 4u
+#define GAPX_WANT_LOCAL_E 8u
+#define GAPX_WANT_LOCAL_V 16u
+
+/* arithmetic of the descriptor x sparse-point GEMMs */
+#define GAPX_PREC_FP64 0    /* plain form: FP64 throughout                          */
+#define GAPX_PREC_BF16X3 1  /* BF16 hi+lo split, 3 MFMA passes, FP32 accumulation   */
+#define GAPX_PREC_FP16X3 2  /* FP16 hi+lo split, 3 MFMA passes, FP32 accumulation   */
+
+typedef struct gapx_model gapx_model;
+typedef struct gapx_nl gapx_nl;
+
+/* Library version; a host binding checks it like quip_lammps_wrapper.f95:13,23-28. */
+int gapx_api_version(void);
+
+/* Message of the calling thread's last error (NUL-terminated, truncated to len). */
+void gapx_last_error(char *buf, int len);
+
+/* Number of visible GPUs. */
+int gapx_device_count(int *count);
+
+/* Load a GAP model from the XML text of a potential file.
+ * Replaces IPModel_GAP_Initialise_str + IPModel_GAP_read_params_xml + gp_readXML
+ * (IPModel_GAP.f95:149-192, 618-944).  `label` selects GAP_params by label
+ * (NULL: the first); sparseX files are read relative to `xml_dir`.  Uploads
+ * the sparse points and coefficients to device `device`. */
+int gapx_model_load(const char *xml_text, int64_t xml_len, const char *label, const char *xml_dir, int device,
+                    gapx_model **out);
+int gapx_model_free(gapx_model *model);
+
+/* IPModel_GAP%cutoff: the largest descriptor cutoff (IPModel_GAP.f95:182-188, IP.f95:704-705). */
+int gapx_model_cutoff(const gapx_model *model, double *cutoff);
+/* E_scale (IPModel_GAP.f95:592-596); default 1. */
+int gapx_model_set_e_scale(gapx_model *model, double e_scale);
+/* GEMM arithmetic, GAPX_PREC_*; default FP64. */
+int gapx_model_set_precision(gapx_model *model, int precision);
+/* Number of gpCoordinates and the descriptor dimension of one. */
+int gapx_model_info(const gapx_model *model, int *n_coord, int coord, int *dim, int *n_sparse);
+
+/* Energy, forces and virials of one structure from host buffers.
+ * Replaces IPModel_GAP_Calc(this, at, e, local_e, f, virial, local_virial)
+ * (IPModel_GAP.f95:233-605) including the calc_connect it relies on
+ * (Potential.f95:844-859 -> Connection.f95:1035-1310).  `nl_cutoff` <= 0 uses the
+ * model cutoff (QUIP's at%cutoff after set_cutoff).  Outputs not requested in
+ * `want` may be NULL. */
+int gapx_calc(gapx_model *model, int n_atoms, const double *pos, const int *Z, const double *lattice,
+              const int *pbc, double nl_cutoff, unsigned want, double *e, double *local_e, double *f,
+              double *virial, double *local_virial);
+
+/* As gapx_calc with positions and every output in device memory (the inputs
+ * are already resident in HBM); Z, lattice and pbc stay host arrays.  Work is
+ * queued on the model's stream; gapx_sync waits for it. */
+int gapx_calc_device(gapx_model *model, int n_atoms, const double *d_pos, const int *Z, const double *lattice,
+                     const int *pbc, double nl_cutoff, unsigned want, double *d_e, double *d_local_e, double *d_f,
+                     double *d_virial, double *d_local_virial);
+int gapx_sync(gapx_model *model);
+
+/* Many structures of uneven size in one call (struct-of-arrays with offsets):
+ * structure s owns atoms [atom_off[s], atom_off[s+1]); lattices[9*s..] and
+ * pbcs[3*s..] per structure; energies[s], virials[9*s..]; forces/local arrays
+ * indexed by global atom.  Same physics as gapx_calc per structure. */
+int gapx_calc_batch(gapx_model *model, int n_struct, const int *atom_off, const double *pos, const int *Z,
+                    const double *lattices, const int *pbcs, double nl_cutoff, unsigned want, double *energies,
+                    double *local_e, double *f, double *virials, double *local_virial);
+
+/* Neighbour tables exactly as QUIP's calc_connect builds them
+ * (Connection.f95:1035-1310): neighbour1(i) = (j >= i, shift, r) in append
+ * order, neighbour2(i) = (j < i, index into neighbour1(j)).  Built on the GPU. */
+int gapx_connect(int device, int n_atoms, const double *pos, const double *lattice, const int *pbc, double cutoff,
+                 gapx_nl **out);
+int gapx_nl_counts(const gapx_nl *nl, int *n1_total, int *n2_total, int *cells);
+int gapx_nl_export(const gapx_nl *nl, int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_off, int *n2_j,
+                   int *n2_idx, int *map_shift);
+int gapx_nl_free(gapx_nl *nl);
+
+/* Descriptor vectors of coordinate `coord` (descriptor calc, IPModel_GAP.f95:423-425):
+ * X[n_desc x dim] unit-normalised SOAP vectors, desc_atom[n_desc] centre atoms. */
+int gapx_descriptors(gapx_model *model, int coord, int n_atoms, const double *pos, const int *Z,
+                     const double *lattice, const int *pbc, double nl_cutoff, double *X, int *desc_atom,
+                     int *n_desc);
+
+/* Time of the last gapx_calc / gapx_calc_device per stage (milliseconds, HIP
+ * events on the model stream): [0] neighbour list, [1] descriptor forward,
+ * [2] GEMM 1 (K, fused pow/alpha), [3] GEMM 2 (G), [4] descriptor adjoint,
+ * [5] distance_2b + gather; requires gapx_model_set_timing(model, 1). */
+int gapx_model_set_timing(gapx_model *model, int on);
+int gapx_last_timings(const gapx_model *model, double *ms6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+// Device-side declarations shared by the kernel and host translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gapx {
+
+constexpr int kMaxSpecies = 16;
+constexpr int kMaxL = 12;
+
+// Neighbour-list geometry, prepared on the host from the lattice (3x3 maths only).
+struct NLGeom {
+    double lat[9];       // lattice(k,m) at lat[3*m+k]: column m is lattice vector m
+    double inv_eff[9];   // inverse of the effective lattice, column-major g(i,j) at [3*j+i]
+    double offset[3];    // lat_offset of non-periodic axes
+    int cells[3];        // cellsNa, cellsNb, cellsNc
+    int img[3];          // cell_image_N{a,b,c} after halving
+    int pbc[3];
+    double cutoff;
+};
+
+struct SoapDev {
+    double rc, ctw, alpha, cw, u00;
+    int n, S, K, LM, dim, L;
+    int species_Z[kMaxSpecies];
+    const double *rb;    // n grid points
+    const double *T;     // n x n transform, T[a*n+b]
+    const int *qmap;     // dim-1 entries: (ia<<16)|(jb<<8)|l
+};
+
+// launchers (kernels.hip)
+int scan_partials_size(int n);
+void launch_scan_i32(const int *in, int n, int *out, int *partials, hipStream_t s);
+void launch_fill_i32(int *p, int n, int v, hipStream_t s);
+void launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
+void launch_nl_bin(int N, const double *pos, const NLGeom &g, int *cell_of, int *map_shift, hipStream_t s);
+void launch_nl_cell_count(int N, const int *cell_of, int *cnt, hipStream_t s);
+void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s);
+void launch_nl_cell_sort(int ncell, const int *start, const int *cnt, int *atoms, hipStream_t s);
+void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
+                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom &g,
+                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, hipStream_t s);
+void launch_nl_n2_count(int N, const int *n1_off, const int *n1_j, int *n2_cnt, hipStream_t s);
+void launch_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_off, int *n2_fill, int *n2_i,
+                       int *n2_k, hipStream_t s);
+void launch_nl_n2_sort(int N, const int *n2_off, int *n2_i, int *n2_k, const int *cell_of, const int *n1_off,
+                       int *n1_rev, hipStream_t s);
+void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off, const int *n1_j,
+                    const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
+                    const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
+                    hipStream_t s);
+void launch_minmax_frac(int N, const double *pos, const double *g, double *out, hipStream_t s);
+
+int soap_supported(int n, int L);
+void launch_soap_fwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
+                     const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
+                     double *cbuf, hipStream_t s);
+void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
+                     const int *nb_j, const double *nb_r, const double *nb_d, const double *X, const double *G,
+                     const double *norm, const double *cbuf, double *pairF, hipStream_t s);
+
+// GEMMs
+// gemm1: K = X (nc x d) . Xs^T (M x d); W = w_s zeta K^(zeta-1); e_part[i*ntile + t]
+void launch_gemm1_f64(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
+                      double *W, double *e_part, int *ntile_out, hipStream_t s);
+// gemm2: G = W (nc x M) . Xs (M x d)
+void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, hipStream_t s);
+// mixed precision forms: Xs pre-split into hi/lo (bf16 or fp16) copies, row- and column-major
+// W is scaled by wscale (a power of two) before its hi/lo split; GEMM2 divides it out.
+void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *X, const void *Xs_hi,
+                        const void *Xs_lo, const double *wvec, double zeta, double wscale, float *Wf,
+                        double *e_part, int *ntile_out, void *xsplit_ws, hipStream_t s);
+void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, const float *Wf, const void *XsT_hi,
+                        const void *XsT_lo, double wscale, double *G, hipStream_t s);
+void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
+                         void *hi, void *lo, hipStream_t s);
+
+void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_part, double *esite, hipStream_t s);
+void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
+                const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
+                const double *nb_d, double *pairF, double *pairE, hipStream_t s);
+void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
+                   const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
+                   double *local_e, double *local_virial, double *block_part, int *nblocks_out, hipStream_t s);
+void launch_final_reduce(int nblocks, const double *block_part, double E_scale, double *e_out, double *virial_out,
+                         hipStream_t s);
+
+}  // namespace gapx

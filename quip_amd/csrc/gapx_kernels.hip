@@ -1,0 +1,1068 @@
+// HIP kernels for the SOAP-GAP energy/force/virial path on gfx950 (MI355X).
+//
+//  neighbour list   cell binning + per-cell lists + per-atom pair search that
+//                   reproduces QUIP's calc_connect tables bit for bit
+//                   (src/libAtoms/Connection.f95:1035-1310, 448-634, 1519-1621)
+//  soap_fwd         one wavefront per centre atom: radial basis and real
+//                   spherical harmonics of a neighbour block staged in LDS,
+//                   density coefficients c_(s,b,lm), power spectrum, norm
+//  soap_bwd         adjoint pass: dE/dp -> dE/dc in LDS, then one lane per
+//                   neighbour re-evaluates the basis and contracts its gradient
+//                   (no grad_data is ever materialised)
+//  gemm1 / gemm2    FP64 MFMA (v_mfma_f64_16x16x4f64) GEMMs for the dot-product
+//                   GP kernel K = X Xs^T with fused (w K^zeta, w zeta K^(zeta-1))
+//                   epilogue, and G = W Xs
+//  d2b              distance_2b ard_se terms per pair
+//  gather           deterministic per-atom segmented reductions for forces,
+//                   local virials and local energies (no floating atomics)
+#include "gapx_kernels.h"
+
+#include <algorithm>
+
+namespace gapx {
+
+#define GAPX_PI 3.14159265358979323846264338327950288
+
+__constant__ double c_ynorm[(kMaxL + 1) * (kMaxL + 2) / 2];   // N_lm (x sqrt 2 for m>0)
+__constant__ double c_dfact[kMaxL + 2];                       // (2m-1)!!
+
+static bool g_const_ready = false;
+static void ensure_constants() {
+    if (g_const_ready) return;
+    double yn[(kMaxL + 1) * (kMaxL + 2) / 2], df[kMaxL + 2];
+    for (int l = 0; l <= kMaxL; l++)
+        for (int m = 0; m <= l; m++) {
+            double r = 1.0;   // (l-m)!/(l+m)!
+            for (int k = l - m + 1; k <= l + m; k++) r /= (double)k;
+            double v = sqrt((2.0 * l + 1.0) / (4.0 * GAPX_PI) * r);
+            if (m > 0) v *= sqrt(2.0);
+            yn[l * (l + 1) / 2 + m] = v;
+        }
+    df[0] = 1.0;
+    for (int m = 1; m < kMaxL + 2; m++) df[m] = df[m - 1] * (2.0 * m - 1.0);
+    hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn));
+    hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df));
+    g_const_ready = true;
+}
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ----------------------------------------------------------------------------
+// small fills (graph-safe replacements for memset)
+// ----------------------------------------------------------------------------
+__global__ void k_fill_i32(int *p, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, size_t n, double v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_i32(int *p, int n, int v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_f64(double *p, size_t n, double v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(cdiv((long)n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+
+// ----------------------------------------------------------------------------
+// exclusive prefix sum of n ints into out[0..n] (out[n] = total), three passes:
+// per-block sums, a single-block scan of those, then per-block local scans.
+// ----------------------------------------------------------------------------
+constexpr int SCAN_BS = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_BS * SCAN_ITEMS;
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int *total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BS; off <<= 1) {
+        int a = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    int incl = sh[t];
+    *total = sh[SCAN_BS - 1];
+    __syncthreads();
+    return incl - v;
+}
+__global__ void k_scan_partials(const int *in, int n, int *partials) {
+    __shared__ int sh[SCAN_BS];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) s += (base + k < n) ? in[base + k] : 0;
+    int tot;
+    block_excl_scan(s, sh, &tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+__global__ void k_scan_top(int *partials, int nb, int *total_out) {
+    __shared__ int sh[SCAN_BS];
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += SCAN_BS) {
+        int i = c0 + threadIdx.x;
+        int v = i < nb ? partials[i] : 0;
+        int tot;
+        int ex = block_excl_scan(v, sh, &tot);
+        if (i < nb) partials[i] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+__global__ void k_scan_apply(const int *in, int n, const int *partials, int *out) {
+    __shared__ int sh[SCAN_BS];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
+    int tot;
+    int ex = block_excl_scan(s, sh, &tot) + partials[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+}
+int scan_partials_size(int n) { return cdiv(n > 0 ? n : 1, SCAN_TILE); }
+void launch_scan_i32(const int *in, int n, int *out, int *partials, hipStream_t s) {
+    const int nb = scan_partials_size(n);
+    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BS), 0, s, in, n, partials);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BS), 0, s, partials, nb, out + n);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BS), 0, s, in, n, partials, out);
+}
+
+// ----------------------------------------------------------------------------
+// neighbour list
+// All floating point below follows the reference's operation order with no
+// contraction, so the in/out decisions and stored distances are bit-exact.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void mult3_exact(const double *g, double p0, double p1, double p2, double *out) {
+#pragma clang fp contract(off)
+    // matrix_product_vect (linearalgebra.f95:671-686): out(i) = ((0 + g(i,1)p1) + g(i,2)p2) + g(i,3)p3
+    for (int i = 0; i < 3; i++) {
+        double t = 0.0;
+        t = t + g[i] * p0;
+        t = t + g[3 + i] * p1;
+        t = t + g[6 + i] * p2;
+        out[i] = t;
+    }
+}
+
+__global__ void k_nl_bin(int N, const double *__restrict__ pos, NLGeom g, int *cell_of, int *map_shift) {
+#pragma clang fp contract(off)
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double lp[3];
+    mult3_exact(g.inv_eff, pos[3 * n], pos[3 * n + 1], pos[3 * n + 2], lp);
+    int ci[3];
+    for (int a = 0; a < 3; a++) {
+        double t = lp[a] - g.offset[a];
+        int ms = g.pbc[a] ? -(int)floor(t + 0.5) : 0;
+        map_shift[3 * n + a] = ms;
+        t = t + (double)ms;
+        int c = (int)floor((double)g.cells[a] * (t + 0.5)) + 1;
+        c = c < 1 ? 1 : (c > g.cells[a] ? g.cells[a] : c);
+        ci[a] = c - 1;
+    }
+    cell_of[n] = (ci[2] * g.cells[1] + ci[1]) * g.cells[0] + ci[0];
+}
+void launch_nl_bin(int N, const double *pos, const NLGeom &g, int *cell_of, int *map_shift, hipStream_t s) {
+    if (N > 0) hipLaunchKernelGGL(k_nl_bin, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pos, g, cell_of, map_shift);
+}
+
+__global__ void k_nl_cell_count(int N, const int *cell_of, int *cnt) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) atomicAdd(&cnt[cell_of[n]], 1);
+}
+void launch_nl_cell_count(int N, const int *cell_of, int *cnt, hipStream_t s) {
+    if (N > 0) hipLaunchKernelGGL(k_nl_cell_count, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, cnt);
+}
+__global__ void k_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    int c = cell_of[n];
+    int slot = atomicAdd(&fillc[c], 1);
+    atoms[start[c] + slot] = n;
+}
+void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s) {
+    if (N > 0) hipLaunchKernelGGL(k_nl_cell_fill, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, start, fillc, atoms);
+}
+// Each cell's list in descending atom index: the order of QUIP's head-inserted
+// linked lists (Connection.f95:1580-1584).
+__global__ void k_nl_cell_sort(int ncell, const int *start, const int *cnt, int *atoms) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int *a = atoms + start[c];
+    int n = cnt[c];
+    for (int i = 1; i < n; i++) {
+        int v = a[i], j = i - 1;
+        while (j >= 0 && a[j] < v) { a[j + 1] = a[j]; j--; }
+        a[j + 1] = v;
+    }
+}
+void launch_nl_cell_sort(int ncell, const int *start, const int *cnt, int *atoms, hipStream_t s) {
+    hipLaunchKernelGGL(k_nl_cell_sort, dim3(cdiv(ncell, 128)), dim3(128), 0, s, ncell, start, cnt, atoms);
+}
+
+// One thread per atom1: the reference's offset loops (k2 slowest, i2 fastest),
+// atom2 over the cell list, test_form_bond.  Pass 1 counts, pass 2 writes
+// neighbour1 in the reference's append order.
+template <bool FILL>
+__global__ void k_nl_pairs(int N, const double *__restrict__ pos, const int *__restrict__ map_shift,
+                           const int *__restrict__ cell_of, const int *__restrict__ cell_start,
+                           const int *__restrict__ cell_cnt, const int *__restrict__ cell_atoms, NLGeom g,
+                           int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r) {
+#pragma clang fp contract(off)
+    int atom1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (atom1 >= N) return;
+    int lin = cell_of[atom1];
+    int cc[3] = {lin % g.cells[0] + 1, (lin / g.cells[0]) % g.cells[1] + 1, lin / (g.cells[0] * g.cells[1]) + 1};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {       // get_min_max_images (Connection.f95:1448-1513)
+        if (g.pbc[a]) { lo[a] = -g.img[a]; hi[a] = g.img[a]; }
+        else {
+            lo[a] = (g.img[a] < cc[a]) ? -g.img[a] : -cc[a] + 1;
+            hi[a] = (g.img[a] < g.cells[a] - cc[a]) ? g.img[a] : g.cells[a] - cc[a];
+        }
+    }
+    const double p1x = pos[3 * atom1], p1y = pos[3 * atom1 + 1], p1z = pos[3 * atom1 + 2];
+    const int ms1x = map_shift[3 * atom1], ms1y = map_shift[3 * atom1 + 1], ms1z = map_shift[3 * atom1 + 2];
+    const double rc = g.cutoff;
+    int count = 0;
+    const int base = FILL ? n1_off[atom1] : 0;
+    int k3 = 1, j3 = 1, i3 = 1;
+    for (int k2 = lo[2]; k2 <= hi[2]; k2++) {
+        if (g.cells[2] > 1) k3 = (cc[2] + k2 - 1 + g.cells[2]) % g.cells[2] + 1;
+        int k4 = (cc[2] + k2 - k3) / g.cells[2];
+        if (k3 < 1 || k3 > g.cells[2]) continue;   // outside the reference's domain of validity
+        for (int j2 = lo[1]; j2 <= hi[1]; j2++) {
+            if (g.cells[1] > 1) j3 = (cc[1] + j2 - 1 + g.cells[1]) % g.cells[1] + 1;
+            int j4 = (cc[1] + j2 - j3) / g.cells[1];
+            if (j3 < 1 || j3 > g.cells[1]) continue;
+            for (int i2 = lo[0]; i2 <= hi[0]; i2++) {
+                if (g.cells[0] > 1) i3 = (cc[0] + i2 - 1 + g.cells[0]) % g.cells[0] + 1;
+                int i4 = (cc[0] + i2 - i3) / g.cells[0];
+                if (i3 < 1 || i3 > g.cells[0]) continue;
+                int lin2 = ((k3 - 1) * g.cells[1] + (j3 - 1)) * g.cells[0] + (i3 - 1);
+                int st = cell_start[lin2], nc = cell_cnt[lin2];
+                bool same_cell = (lin2 == lin);
+                for (int q = 0; q < nc; q++) {
+                    int atom2 = cell_atoms[st + q];
+                    if (atom2 < atom1) break;           // list is descending: nothing further qualifies
+                    if (atom2 == atom1 && i4 == 0 && j4 == 0 && k4 == 0 && same_cell) continue;
+                    int sh0 = i4 - ms1x + map_shift[3 * atom2];
+                    int sh1 = j4 - ms1y + map_shift[3 * atom2 + 1];
+                    int sh2 = k4 - ms1z + map_shift[3 * atom2 + 2];
+                    double d0 = pos[3 * atom2] - p1x, d1 = pos[3 * atom2 + 1] - p1y, d2 = pos[3 * atom2 + 2] - p1z;
+                    // dd(k) = dd(k) + lattice(k,m) * shift(m), m = 1..3
+                    d0 = d0 + g.lat[0] * (double)sh0; d1 = d1 + g.lat[1] * (double)sh0; d2 = d2 + g.lat[2] * (double)sh0;
+                    d0 = d0 + g.lat[3] * (double)sh1; d1 = d1 + g.lat[4] * (double)sh1; d2 = d2 + g.lat[5] * (double)sh1;
+                    d0 = d0 + g.lat[6] * (double)sh2; d1 = d1 + g.lat[7] * (double)sh2; d2 = d2 + g.lat[8] * (double)sh2;
+                    if (d0 > rc || d1 > rc || d2 > rc) continue;
+                    double dsq = d0 * d0;
+                    dsq = dsq + d1 * d1;
+                    dsq = dsq + d2 * d2;
+                    double d = sqrt(dsq);
+                    if (!(d < rc)) continue;
+                    if (FILL) {
+                        int e = base + count;
+                        n1_j[e] = atom2;
+                        n1_shift[3 * e] = sh0; n1_shift[3 * e + 1] = sh1; n1_shift[3 * e + 2] = sh2;
+                        n1_r[e] = d;
+                    }
+                    count++;
+                }
+            }
+        }
+    }
+    if (!FILL) n1_cnt[atom1] = count;
+}
+void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
+                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom &g,
+                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, hipStream_t s) {
+    if (N <= 0) return;
+    if (fill)
+        hipLaunchKernelGGL(k_nl_pairs<true>, dim3(cdiv(N, 128)), dim3(128), 0, s, N, pos, map_shift, cell_of,
+                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r);
+    else
+        hipLaunchKernelGGL(k_nl_pairs<false>, dim3(cdiv(N, 128)), dim3(128), 0, s, N, pos, map_shift, cell_of,
+                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r);
+}
+
+__global__ void k_nl_n2_count(int N, const int *n1_off, const int *n1_j, int *n2_cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int e = n1_off[i]; e < n1_off[i + 1]; e++) {
+        int j = n1_j[e];
+        if (j != i) atomicAdd(&n2_cnt[j], 1);
+    }
+}
+void launch_nl_n2_count(int N, const int *n1_off, const int *n1_j, int *n2_cnt, hipStream_t s) {
+    if (N > 0) hipLaunchKernelGGL(k_nl_n2_count, dim3(cdiv(N, 256)), dim3(256), 0, s, N, n1_off, n1_j, n2_cnt);
+}
+__global__ void k_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_off, int *n2_fill, int *n2_i,
+                             int *n2_k) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int e = n1_off[i]; e < n1_off[i + 1]; e++) {
+        int j = n1_j[e];
+        if (j == i) continue;
+        int slot = atomicAdd(&n2_fill[j], 1);
+        n2_i[n2_off[j] + slot] = i;
+        n2_k[n2_off[j] + slot] = e - n1_off[i];
+    }
+}
+void launch_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_off, int *n2_fill, int *n2_i,
+                       int *n2_k, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_nl_n2_fill, dim3(cdiv(N, 256)), dim3(256), 0, s, N, n1_off, n1_j, n2_off, n2_fill, n2_i, n2_k);
+}
+// neighbour2(j) in the reference's append order: the global loop visits atom1
+// by (cell linear index, position in its descending cell list), then by its
+// neighbour1 entry index.  Also records, for every neighbour1 entry, the slot of
+// its mirror in neighbour2.
+__global__ void k_nl_n2_sort(int N, const int *n2_off, int *n2_i, int *n2_k, const int *cell_of, const int *n1_off,
+                             int *n1_rev) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    int b = n2_off[j], n = n2_off[j + 1] - b;
+    int *I = n2_i + b, *Kk = n2_k + b;
+    for (int x = 1; x < n; x++) {
+        int vi = I[x], vk = Kk[x], vc = cell_of[vi];
+        int y = x - 1;
+        while (y >= 0) {
+            int ci = cell_of[I[y]];
+            bool after = (ci > vc) || (ci == vc && (I[y] < vi || (I[y] == vi && Kk[y] > vk)));
+            if (!after) break;
+            I[y + 1] = I[y]; Kk[y + 1] = Kk[y];
+            y--;
+        }
+        I[y + 1] = vi; Kk[y + 1] = vk;
+    }
+    for (int x = 0; x < n; x++) n1_rev[n1_off[I[x]] + Kk[x]] = x;
+}
+void launch_nl_n2_sort(int N, const int *n2_off, int *n2_i, int *n2_k, const int *cell_of, const int *n1_off,
+                       int *n1_rev, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_nl_n2_sort, dim3(cdiv(N, 128)), dim3(128), 0, s, N, n2_off, n2_i, n2_k, cell_of, n1_off, n1_rev);
+}
+
+// Full per-atom neighbour view in connection_neighbour order (neighbour2 first,
+// Connection.f95:2038-2051), with diff = pos_j - pos_i + lattice.shift summed
+// m = 1..3 (:2120-2125) and the index of the mirror pair.
+__global__ void k_nl_full(int N, const double *__restrict__ pos, NLGeom g, const int *n1_off, const int *n1_j,
+                          const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i,
+                          const int *n2_k, const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r,
+                          double *nb_d) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    int base = n1_off[i] + n2_off[i];
+    nb_off[i] = base;
+    if (i == N) return;
+    int n2c = n2_off[i + 1] - n2_off[i];
+    int n1c = n1_off[i + 1] - n1_off[i];
+    for (int q = 0; q < n2c + n1c; q++) {
+        int j, e1, sgn, rev;
+        if (q < n2c) {
+            j = n2_i[n2_off[i] + q];
+            e1 = n1_off[j] + n2_k[n2_off[i] + q];
+            sgn = -1;
+            rev = n1_off[j] + n2_off[j] + (n2_off[j + 1] - n2_off[j]) + n2_k[n2_off[i] + q];
+        } else {
+            e1 = n1_off[i] + (q - n2c);
+            j = n1_j[e1];
+            sgn = 1;
+            if (j != i) rev = n1_off[j] + n2_off[j] + n1_rev[e1];
+            else {
+                rev = -1;
+                for (int t = n1_off[i]; t < n1_off[i + 1]; t++)
+                    if (n1_j[t] == i && n1_shift[3 * t] == -n1_shift[3 * e1] && n1_shift[3 * t + 1] == -n1_shift[3 * e1 + 1] &&
+                        n1_shift[3 * t + 2] == -n1_shift[3 * e1 + 2]) { rev = base + n2c + (t - n1_off[i]); break; }
+                if (rev < 0) rev = base + q;   // mirror image missing (cannot happen for an exact cutoff test)
+            }
+        }
+        int s0 = sgn * n1_shift[3 * e1], s1 = sgn * n1_shift[3 * e1 + 1], s2 = sgn * n1_shift[3 * e1 + 2];
+        double d0 = pos[3 * j] - pos[3 * i], d1 = pos[3 * j + 1] - pos[3 * i + 1], d2 = pos[3 * j + 2] - pos[3 * i + 2];
+        d0 = d0 + g.lat[0] * (double)s0; d1 = d1 + g.lat[1] * (double)s0; d2 = d2 + g.lat[2] * (double)s0;
+        d0 = d0 + g.lat[3] * (double)s1; d1 = d1 + g.lat[4] * (double)s1; d2 = d2 + g.lat[5] * (double)s1;
+        d0 = d0 + g.lat[6] * (double)s2; d1 = d1 + g.lat[7] * (double)s2; d2 = d2 + g.lat[8] * (double)s2;
+        int p = base + q;
+        nb_j[p] = j;
+        nb_rev[p] = rev;
+        nb_r[p] = n1_r[e1];
+        nb_d[3 * p] = d0; nb_d[3 * p + 1] = d1; nb_d[3 * p + 2] = d2;
+    }
+}
+void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off, const int *n1_j,
+                    const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
+                    const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 128)), dim3(128), 0, s, N, pos, g, n1_off, n1_j, n1_shift, n1_r,
+                       n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d);
+}
+
+// min / max of fractional coordinates (g .mult. pos) per axis, for calc_lat_eff.
+__global__ void k_minmax_frac(int N, const double *pos, const double *gmat, double *out) {
+#pragma clang fp contract(off)
+    __shared__ double smin[3][256], smax[3][256];
+    double g[9];
+    for (int k = 0; k < 9; k++) g[k] = gmat[k];
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+        double lp[3];
+        mult3_exact(g, pos[3 * n], pos[3 * n + 1], pos[3 * n + 2], lp);
+        for (int a = 0; a < 3; a++) { mn[a] = fmin(mn[a], lp[a]); mx[a] = fmax(mx[a], lp[a]); }
+    }
+    for (int a = 0; a < 3; a++) { smin[a][threadIdx.x] = mn[a]; smax[a][threadIdx.x] = mx[a]; }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int a = 0; a < 3; a++) {
+                smin[a][threadIdx.x] = fmin(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+                smax[a][threadIdx.x] = fmax(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; a++) { out[a] = smin[a][0]; out[3 + a] = smax[a][0]; }
+}
+void launch_minmax_frac(int N, const double *pos, const double *g, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_minmax_frac, dim3(1), dim3(256), 0, s, N, pos, g, out);
+}
+
+// ----------------------------------------------------------------------------
+// SOAP
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double fcut_d(double r, double rc, double w) {
+    if (r > rc) return 0.0;
+    if (r > rc - w) return 0.5 * (cos(GAPX_PI * (r - rc + w) / w) + 1.0);
+    return 1.0;
+}
+__device__ __forceinline__ double dfcut_d(double r, double rc, double w) {
+    if (r > rc) return 0.0;
+    if (r > rc - w) return -0.5 * GAPX_PI * sin(GAPX_PI * (r - rc + w) / w) / w;
+    return 0.0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    // fixed butterfly tree over the 64 lanes of a wavefront (deterministic)
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int species_of(const SoapDev &sp, int z) {
+    for (int s = 0; s < sp.S; s++)
+        if (sp.species_Z[s] == z || sp.species_Z[s] == 0) return s;
+    return -1;
+}
+
+// Radial functions f_l(r; r_a) = exp(-alpha(r^2+r_a^2)) i_l(2 alpha r r_a) by the
+// upward recursion i_(l+1) = i_(l-1) - (2l+1)/x i_l started from
+// i_(-1) = cosh(x)/x and i_0 = sinh(x)/x.  State per grid point: F_l and F_(l+1).
+template <int NMAX>
+struct RadialState {
+    double F0[NMAX], F1[NMAX], invx[NMAX];
+    __device__ __forceinline__ void init(const SoapDev &sp, double r) {
+#pragma unroll
+        for (int a = 0; a < NMAX; a++) {
+            if (a >= sp.n) { F0[a] = F1[a] = invx[a] = 0.0; continue; }
+            double ra = sp.rb[a];
+            double x = 2.0 * sp.alpha * r * ra;
+            if (x == 0.0) {
+                F0[a] = exp(-sp.alpha * (ra * ra + r * r));
+                F1[a] = 0.0;
+                invx[a] = 0.0;
+            } else {
+                double ep = exp(-sp.alpha * (r + ra) * (r + ra)), em = exp(-sp.alpha * (r - ra) * (r - ra));
+                double ix = 1.0 / x;
+                double fm = 0.5 * (em + ep) * ix;
+                F0[a] = 0.5 * (em - ep) * ix;
+                F1[a] = fm - F0[a] * ix;
+                invx[a] = ix;
+            }
+        }
+    }
+    // advance from (F_l, F_(l+1)) to (F_(l+1), F_(l+2))
+    __device__ __forceinline__ void step(int l) {
+#pragma unroll
+        for (int a = 0; a < NMAX; a++) {
+            double n2 = F0[a] - (2.0 * l + 3.0) * invx[a] * F1[a];
+            F0[a] = F1[a];
+            F1[a] = (invx[a] == 0.0) ? 0.0 : n2;
+        }
+    }
+};
+
+template <int L, int NMAX>
+__global__ __launch_bounds__(64) void k_soap_fwd(SoapDev sp, const int *__restrict__ centres, const int *__restrict__ Z,
+                                                 const int *__restrict__ nb_off, const int *__restrict__ nb_j,
+                                                 const double *__restrict__ nb_r, const double *__restrict__ nb_d,
+                                                 double *X, double *norm_out, double *cbuf, int NB, int rows_cap) {
+    extern __shared__ double lds[];
+    constexpr int LM = (L + 1) * (L + 1);
+    const int n = sp.n, KLM = sp.K * LM, RL = n * (L + 1) + LM;
+    double *cL = lds;                         // K*LM coefficients
+    double *rows = lds + KLM;                 // NB rows of [g (n*(L+1)) | Y (LM)], later the power spectrum
+    int *spc = (int *)(rows + rows_cap);
+    const int lane = threadIdx.x;
+    const int desc = blockIdx.x;
+    const int ci = centres[desc];
+    for (int e = lane; e < KLM; e += 64) cL[e] = 0.0;
+    __syncthreads();
+    if (lane == 0) {
+        int sc = species_of(sp, Z[ci]);
+        if (sc >= 0) cL[(sc * n) * LM] += sp.cw * sp.u00 * c_ynorm[0];
+    }
+    const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
+    for (int c0 = 0; c0 < np; c0 += NB) {
+        const int nk = min(NB, np - c0);
+        __syncthreads();
+        if (lane < nk) {
+            const int p = p0 + c0 + lane;
+            double r = nb_r[p];
+            int s = species_of(sp, Z[nb_j[p]]);
+            double *row = rows + (size_t)lane * RL;
+            if (r > sp.rc || s < 0) {
+                s = -1;
+            } else {
+                double fc = fcut_d(r, sp.rc, sp.ctw);
+                RadialState<NMAX> rs;
+                rs.init(sp, r);
+                for (int l = 0; l <= L; l++) {
+                    for (int b = 0; b < n; b++) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int a = 0; a < NMAX; a++)
+                            if (a < n) acc += rs.F0[a] * sp.T[a * n + b];
+                        row[b * (L + 1) + l] = fc * acc;
+                    }
+                    rs.step(l);
+                }
+                // real spherical harmonics of u = d/r
+                double ir = 1.0 / r;
+                double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
+                double Pm1[L + 2], Pm2[L + 2], Pc[L + 2];
+                double *Yr = row + n * (L + 1);
+#pragma unroll
+                for (int l = 0; l <= L; l++) {
+#pragma unroll
+                    for (int m = 0; m <= l; m++) {
+                        if (m == l) Pc[m] = c_dfact[m];
+                        else if (m == l - 1) Pc[m] = (2.0 * l - 1.0) * uz * Pm1[m];
+                        else Pc[m] = ((2.0 * l - 1.0) * uz * Pm1[m] - (double)(l + m - 1) * Pm2[m]) * (1.0 / (double)(l - m));
+                    }
+                    double A = 1.0, B = 0.0;
+#pragma unroll
+                    for (int m = 0; m <= l; m++) {
+                        if (m > 0) { double An = A * ux - B * uy; B = A * uy + B * ux; A = An; }
+                        double N = c_ynorm[l * (l + 1) / 2 + m] * Pc[m];
+                        if (m == 0) Yr[l * l + l] = N;
+                        else { Yr[l * l + l + m] = N * A; Yr[l * l + l - m] = N * B; }
+                    }
+#pragma unroll
+                    for (int m = 0; m <= l; m++) { Pm2[m] = Pm1[m]; Pm1[m] = Pc[m]; }
+                }
+            }
+            spc[lane] = s;
+        }
+        __syncthreads();
+        for (int e = lane; e < KLM; e += 64) {
+            int s = e / (n * LM), rem = e % (n * LM), b = rem / LM, lm = rem % LM;
+            int l = (int)sqrt((double)lm + 0.5);
+            if (l * l > lm) l--;
+            double acc = 0.0;
+            for (int k = 0; k < nk; k++) {
+                if (spc[k] != s) continue;
+                const double *row = rows + (size_t)k * RL;
+                acc += row[b * (L + 1) + l] * row[n * (L + 1) + lm];
+            }
+            cL[e] += acc;
+        }
+    }
+    __syncthreads();
+    // power spectrum -> rows area (reused), then normalise
+    double *pb = rows;
+    const int d = sp.dim;
+    double ss = 0.0;
+    for (int q = lane; q < d - 1; q += 64) {
+        int code = sp.qmap[q];
+        int ia = code >> 16, jb = (code >> 8) & 0xff, l = code & 0xff;
+        double v = 0.0;
+        for (int m = -l; m <= l; m++) v += cL[ia * LM + l * l + l + m] * cL[jb * LM + l * l + l + m];
+        v *= (ia != jb ? 1.4142135623730950488 : 1.0) / sqrt(2.0 * l + 1.0);
+        pb[q] = v;
+        ss += v * v;
+    }
+    ss = wave_sum(ss);
+    double nrm = sqrt(ss);
+    double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    double *xo = X + (size_t)desc * d;
+    for (int q = lane; q < d - 1; q += 64) xo[q] = pb[q] * inv;
+    if (lane == 0) { xo[d - 1] = 0.0; norm_out[desc] = nrm; }
+    double *co = cbuf + (size_t)desc * KLM;
+    for (int e = lane; e < KLM; e += 64) co[e] = cL[e];
+}
+
+template <int L, int NMAX>
+__global__ __launch_bounds__(64) void k_soap_bwd(SoapDev sp, const int *__restrict__ centres, const int *__restrict__ Z,
+                                                 const int *__restrict__ nb_off, const int *__restrict__ nb_j,
+                                                 const double *__restrict__ nb_r, const double *__restrict__ nb_d,
+                                                 const double *__restrict__ X, const double *__restrict__ G,
+                                                 const double *__restrict__ norm, const double *__restrict__ cbuf,
+                                                 double *pairF) {
+    extern __shared__ double lds[];
+    constexpr int LM = (L + 1) * (L + 1);
+    const int n = sp.n, K = sp.K, KLM = K * LM, d = sp.dim;
+    double *cL = lds;              // K*LM
+    double *dcL = lds + KLM;       // K*LM
+    double *dpL = lds + 2 * KLM;   // d
+    const int lane = threadIdx.x;
+    const int desc = blockIdx.x;
+    const int ci = centres[desc];
+    const double *x = X + (size_t)desc * d, *gg = G + (size_t)desc * d;
+    double dot = 0.0;
+    for (int q = lane; q < d; q += 64) dot += x[q] * gg[q];
+    dot = wave_sum(dot);
+    const double nrm = norm[desc];
+    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
+    for (int e = lane; e < KLM; e += 64) cL[e] = cbuf[(size_t)desc * KLM + e];
+    __syncthreads();
+    // dE/dc_(ia,lm) = sum_jb W_l(ia,jb) c_(jb,lm); W_l = dp * (2 on the diagonal, sqrt2 off it) / sqrt(2l+1)
+    for (int e = lane; e < KLM; e += 64) {
+        int ia = e / LM, lm = e % LM;
+        int l = (int)sqrt((double)lm + 0.5);
+        if (l * l > lm) l--;
+        double acc = 0.0;
+        for (int jb = 0; jb < K; jb++) {
+            int hi = ia > jb ? ia : jb, lo = ia > jb ? jb : ia;
+            int q = (hi * (hi + 1) / 2 + lo) * (L + 1) + l;
+            double w = dpL[q] * (ia == jb ? 2.0 : 1.4142135623730950488);
+            acc += w * cL[jb * LM + lm];
+        }
+        dcL[e] = acc / sqrt(2.0 * l + 1.0);
+    }
+    __syncthreads();
+    const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
+    for (int k = lane; k < np; k += 64) {
+        const int p = p0 + k;
+        const double r = nb_r[p];
+        const int s = species_of(sp, Z[nb_j[p]]);
+        if (r > sp.rc || s < 0) continue;
+        const double fc = fcut_d(r, sp.rc, sp.ctw), dfc = dfcut_d(r, sp.rc, sp.ctw);
+        const double ir = 1.0 / r;
+        const double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
+        const double *dcs = dcL + (size_t)s * n * LM;
+        RadialState<NMAX> rs;
+        rs.init(sp, r);
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        double Pm1[L + 2], Pm2[L + 2], Pc[L + 2];
+#pragma unroll
+        for (int l = 0; l <= L; l++) {
+            double gb[NMAX], dgb[NMAX];
+            const double cl = -2.0 * sp.alpha * r + (double)l * ir;
+#pragma unroll
+            for (int b = 0; b < NMAX; b++) {
+                double g0 = 0.0, q1 = 0.0;
+                if (b < n) {
+#pragma unroll
+                    for (int a = 0; a < NMAX; a++)
+                        if (a < n) {
+                            double t = sp.T[a * n + b];
+                            g0 += rs.F0[a] * t;
+                            q1 += sp.rb[a] * rs.F1[a] * t;
+                        }
+                }
+                double dg0 = cl * g0 + 2.0 * sp.alpha * q1;
+                gb[b] = fc * g0;
+                dgb[b] = fc * dg0 + dfc * g0;
+            }
+            rs.step(l);
+#pragma unroll
+            for (int m = 0; m <= l; m++) {
+                if (m == l) Pc[m] = c_dfact[m];
+                else if (m == l - 1) Pc[m] = (2.0 * l - 1.0) * uz * Pm1[m];
+                else Pc[m] = ((2.0 * l - 1.0) * uz * Pm1[m] - (double)(l + m - 1) * Pm2[m]) * (1.0 / (double)(l - m));
+            }
+            Pc[l + 1] = 0.0;
+            double A = 1.0, B = 0.0, Am = 0.0, Bm = 0.0;
+#pragma unroll
+            for (int m = 0; m <= l; m++) {
+                if (m > 0) { Am = A; Bm = B; double An = A * ux - B * uy; B = A * uy + B * ux; A = An; }
+                const double N = c_ynorm[l * (l + 1) / 2 + m];
+                const double P = Pc[m], Pz = Pc[m + 1];
+#pragma unroll
+                for (int sgn = 0; sgn < 2; sgn++) {
+                    if (m == 0 && sgn == 1) continue;
+                    double Y, Fx, Fy, Fz;
+                    if (m == 0) { Y = N * P; Fx = 0.0; Fy = 0.0; Fz = N * Pz; }
+                    else if (sgn == 0) { Y = N * P * A; Fx = N * P * m * Am; Fy = -N * P * m * Bm; Fz = N * Pz * A; }
+                    else { Y = N * P * B; Fx = N * P * m * Bm; Fy = N * P * m * Am; Fz = N * Pz * B; }
+                    const int lm = l * l + l + (sgn ? -m : m);
+                    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NMAX; b++)
+                        if (b < n) {
+                            double dc = dcs[b * LM + lm];
+                            s1 += dc * dgb[b];
+                            s2 += dc * gb[b];
+                        }
+                    // grad of Y(r/|r|) = (grad F - u (u . grad F)) / r
+                    const double ud = ux * Fx + uy * Fy + uz * Fz;
+                    const double a1 = s1 * Y - s2 * ud * ir, a2 = s2 * ir;
+                    fx += a1 * ux + a2 * Fx;
+                    fy += a1 * uy + a2 * Fy;
+                    fz += a1 * uz + a2 * Fz;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m <= l; m++) { Pm2[m] = Pm1[m]; Pm1[m] = Pc[m]; }
+        }
+        pairF[3 * p] += fx;
+        pairF[3 * p + 1] += fy;
+        pairF[3 * p + 2] += fz;
+    }
+}
+
+// instantiation table
+#define SOAP_CASES(X) X(2, 8) X(3, 8) X(4, 8) X(5, 8) X(6, 8) X(7, 8) X(8, 8) X(10, 8) X(12, 8) \
+                      X(2, 16) X(3, 16) X(4, 16) X(5, 16) X(6, 16) X(7, 16) X(8, 16) X(10, 16) X(12, 16)
+
+int soap_supported(int n, int L) {
+    if (n < 1 || n > 16) return 0;
+#define CHK(LL, NN) if (L == LL) return 1;
+    SOAP_CASES(CHK)
+#undef CHK
+    return 0;
+}
+
+static int pick_nmax(int n) { return n <= 8 ? 8 : 16; }
+
+void launch_soap_fwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
+                     const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
+                     double *cbuf, hipStream_t s) {
+    ensure_constants();
+    if (nc <= 0) return;
+    const int LM = (sp.L + 1) * (sp.L + 1);
+    const int RL = sp.n * (sp.L + 1) + LM;
+    const int KLM = sp.K * LM;
+    int NB = 64;
+    auto need = [&](int nb) { return (size_t)(KLM + std::max((size_t)nb * RL, (size_t)sp.dim)) * 8 + nb * 4; };
+    while (NB > 8 && need(NB) > 60 * 1024) NB /= 2;
+    const int rows_cap = (int)std::max((size_t)NB * RL, (size_t)sp.dim);
+    size_t sh = need(NB);
+    const int nm = pick_nmax(sp.n);
+#define LAUNCH(LL, NN)                                                                                          \
+    if (sp.L == LL && nm == NN) {                                                                               \
+        hipLaunchKernelGGL((k_soap_fwd<LL, NN>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j, nb_r, \
+                           nb_d, X, norm, cbuf, NB, rows_cap);                                                            \
+        return;                                                                                                 \
+    }
+    SOAP_CASES(LAUNCH)
+#undef LAUNCH
+}
+
+void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
+                     const int *nb_j, const double *nb_r, const double *nb_d, const double *X, const double *G,
+                     const double *norm, const double *cbuf, double *pairF, hipStream_t s) {
+    ensure_constants();
+    if (nc <= 0) return;
+    const int LM = (sp.L + 1) * (sp.L + 1);
+    size_t sh = (size_t)(2 * sp.K * LM + sp.dim) * 8;
+    const int nm = pick_nmax(sp.n);
+#define LAUNCH(LL, NN)                                                                                          \
+    if (sp.L == LL && nm == NN) {                                                                               \
+        hipLaunchKernelGGL((k_soap_bwd<LL, NN>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j, nb_r, \
+                           nb_d, X, G, norm, cbuf, pairF);                                                      \
+        return;                                                                                                 \
+    }
+    SOAP_CASES(LAUNCH)
+#undef LAUNCH
+}
+
+// ----------------------------------------------------------------------------
+// FP64 MFMA GEMMs.  Block tile 64x64, k-step 16, 4 wavefronts in a 2x2 grid,
+// each wavefront owns 32x32 = 2x2 tiles of v_mfma_f64_16x16x4f64.
+// Operand layout of the 16x16x4 f64 instruction (wave64):
+//   A: lane l holds A[l%16][l/16];  B: lane l holds B[l/16][l%16];
+//   C: lane l holds C[(l/16) + 4*v][l%16], v = 0..3 (rows interleaved by 4).
+// Tiles are staged k-major in LDS so both fragment reads are unit-stride.
+// ----------------------------------------------------------------------------
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 4;
+
+// load a 64 x 16 tile of a row-major matrix (rows r0.., cols k0..) into k-major LDS
+__device__ __forceinline__ void load_rows_kmajor(double (*dst)[GBM + GPAD], const double *src, int ld, int r0,
+                                                 int nrows, int k0, int nk) {
+    const int t = threadIdx.x;
+    const int row = t >> 2, kq = (t & 3) * 4;
+    const int gr = r0 + row;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int gk = k0 + kq + j;
+        dst[kq + j][row] = (gr < nrows && gk < nk) ? src[(size_t)gr * ld + gk] : 0.0;
+    }
+}
+// load a 16 x 64 tile of a row-major K x N matrix (rows k0.., cols c0..) as-is
+__device__ __forceinline__ void load_kxn(double (*dst)[GBN + GPAD], const double *src, int ld, int k0, int nk, int c0,
+                                         int ncols) {
+    const int t = threadIdx.x;
+    const int k = t >> 4, cq = (t & 15) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int gc = c0 + cq + j;
+        dst[k][cq + j] = (k0 + k < nk && gc < ncols) ? src[(size_t)(k0 + k) * ld + gc] : 0.0;
+    }
+}
+
+template <bool BT, bool EPI>
+__global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, const double *__restrict__ A,
+                                                  const double *__restrict__ B, double *__restrict__ C,
+                                                  const double *__restrict__ wvec, double zeta, int izeta,
+                                                  double *__restrict__ e_part) {
+    __shared__ double As[GBK][GBM + GPAD];
+    __shared__ double Bs[GBK][GBN + GPAD];
+    __shared__ double red[2][GBM];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
+    double4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < Kd; k0 += GBK) {
+        load_rows_kmajor(As, A, Kd, m0, Mr, k0, Kd);
+        if (BT) load_rows_kmajor(Bs, B, Kd, n0, Nc, k0, Kd);
+        else load_kxn(Bs, B, Nc, k0, Kd, n0, Nc);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; j++) b[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue
+    double rowsum[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) rowsum[i][v] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * v;
+                double kv = acc[i][j][v];
+                if (EPI) {
+                    double wv = (col < Nc) ? wvec[col] : 0.0;
+                    double kz1, kz;
+                    if (izeta > 0) {
+                        kz1 = 1.0;
+                        for (int t = 1; t < izeta; t++) kz1 *= kv;
+                        kz = kz1 * kv;
+                    } else {
+                        kz1 = pow(kv, zeta - 1.0);
+                        kz = pow(kv, zeta);
+                    }
+                    rowsum[i][v] += wv * kz;
+                    kv = wv * zeta * kz1;
+                }
+                if (row < Mr && col < Nc) C[(size_t)row * Nc + col] = kv;
+            }
+        }
+    if (EPI) {
+        // sum over the 16 columns held by lanes with equal lane/16, then over the two wn halves
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                double s = rowsum[i][v];
+                for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+                rowsum[i][v] = s;
+            }
+        if ((lane & 15) == 0)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) red[wn][wm * 32 + i * 16 + (lane >> 4) + 4 * v] = rowsum[i][v];
+        __syncthreads();
+        if (threadIdx.x < GBM) {
+            const int row = m0 + threadIdx.x;
+            if (row < Mr) e_part[(size_t)row * gridDim.y + blockIdx.y] = red[0][threadIdx.x] + red[1][threadIdx.x];
+        }
+    }
+}
+
+static int int_zeta(double z) {
+    double r = floor(z + 0.5);
+    return (fabs(z - r) == 0.0 && r >= 1.0 && r <= 16.0) ? (int)r : 0;
+}
+
+void launch_gemm1_f64(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
+                      double *W, double *e_part, int *ntile_out, hipStream_t s) {
+    dim3 grid(cdiv(nc, GBM), cdiv(M, GBN));
+    *ntile_out = grid.y;
+    if (nc <= 0) return;
+    hipLaunchKernelGGL((k_gemm_f64<true, true>), grid, dim3(256), 0, s, nc, M, d, X, Xs, W, wvec, zeta,
+                       int_zeta(zeta), e_part);
+}
+void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, hipStream_t s) {
+    dim3 grid(cdiv(nc, GBM), cdiv(d, GBN));
+    if (nc <= 0) return;
+    hipLaunchKernelGGL((k_gemm_f64<false, false>), grid, dim3(256), 0, s, nc, d, M, W, Xs, G, (const double *)nullptr,
+                       0.0, 0, (double *)nullptr);
+}
+
+// e_i = sum over column tiles, fixed order
+__global__ void k_energy_rows(int nc, int ntile, const int *centres, const double *e_part, double *esite) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double s = 0.0;
+    for (int t = 0; t < ntile; t++) s += e_part[(size_t)i * ntile + t];
+    esite[centres[i]] += s;
+}
+void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_part, double *esite, hipStream_t s) {
+    if (nc > 0) hipLaunchKernelGGL(k_energy_rows, dim3(cdiv(nc, 256)), dim3(256), 0, s, nc, ntile, centres, e_part, esite);
+}
+
+// ----------------------------------------------------------------------------
+// distance_2b with the ard_se kernel, one thread per atom over its pairs
+// ----------------------------------------------------------------------------
+__global__ void k_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
+                      const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
+                      const double *nb_d, double *pairF, double *pairE) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const bool zi1 = (Z1 == 0) || (Z[i] == Z1), zi2 = (Z2 == 0) || (Z[i] == Z2);
+    const double it2 = 1.0 / (theta[0] * theta[0]);
+    for (int p = nb_off[i]; p < nb_off[i + 1]; p++) {
+        const double r = nb_r[p];
+        if (r > rc) continue;
+        const int j = nb_j[p];
+        const bool zj1 = (Z1 == 0) || (Z[j] == Z1), zj2 = (Z2 == 0) || (Z[j] == Z2);
+        if (!((zi1 && zj2) || (zi2 && zj1))) continue;
+        double e = 0.0, de = 0.0;
+        for (int s = 0; s < M; s++) {
+            double t = r - xs[s];
+            double k = wvec[s] * exp(-0.5 * t * t * it2);
+            e += k;
+            de -= k * t * it2;
+        }
+        const double fc = fcut_d(r, rc, ctw), dfc = dfcut_d(r, rc, ctw);
+        const double dE = de * fc + e * dfc;
+        const double ir = 1.0 / r;
+        pairE[p] += e * fc;
+        pairF[3 * p] += dE * nb_d[3 * p] * ir;
+        pairF[3 * p + 1] += dE * nb_d[3 * p + 1] * ir;
+        pairF[3 * p + 2] += dE * nb_d[3 * p + 2] * ir;
+    }
+}
+void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
+                const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
+                const double *nb_d, double *pairF, double *pairE, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_d2b, dim3(cdiv(N, 128)), dim3(128), 0, s, N, Z, Z1, Z2, rc, ctw, M, xs, wvec, theta,
+                           nb_off, nb_j, nb_r, nb_d, pairF, pairE);
+}
+
+// ----------------------------------------------------------------------------
+// per-atom gather: F_j = sum_q (pairF[q] - pairF[rev q]),
+// local_virial_j(a,b) = -sum_q d_(rev q),a pairF_(rev q),b,
+// local_e_j = e_site + (pairE[q] + pairE[rev q])/2 + e0(Z_j); block partial sums.
+// ----------------------------------------------------------------------------
+constexpr int GATHER_BS = 256;
+__global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const double *e0, double E_scale,
+                                                      const int *nb_off, const int *nb_rev, const double *nb_d,
+                                                      const double *pairF, const double *pairE, const double *esite,
+                                                      double *f, double *local_e, double *local_virial,
+                                                      double *block_part) {
+    __shared__ double sred[10][GATHER_BS];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double le = 0.0, lv[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) lv[k] = 0.0;
+    if (j < N) {
+        double fx = 0.0, fy = 0.0, fz = 0.0, e2 = 0.0;
+        for (int q = nb_off[j]; q < nb_off[j + 1]; q++) {
+            const int r = nb_rev[q];
+            const double gx = pairF[3 * r], gy = pairF[3 * r + 1], gz = pairF[3 * r + 2];
+            fx += pairF[3 * q] - gx;
+            fy += pairF[3 * q + 1] - gy;
+            fz += pairF[3 * q + 2] - gz;
+            const double dx = nb_d[3 * r], dy = nb_d[3 * r + 1], dz = nb_d[3 * r + 2];
+            lv[0] -= dx * gx; lv[1] -= dy * gx; lv[2] -= dz * gx;
+            lv[3] -= dx * gy; lv[4] -= dy * gy; lv[5] -= dz * gy;
+            lv[6] -= dx * gz; lv[7] -= dy * gz; lv[8] -= dz * gz;
+            if (pairE) e2 += pairE[q] + pairE[r];
+        }
+        le = esite[j] + 0.5 * e2 + e0[Z[j]];
+        le *= E_scale;
+#pragma unroll
+        for (int k = 0; k < 9; k++) lv[k] *= E_scale;
+        if (f) { f[3 * j] = E_scale * fx; f[3 * j + 1] = E_scale * fy; f[3 * j + 2] = E_scale * fz; }
+        if (local_e) local_e[j] = le;
+        if (local_virial)
+#pragma unroll
+            for (int k = 0; k < 9; k++) local_virial[9 * (size_t)j + k] = lv[k];
+    }
+    sred[0][threadIdx.x] = le;
+#pragma unroll
+    for (int k = 0; k < 9; k++) sred[1 + k][threadIdx.x] = lv[k];
+    __syncthreads();
+    for (int s = GATHER_BS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int k = 0; k < 10; k++) sred[k][threadIdx.x] += sred[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 10) block_part[(size_t)blockIdx.x * 10 + threadIdx.x] = sred[threadIdx.x][0];
+}
+void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
+                   const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
+                   double *local_e, double *local_virial, double *block_part, int *nblocks_out, hipStream_t s) {
+    int nb = cdiv(N > 0 ? N : 1, GATHER_BS);
+    *nblocks_out = nb;
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev, nb_d, pairF,
+                       pairE, esite, f, local_e, local_virial, block_part);
+}
+__global__ void k_final_reduce(int nblocks, const double *block_part, double *e_out, double *virial_out) {
+    __shared__ double sred[10][256];
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+        for (int k = 0; k < 10; k++) acc[k] += block_part[(size_t)b * 10 + k];
+#pragma unroll
+    for (int k = 0; k < 10; k++) sred[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int k = 0; k < 10; k++) sred[k][threadIdx.x] += sred[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (e_out) e_out[0] = sred[0][0];
+        if (virial_out)
+            for (int k = 0; k < 9; k++) virial_out[k] = sred[1 + k][0];
+    }
+}
+void launch_final_reduce(int nblocks, const double *block_part, double E_scale, double *e_out, double *virial_out,
+                         hipStream_t s) {
+    (void)E_scale;
+    hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(256), 0, s, nblocks, block_part, e_out, virial_out);
+}
+
+}  // namespace gapx

@@ -1,0 +1,269 @@
+// Split-precision (hi+lo) GEMMs for the dot-product GP kernel on gfx950 MFMA.
+//
+// A FP64 value x is carried as hi = round(x), lo = round(x - hi) in BF16 (or
+// FP16); products use three MFMA passes hi*hi + hi*lo + lo*hi with FP32
+// accumulation (v_mfma_f32_16x16x32_bf16 / _f16).  The descriptor
+// normalisation, the K^zeta epilogue, the energy sum and every force/virial
+// reduction stay FP64.
+//
+// Operand layout of the 16x16x32 instructions (wave64): lane l holds 8
+// consecutive k of row l%16 of A (k = 8*(l/16) .. +7) and 8 consecutive k of
+// column l%16 of B; C lane l holds rows 4*(l/16)+v, column l%16.  Both
+// operands are therefore read as 16-byte vectors from k-contiguous storage,
+// straight from global memory (L2-resident), without LDS staging.
+#include "gapx_kernels.h"
+
+#include <algorithm>
+
+namespace gapx {
+
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef short short8_t __attribute__((ext_vector_type(8)));
+
+static inline int cdivm(long a, long b) { return (int)((a + b - 1) / b); }
+
+template <int PREC>
+struct Half;
+template <>
+struct Half<1> {   // bf16
+    typedef __bf16 T;
+    typedef bf16x8_t V;
+    static __device__ __forceinline__ T from(double x) { return (T)(float)x; }
+    static __device__ __forceinline__ double to(T x) { return (double)(float)x; }
+    static __device__ __forceinline__ float4_t mfma(V a, V b, float4_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Half<2> {   // fp16
+    typedef _Float16 T;
+    typedef f16x8_t V;
+    static __device__ __forceinline__ T from(double x) { return (T)x; }
+    static __device__ __forceinline__ double to(T x) { return (double)x; }
+    static __device__ __forceinline__ float4_t mfma(V a, V b, float4_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// split rows x cols FP64 (row-major, ld src_ld) into hi/lo with leading dimension
+// ld_out; transpose=true writes the transpose (cols x rows).  Padding is zero.
+template <int PREC>
+__global__ void k_split(int rows, int cols, int ld_out, int rows_pad, const double *src, int src_ld, bool transpose,
+                        typename Half<PREC>::T *hi, typename Half<PREC>::T *lo, double scale) {
+    typedef Half<PREC> H;
+    const size_t tot = transpose ? (size_t)cols * ld_out : (size_t)rows_pad * ld_out;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
+        int r, c;
+        if (!transpose) { r = (int)(idx / ld_out); c = (int)(idx % ld_out); }
+        else { c = (int)(idx / ld_out); r = (int)(idx % ld_out); }
+        double x = (r < rows && c < cols) ? src[(size_t)r * src_ld + c] * scale : 0.0;
+        typename H::T h = H::from(x);
+        typename H::T l = H::from(x - H::to(h));
+        hi[idx] = h;
+        lo[idx] = l;
+    }
+}
+
+void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
+                         void *hi, void *lo, hipStream_t s) {
+    const int rows_pad = (rows + 63) / 64 * 64;
+    const size_t tot = transpose ? (size_t)cols * ld_out : (size_t)rows_pad * ld_out;
+    const int grid = (int)std::min<size_t>((tot + 255) / 256, 65536);
+    if (prec == 1)
+        hipLaunchKernelGGL(k_split<1>, dim3(grid), dim3(256), 0, s, rows, cols, ld_out, rows_pad, src, src_ld, transpose,
+                           (__bf16 *)hi, (__bf16 *)lo, 1.0);
+    else
+        hipLaunchKernelGGL(k_split<2>, dim3(grid), dim3(256), 0, s, rows, cols, ld_out, rows_pad, src, src_ld, transpose,
+                           (_Float16 *)hi, (_Float16 *)lo, 1.0);
+}
+
+// C[row, col] = sum_k A[row, k] B[col, k] with A, B hi/lo split, k-contiguous,
+// both with leading dimension ldk (a multiple of 32, zero padded).
+// Block = 4 wavefronts, 64x64 output tile, each wave 32x32 (2x2 MFMA tiles).
+template <int PREC>
+__device__ __forceinline__ void mma_tile(const typename Half<PREC>::T *Ahi, const typename Half<PREC>::T *Alo,
+                                         const typename Half<PREC>::T *Bhi, const typename Half<PREC>::T *Blo, int ldk,
+                                         int arow0, int amax, int bcol0, int bmax, int kdim, float4_t acc[2][2]) {
+    typedef Half<PREC> H;
+    typedef typename H::V V;
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kg = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < kdim; k0 += 32) {
+        V ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            int row = arow0 + i * 16 + r;
+            if (row < amax) {
+                ah[i] = *(const V *)(Ahi + (size_t)row * ldk + k0 + kg);
+                al[i] = *(const V *)(Alo + (size_t)row * ldk + k0 + kg);
+            } else {
+                ah[i] = (V){};
+                al[i] = (V){};
+            }
+            int col = bcol0 + i * 16 + r;
+            if (col < bmax) {
+                bh[i] = *(const V *)(Bhi + (size_t)col * ldk + k0 + kg);
+                bl[i] = *(const V *)(Blo + (size_t)col * ldk + k0 + kg);
+            } else {
+                bh[i] = (V){};
+                bl[i] = (V){};
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                acc[i][j] = H::mfma(al[i], bh[j], acc[i][j]);
+                acc[i][j] = H::mfma(ah[i], bl[j], acc[i][j]);
+                acc[i][j] = H::mfma(ah[i], bh[j], acc[i][j]);
+            }
+    }
+}
+
+// GEMM1: K = X Xs^T; epilogue w K^zeta -> e_part (FP64), W = w zeta K^(zeta-1)
+// scaled by wscale, split into hi/lo (nc x Mpad) for GEMM2.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_gemm1_split(int nc, int M, int dpad, int Mpad,
+                                                      const typename Half<PREC>::T *Xhi, const typename Half<PREC>::T *Xlo,
+                                                      const typename Half<PREC>::T *Shi, const typename Half<PREC>::T *Slo,
+                                                      const double *wvec, double zeta, int izeta, double wscale,
+                                                      typename Half<PREC>::T *Whi, typename Half<PREC>::T *Wlo,
+                                                      double *e_part) {
+    typedef Half<PREC> H;
+    __shared__ double red[2][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    float4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    mma_tile<PREC>(Xhi, Xlo, Shi, Slo, dpad, m0 + wm * 32, nc, n0 + wn * 32, M, dpad, acc);
+    double rowsum[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) rowsum[i][v] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+            const double wv = col < M ? wvec[col] : 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
+                const double kv = (double)acc[i][j][v];
+                double kz1, kz;
+                if (izeta > 0) {
+                    kz1 = 1.0;
+                    for (int t = 1; t < izeta; t++) kz1 *= kv;
+                    kz = kz1 * kv;
+                } else {
+                    kz1 = pow(kv, zeta - 1.0);
+                    kz = pow(kv, zeta);
+                }
+                rowsum[i][v] += wv * kz;
+                if (row < nc && col < Mpad) {
+                    const double wk = col < M ? wv * zeta * kz1 * wscale : 0.0;
+                    typename H::T h = H::from(wk);
+                    Whi[(size_t)row * Mpad + col] = h;
+                    Wlo[(size_t)row * Mpad + col] = H::from(wk - H::to(h));
+                }
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double s = rowsum[i][v];
+            for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            rowsum[i][v] = s;
+        }
+    if ((lane & 15) == 0)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[wn][wm * 32 + i * 16 + 4 * (lane >> 4) + v] = rowsum[i][v];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int row = m0 + threadIdx.x;
+        if (row < nc) e_part[(size_t)row * gridDim.y + blockIdx.y] = red[0][threadIdx.x] + red[1][threadIdx.x];
+    }
+}
+
+// GEMM2: G = W Xs = sum_s W[i,s] XsT[q,s]; output FP64 (nc x d), unscaled.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_gemm2_split(int nc, int d, int Mpad, const typename Half<PREC>::T *Whi,
+                                                      const typename Half<PREC>::T *Wlo,
+                                                      const typename Half<PREC>::T *Thi,
+                                                      const typename Half<PREC>::T *Tlo, double inv_wscale, double *G) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    float4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    mma_tile<PREC>(Whi, Wlo, Thi, Tlo, Mpad, m0 + wm * 32, nc, n0 + wn * 32, d, Mpad, acc);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
+                if (row < nc && col < d) G[(size_t)row * d + col] = (double)acc[i][j][v] * inv_wscale;
+            }
+        }
+}
+
+static int int_zeta_m(double z) {
+    double r = floor(z + 0.5);
+    return (fabs(z - r) == 0.0 && r >= 1.0 && r <= 16.0) ? (int)r : 0;
+}
+
+
+void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *X, const void *Xs_hi,
+                        const void *Xs_lo, const double *wvec, double zeta, double wscale, float *Wf,
+                        double *e_part, int *ntile_out, void *xsplit_ws, hipStream_t s) {
+    const int Mpad = (M + 31) / 32 * 32;
+    dim3 grid(cdivm(nc, 64), cdivm(M, 64));
+    *ntile_out = grid.y;
+    if (nc <= 0) return;
+    const size_t xsz = (size_t)((nc + 63) / 64 * 64) * dpad;
+    char *xs = (char *)xsplit_ws;
+    void *Xhi = xs, *Xlo = xs + xsz * 2;
+    launch_split_matrix(prec, nc, d, dpad, X, d, false, Xhi, Xlo, s);
+    char *wf = (char *)Wf;
+    void *Whi = wf, *Wlo = wf + (size_t)nc * Mpad * 2;
+    if (prec == 1)
+        hipLaunchKernelGGL(k_gemm1_split<1>, grid, dim3(256), 0, s, nc, M, dpad, Mpad, (const __bf16 *)Xhi,
+                           (const __bf16 *)Xlo, (const __bf16 *)Xs_hi, (const __bf16 *)Xs_lo, wvec, zeta,
+                           int_zeta_m(zeta), wscale, (__bf16 *)Whi, (__bf16 *)Wlo, e_part);
+    else
+        hipLaunchKernelGGL(k_gemm1_split<2>, grid, dim3(256), 0, s, nc, M, dpad, Mpad, (const _Float16 *)Xhi,
+                           (const _Float16 *)Xlo, (const _Float16 *)Xs_hi, (const _Float16 *)Xs_lo, wvec, zeta,
+                           int_zeta_m(zeta), wscale, (_Float16 *)Whi, (_Float16 *)Wlo, e_part);
+}
+
+void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, const float *Wf, const void *XsT_hi,
+                        const void *XsT_lo, double wscale, double *G, hipStream_t s) {
+    (void)M; (void)dpad;
+    dim3 grid(cdivm(nc, 64), cdivm(d, 64));
+    if (nc <= 0) return;
+    const char *wf = (const char *)Wf;
+    const void *Whi = wf, *Wlo = wf + (size_t)nc * Mpad * 2;
+    const double inv = 1.0 / wscale;
+    if (prec == 1)
+        hipLaunchKernelGGL(k_gemm2_split<1>, grid, dim3(256), 0, s, nc, d, Mpad, (const __bf16 *)Whi,
+                           (const __bf16 *)Wlo, (const __bf16 *)XsT_hi, (const __bf16 *)XsT_lo, inv, G);
+    else
+        hipLaunchKernelGGL(k_gemm2_split<2>, grid, dim3(256), 0, s, nc, d, Mpad, (const _Float16 *)Whi,
+                           (const _Float16 *)Wlo, (const _Float16 *)XsT_hi, (const _Float16 *)XsT_lo, inv, G);
+}
+
+}  // namespace gapx

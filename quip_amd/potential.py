@@ -1,0 +1,217 @@
+"""Host-side mirror of QUIP's Potential / IPModel_GAP interface over libgapx.so.
+
+``Potential("IP GAP", param_filename="gp.xml")`` loads a GAP XML model the way
+quippy.potential.Potential does (quippy/quippy/potential.py:43-90 ->
+Potential_Filename_Initialise, Potential.f95:438-468: the sparseX files are
+resolved relative to the XML file).  ``calc(at, args_str=...)`` follows
+IPModel_GAP_Calc (IPModel_GAP.f95:233-605): the args string names the outputs
+(``energy``, ``force``, ``virial``, ``local_energy``, ``local_virial``), unknown
+keys are ignored like ParamReader's ignore_unknown, results are stored on the
+Atoms object (info for per-config values, arrays for per-atom values) and also
+returned.  The ASE-calculator conventions of quippy are kept: stress =
+-virial / volume in Voigt order (potential.py:281-284).
+
+All arithmetic runs in the HIP library on the GPU; there is no CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib as L
+from .gap_xml import parse_args_str
+
+
+class Potential:
+    def __init__(self, args_str="IP GAP", param_filename=None, param_str=None, label=None, device=0,
+                 precision="fp64", calc_args=""):
+        kv = parse_args_str(args_str or "")
+        if label is None:
+            label = kv.get("label")
+        if param_filename is not None:
+            with open(param_filename) as fh:
+                xml = fh.read()
+            xml_dir = os.path.dirname(os.path.abspath(param_filename))
+        elif param_str is not None:
+            xml, xml_dir = param_str, os.getcwd()
+        else:
+            raise ValueError("Potential needs param_filename or param_str")
+        lib = L.lib()
+        if lib.gapx_api_version() != 1:
+            raise L.GapxError("libgapx API version %d, expected 1" % lib.gapx_api_version())
+        h = C.c_void_p()
+        raw = xml.encode()
+        L.check(lib.gapx_model_load(raw, len(raw), label.encode() if label else None, xml_dir.encode(), device,
+                                    C.byref(h)), "gapx_model_load")
+        self._h = h
+        self.calc_args = calc_args
+        self.set_precision(precision)
+        if "E_scale" in kv:
+            L.check(lib.gapx_model_set_e_scale(h, float(kv["E_scale"])), "set_e_scale")
+        self.results = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L._lib is not None:
+            L._lib.gapx_model_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_precision(self, precision):
+        L.check(L.lib().gapx_model_set_precision(self._h, L.PREC[precision]), "set_precision")
+        self.precision = precision
+
+    def cutoff(self):
+        c = C.c_double()
+        L.check(L.lib().gapx_model_cutoff(self._h, C.byref(c)), "gapx_model_cutoff")
+        return c.value
+
+    def n_coordinates(self):
+        n = C.c_int()
+        L.check(L.lib().gapx_model_info(self._h, C.byref(n), -1, None, None), "gapx_model_info")
+        return n.value
+
+    def set_timing(self, on=True):
+        L.check(L.lib().gapx_model_set_timing(self._h, 1 if on else 0), "set_timing")
+
+    def last_timings(self):
+        t = np.zeros(6)
+        L.check(L.lib().gapx_last_timings(self._h, L.ptr(t)), "last_timings")
+        return dict(zip(["neighbour_list", "soap_forward", "gemm1", "gemm2", "soap_adjoint", "d2b_gather"], t))
+
+    def calc(self, at, energy=None, force=None, virial=None, local_energy=None, local_virial=None, args_str="",
+             cutoff=None):
+        """IPModel_GAP_Calc.  `energy`, `force`... may be True (allocate) or caller arrays to fill."""
+        kv = parse_args_str((self.calc_args or "") + " " + (args_str or ""))
+        if "r_scale" in kv or "E_scale" in kv:
+            raise ValueError("IPModel_GAP_Calc: r_scale / E_scale cannot be given at calc time "
+                             "(IPModel_GAP.f95:348-350)")
+        want = 0
+        N = len(at)
+        out = {}
+
+        def req(flag, name, val, shape):
+            nonlocal want
+            on = (val is not None and val is not False) or name in kv
+            if not on:
+                return None
+            want |= flag
+            if isinstance(val, np.ndarray):
+                if val.size != int(np.prod(shape)):
+                    raise ValueError("%s has %d elements, expected %s" % (name, val.size, shape))
+                return val
+            return np.zeros(shape)
+
+        e = req(L.WANT_E, "energy", energy, (1,))
+        if e is None:
+            e = np.zeros(1)
+            want |= L.WANT_E
+        f = req(L.WANT_F, "force", force, (N, 3))
+        v = req(L.WANT_V, "virial", virial, (3, 3))
+        le = req(L.WANT_LOCAL_E, "local_energy", local_energy, (N,))
+        lv = req(L.WANT_LOCAL_V, "local_virial", local_virial, (N, 9))
+        pos, Z = L.f64(at.positions), L.i32(at.numbers)
+        lat, pbc = L.f64(at.cell), L.i32(at.pbc)
+        buf = {k: (np.zeros(x.shape) if x is not None and not x.flags["C_CONTIGUOUS"] else x)
+               for k, x in (("f", f), ("v", v), ("le", le), ("lv", lv))}
+        vbuf = np.zeros(9) if v is not None else None
+        rc = float(cutoff) if cutoff is not None else float(kv.get("cutoff", 0.0))
+        L.check(L.lib().gapx_calc(self._h, N, L.ptr(pos), L.ptr(Z), L.ptr(lat), L.ptr(pbc), rc, want, L.ptr(e),
+                                  L.ptr(buf["le"]), L.ptr(buf["f"]), L.ptr(vbuf), L.ptr(buf["lv"])), "gapx_calc")
+        out["energy"] = float(e[0])
+        at.info["energy"] = out["energy"]
+        if f is not None:
+            f[...] = buf["f"].reshape(f.shape)
+            out["force"] = f.reshape(N, 3)
+            at.arrays["force"] = out["force"]
+        if v is not None:
+            vv = vbuf.reshape(3, 3).T      # stored column-major: vbuf[3*b + a] = virial(a, b)
+            v[...] = vv.reshape(v.shape)
+            out["virial"] = vv
+            at.info["virial"] = vv
+        if le is not None:
+            le[...] = buf["le"].reshape(le.shape)
+            out["local_energy"] = le
+            at.arrays["local_energy"] = le
+        if lv is not None:
+            lv[...] = buf["lv"].reshape(lv.shape)
+            out["local_virial"] = lv.reshape(N, 9)
+            at.arrays["local_virial"] = out["local_virial"]
+        self.results = out
+        return out
+
+    # --- ASE-calculator style accessors (quippy/quippy/potential.py:177-300) ---
+    def get_property(self, name, at):
+        r = self.calc(at, force=True, virial=True)
+        if name == "energy":
+            return r["energy"]
+        if name == "forces":
+            return r["force"]
+        if name == "stress":
+            s = -r["virial"] / at.get_volume()
+            return np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])
+        raise KeyError(name)
+
+    def calc_batch(self, frames, want=("energy", "force", "virial"), cutoff=0.0):
+        """Many structures in one library call; returns per-structure dicts."""
+        off = np.zeros(len(frames) + 1, np.int32)
+        off[1:] = np.cumsum([len(a) for a in frames])
+        pos = L.f64(np.vstack([a.positions for a in frames]))
+        Z = L.i32(np.concatenate([a.numbers for a in frames]))
+        lats = L.f64(np.stack([a.cell for a in frames]))
+        pbcs = L.i32(np.stack([a.pbc for a in frames]))
+        n = off[-1]
+        E = np.zeros(len(frames))
+        F = np.zeros((n, 3)) if "force" in want else None
+        V = np.zeros((len(frames), 9)) if "virial" in want else None
+        w = L.WANT_E | (L.WANT_F if F is not None else 0) | (L.WANT_V if V is not None else 0)
+        L.check(L.lib().gapx_calc_batch(self._h, len(frames), L.ptr(off), L.ptr(pos), L.ptr(Z), L.ptr(lats),
+                                        L.ptr(pbcs), cutoff, w, L.ptr(E), None, L.ptr(F), L.ptr(V), None),
+                "gapx_calc_batch")
+        res = []
+        for s in range(len(frames)):
+            r = {"energy": E[s]}
+            if F is not None:
+                r["force"] = F[off[s]:off[s + 1]]
+            if V is not None:
+                r["virial"] = V[s].reshape(3, 3).T
+            res.append(r)
+        return res
+
+    def descriptors(self, at, coord=0, cutoff=0.0):
+        """SOAP vectors of coordinate `coord` for every centre atom (descriptor calc)."""
+        d = C.c_int()
+        L.check(L.lib().gapx_model_info(self._h, None, coord, C.byref(d), None), "gapx_model_info")
+        N = len(at)
+        X = np.zeros((max(N, 1), d.value))
+        idx = np.zeros(max(N, 1), np.int32)
+        nd = C.c_int()
+        pos, Z, lat, pbc = L.f64(at.positions), L.i32(at.numbers), L.f64(at.cell), L.i32(at.pbc)
+        L.check(L.lib().gapx_descriptors(self._h, coord, N, L.ptr(pos), L.ptr(Z), L.ptr(lat), L.ptr(pbc), cutoff,
+                                         L.ptr(X), L.ptr(idx), C.byref(nd)), "gapx_descriptors")
+        return X[:nd.value], idx[:nd.value]
+
+
+def connect(at, cutoff, device=0):
+    """QUIP calc_connect tables built on the GPU (same layout as the oracle's)."""
+    lib = L.lib()
+    pos, lat, pbc = L.f64(at.positions), L.f64(at.cell), L.i32(at.pbc)
+    h = C.c_void_p()
+    L.check(lib.gapx_connect(device, len(at), L.ptr(pos), L.ptr(lat), L.ptr(pbc), cutoff, C.byref(h)), "gapx_connect")
+    try:
+        t1, t2 = C.c_int(), C.c_int()
+        cells = np.zeros(3, np.int32)
+        L.check(lib.gapx_nl_counts(h, C.byref(t1), C.byref(t2), L.ptr(cells)), "gapx_nl_counts")
+        N = len(at)
+        out = dict(n1_off=np.zeros(N + 1, np.int32), n1_j=np.zeros(t1.value, np.int32),
+                   n1_shift=np.zeros((t1.value, 3), np.int32), n1_r=np.zeros(t1.value),
+                   n2_off=np.zeros(N + 1, np.int32), n2_j=np.zeros(t2.value, np.int32),
+                   n2_idx=np.zeros(t2.value, np.int32), map_shift=np.zeros((N, 3), np.int32), cells=cells)
+        L.check(lib.gapx_nl_export(h, *[L.ptr(out[k]) for k in ("n1_off", "n1_j", "n1_shift", "n1_r", "n2_off",
+                                                                  "n2_j", "n2_idx", "map_shift")]), "gapx_nl_export")
+        return out
+    finally:
+        lib.gapx_nl_free(h)

@@ -1,0 +1,268 @@
+"""HIP path (libgapx.so through its C-ABI) against the oracle and the reference fixtures.
+
+Tolerances: FP64 plain form 1e-10 eV/atom in energy and 1e-8 eV/A in every
+force component; bf16x3 / fp16x3 forms 1 meV/atom and 10 meV/A force RMSE.
+Neighbour tables are compared bit for bit (integers and stored distances).
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from quip_amd import configs
+from quip_amd import _lib as L
+from quip_amd.atoms import Atoms, read_xyz
+from quip_amd.gap_xml import GAPModel, GPCoordinate, Descriptor, read_gap_xml, write_gap_xml
+from quip_amd.potential import Potential, connect
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-10      # eV/atom
+F_TOL = 1e-8       # eV/A
+
+
+def _desc_fn(d, at):
+    return O.soap_descriptors(GPCoordinate("t", d, 2, 1, 0, 4, np.zeros((1, d.dim)), np.zeros(1), np.ones(1)), at)[0]
+
+
+@pytest.fixture(scope="module")
+def models(tmp_path_factory):
+    out = tmp_path_factory.mktemp("models")
+    res = {}
+    for name, builder in (("si64", configs.model_si64), ("si4096", configs.model_si4096)):
+        m = builder(_desc_fn)
+        path = str(out / (name + ".xml"))
+        write_gap_xml(path, m)
+        res[name] = (m, path)
+    # two-species model with the SiC hyperparameters (n_max 12, l_max 8) and few sparse points
+    train = [configs.antisite_swaps(configs.perturb(configs.diamond(4.36, (1, 1, 1), species=(14, 6)), 0.1,
+                                                    configs.rng_for(2, 500 + k)), 1, configs.rng_for(2, 600 + k))
+             for k in range(6)]
+    strs = [configs.soap_string(n_max=12, l_max=8, species=(14, 6), centre=14),
+            configs.soap_string(n_max=12, l_max=8, species=(14, 6), centre=6)]
+    m = configs.soap_model(_desc_fn, train, 20, 2, strs, "GAP_sic_small", alpha_std=1e-2)
+    path = str(out / "sic_small.xml")
+    write_gap_xml(path, m)
+    res["sic_small"] = (m, path)
+    return res
+
+
+def _geoms():
+    rng = np.random.default_rng(5)
+    yield "si64", configs.si64(), 5.0
+    yield "small_cell", Atoms([14, 14], [[0, 0, 0], [1.1, 1.2, 0.9]], [[2.6, 0.1, 0], [0.3, 2.4, 0], [0.2, -0.1, 2.9]]), 4.0
+    yield "skew", Atoms(np.full(12, 14), rng.uniform(0, 6, (12, 3)), [[6.0, 0, 0], [2.5, 5.5, 0], [1.0, 1.5, 6.5]]), 3.5
+    yield "slab_nonperiodic", Atoms(np.full(10, 14), rng.uniform(0, 5, (10, 3)), np.diag([5.0, 5.0, 5.0]),
+                                    (True, True, False)), 3.0
+    yield "offset", Atoms(np.full(6, 14), rng.uniform(0, 5, (6, 3)) + np.array([40.0, -17.0, 3.0]),
+                          np.diag([5.0, 5.0, 5.0])), 3.0
+    yield "si4096", configs.si4096(), 5.0
+    yield "si4096_skin", configs.si4096(), 5.5
+    yield "empty", Atoms(np.zeros(0, np.int32), np.zeros((0, 3)), np.eye(3) * 5.0), 3.0
+    yield "single_atom", Atoms([14], [[0.3, 0.2, 0.1]], np.eye(3) * 2.2), 5.0
+
+
+@pytest.mark.parametrize("case", [g[0] for g in _geoms()])
+def test_neighbour_tables_bit_exact(case):
+    name, at, rc = [g for g in _geoms() if g[0] == case][0]
+    print("case", name, len(at), rc, flush=True)
+    a = connect(at, rc)
+    b = O.neighbour_list(at, rc)
+    for k in ("n1_off", "n1_j", "n1_shift", "n2_off", "n2_j", "n2_idx", "map_shift", "cells"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["n1_r"].view(np.int64), b["n1_r"].view(np.int64))
+
+
+def test_soap_vectors_match_reference_sparse_points(golden, tmp_path):
+    g = np.load(os.path.join(golden, "si_soap_cur_points.npz"))
+    d = Descriptor.from_string("soap atom_sigma=0.5 central_weight=1.0 cutoff=4.0 cutoff_transition_width=1.0 "
+                               "l_max=8 n_max=8")
+    m = GAPModel("T", 1527075646, np.zeros(119),
+                 [GPCoordinate("T1", d, 2, 1.0, 0.0, 4.0, g["sparsex"][:2], np.zeros(2), np.ones(2))])
+    write_gap_xml(str(tmp_path / "t.xml"), m)
+    pot = Potential(param_filename=str(tmp_path / "t.xml"))
+    X, off = [], 0
+    for f, n in enumerate(g["natoms"]):
+        at = Atoms(g["numbers"][off:off + n], g["positions"][off:off + n], g["lattices"][f])
+        off += n
+        X.append(pot.descriptors(at)[0])
+    X = np.vstack(X)
+    assert np.abs(X[g["index"] - 1] - g["sparsex"]).max() < 1e-12
+
+
+def test_distance_2b_water_reference(golden):
+    pot = Potential(param_filename=os.path.join(golden, "gap_water", "GAP.xml"))
+    at = read_xyz(os.path.join(golden, "gap_water", "gap_sample.xyz"), 0)
+    e_ref, f_ref = at.info["energy"], at.arrays["force"].copy()
+    r = pot.calc(at, force=True, virial=True)
+    assert abs(r["energy"] - e_ref) < 1e-7          # reference test tolerance 1e-5
+    assert np.abs(r["force"] - f_ref).max() < 1e-7  # reference test tolerance 1e-6
+    ro = O.gap_calc(read_gap_xml(os.path.join(golden, "gap_water", "GAP.xml")), at)
+    assert np.abs(r["virial"] - ro["virial"]).max() < 1e-7
+
+
+def test_cells_smaller_than_cutoff(golden):
+    ref = json.load(open(os.path.join(golden, "potential_cell.json")))
+    pot = Potential(param_filename=os.path.join(golden, "gap_water", "GAP.xml"))
+    for c, e in zip(ref["cell_sizes"], ref["energies"]):
+        at = Atoms(ref["numbers"], ref["positions"], np.eye(3) * c)
+        assert abs(pot.calc(at)["energy"] - e) < 1e-8
+
+
+def _compare(r, ro, N, e_tol=E_TOL, f_tol=F_TOL):
+    assert abs(r["energy"] - ro["energy"]) / N < e_tol
+    assert np.abs(r["force"] - ro["forces"]).max() < f_tol
+    assert np.abs(r["virial"] - ro["virial"]).max() < f_tol * 10
+    assert np.abs(r["local_energy"] - ro["local_energy"]).max() < e_tol * 10
+    assert np.abs(r["local_virial"] - ro["local_virial"]).max() < f_tol * 10
+
+
+@pytest.mark.parametrize("name", ["si64", "si4096", "sic_small"])
+def test_fp64_matches_oracle(models, name):
+    m, path = models[name]
+    at = {"si64": configs.si64(), "si4096": configs.si4096(),
+          "sic_small": configs.antisite_swaps(configs.perturb(configs.diamond(4.36, (2, 2, 2), species=(14, 6)), 0.1,
+                                                              np.random.default_rng(9)), 3, np.random.default_rng(10))}[name]
+    print("case", name, len(at), flush=True)
+    pot = Potential(param_filename=path)
+    r = pot.calc(at.copy(), force=True, virial=True, local_energy=True, local_virial=True)
+    ro = O.gap_calc(m, at)
+    _compare(r, ro, len(at))
+
+
+def test_skin_and_nonperiodic(models):
+    """A neighbour cutoff above the model cutoff (QUIP's cutoff_skin) and a non-periodic axis."""
+    m, path = models["si64"]
+    pot = Potential(param_filename=path)
+    at = configs.si64()
+    r = pot.calc(at.copy(), force=True, virial=True, local_energy=True, local_virial=True, cutoff=5.7)
+    _compare(r, O.gap_calc(m, at, nl_cutoff=5.7), len(at))
+    slab = configs.si64(3)
+    slab.pbc = np.array([True, True, False])
+    slab.cell[2, 2] = 30.0
+    r = pot.calc(slab.copy(), force=True, virial=True, local_energy=True, local_virial=True)
+    _compare(r, O.gap_calc(m, slab), len(slab))
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+def test_split_precision_within_budget(models, prec):
+    m, path = models["si4096"]
+    at = configs.si4096()
+    pot = Potential(param_filename=path, precision=prec)
+    r = pot.calc(at.copy(), force=True, virial=True, local_energy=True)
+    ro = O.gap_calc(m, at)
+    de = abs(r["energy"] - ro["energy"]) / len(at)
+    frmse = np.sqrt(np.mean((r["force"] - ro["forces"]) ** 2))
+    print(prec, "dE/atom", de, "F rmse", frmse)
+    assert de < 1e-3 and frmse < 1e-2
+    # per-atom energies sit on the right atoms (catches accumulator layout errors)
+    assert np.abs(r["local_energy"] - ro["local_energy"]).max() < 1e-4
+
+
+def test_bitwise_repeatable(models):
+    m, path = models["si4096"]
+    pot = Potential(param_filename=path)
+    at = configs.si4096()
+    a = pot.calc(at.copy(), force=True, virial=True, local_virial=True)
+    b = pot.calc(at.copy(), force=True, virial=True, local_virial=True)
+    assert a["energy"] == b["energy"]
+    assert np.array_equal(a["force"], b["force"]) and np.array_equal(a["virial"], b["virial"])
+    assert np.array_equal(a["local_virial"], b["local_virial"])
+
+
+def test_batch_matches_single(models):
+    m, path = models["si4096"]
+    pot = Potential(param_filename=path)
+    frames = configs.batch512(n_struct=12)
+    res = pot.calc_batch(frames)
+    for at, rb in zip(frames, res):
+        r = pot.calc(at.copy(), force=True, virial=True)
+        assert rb["energy"] == r["energy"]
+        assert np.array_equal(rb["force"], r["force"])
+        assert np.array_equal(rb["virial"], r["virial"])
+
+
+def test_finite_differences_on_gpu(models):
+    m, path = models["si64"]
+    pot = Potential(param_filename=path)
+    at = configs.si64(5)
+    r = pot.calc(at.copy(), force=True)
+    h = 1e-5
+    for i, k in ((0, 0), (7, 1), (33, 2)):
+        p, q = at.copy(), at.copy()
+        p.positions[i, k] += h
+        q.positions[i, k] -= h
+        fd = -(pot.calc(p)["energy"] - pot.calc(q)["energy"]) / (2 * h)
+        assert abs(fd - r["force"][i, k]) < 1e-7
+
+
+def test_rotation_translation_invariance(models):
+    m, path = models["si64"]
+    pot = Potential(param_filename=path)
+    at = configs.si64(2)
+    e0 = pot.calc(at.copy(), force=True)
+    R = np.array([[0.36, 0.48, -0.8], [-0.8, 0.6, 0.0], [0.48, 0.64, 0.6]])
+    rot = Atoms(at.numbers, at.positions @ R.T + np.array([1.0, -2.0, 0.5]), at.cell @ R.T)
+    e1 = pot.calc(rot, force=True)
+    assert abs(e1["energy"] - e0["energy"]) < 1e-10
+    assert np.abs(e1["force"] - e0["force"] @ R.T).max() < 1e-9
+
+
+def _hip():
+    return C.CDLL("libamdhip64.so")
+
+
+def test_device_resident_entry_point(models):
+    """gapx_calc_device with positions and outputs in HBM equals gapx_calc."""
+    m, path = models["si64"]
+    pot = Potential(param_filename=path)
+    at = configs.si64()
+    ref = pot.calc(at.copy(), force=True, virial=True)
+    hip = _hip()
+    N = len(at)
+    ptrs = {}
+    for k, nb in (("pos", N * 24), ("e", 8), ("f", N * 24), ("v", 72)):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(nb)) == 0
+        ptrs[k] = p
+    try:
+        pos = L.f64(at.positions)
+        assert hip.hipMemcpy(ptrs["pos"], L.ptr(pos), C.c_size_t(N * 24), 1) == 0
+        lib = L.lib()
+        L.check(lib.gapx_calc_device(pot.handle, N, ptrs["pos"], L.ptr(L.i32(at.numbers)), L.ptr(L.f64(at.cell)),
+                                     L.ptr(L.i32(at.pbc)), 0.0, L.WANT_E | L.WANT_F | L.WANT_V, ptrs["e"], None,
+                                     ptrs["f"], ptrs["v"], None), "gapx_calc_device")
+        L.check(lib.gapx_sync(pot.handle), "gapx_sync")
+        e, f, v = np.zeros(1), np.zeros((N, 3)), np.zeros(9)
+        for host, k, nb in ((e, "e", 8), (f, "f", N * 24), (v, "v", 72)):
+            assert hip.hipMemcpy(L.ptr(host), ptrs[k], C.c_size_t(nb), 2) == 0
+        assert e[0] == ref["energy"]
+        assert np.array_equal(f, ref["force"])
+        assert np.array_equal(v.reshape(3, 3).T, ref["virial"])
+    finally:
+        for p in ptrs.values():
+            hip.hipFree(p)
+
+
+def test_errors_are_status_codes(tmp_path, golden):
+    lib = L.lib()
+    h = C.c_void_p()
+    bad = b"<GAP><GAP_params label='x'><gpSparse>"
+    st = lib.gapx_model_load(bad, len(bad), None, b".", 0, C.byref(h))
+    assert st == 2 and "XML" in L.last_error()
+    xml = open(os.path.join(golden, "gap_water", "GAP.xml")).read().encode()
+    st = lib.gapx_model_load(xml, len(xml), None, str(tmp_path).encode(), 0, C.byref(h))
+    assert st == 3 and "sparseX" in L.last_error()
+    st = lib.gapx_model_load(xml, len(xml), b"no_such_label", os.path.join(golden, "gap_water").encode(), 0,
+                             C.byref(h))
+    assert st == 2
+    pot = Potential(param_filename=os.path.join(golden, "gap_water", "GAP.xml"))
+    with pytest.raises(L.GapxError):
+        pot.calc(Atoms([1, 1], [[0, 0, 0], [1, 1, 1]], np.zeros((3, 3))))      # singular lattice
+    with pytest.raises(L.GapxError):
+        pot.calc(Atoms([1, 1], [[0, 0, 0], [1, 1, 1]], np.eye(3) * 5), cutoff=1.0)   # below model cutoff
+    with pytest.raises(ValueError):
+        pot.calc(Atoms([1, 1], [[0, 0, 0], [1, 1, 1]], np.eye(3) * 5), args_str="E_scale=2")
