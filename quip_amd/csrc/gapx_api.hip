@@ -265,7 +265,7 @@ struct Coord {
 
 struct Workspace {
     DBuf pos, Z, cell_of, map_shift, cell_cnt, cell_start, cell_fill, cell_atoms, scan_part;
-    DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k;
+    DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k, n2_is, n2_ks;
     DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite, centres;
     DBuf X, norm, cbuf, W, Wf, e_part, G, split_ws, split_ws2;
     DBuf f, le, lv, e, vir, block_part, minmax;
@@ -723,6 +723,8 @@ static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat
     CHK(w.n1_rev.ensure(T1 * 4));
     CHK(w.n2_i.ensure(T1 * 4));
     CHK(w.n2_k.ensure(T1 * 4));
+    CHK(w.n2_is.ensure(T1 * 4));
+    CHK(w.n2_ks.ensure(T1 * 4));
     const size_t P = 2 * (size_t)n1_total + 1;
     CHK(w.nb_j.ensure(P * 4));
     CHK(w.nb_rev.ensure(P * 4));
@@ -738,9 +740,9 @@ static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat
     launch_nl_n2_fill(N, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n2_off.as<int>(), w.n2_fill.as<int>(),
                       w.n2_i.as<int>(), w.n2_k.as<int>(), s);
     launch_nl_n2_sort(N, w.n2_off.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), w.cell_of.as<int>(),
-                      w.n1_off.as<int>(), w.n1_rev.as<int>(), s);
+                      w.n1_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), s);
     launch_nl_full(N, d_pos, g, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(),
-                   w.n2_off.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), w.n1_rev.as<int>(), w.nb_off.as<int>(),
+                   w.n2_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), w.nb_off.as<int>(),
                    w.nb_j.as<int>(), w.nb_rev.as<int>(), w.nb_r.as<double>(), w.nb_d.as<double>(), s);
     m->n1_total_last = n1_total;
     *P_out = (int)(P - 1);
@@ -988,8 +990,8 @@ extern "C" int gapx_connect(int device, int N, const double *pos, const double *
         const int T2 = nl->n2_off[N];
         nl->n2_j.resize(T2); nl->n2_idx.resize(T2);
         if (T2) {
-            hipMemcpy(nl->n2_j.data(), tmp.ws.n2_i.p, (size_t)T2 * 4, hipMemcpyDeviceToHost);
-            hipMemcpy(nl->n2_idx.data(), tmp.ws.n2_k.p, (size_t)T2 * 4, hipMemcpyDeviceToHost);
+            hipMemcpy(nl->n2_j.data(), tmp.ws.n2_is.p, (size_t)T2 * 4, hipMemcpyDeviceToHost);
+            hipMemcpy(nl->n2_idx.data(), tmp.ws.n2_ks.p, (size_t)T2 * 4, hipMemcpyDeviceToHost);
         }
         NLGeom g;
         double mm[6] = {0, 0, 0, 0, 0, 0};

@@ -43,8 +43,8 @@ void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, 
 void launch_nl_n2_count(int N, const int *n1_off, const int *n1_j, int *n2_cnt, hipStream_t s);
 void launch_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_off, int *n2_fill, int *n2_i,
                        int *n2_k, hipStream_t s);
-void launch_nl_n2_sort(int N, const int *n2_off, int *n2_i, int *n2_k, const int *cell_of, const int *n1_off,
-                       int *n1_rev, hipStream_t s);
+void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
+                       const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s);
 void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off, const int *n1_j,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
@@ -52,6 +52,13 @@ void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off
 void launch_minmax_frac(int N, const double *pos, const double *g, double *out, hipStream_t s);
 
 int soap_supported(int n, int L);
+int soap_fast_supported(int n, int L);
+bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
+                          const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
+                          double *cbuf, hipStream_t s);
+bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
+                          const int *nb_j, const double *nb_r, const double *nb_d, const double *X, const double *G,
+                          const double *norm, const double *cbuf, double *pairF, hipStream_t s);
 void launch_soap_fwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                      const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
                      double *cbuf, hipStream_t s);

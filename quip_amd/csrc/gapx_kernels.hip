@@ -274,15 +274,124 @@ __global__ void k_nl_pairs(int N, const double *__restrict__ pos, const int *__r
     }
     if (!FILL) n1_cnt[atom1] = count;
 }
+
+// Wave-parallel form of the same search: one wavefront per atom1, lanes over
+// the neighbour-cell offsets in the reference's loop order (k2 slowest, i2
+// fastest).  Each lane scans one cell list; an exclusive wave scan of the lane
+// counts places every lane's pairs, so neighbour1 keeps the append order.
+__device__ __forceinline__ int wave_excl_scan_i(int v, int *total) {
+    const int lane = threadIdx.x & 63;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    *total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restrict__ pos,
+                                                    const int *__restrict__ map_shift, const int *__restrict__ cell_of,
+                                                    const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
+                                                    const int *__restrict__ cell_atoms, NLGeom g, int *n1_cnt,
+                                                    const int *n1_off, int *n1_j, int *n1_shift, double *n1_r) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int atom1 = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (atom1 >= N) return;   // whole wavefront leaves together
+    const int lin = cell_of[atom1];
+    const int cc[3] = {lin % g.cells[0] + 1, (lin / g.cells[0]) % g.cells[1] + 1, lin / (g.cells[0] * g.cells[1]) + 1};
+    int lo[3], hi[3], span[3];
+    for (int a = 0; a < 3; a++) {
+        if (g.pbc[a]) { lo[a] = -g.img[a]; hi[a] = g.img[a]; }
+        else {
+            lo[a] = (g.img[a] < cc[a]) ? -g.img[a] : -cc[a] + 1;
+            hi[a] = (g.img[a] < g.cells[a] - cc[a]) ? g.img[a] : g.cells[a] - cc[a];
+        }
+        span[a] = hi[a] - lo[a] + 1;
+    }
+    const int noff = span[0] * span[1] * span[2];
+    const double p1x = pos[3 * atom1], p1y = pos[3 * atom1 + 1], p1z = pos[3 * atom1 + 2];
+    const int ms1x = map_shift[3 * atom1], ms1y = map_shift[3 * atom1 + 1], ms1z = map_shift[3 * atom1 + 2];
+    const double rc = g.cutoff;
+    int base = FILL ? n1_off[atom1] : 0;
+    int total_all = 0;
+    for (int o0 = 0; o0 < noff; o0 += 64) {
+        const int o = o0 + lane;
+        int cnt = 0;
+        int lin2 = -1, i4 = 0, j4 = 0, k4 = 0;
+        bool same_cell = false;
+        if (o < noff) {
+            const int i2 = lo[0] + o % span[0], j2 = lo[1] + (o / span[0]) % span[1], k2 = lo[2] + o / (span[0] * span[1]);
+            int k3 = 1, j3 = 1, i3 = 1;
+            if (g.cells[2] > 1) k3 = (cc[2] + k2 - 1 + g.cells[2]) % g.cells[2] + 1;
+            if (g.cells[1] > 1) j3 = (cc[1] + j2 - 1 + g.cells[1]) % g.cells[1] + 1;
+            if (g.cells[0] > 1) i3 = (cc[0] + i2 - 1 + g.cells[0]) % g.cells[0] + 1;
+            k4 = (cc[2] + k2 - k3) / g.cells[2];
+            j4 = (cc[1] + j2 - j3) / g.cells[1];
+            i4 = (cc[0] + i2 - i3) / g.cells[0];
+            if (k3 >= 1 && k3 <= g.cells[2] && j3 >= 1 && j3 <= g.cells[1] && i3 >= 1 && i3 <= g.cells[0]) {
+                lin2 = ((k3 - 1) * g.cells[1] + (j3 - 1)) * g.cells[0] + (i3 - 1);
+                same_cell = (lin2 == lin);
+            }
+        }
+        // two sweeps over this lane's cell: count, then (FILL) write at the scanned offset
+        for (int pass = 0; pass < (FILL ? 2 : 1); pass++) {
+            int w = 0, out0 = 0;
+            if (pass == 1) {
+                int tot;
+                out0 = base + wave_excl_scan_i(cnt, &tot);
+                base += tot;
+            }
+            if (lin2 >= 0) {
+                const int st = cell_start[lin2], nc = cell_cnt[lin2];
+                for (int q = 0; q < nc; q++) {
+                    const int atom2 = cell_atoms[st + q];
+                    if (atom2 < atom1) break;
+                    if (atom2 == atom1 && i4 == 0 && j4 == 0 && k4 == 0 && same_cell) continue;
+                    const int sh0 = i4 - ms1x + map_shift[3 * atom2];
+                    const int sh1 = j4 - ms1y + map_shift[3 * atom2 + 1];
+                    const int sh2 = k4 - ms1z + map_shift[3 * atom2 + 2];
+                    double d0 = pos[3 * atom2] - p1x, d1 = pos[3 * atom2 + 1] - p1y, d2 = pos[3 * atom2 + 2] - p1z;
+                    d0 = d0 + g.lat[0] * (double)sh0; d1 = d1 + g.lat[1] * (double)sh0; d2 = d2 + g.lat[2] * (double)sh0;
+                    d0 = d0 + g.lat[3] * (double)sh1; d1 = d1 + g.lat[4] * (double)sh1; d2 = d2 + g.lat[5] * (double)sh1;
+                    d0 = d0 + g.lat[6] * (double)sh2; d1 = d1 + g.lat[7] * (double)sh2; d2 = d2 + g.lat[8] * (double)sh2;
+                    if (d0 > rc || d1 > rc || d2 > rc) continue;
+                    double dsq = d0 * d0;
+                    dsq = dsq + d1 * d1;
+                    dsq = dsq + d2 * d2;
+                    const double d = sqrt(dsq);
+                    if (!(d < rc)) continue;
+                    if (pass == 1) {
+                        const int e = out0 + w;
+                        n1_j[e] = atom2;
+                        n1_shift[3 * e] = sh0; n1_shift[3 * e + 1] = sh1; n1_shift[3 * e + 2] = sh2;
+                        n1_r[e] = d;
+                    }
+                    w++;
+                }
+            }
+            if (pass == 0) cnt = w;
+        }
+        if (!FILL) {
+            int tot;
+            wave_excl_scan_i(cnt, &tot);
+            total_all += tot;
+        }
+    }
+    if (!FILL && lane == 0) n1_cnt[atom1] = total_all;
+}
+
 void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
                      const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom &g,
                      int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, hipStream_t s) {
     if (N <= 0) return;
     if (fill)
-        hipLaunchKernelGGL(k_nl_pairs<true>, dim3(cdiv(N, 128)), dim3(128), 0, s, N, pos, map_shift, cell_of,
+        hipLaunchKernelGGL(k_nl_pairs_w<true>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
                            cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r);
     else
-        hipLaunchKernelGGL(k_nl_pairs<false>, dim3(cdiv(N, 128)), dim3(128), 0, s, N, pos, map_shift, cell_of,
+        hipLaunchKernelGGL(k_nl_pairs_w<false>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
                            cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r);
 }
 
@@ -316,50 +425,55 @@ void launch_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_
 }
 // neighbour2(j) in the reference's append order: the global loop visits atom1
 // by (cell linear index, position in its descending cell list), then by its
-// neighbour1 entry index.  Also records, for every neighbour1 entry, the slot of
-// its mirror in neighbour2.
-__global__ void k_nl_n2_sort(int N, const int *n2_off, int *n2_i, int *n2_k, const int *cell_of, const int *n1_off,
-                             int *n1_rev) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+// neighbour1 entry index.  One wavefront per atom ranks its entries by that key
+// (all keys distinct) and writes them in order; it also records, for every
+// neighbour1 entry, the slot of its mirror in neighbour2.
+__global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k,
+                                                    const int *cell_of, const int *n1_off, int *n2_is, int *n2_ks,
+                                                    int *n1_rev) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= N) return;
-    int b = n2_off[j], n = n2_off[j + 1] - b;
-    int *I = n2_i + b, *Kk = n2_k + b;
-    for (int x = 1; x < n; x++) {
-        int vi = I[x], vk = Kk[x], vc = cell_of[vi];
-        int y = x - 1;
-        while (y >= 0) {
-            int ci = cell_of[I[y]];
-            bool after = (ci > vc) || (ci == vc && (I[y] < vi || (I[y] == vi && Kk[y] > vk)));
-            if (!after) break;
-            I[y + 1] = I[y]; Kk[y + 1] = Kk[y];
-            y--;
+    const int b = n2_off[j], n = n2_off[j + 1] - b;
+    for (int x = lane; x < n; x += 64) {
+        const int vi = n2_i[b + x], vk = n2_k[b + x], vc = cell_of[vi];
+        int rank = 0;
+        for (int y = 0; y < n; y++) {
+            const int yi = n2_i[b + y], yk = n2_k[b + y], yc = cell_of[yi];
+            const bool before = (yc < vc) || (yc == vc && (yi > vi || (yi == vi && yk < vk)));
+            rank += before ? 1 : 0;
         }
-        I[y + 1] = vi; Kk[y + 1] = vk;
+        n2_is[b + rank] = vi;
+        n2_ks[b + rank] = vk;
+        n1_rev[n1_off[vi] + vk] = rank;
     }
-    for (int x = 0; x < n; x++) n1_rev[n1_off[I[x]] + Kk[x]] = x;
 }
-void launch_nl_n2_sort(int N, const int *n2_off, int *n2_i, int *n2_k, const int *cell_of, const int *n1_off,
-                       int *n1_rev, hipStream_t s) {
+void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
+                       const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s) {
     if (N > 0)
-        hipLaunchKernelGGL(k_nl_n2_sort, dim3(cdiv(N, 128)), dim3(128), 0, s, N, n2_off, n2_i, n2_k, cell_of, n1_off, n1_rev);
+        hipLaunchKernelGGL(k_nl_n2_sort, dim3(cdiv(N, 4)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of, n1_off,
+                           n2_is, n2_ks, n1_rev);
 }
 
 // Full per-atom neighbour view in connection_neighbour order (neighbour2 first,
 // Connection.f95:2038-2051), with diff = pos_j - pos_i + lattice.shift summed
-// m = 1..3 (:2120-2125) and the index of the mirror pair.
-__global__ void k_nl_full(int N, const double *__restrict__ pos, NLGeom g, const int *n1_off, const int *n1_j,
-                          const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i,
-                          const int *n2_k, const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r,
-                          double *nb_d) {
+// m = 1..3 (:2120-2125) and the index of the mirror pair.  One wavefront per
+// atom, lanes over its entries.
+__global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict__ pos, NLGeom g,
+                                                 const int *n1_off, const int *n1_j, const int *n1_shift,
+                                                 const double *n1_r, const int *n2_off, const int *n2_i,
+                                                 const int *n2_k, const int *n1_rev, int *nb_off, int *nb_j,
+                                                 int *nb_rev, double *nb_r, double *nb_d) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i > N) return;
-    int base = n1_off[i] + n2_off[i];
-    nb_off[i] = base;
+    const int base = n1_off[i] + n2_off[i];
+    if (lane == 0) nb_off[i] = base;
     if (i == N) return;
-    int n2c = n2_off[i + 1] - n2_off[i];
-    int n1c = n1_off[i + 1] - n1_off[i];
-    for (int q = 0; q < n2c + n1c; q++) {
+    const int n2c = n2_off[i + 1] - n2_off[i];
+    const int n1c = n1_off[i + 1] - n1_off[i];
+    for (int q = lane; q < n2c + n1c; q += 64) {
         int j, e1, sgn, rev;
         if (q < n2c) {
             j = n2_i[n2_off[i] + q];
@@ -379,12 +493,12 @@ __global__ void k_nl_full(int N, const double *__restrict__ pos, NLGeom g, const
                 if (rev < 0) rev = base + q;   // mirror image missing (cannot happen for an exact cutoff test)
             }
         }
-        int s0 = sgn * n1_shift[3 * e1], s1 = sgn * n1_shift[3 * e1 + 1], s2 = sgn * n1_shift[3 * e1 + 2];
+        const int s0 = sgn * n1_shift[3 * e1], s1 = sgn * n1_shift[3 * e1 + 1], s2 = sgn * n1_shift[3 * e1 + 2];
         double d0 = pos[3 * j] - pos[3 * i], d1 = pos[3 * j + 1] - pos[3 * i + 1], d2 = pos[3 * j + 2] - pos[3 * i + 2];
         d0 = d0 + g.lat[0] * (double)s0; d1 = d1 + g.lat[1] * (double)s0; d2 = d2 + g.lat[2] * (double)s0;
         d0 = d0 + g.lat[3] * (double)s1; d1 = d1 + g.lat[4] * (double)s1; d2 = d2 + g.lat[5] * (double)s1;
         d0 = d0 + g.lat[6] * (double)s2; d1 = d1 + g.lat[7] * (double)s2; d2 = d2 + g.lat[8] * (double)s2;
-        int p = base + q;
+        const int p = base + q;
         nb_j[p] = j;
         nb_rev[p] = rev;
         nb_r[p] = n1_r[e1];
@@ -395,7 +509,7 @@ void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 128)), dim3(128), 0, s, N, pos, g, n1_off, n1_j, n1_shift, n1_r,
+    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 4)), dim3(256), 0, s, N, pos, g, n1_off, n1_j, n1_shift, n1_r,
                        n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d);
 }
 
@@ -739,6 +853,7 @@ static int pick_nmax(int n) { return n <= 8 ? 8 : 16; }
 void launch_soap_fwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                      const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
                      double *cbuf, hipStream_t s) {
+    if (launch_soap_fwd_fast(sp, nc, centres, Z, nb_off, nb_j, nb_r, nb_d, X, norm, cbuf, s)) return;
     ensure_constants();
     if (nc <= 0) return;
     const int LM = (sp.L + 1) * (sp.L + 1);
@@ -763,6 +878,7 @@ void launch_soap_fwd(const SoapDev &sp, int nc, const int *centres, const int *Z
 void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                      const int *nb_j, const double *nb_r, const double *nb_d, const double *X, const double *G,
                      const double *norm, const double *cbuf, double *pairF, hipStream_t s) {
+    if (launch_soap_bwd_fast(sp, nc, centres, Z, nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, s)) return;
     ensure_constants();
     if (nc <= 0) return;
     const int LM = (sp.L + 1) * (sp.L + 1);

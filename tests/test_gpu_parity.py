@@ -266,3 +266,28 @@ def test_errors_are_status_codes(tmp_path, golden):
         pot.calc(Atoms([1, 1], [[0, 0, 0], [1, 1, 1]], np.eye(3) * 5), cutoff=1.0)   # below model cutoff
     with pytest.raises(ValueError):
         pot.calc(Atoms([1, 1], [[0, 0, 0], [1, 1, 1]], np.eye(3) * 5), args_str="E_scale=2")
+
+
+def test_fortran_host_module(models, tmp_path):
+    """The amdflang IPModel_GAPX module over the C-ABI gives the Python host's numbers."""
+    import subprocess
+    drv = os.path.join(os.path.dirname(L.LIB_PATH), "fortran", "gapx_calc_driver")
+    if not os.path.exists(drv):
+        pytest.skip("Fortran driver not built (amdflang absent)")
+    m, path = models["si64"]
+    at = configs.si64(4)
+    inp = tmp_path / "in.txt"
+    with open(inp, "w") as fh:
+        fh.write(path + "\n%d\n" % len(at))
+        fh.write(" ".join("%.17e" % x for x in at.cell.ravel()) + "\n1 1 1\n")
+        for z, p in zip(at.numbers, at.positions):
+            fh.write("%d %.17e %.17e %.17e\n" % (z, p[0], p[1], p[2]))
+    out = subprocess.run([drv, str(inp)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    e = float([ln.split()[1] for ln in out.stdout.splitlines() if ln.startswith("ENERGY")][0])
+    f = np.array([[float(x) for x in ln.split()[2:5]] for ln in out.stdout.splitlines() if ln.startswith("FORCE")])
+    v = np.array([[float(x) for x in ln.split()[1:4]] for ln in out.stdout.splitlines() if ln.startswith("VIRIAL")])
+    r = Potential(param_filename=path).calc(at.copy(), force=True, virial=True)
+    assert abs(e - r["energy"]) < 1e-12
+    assert np.abs(f - r["force"]).max() < 1e-14
+    assert np.abs(v - r["virial"]).max() < 1e-13
