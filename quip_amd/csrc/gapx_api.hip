@@ -267,7 +267,7 @@ struct Workspace {
     DBuf pos, Z, cell_of, map_shift, cell_cnt, cell_start, cell_fill, cell_atoms, scan_part;
     DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k, n2_is, n2_ks;
     DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite, centres;
-    DBuf X, norm, cbuf, W, Wf, e_part, G, split_ws, split_ws2;
+    DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws, split_ws2;
     DBuf f, le, lv, e, vir, block_part, minmax;
     std::vector<int> h_centres;
 };
@@ -830,7 +830,7 @@ static int eval_device(gapx_model *m, int N, const double *d_pos, const int *Z, 
         CHK(w.cbuf.ensure((size_t)nc * KLM * 8));
         CHK(w.W.ensure((size_t)nc * c.M * 8));
         CHK(w.G.ensure((size_t)nc * c.dim * 8));
-        CHK(w.e_part.ensure((size_t)nc * ((c.M + 63) / 64 + 1) * 8));
+        CHK(w.e_part.ensure((size_t)nc * ((c.M + 31) / 32 + 1) * 8));
         SoapDev sp = soap_dev(c);
         hipEvent_t e0 = m->ev[2], e1 = m->ev[3], e2 = m->ev[4], e3 = m->ev[5], e4 = m->ev[6];
         if (m->timing) hipEventRecord(e0, s);
@@ -853,8 +853,11 @@ static int eval_device(gapx_model *m, int N, const double *d_pos, const int *Z, 
         launch_energy_rows(nc, ntile, w.centres.as<int>(), w.e_part.as<double>(), w.esite.as<double>(), s);
         if (m->timing) hipEventRecord(e2, s);
         if (grad) {
-            if (m->precision == GAPX_PREC_FP64)
-                launch_gemm2_f64(nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(), w.G.as<double>(), s);
+            if (m->precision == GAPX_PREC_FP64) {
+                CHK(w.Gparts.ensure((size_t)gemm2_parts(nc, c.dim) * nc * c.dim * 8 + 8));
+                launch_gemm2_f64(nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(), w.G.as<double>(),
+                                 w.Gparts.as<double>(), s);
+            }
             else
                 launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.Wf.as<float>(), c.d_XsT_hi.p,
                                    c.d_XsT_lo.p, c.wscale, w.G.as<double>(), s);

@@ -70,8 +70,10 @@ void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z
 // gemm1: K = X (nc x d) . Xs^T (M x d); W = w_s zeta K^(zeta-1); e_part[i*ntile + t]
 void launch_gemm1_f64(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
                       double *W, double *e_part, int *ntile_out, hipStream_t s);
-// gemm2: G = W (nc x M) . Xs (M x d)
-void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, hipStream_t s);
+// gemm2: G = W (nc x M) . Xs (M x d); split-K partials in Gparts (gemm2_parts(nc,d) x nc x d)
+int gemm2_parts(int nc, int d);
+void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, double *Gparts,
+                      hipStream_t s);
 // mixed precision forms: Xs pre-split into hi/lo (bf16 or fp16) copies, row- and column-major
 // W is scaled by wscale (a power of two) before its hi/lo split; GEMM2 divides it out.
 void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *X, const void *Xs_hi,

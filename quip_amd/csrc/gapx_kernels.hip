@@ -904,88 +904,112 @@ void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z
 // ----------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 4;
+constexpr int GBK = 16, GPAD = 4;
 
-// load a 64 x 16 tile of a row-major matrix (rows r0.., cols k0..) into k-major LDS
-__device__ __forceinline__ void load_rows_kmajor(double (*dst)[GBM + GPAD], const double *src, int ld, int r0,
-                                                 int nrows, int k0, int nk) {
-    const int t = threadIdx.x;
-    const int row = t >> 2, kq = (t & 3) * 4;
-    const int gr = r0 + row;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int gk = k0 + kq + j;
-        dst[kq + j][row] = (gr < nrows && gk < nk) ? src[(size_t)gr * ld + gk] : 0.0;
-    }
-}
-// load a 16 x 64 tile of a row-major K x N matrix (rows k0.., cols c0..) as-is
-__device__ __forceinline__ void load_kxn(double (*dst)[GBN + GPAD], const double *src, int ld, int k0, int nk, int c0,
-                                         int ncols) {
-    const int t = threadIdx.x;
-    const int k = t >> 4, cq = (t & 15) * 4;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int gc = c0 + cq + j;
-        dst[k][cq + j] = (k0 + k < nk && gc < ncols) ? src[(size_t)(k0 + k) * ld + gc] : 0.0;
-    }
-}
-
-template <bool BT, bool EPI>
-__global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, const double *__restrict__ A,
+// C = A . B (+ fused GP epilogue).  A is row-major (rows x K, k contiguous);
+// B is row-major (cols x K) when BT, else (K x cols).  4 wavefronts in a 2x2
+// grid, each owning (16 WM) x (16 WN) outputs, block tile (32 WM) x (32 WN).
+// The next k tile is fetched into registers while the current one feeds the
+// MFMAs from LDS.  blockIdx.z selects a k range (split-K) whose partial result
+// goes to C + z * rows * cols.
+template <bool BT, bool EPI, int WM, int WN>
+__global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, int kchunk, const double *__restrict__ A,
                                                   const double *__restrict__ B, double *__restrict__ C,
                                                   const double *__restrict__ wvec, double zeta, int izeta,
                                                   double *__restrict__ e_part) {
-    __shared__ double As[GBK][GBM + GPAD];
-    __shared__ double Bs[GBK][GBN + GPAD];
-    __shared__ double red[2][GBM];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    constexpr int AL = BM * GBK / 256;   // A elements per thread per k tile
+    constexpr int BL = BN * GBK / 256;
+    __shared__ double As[GBK][BM + GPAD];
+    __shared__ double Bs[GBK][BN + GPAD];
+    __shared__ double red[2][BM];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
-    const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
-    double4_t acc[2][2];
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int kbeg = blockIdx.z * kchunk, kend = min(Kd, kbeg + kchunk);
+    C += (size_t)blockIdx.z * Mr * Nc;
+    double4_t acc[WM][WN];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < Kd; k0 += GBK) {
-        load_rows_kmajor(As, A, Kd, m0, Mr, k0, Kd);
-        if (BT) load_rows_kmajor(Bs, B, Kd, n0, Nc, k0, Kd);
-        else load_kxn(Bs, B, Nc, k0, Kd, n0, Nc);
+        for (int j = 0; j < WN; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    double ra[AL], rb[BL];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < AL; e++) {      // A: row = (t + 256 e) / 16 ... k contiguous pairs of 4
+            const int idx = t + 256 * e;
+            const int row = idx >> 4, kk = idx & 15;
+            const int gr = m0 + row, gk = k0 + kk;
+            ra[e] = (gr < Mr && gk < kend) ? A[(size_t)gr * Kd + gk] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < BL; e++) {
+            const int idx = t + 256 * e;
+            if (BT) {
+                const int col = idx >> 4, kk = idx & 15;
+                const int gc = n0 + col, gk = k0 + kk;
+                rb[e] = (gc < Nc && gk < kend) ? B[(size_t)gc * Kd + gk] : 0.0;
+            } else {
+                const int kk = idx / BN, col = idx % BN;
+                const int gc = n0 + col, gk = k0 + kk;
+                rb[e] = (gc < Nc && gk < kend) ? B[(size_t)gk * Nc + gc] : 0.0;
+            }
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int e = 0; e < AL; e++) {
+            const int idx = t + 256 * e;
+            As[idx & 15][idx >> 4] = ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < BL; e++) {
+            const int idx = t + 256 * e;
+            if (BT) Bs[idx & 15][idx >> 4] = rb[e];
+            else Bs[idx / BN][idx % BN] = rb[e];
+        }
+    };
+    if (kbeg < kend) fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += GBK) {
         __syncthreads();
+        stash();
+        __syncthreads();
+        if (k0 + GBK < kend) fetch(k0 + GBK);
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 4) {
-            double a[2], b[2];
+            double a[WM], b[WN];
 #pragma unroll
-            for (int i = 0; i < 2; i++) a[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+            for (int i = 0; i < WM; i++) a[i] = As[kk + (lane >> 4)][wm * 16 * WM + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 2; j++) b[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+            for (int j = 0; j < WN; j++) b[j] = Bs[kk + (lane >> 4)][wn * 16 * WN + j * 16 + (lane & 15)];
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < WM; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < WN; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-        __syncthreads();
     }
-    // epilogue
-    double rowsum[2][4];
+    // epilogue: accumulator lane l, element v holds row (l/16) + 4v, column l%16
+    double rowsum[WM][4];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WM; i++)
 #pragma unroll
         for (int v = 0; v < 4; v++) rowsum[i][v] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < WM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+        for (int j = 0; j < WN; j++) {
+            const int col = n0 + wn * 16 * WN + j * 16 + (lane & 15);
+            double wv = 0.0;
+            if (EPI) wv = (col < Nc) ? wvec[col] : 0.0;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * v;
+                const int row = m0 + wm * 16 * WM + i * 16 + (lane >> 4) + 4 * v;
                 double kv = acc[i][j][v];
                 if (EPI) {
-                    double wv = (col < Nc) ? wvec[col] : 0.0;
                     double kz1, kz;
                     if (izeta > 0) {
                         kz1 = 1.0;
-                        for (int t = 1; t < izeta; t++) kz1 *= kv;
+                        for (int q = 1; q < izeta; q++) kz1 *= kv;
                         kz = kz1 * kv;
                     } else {
                         kz1 = pow(kv, zeta - 1.0);
@@ -998,24 +1022,23 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, const 
             }
         }
     if (EPI) {
-        // sum over the 16 columns held by lanes with equal lane/16, then over the two wn halves
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < WM; i++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                double s = rowsum[i][v];
-                for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-                rowsum[i][v] = s;
+                double sm = rowsum[i][v];
+                for (int off = 8; off > 0; off >>= 1) sm += __shfl_xor(sm, off, 64);
+                rowsum[i][v] = sm;
             }
         if ((lane & 15) == 0)
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < WM; i++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) red[wn][wm * 32 + i * 16 + (lane >> 4) + 4 * v] = rowsum[i][v];
+                for (int v = 0; v < 4; v++) red[wn][wm * 16 * WM + i * 16 + (lane >> 4) + 4 * v] = rowsum[i][v];
         __syncthreads();
-        if (threadIdx.x < GBM) {
-            const int row = m0 + threadIdx.x;
-            if (row < Mr) e_part[(size_t)row * gridDim.y + blockIdx.y] = red[0][threadIdx.x] + red[1][threadIdx.x];
+        for (int r = t; r < BM; r += 256) {
+            const int row = m0 + r;
+            if (row < Mr) e_part[(size_t)row * gridDim.y + blockIdx.y] = red[0][r] + red[1][r];
         }
     }
 }
@@ -1027,17 +1050,41 @@ static int int_zeta(double z) {
 
 void launch_gemm1_f64(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
                       double *W, double *e_part, int *ntile_out, hipStream_t s) {
-    dim3 grid(cdiv(nc, GBM), cdiv(M, GBN));
+    dim3 grid(cdiv(nc, 128), cdiv(M, 64), 1);
     *ntile_out = grid.y;
     if (nc <= 0) return;
-    hipLaunchKernelGGL((k_gemm_f64<true, true>), grid, dim3(256), 0, s, nc, M, d, X, Xs, W, wvec, zeta,
+    hipLaunchKernelGGL((k_gemm_f64<true, true, 4, 2>), grid, dim3(256), 0, s, nc, M, d, d, X, Xs, W, wvec, zeta,
                        int_zeta(zeta), e_part);
 }
-void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, hipStream_t s) {
-    dim3 grid(cdiv(nc, GBM), cdiv(d, GBN));
+
+// split-K partial sums, reduced in a fixed order
+__global__ void k_sum_parts(size_t n, int parts, const double *P, double *out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int z = 0; z < parts; z++) s += P[z * n + i];
+    out[i] = s;
+}
+
+int gemm2_parts(int nc, int d) {
+    const long tiles = (long)cdiv(nc, 64) * cdiv(d, 64);
+    int parts = 1;
+    while (tiles * parts < 1024 && parts < 8) parts *= 2;
+    return parts;
+}
+
+void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, double *Gparts,
+                      hipStream_t s) {
     if (nc <= 0) return;
-    hipLaunchKernelGGL((k_gemm_f64<false, false>), grid, dim3(256), 0, s, nc, d, M, W, Xs, G, (const double *)nullptr,
-                       0.0, 0, (double *)nullptr);
+    const int parts = gemm2_parts(nc, d);
+    const int kchunk = (cdiv(M, parts) + GBK - 1) / GBK * GBK;
+    dim3 grid(cdiv(nc, 64), cdiv(d, 64), parts);
+    hipLaunchKernelGGL((k_gemm_f64<false, false, 2, 2>), grid, dim3(256), 0, s, nc, d, M, kchunk, W, Xs,
+                       parts > 1 ? Gparts : G, (const double *)nullptr, 0.0, 0, (double *)nullptr);
+    if (parts > 1) {
+        const size_t n = (size_t)nc * d;
+        hipLaunchKernelGGL(k_sum_parts, dim3(cdiv((long)n, 256)), dim3(256), 0, s, n, parts, Gparts, G);
+    }
 }
 
 // e_i = sum over column tiles, fixed order
