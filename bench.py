@@ -138,6 +138,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,bf16x3,fp16x3", help="comma list of GEMM precision forms to time")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: --gpus %d is not supported (the SOAP-GAP path shards over centre "
@@ -155,7 +158,17 @@ def main():
     dev.h2d(bufs["pos"], L.f64(at.positions))
     forms = {}
     results = {}
-    for prec in ("fp64", "bf16x3", "fp16x3"):
+    form_list = [f for f in args.forms.split(",") if f]
+    if "fp64" not in form_list:
+        form_list.insert(0, "fp64")
+    if args.copy_probe_mb > 0:
+        nb = args.copy_probe_mb << 20
+        src, dst = dev.alloc(nb), dev.alloc(nb)
+        dev.hip.hipMemset(src, 1, C.c_size_t(nb))
+        dev.hip.hipMemcpy(dst, src, C.c_size_t(nb), 3)
+        dev.hip.hipDeviceSynchronize()
+        log("copy probe: %d bytes device-to-device" % nb)
+    for prec in form_list:
         pot = Potential(param_filename=path, precision=prec)
         log("timing", prec)
         dt, res, stages = run_form(pot, at, dev, bufs, args.steps, args.warmup)
@@ -164,7 +177,7 @@ def main():
         results[prec] = res
         del pot
     ref = results["fp64"]
-    for prec in ("bf16x3", "fp16x3"):
+    for prec in [f for f in form_list if f != "fp64"]:
         r = results[prec]
         forms[prec]["dE_per_atom_vs_fp64"] = abs(r["energy"] - ref["energy"]) / N
         forms[prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - ref["force"]) ** 2)))

@@ -264,7 +264,8 @@ struct Coord {
 };
 
 struct Workspace {
-    DBuf pos, Z, cell_of, map_shift, cell_cnt, cell_start, cell_fill, cell_atoms, scan_part;
+    DBuf pos, Z, cell_of, map_shift, cell_cnt, cell_start, cell_fill, cell_atoms, cell_unsorted, scan_part,
+        scan_part2;
     DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k, n2_is, n2_ks;
     DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite, centres;
     DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws, split_ws2;
@@ -695,23 +696,31 @@ static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat
     CHK(w.cell_start.ensure((size_t)(NC + 1) * 4));
     CHK(w.cell_atoms.ensure((size_t)N * 4 + 4));
     CHK(w.scan_part.ensure((size_t)scan_partials_size(std::max(NC, N) + 1) * 4 + 64));
+    CHK(w.scan_part2.ensure((size_t)scan_partials_size(N + 1) * 4 + 64));
     CHK(w.n1_cnt.ensure((size_t)N * 4 + 4));
     CHK(w.n1_off.ensure((size_t)(N + 1) * 4));
     CHK(w.n2_cnt.ensure((size_t)N * 4 + 4));
     CHK(w.n2_off.ensure((size_t)(N + 1) * 4));
     CHK(w.n2_fill.ensure((size_t)N * 4 + 4));
     CHK(w.nb_off.ensure((size_t)(N + 1) * 4));
+    CHK(w.cell_unsorted.ensure((size_t)N * 4 + 4));
+    // per-call counters start at zero
     launch_fill_i32(w.cell_cnt.as<int>(), NC, 0, s);
     launch_fill_i32(w.cell_fill.as<int>(), NC, 0, s);
+    launch_fill_i32(w.n2_cnt.as<int>(), N, 0, s);
+    launch_fill_i32(w.n2_fill.as<int>(), N, 0, s);
     launch_nl_bin(N, d_pos, g, w.cell_of.as<int>(), w.map_shift.as<int>(), s);
     launch_nl_cell_count(N, w.cell_of.as<int>(), w.cell_cnt.as<int>(), s);
     launch_scan_i32(w.cell_cnt.as<int>(), NC, w.cell_start.as<int>(), w.scan_part.as<int>(), s);
-    launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(), w.cell_atoms.as<int>(), s);
-    launch_nl_cell_sort(NC, w.cell_start.as<int>(), w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), s);
+    launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
+                        w.cell_unsorted.as<int>(), s);
+    launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
+                        w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
     launch_nl_pairs(false, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
                     w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), g, w.n1_cnt.as<int>(), nullptr, nullptr, nullptr,
-                    nullptr, s);
+                    nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, s);
     launch_scan_i32(w.n1_cnt.as<int>(), N, w.n1_off.as<int>(), w.scan_part.as<int>(), s);
+    launch_scan_i32(w.n2_cnt.as<int>(), N, w.n2_off.as<int>(), w.scan_part2.as<int>(), s);
     int n1_total = 0;
     HIPCHK(hipMemcpyAsync(&n1_total, w.n1_off.as<int>() + N, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -732,13 +741,8 @@ static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat
     CHK(w.nb_d.ensure(P * 24));
     launch_nl_pairs(true, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
                     w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), g, nullptr, w.n1_off.as<int>(), w.n1_j.as<int>(),
-                    w.n1_shift.as<int>(), w.n1_r.as<double>(), s);
-    launch_fill_i32(w.n2_cnt.as<int>(), N, 0, s);
-    launch_fill_i32(w.n2_fill.as<int>(), N, 0, s);
-    launch_nl_n2_count(N, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n2_cnt.as<int>(), s);
-    launch_scan_i32(w.n2_cnt.as<int>(), N, w.n2_off.as<int>(), w.scan_part.as<int>(), s);
-    launch_nl_n2_fill(N, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n2_off.as<int>(), w.n2_fill.as<int>(),
-                      w.n2_i.as<int>(), w.n2_k.as<int>(), s);
+                    w.n1_shift.as<int>(), w.n1_r.as<double>(), nullptr, w.n2_off.as<int>(), w.n2_fill.as<int>(),
+                    w.n2_i.as<int>(), w.n2_k.as<int>(), s);
     launch_nl_n2_sort(N, w.n2_off.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), w.cell_of.as<int>(),
                       w.n1_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), s);
     launch_nl_full(N, d_pos, g, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(),
@@ -882,7 +886,7 @@ static int eval_device(gapx_model *m, int N, const double *d_pos, const int *Z, 
     // outputs not requested go to scratch (already sized when they are the caller's outputs)
     CHK(ensure_outputs(w, N));
     int nblk = 0;
-    CHK(w.block_part.ensure((size_t)((N + 255) / 256 + 1) * 80));
+    CHK(w.block_part.ensure((size_t)((N + 3) / 4 + 1) * 80));
     launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
                   w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>(),
                   (want & GAPX_WANT_F) && d_f ? d_f : w.f.as<double>(),

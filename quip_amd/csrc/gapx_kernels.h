@@ -36,13 +36,12 @@ void launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
 void launch_nl_bin(int N, const double *pos, const NLGeom &g, int *cell_of, int *map_shift, hipStream_t s);
 void launch_nl_cell_count(int N, const int *cell_of, int *cnt, hipStream_t s);
 void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s);
-void launch_nl_cell_sort(int ncell, const int *start, const int *cnt, int *atoms, hipStream_t s);
+void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
+                         int *atoms, hipStream_t s);
 void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
                      const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom &g,
-                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, hipStream_t s);
-void launch_nl_n2_count(int N, const int *n1_off, const int *n1_j, int *n2_cnt, hipStream_t s);
-void launch_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_off, int *n2_fill, int *n2_i,
-                       int *n2_k, hipStream_t s);
+                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt,
+                     const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, hipStream_t s);
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
                        const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s);
 void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off, const int *n1_j,

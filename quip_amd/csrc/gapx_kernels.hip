@@ -185,94 +185,21 @@ void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc
     if (N > 0) hipLaunchKernelGGL(k_nl_cell_fill, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, start, fillc, atoms);
 }
 // Each cell's list in descending atom index: the order of QUIP's head-inserted
-// linked lists (Connection.f95:1580-1584).
-__global__ void k_nl_cell_sort(int ncell, const int *start, const int *cnt, int *atoms) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    int *a = atoms + start[c];
-    int n = cnt[c];
-    for (int i = 1; i < n; i++) {
-        int v = a[i], j = i - 1;
-        while (j >= 0 && a[j] < v) { a[j + 1] = a[j]; j--; }
-        a[j + 1] = v;
-    }
+// linked lists (Connection.f95:1580-1584).  Atom n's slot is the number of atoms
+// of its cell with a larger index.
+__global__ void k_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
+                               int *atoms) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int c = cell_of[n], st = start[c], m = cnt[c];
+    int rank = 0;
+    for (int q = 0; q < m; q++) rank += unsorted[st + q] > n ? 1 : 0;
+    atoms[st + rank] = n;
 }
-void launch_nl_cell_sort(int ncell, const int *start, const int *cnt, int *atoms, hipStream_t s) {
-    hipLaunchKernelGGL(k_nl_cell_sort, dim3(cdiv(ncell, 128)), dim3(128), 0, s, ncell, start, cnt, atoms);
-}
-
-// One thread per atom1: the reference's offset loops (k2 slowest, i2 fastest),
-// atom2 over the cell list, test_form_bond.  Pass 1 counts, pass 2 writes
-// neighbour1 in the reference's append order.
-template <bool FILL>
-__global__ void k_nl_pairs(int N, const double *__restrict__ pos, const int *__restrict__ map_shift,
-                           const int *__restrict__ cell_of, const int *__restrict__ cell_start,
-                           const int *__restrict__ cell_cnt, const int *__restrict__ cell_atoms, NLGeom g,
-                           int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r) {
-#pragma clang fp contract(off)
-    int atom1 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (atom1 >= N) return;
-    int lin = cell_of[atom1];
-    int cc[3] = {lin % g.cells[0] + 1, (lin / g.cells[0]) % g.cells[1] + 1, lin / (g.cells[0] * g.cells[1]) + 1};
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; a++) {       // get_min_max_images (Connection.f95:1448-1513)
-        if (g.pbc[a]) { lo[a] = -g.img[a]; hi[a] = g.img[a]; }
-        else {
-            lo[a] = (g.img[a] < cc[a]) ? -g.img[a] : -cc[a] + 1;
-            hi[a] = (g.img[a] < g.cells[a] - cc[a]) ? g.img[a] : g.cells[a] - cc[a];
-        }
-    }
-    const double p1x = pos[3 * atom1], p1y = pos[3 * atom1 + 1], p1z = pos[3 * atom1 + 2];
-    const int ms1x = map_shift[3 * atom1], ms1y = map_shift[3 * atom1 + 1], ms1z = map_shift[3 * atom1 + 2];
-    const double rc = g.cutoff;
-    int count = 0;
-    const int base = FILL ? n1_off[atom1] : 0;
-    int k3 = 1, j3 = 1, i3 = 1;
-    for (int k2 = lo[2]; k2 <= hi[2]; k2++) {
-        if (g.cells[2] > 1) k3 = (cc[2] + k2 - 1 + g.cells[2]) % g.cells[2] + 1;
-        int k4 = (cc[2] + k2 - k3) / g.cells[2];
-        if (k3 < 1 || k3 > g.cells[2]) continue;   // outside the reference's domain of validity
-        for (int j2 = lo[1]; j2 <= hi[1]; j2++) {
-            if (g.cells[1] > 1) j3 = (cc[1] + j2 - 1 + g.cells[1]) % g.cells[1] + 1;
-            int j4 = (cc[1] + j2 - j3) / g.cells[1];
-            if (j3 < 1 || j3 > g.cells[1]) continue;
-            for (int i2 = lo[0]; i2 <= hi[0]; i2++) {
-                if (g.cells[0] > 1) i3 = (cc[0] + i2 - 1 + g.cells[0]) % g.cells[0] + 1;
-                int i4 = (cc[0] + i2 - i3) / g.cells[0];
-                if (i3 < 1 || i3 > g.cells[0]) continue;
-                int lin2 = ((k3 - 1) * g.cells[1] + (j3 - 1)) * g.cells[0] + (i3 - 1);
-                int st = cell_start[lin2], nc = cell_cnt[lin2];
-                bool same_cell = (lin2 == lin);
-                for (int q = 0; q < nc; q++) {
-                    int atom2 = cell_atoms[st + q];
-                    if (atom2 < atom1) break;           // list is descending: nothing further qualifies
-                    if (atom2 == atom1 && i4 == 0 && j4 == 0 && k4 == 0 && same_cell) continue;
-                    int sh0 = i4 - ms1x + map_shift[3 * atom2];
-                    int sh1 = j4 - ms1y + map_shift[3 * atom2 + 1];
-                    int sh2 = k4 - ms1z + map_shift[3 * atom2 + 2];
-                    double d0 = pos[3 * atom2] - p1x, d1 = pos[3 * atom2 + 1] - p1y, d2 = pos[3 * atom2 + 2] - p1z;
-                    // dd(k) = dd(k) + lattice(k,m) * shift(m), m = 1..3
-                    d0 = d0 + g.lat[0] * (double)sh0; d1 = d1 + g.lat[1] * (double)sh0; d2 = d2 + g.lat[2] * (double)sh0;
-                    d0 = d0 + g.lat[3] * (double)sh1; d1 = d1 + g.lat[4] * (double)sh1; d2 = d2 + g.lat[5] * (double)sh1;
-                    d0 = d0 + g.lat[6] * (double)sh2; d1 = d1 + g.lat[7] * (double)sh2; d2 = d2 + g.lat[8] * (double)sh2;
-                    if (d0 > rc || d1 > rc || d2 > rc) continue;
-                    double dsq = d0 * d0;
-                    dsq = dsq + d1 * d1;
-                    dsq = dsq + d2 * d2;
-                    double d = sqrt(dsq);
-                    if (!(d < rc)) continue;
-                    if (FILL) {
-                        int e = base + count;
-                        n1_j[e] = atom2;
-                        n1_shift[3 * e] = sh0; n1_shift[3 * e + 1] = sh1; n1_shift[3 * e + 2] = sh2;
-                        n1_r[e] = d;
-                    }
-                    count++;
-                }
-            }
-        }
-    }
-    if (!FILL) n1_cnt[atom1] = count;
+void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
+                         int *atoms, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_nl_cell_rank, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, start, cnt, unsorted, atoms);
 }
 
 // Wave-parallel form of the same search: one wavefront per atom1, lanes over
@@ -295,7 +222,9 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
                                                     const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
                                                     const int *__restrict__ cell_atoms, NLGeom g, int *n1_cnt,
-                                                    const int *n1_off, int *n1_j, int *n1_shift, double *n1_r) {
+                                                    const int *n1_off, int *n1_j, int *n1_shift, double *n1_r,
+                                                    int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i,
+                                                    int *n2_k) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
     const int atom1 = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -368,6 +297,13 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
                         n1_j[e] = atom2;
                         n1_shift[3 * e] = sh0; n1_shift[3 * e + 1] = sh1; n1_shift[3 * e + 2] = sh2;
                         n1_r[e] = d;
+                        if (atom2 != atom1) {      // mirror entry in neighbour2(atom2), ordered later
+                            const int slot = atomicAdd(&n2_fill[atom2], 1);
+                            n2_i[n2_off[atom2] + slot] = atom1;
+                            n2_k[n2_off[atom2] + slot] = e - n1_off[atom1];
+                        }
+                    } else if (!FILL && atom2 != atom1) {
+                        atomicAdd(&n2_cnt[atom2], 1);
                     }
                     w++;
                 }
@@ -385,44 +321,19 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
 
 void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
                      const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom &g,
-                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, hipStream_t s) {
+                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt,
+                     const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, hipStream_t s) {
     if (N <= 0) return;
     if (fill)
         hipLaunchKernelGGL(k_nl_pairs_w<true>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
-                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r);
+                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r, n2_cnt, n2_off,
+                           n2_fill, n2_i, n2_k);
     else
         hipLaunchKernelGGL(k_nl_pairs_w<false>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
-                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r);
+                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r, n2_cnt, n2_off,
+                           n2_fill, n2_i, n2_k);
 }
 
-__global__ void k_nl_n2_count(int N, const int *n1_off, const int *n1_j, int *n2_cnt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    for (int e = n1_off[i]; e < n1_off[i + 1]; e++) {
-        int j = n1_j[e];
-        if (j != i) atomicAdd(&n2_cnt[j], 1);
-    }
-}
-void launch_nl_n2_count(int N, const int *n1_off, const int *n1_j, int *n2_cnt, hipStream_t s) {
-    if (N > 0) hipLaunchKernelGGL(k_nl_n2_count, dim3(cdiv(N, 256)), dim3(256), 0, s, N, n1_off, n1_j, n2_cnt);
-}
-__global__ void k_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_off, int *n2_fill, int *n2_i,
-                             int *n2_k) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    for (int e = n1_off[i]; e < n1_off[i + 1]; e++) {
-        int j = n1_j[e];
-        if (j == i) continue;
-        int slot = atomicAdd(&n2_fill[j], 1);
-        n2_i[n2_off[j] + slot] = i;
-        n2_k[n2_off[j] + slot] = e - n1_off[i];
-    }
-}
-void launch_nl_n2_fill(int N, const int *n1_off, const int *n1_j, const int *n2_off, int *n2_fill, int *n2_i,
-                       int *n2_k, hipStream_t s) {
-    if (N > 0)
-        hipLaunchKernelGGL(k_nl_n2_fill, dim3(cdiv(N, 256)), dim3(256), 0, s, N, n1_off, n1_j, n2_off, n2_fill, n2_i, n2_k);
-}
 // neighbour2(j) in the reference's append order: the global loop visits atom1
 // by (cell linear index, position in its descending cell list), then by its
 // neighbour1 entry index.  One wavefront per atom ranks its entries by that key
@@ -1144,57 +1055,69 @@ void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int 
 // local_virial_j(a,b) = -sum_q d_(rev q),a pairF_(rev q),b,
 // local_e_j = e_site + (pairE[q] + pairE[rev q])/2 + e0(Z_j); block partial sums.
 // ----------------------------------------------------------------------------
-constexpr int GATHER_BS = 256;
+constexpr int GATHER_BS = 256;   // 4 wavefronts, one atom each
+__device__ __forceinline__ double wave_sum_g(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const double *e0, double E_scale,
                                                       const int *nb_off, const int *nb_rev, const double *nb_d,
                                                       const double *pairF, const double *pairE, const double *esite,
                                                       double *f, double *local_e, double *local_virial,
                                                       double *block_part) {
-    __shared__ double sred[10][GATHER_BS];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sred[10][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = blockIdx.x * 4 + w;
+    double v[13];
+#pragma unroll
+    for (int k = 0; k < 13; k++) v[k] = 0.0;
+    if (j < N) {
+        // lanes over the pairs of atom j; fixed lane assignment and a fixed reduction tree
+        for (int q = nb_off[j] + lane; q < nb_off[j + 1]; q += 64) {
+            const int r = nb_rev[q];
+            const double gx = pairF[3 * r], gy = pairF[3 * r + 1], gz = pairF[3 * r + 2];
+            v[0] += pairF[3 * q] - gx;
+            v[1] += pairF[3 * q + 1] - gy;
+            v[2] += pairF[3 * q + 2] - gz;
+            const double dx = nb_d[3 * r], dy = nb_d[3 * r + 1], dz = nb_d[3 * r + 2];
+            v[3] -= dx * gx; v[4] -= dy * gx; v[5] -= dz * gx;
+            v[6] -= dx * gy; v[7] -= dy * gy; v[8] -= dz * gy;
+            v[9] -= dx * gz; v[10] -= dy * gz; v[11] -= dz * gz;
+            v[12] += pairE[q] + pairE[r];
+        }
+#pragma unroll
+        for (int k = 0; k < 13; k++) v[k] = wave_sum_g(v[k]);
+    }
     double le = 0.0, lv[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) lv[k] = 0.0;
     if (j < N) {
-        double fx = 0.0, fy = 0.0, fz = 0.0, e2 = 0.0;
-        for (int q = nb_off[j]; q < nb_off[j + 1]; q++) {
-            const int r = nb_rev[q];
-            const double gx = pairF[3 * r], gy = pairF[3 * r + 1], gz = pairF[3 * r + 2];
-            fx += pairF[3 * q] - gx;
-            fy += pairF[3 * q + 1] - gy;
-            fz += pairF[3 * q + 2] - gz;
-            const double dx = nb_d[3 * r], dy = nb_d[3 * r + 1], dz = nb_d[3 * r + 2];
-            lv[0] -= dx * gx; lv[1] -= dy * gx; lv[2] -= dz * gx;
-            lv[3] -= dx * gy; lv[4] -= dy * gy; lv[5] -= dz * gy;
-            lv[6] -= dx * gz; lv[7] -= dy * gz; lv[8] -= dz * gz;
-            if (pairE) e2 += pairE[q] + pairE[r];
+        le = E_scale * (esite[j] + 0.5 * v[12] + e0[Z[j]]);
+#pragma unroll
+        for (int k = 0; k < 9; k++) lv[k] = E_scale * v[3 + k];
+        if (lane < 3 && f) f[3 * j + lane] = E_scale * (lane == 0 ? v[0] : (lane == 1 ? v[1] : v[2]));
+        if (lane == 0 && local_e) local_e[j] = le;
+        if (lane < 9 && local_virial) {
+            double x = lv[0];
+#pragma unroll
+            for (int k = 1; k < 9; k++) x = (lane == k) ? lv[k] : x;
+            local_virial[9 * (size_t)j + lane] = x;
         }
-        le = esite[j] + 0.5 * e2 + e0[Z[j]];
-        le *= E_scale;
-#pragma unroll
-        for (int k = 0; k < 9; k++) lv[k] *= E_scale;
-        if (f) { f[3 * j] = E_scale * fx; f[3 * j + 1] = E_scale * fy; f[3 * j + 2] = E_scale * fz; }
-        if (local_e) local_e[j] = le;
-        if (local_virial)
-#pragma unroll
-            for (int k = 0; k < 9; k++) local_virial[9 * (size_t)j + k] = lv[k];
     }
-    sred[0][threadIdx.x] = le;
+    if (lane == 0) {
+        sred[0][w] = le;
 #pragma unroll
-    for (int k = 0; k < 9; k++) sred[1 + k][threadIdx.x] = lv[k];
+        for (int k = 0; k < 9; k++) sred[1 + k][w] = lv[k];
+    }
     __syncthreads();
-    for (int s = GATHER_BS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-#pragma unroll
-            for (int k = 0; k < 10; k++) sred[k][threadIdx.x] += sred[k][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 10) block_part[(size_t)blockIdx.x * 10 + threadIdx.x] = sred[threadIdx.x][0];
+    if (threadIdx.x < 10)
+        block_part[(size_t)blockIdx.x * 10 + threadIdx.x] =
+            (sred[threadIdx.x][0] + sred[threadIdx.x][1]) + (sred[threadIdx.x][2] + sred[threadIdx.x][3]);
 }
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
                    double *local_e, double *local_virial, double *block_part, int *nblocks_out, hipStream_t s) {
-    int nb = cdiv(N > 0 ? N : 1, GATHER_BS);
+    int nb = cdiv(N > 0 ? N : 1, 4);
     *nblocks_out = nb;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev, nb_d, pairF,
                        pairE, esite, f, local_e, local_virial, block_part);

@@ -188,21 +188,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
             spc[lane] = s;
         }
         __syncthreads();
-        for (int e = lane; e < KLM; e += 64) {
-            const int s = e / (N * LM), rem = e - s * (N * LM), b = rem / LM, lm = rem - b * LM;
+        // lanes over lm; each lane accumulates all radial channels of one species at a time.
+        // spc[k] is the same for every lane, so the species test is a uniform branch.
+        for (int lm = lane; lm < LM; lm += 64) {
             int l = (int)sqrt((double)lm + 0.5);
             if (l * l > lm) l--;
-            const double *gcol = rows + b * (L + 1) + l, *ycol = rows + GL + lm;
-            double acc0 = 0.0, acc1 = 0.0;
-            int k = 0;
-            for (; k + 1 < nk; k += 2) {
-                const double t0 = (spc[k] == s) ? gcol[(size_t)k * RL] * ycol[(size_t)k * RL] : 0.0;
-                const double t1 = (spc[k + 1] == s) ? gcol[(size_t)(k + 1) * RL] * ycol[(size_t)(k + 1) * RL] : 0.0;
-                acc0 += t0;
-                acc1 += t1;
+            for (int sp_i = 0; sp_i < K / N; sp_i++) {
+                double acc[N];
+#pragma unroll
+                for (int b = 0; b < N; b++) acc[b] = 0.0;
+                bool any = false;
+                for (int k = 0; k < nk; k++) {
+                    if (spc[k] != sp_i) continue;
+                    any = true;
+                    const double *row = rows + (size_t)k * RL;
+                    const double y = row[GL + lm];
+#pragma unroll
+                    for (int b = 0; b < N; b++) acc[b] += row[b * (L + 1) + l] * y;
+                }
+                if (any) {
+#pragma unroll
+                    for (int b = 0; b < N; b++) cL[(sp_i * N + b) * LM + lm] += acc[b];
+                }
             }
-            if (k < nk && spc[k] == s) acc0 += gcol[(size_t)k * RL] * ycol[(size_t)k * RL];
-            cL[e] += acc0 + acc1;
         }
     }
     __syncthreads();
@@ -232,35 +240,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
 }
 
 template <int N, int L>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))) void k_soap_bwd_x(SoapDev sp, const int *__restrict__ centres,
-                                                   const int *__restrict__ Z, const int *__restrict__ nb_off,
-                                                   const int *__restrict__ nb_j, const double *__restrict__ nb_r,
-                                                   const double *__restrict__ nb_d, const double *__restrict__ X,
-                                                   const double *__restrict__ G, const double *__restrict__ norm,
-                                                   const double *__restrict__ cbuf, double *pairF) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))) void k_soap_bwd_x(
+    SoapDev sp, int nc, int cpb, const int *__restrict__ centres, const int *__restrict__ Z, const int *__restrict__ nb_off,
+    const int *__restrict__ nb_j, const double *__restrict__ nb_r, const double *__restrict__ nb_d,
+    const double *__restrict__ X, const double *__restrict__ G, const double *__restrict__ norm,
+    const double *__restrict__ cbuf, double *pairF) {
+    // Two centres per wavefront: a centre has ~30 neighbours, so one lane per
+    // neighbour over 32 lanes keeps the whole wavefront busy.
     extern __shared__ double lds[];
     constexpr int LM = (L + 1) * (L + 1);
     const int K = sp.K, KLM = K * LM, d = sp.dim;
+    const int per = 2 * KLM + d;
     double *Ts = lds;              // N*N
     double *rbs = Ts + N * N;      // N
-    double *cL = rbs + N;          // K*LM
-    double *dcL = cL + KLM;        // K*LM
-    double *dpL = dcL + KLM;       // d
+    double *cen = rbs + N;         // 2 x [cL (K*LM) | dcL (K*LM) | dpL (d)]
     const int lane = threadIdx.x;
-    const int desc = blockIdx.x;
-    const int ci = centres[desc];
-    const double *x = X + (size_t)desc * d, *gg = G + (size_t)desc * d;
-    double dot = 0.0;
-    for (int q = lane; q < d; q += 64) dot += x[q] * gg[q];
-    dot = wave_sum_s(dot);
-    const double nrm = norm[desc];
-    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-    for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
-    for (int e = lane; e < KLM; e += 64) cL[e] = cbuf[(size_t)desc * KLM + e];
+    const int d0 = cpb * blockIdx.x;
+    const int nh = min(cpb, nc - d0);
     for (int e = lane; e < N * N; e += 64) Ts[e] = sp.T[e];
     if (lane < N) rbs[lane] = sp.rb[lane];
+    for (int h = 0; h < nh; h++) {
+        const int desc = d0 + h;
+        double *cL = cen + h * per, *dpL = cL + 2 * KLM;
+        const double *x = X + (size_t)desc * d, *gg = G + (size_t)desc * d;
+        double dot = 0.0;
+        for (int q = lane; q < d; q += 64) dot += x[q] * gg[q];
+        dot = wave_sum_s(dot);
+        const double nrm = norm[desc];
+        const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
+        for (int e = lane; e < KLM; e += 64) cL[e] = cbuf[(size_t)desc * KLM + e];
+    }
     __syncthreads();
-    for (int e = lane; e < KLM; e += 64) {
+    for (int it = lane; it < nh * KLM; it += 64) {
+        const int h = it / KLM, e = it - h * KLM;
+        const double *cL = cen + h * per, *dpL = cL + 2 * KLM;
+        double *dcL = cen + h * per + KLM;
         const int ia = e / LM, lm = e - ia * LM;
         int l = (int)sqrt((double)lm + 0.5);
         if (l * l > lm) l--;
@@ -273,8 +288,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
         dcL[e] = acc / sqrt(2.0 * l + 1.0);
     }
     __syncthreads();
+    const int lanes_per = 64 / cpb;
+    const int h = lane / lanes_per, sub = lane - h * lanes_per;
+    if (h >= nh) return;
+    const int ci = centres[d0 + h];
+    const double *dcL = cen + h * per + KLM;
     const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
-    for (int k = lane; k < np; k += 64) {
+    for (int k = sub; k < np; k += lanes_per) {
         const int p = p0 + k;
         const double r = nb_r[p];
         const int s = species_s(sp, Z[nb_j[p]]);
@@ -401,11 +421,14 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     ensure_soap_constants();
     if (nc <= 0) return true;
     const int LM = (sp.L + 1) * (sp.L + 1);
-    const size_t sh = (size_t)(sp.n * sp.n + sp.n + 2 * sp.K * LM + sp.dim) * 8;
+    const size_t one = (size_t)(2 * sp.K * LM + sp.dim) * 8;
+    const int cpb = (sp.n * sp.n + sp.n) * 8 + 2 * one <= 48 * 1024 ? 2 : 1;   // two centres per wave when LDS allows
+    const size_t sh = (size_t)(sp.n * sp.n + sp.n) * 8 + cpb * one;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
-        hipLaunchKernelGGL((k_soap_bwd_x<NN, LL>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j,     \
-                           nb_r, nb_d, X, G, norm, cbuf, pairF);                                                \
+        hipLaunchKernelGGL((k_soap_bwd_x<NN, LL>), dim3((nc + cpb - 1) / cpb), dim3(64), sh, s, sp, nc, cpb,     \
+                           centres, Z,                                                                          \
+                           nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF);                                  \
         return true;                                                                                            \
     }
     SOAP_FAST(LAUNCH)
