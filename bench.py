@@ -80,6 +80,8 @@ class DeviceArrays:
 
 
 def run_form(pot, at, dev, bufs, steps, warmup):
+    """FP64/split plain form on one structure; stage times from HIP events recorded
+    during the timed steps themselves (no extra synchronisation)."""
     lib = L.lib()
     N = len(at)
     Z, lat, pbc = L.i32(at.numbers), L.f64(at.cell), L.i32(at.pbc)
@@ -92,26 +94,57 @@ def run_form(pot, at, dev, bufs, steps, warmup):
     for _ in range(warmup):
         step()
     L.check(lib.gapx_sync(pot.handle), "sync")
+    pot.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     L.check(lib.gapx_sync(pot.handle), "sync")
     dt = time.perf_counter() - t0
+    tot, n = pot.timing_totals()
+    pot.set_timing(False)
     e, f, v = np.zeros(1), np.zeros((N, 3)), np.zeros(9)
     dev.d2h(e, bufs["e"])
     dev.d2h(f, bufs["f"])
     dev.d2h(v, bufs["v"])
-    # per-stage times (HIP events on the library stream), separate pass
-    pot.set_timing(True)
-    acc = None
-    nt = 5
-    for _ in range(nt):
-        step()
-        t = pot.last_timings()
-        acc = t if acc is None else {k: acc[k] + t[k] for k in t}
-    pot.set_timing(False)
-    stages = {k: acc[k] / nt for k in acc}
+    stages = {k: tot[k] / max(n, 1) for k in tot}
     return dt, dict(energy=e[0], force=f, virial=v.reshape(3, 3).T), stages
+
+
+def run_batch(pot, frames, dev, steps, warmup):
+    """Batching form: every structure of `frames` in one pass (gapx_calc_batch_device)."""
+    lib = L.lib()
+    S = len(frames)
+    off = np.zeros(S + 1, np.int32)
+    off[1:] = np.cumsum([len(a) for a in frames])
+    N = int(off[-1])
+    Z = L.i32(np.concatenate([a.numbers for a in frames]))
+    lats = L.f64(np.stack([a.cell for a in frames]))
+    pbcs = L.i32(np.stack([a.pbc for a in frames]))
+    pos = L.f64(np.vstack([a.positions for a in frames]))
+    b = dict(pos=dev.alloc(N * 24), e=dev.alloc(S * 8), f=dev.alloc(N * 24), v=dev.alloc(S * 72))
+    dev.h2d(b["pos"], pos)
+    want = L.WANT_E | L.WANT_F | L.WANT_V
+
+    def step():
+        L.check(lib.gapx_calc_batch_device(pot.handle, S, L.ptr(off), b["pos"], L.ptr(Z), L.ptr(lats), L.ptr(pbcs),
+                                           0.0, want, b["e"], None, b["f"], b["v"], None), "gapx_calc_batch_device")
+
+    for _ in range(warmup):
+        step()
+    L.check(lib.gapx_sync(pot.handle), "sync")
+    pot.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    L.check(lib.gapx_sync(pot.handle), "sync")
+    dt = time.perf_counter() - t0
+    tot, n = pot.timing_totals()
+    pot.set_timing(False)
+    E, F, V = np.zeros(S), np.zeros((N, 3)), np.zeros(S * 9)
+    dev.d2h(E, b["e"])
+    dev.d2h(F, b["f"])
+    dev.d2h(V, b["v"])
+    return dt, N, dict(energy=E, force=F, natoms=np.diff(off)), {k: tot[k] / max(n, 1) for k in tot}
 
 
 def cpu_baseline(model, at, budget_s=12.0):
@@ -132,6 +165,80 @@ def cpu_baseline(model, at, budget_s=12.0):
                        "oracle/liboracle.so FP64 with OpenMP" % reps), r
 
 
+def run_serving(path, prec, n_pool, duration, fracs, n_slots, max_batch_atoms):
+    """Serving form: saturation throughput (the whole pool submitted at once), then
+    Poisson arrivals at fractions of it; latency is submission -> completion."""
+    import threading
+    from quip_amd.serve import Server
+    pool = configs.sic_requests(n_pool)
+    pot = Potential(param_filename=path, precision=prec)
+    out = {}
+    with Server(pot, n_slots=n_slots, hbm_budget=96 << 30, max_batch_atoms=max_batch_atoms) as srv:
+        for _ in range(2):   # warm-up: workspaces grow to their working size
+            for t in [srv.submit(a) for a in pool]:
+                srv.wait(t)
+        t0 = time.perf_counter()
+        tickets = [srv.submit(a) for a in pool]
+        lat = [srv.wait(t)["latency_s"] for t in tickets]
+        dt = time.perf_counter() - t0
+        atoms = sum(len(a) for a in pool)
+        sat = len(pool) / dt
+        out["saturation"] = dict(requests_per_s=sat, atoms_per_s=atoms / dt, requests=len(pool),
+                                 mean_atoms=atoms / len(pool))
+        rng = np.random.default_rng(44)
+        for frac in fracs:
+            rate = frac * sat
+            n = max(50, int(rate * duration))
+            gaps = rng.exponential(1.0 / rate, n)
+            picks = rng.integers(0, len(pool), n)
+            tick = [None] * n
+            lats = [0.0] * n
+            ready = threading.Semaphore(0)
+
+            def waiter():
+                for k in range(n):
+                    ready.acquire()
+                    lats[k] = srv.wait(tick[k])["latency_s"]
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            st0 = srv.stats()
+            t0 = time.perf_counter()
+            t_next = t0
+            for k in range(n):
+                t_next += gaps[k]
+                while True:
+                    now = time.perf_counter()
+                    if now >= t_next:
+                        break
+                    time.sleep(min(t_next - now, 5e-4))
+                tick[k] = srv.submit(pool[picks[k]])
+                ready.release()
+            th.join()
+            dt = time.perf_counter() - t0
+            st1 = srv.stats()
+            la = np.array(lats) * 1e3
+            out["rate_%.2f" % frac] = dict(
+                offered_requests_per_s=rate, requests=n, atoms_per_s=(st1["atoms"] - st0["atoms"]) / dt,
+                p50_ms=float(np.percentile(la, 50)), p99_ms=float(np.percentile(la, 99)),
+                mean_batch=(st1["requests"] - st0["requests"]) / max(1, st1["batches"] - st0["batches"]))
+            log("serving %s rate %.0f/s: p50 %.2f ms p99 %.2f ms" % (prec, rate, out["rate_%.2f" % frac]["p50_ms"],
+                                                                    out["rate_%.2f" % frac]["p99_ms"]))
+    return out
+
+
+def gemm1_traffic():
+    """HBM bytes per GEMM 1 launch from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration it records), or None."""
+    path = os.path.join(ROOT, "profiles", "k_gemm_f64_gemm1_si4096_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch"):
+            return float(line.split()[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +246,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,bf16x3,fp16x3", help="comma list of GEMM precision forms to time")
+    ap.add_argument("--batch", default="fp64,bf16x3",
+                    help="precision forms of the 512-structure batching form ('' to skip)")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--serve", default="bf16x3,fp64",
+                    help="precision forms of the serving form ('' to skip); the first gets the three arrival rates")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of each Poisson stream")
+    ap.add_argument("--serve-pool", type=int, default=256, help="distinct requests the streams draw from")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -182,14 +296,51 @@ def main():
         forms[prec]["dE_per_atom_vs_fp64"] = abs(r["energy"] - ref["energy"]) / N
         forms[prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - ref["force"]) ** 2)))
         forms[prec]["speedup_vs_fp64"] = forms[prec]["atoms_per_s"] / forms["fp64"]["atoms_per_s"]
+    batch_list = [f for f in args.batch.split(",") if f]
+    if batch_list:
+        if "fp64" not in batch_list:
+            batch_list.insert(0, "fp64")
+        frames = configs.batch512()
+        bres = {}
+        for prec in batch_list:
+            pot = Potential(param_filename=path, precision=prec)
+            log("timing batch", prec)
+            dt, NB, r, stages = run_batch(pot, frames, dev, args.batch_steps, 2)
+            bres[prec] = r
+            forms["batch512_" + prec] = dict(
+                atoms_per_s=NB * args.batch_steps / dt, ms_per_step=1e3 * dt / args.batch_steps,
+                structures=len(frames), atoms=NB, stage_ms={k: round(v, 4) for k, v in stages.items()},
+                speedup_vs_fp64_plain=NB * args.batch_steps / dt / forms["fp64"]["atoms_per_s"])
+            del pot
+        rb = bres["fp64"]
+        for prec in batch_list[1:]:
+            r = bres[prec]
+            forms["batch512_" + prec]["max_dE_per_atom_vs_fp64"] = float(
+                np.max(np.abs(r["energy"] - rb["energy"]) / rb["natoms"]))
+            forms["batch512_" + prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - rb["force"]) ** 2)))
+    serving = None
+    serve_list = [f for f in args.serve.split(",") if f]
+    if serve_list:
+        log("building the two-species serving model from GPU descriptors")
+        smodel = configs.model_sic(gpu_descriptor_fn(tmp))
+        spath = os.path.join(tmp, "sic.xml")
+        write_gap_xml(spath, smodel)
+        serving = {"model": "Si/C two coordinates (centre Si, centre C), SOAP n_max 12 l_max 8 cutoff 5.0 (d=%d), "
+                            "2 x 2000 sparse points, zeta 4" % smodel.coords[0].descriptor.dim,
+                   "requests": "Si/C zincblende, N log-uniform on [16, 4096]; bulk / (111) slab / vacancies",
+                   "slots": 2, "max_batch_atoms": 16384}
+        for k, prec in enumerate(serve_list):
+            log("serving", prec)
+            serving[prec] = run_serving(spath, prec, args.serve_pool, args.serve_seconds,
+                                        (0.5, 0.8, 0.95) if k == 0 else (), 2, 16384)
     st = forms["fp64"]["stage_ms"]
     nc, M, d = N, model.coords[0].n_sparse, model.coords[0].descriptor.dim
     gemm_flops = 2.0 * nc * M * d
-    dom = "gemm2" if st["gemm2"] >= st["gemm1"] else "gemm1"
-    achieved = gemm_flops / (st[dom] * 1e-3)
-    roofline = dict(bound="mfma", kernel="k_gemm_f64 (%s, FP64 MFMA 16x16x4)" % dom, achieved=achieved / 1e12,
-                    peak=MFMA_FP64_PEAK / 1e12, unit="TFLOP/s", frac=achieved / MFMA_FP64_PEAK, traffic=None,
-                    flops_per_launch=gemm_flops)
+    achieved = gemm_flops / (st["gemm1"] * 1e-3)
+    roofline = dict(bound="mfma", kernel="k_gemm_f64 GEMM 1 (K = X Xs^T, fused pow/alpha epilogue; FP64 MFMA "
+                    "16x16x4)", achieved=achieved / 1e12, peak=MFMA_FP64_PEAK / 1e12, unit="TFLOP/s",
+                    frac=achieved / MFMA_FP64_PEAK, traffic=gemm1_traffic(), flops_per_launch=gemm_flops,
+                    avg_launch_ms=st["gemm1"])
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -198,6 +349,7 @@ def main():
         forms["fp64"]["max_dF_vs_oracle"] = float(np.abs(ref["force"] - ro["forces"]).max())
     dev.free()
     fp = forms["fp64"]
+    mixed = [k for k in forms if k.startswith("batch512_") and not k.endswith("fp64")]
     line = {
         "metric": "SOAP-GAP atoms/s (E+F+virial), mixed vs FP64 GPU; p50/p99 served latency",
         "value": fp["atoms_per_s"], "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -208,7 +360,9 @@ def main():
                    "atoms": N, "sparse_points": M, "descriptor_dim": d},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed_batched_vs_fp64_plain": max([forms[k]["speedup_vs_fp64_plain"] for k in mixed], default=None),
         "forms": forms,
+        "serving": serving,
     }
     print(json.dumps(line))
 

@@ -51,6 +51,7 @@ extern "C" {
 
 typedef struct gapx_model gapx_model;
 typedef struct gapx_nl gapx_nl;
+typedef struct gapx_server gapx_server;
 
 /* Library version; a host binding checks it like quip_lammps_wrapper.f95:13,23-28. */
 int gapx_api_version(void);
@@ -97,13 +98,45 @@ int gapx_calc_device(gapx_model *model, int n_atoms, const double *d_pos, const 
                      double *d_virial, double *d_local_virial);
 int gapx_sync(gapx_model *model);
 
-/* Many structures of uneven size in one call (struct-of-arrays with offsets):
- * structure s owns atoms [atom_off[s], atom_off[s+1]); lattices[9*s..] and
- * pbcs[3*s..] per structure; energies[s], virials[9*s..]; forces/local arrays
- * indexed by global atom.  Same physics as gapx_calc per structure. */
+/* Many structures of uneven size in one pass (struct-of-arrays with offsets):
+ * structure s owns atoms [atom_off[s], atom_off[s+1]), atom_off[0] = 0;
+ * lattices[9*s..] and pbcs[3*s..] per structure; energies[s], virials[9*s..];
+ * forces/local arrays indexed by global atom.  One neighbour-list build, one
+ * descriptor launch, one GEMM pair and one gather cover every structure; the
+ * physics is IPModel_GAP_Calc per structure (IPModel_GAP.f95:233-605). */
 int gapx_calc_batch(gapx_model *model, int n_struct, const int *atom_off, const double *pos, const int *Z,
                     const double *lattices, const int *pbcs, double nl_cutoff, unsigned want, double *energies,
                     double *local_e, double *f, double *virials, double *local_virial);
+/* As gapx_calc_batch with positions and every output in device memory; queued on
+ * the model's stream like gapx_calc_device. */
+int gapx_calc_batch_device(gapx_model *model, int n_struct, const int *atom_off, const double *d_pos, const int *Z,
+                           const double *lattices, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
+                           double *d_local_e, double *d_f, double *d_virials, double *d_local_virial);
+
+/* Serving loop (continuous batching).  No counterpart in the reference, whose
+ * IPModel_GAP_Calc serves one structure per call; each request is evaluated
+ * with the physics of gapx_calc.
+ * gapx_server_create starts n_slots engines (HIP stream + device workspace +
+ * host thread each).  A free slot takes queued requests first-in first-out
+ * into one batch while their estimated device footprint fits
+ * hbm_budget / n_slots and the batch has at most max_batch_atoms atoms, and
+ * evaluates the batch in one pass; slots overlap on the device.  The model's
+ * precision must not change while a server runs.
+ * gapx_server_submit copies the structure and returns a ticket at once
+ * (GAPX_E_CAPACITY if it can never fit a slot).  gapx_server_wait blocks until
+ * that request is done, copies the requested outputs (as gapx_calc) and gives
+ * its latency from submission to completion; a ticket is waited on once.
+ * gapx_server_free finishes the queued requests and stops the threads. */
+int gapx_server_create(gapx_model *model, int n_slots, int64_t hbm_budget_bytes, int max_batch_atoms,
+                       gapx_server **out);
+int gapx_server_submit(gapx_server *server, int n_atoms, const double *pos, const int *Z, const double *lattice,
+                       const int *pbc, unsigned want, int64_t *ticket);
+int gapx_server_poll(gapx_server *server, int64_t ticket, int *done);
+int gapx_server_wait(gapx_server *server, int64_t ticket, double *e, double *local_e, double *f, double *virial,
+                     double *local_virial, double *latency_s);
+int gapx_server_stats(gapx_server *server, int64_t *n_done, int64_t *n_batches, int64_t *atoms_done,
+                      double *busy_s);
+int gapx_server_free(gapx_server *server);
 
 /* Neighbour tables exactly as QUIP's calc_connect builds them
  * (Connection.f95:1035-1310): neighbour1(i) = (j >= i, shift, r) in append
@@ -121,12 +154,29 @@ int gapx_descriptors(gapx_model *model, int coord, int n_atoms, const double *po
                      const double *lattice, const int *pbc, double nl_cutoff, double *X, int *desc_atom,
                      int *n_desc);
 
-/* Time of the last gapx_calc / gapx_calc_device per stage (milliseconds, HIP
- * events on the model stream): [0] neighbour list, [1] descriptor forward,
- * [2] GEMM 1 (K, fused pow/alpha), [3] GEMM 2 (G), [4] descriptor adjoint,
- * [5] distance_2b + gather; requires gapx_model_set_timing(model, 1). */
+/* FP64 GEMM autotuner (on by default).  No counterpart in the reference.  For
+ * each gpCoordinate and centre-count bucket b (2^(b-1) < n_centres <= 2^b) the
+ * first evaluation times every GEMM tile shape (tile t: (WM,WN) = (4,2), (2,2),
+ * (2,4), (4,4) wavefront tiles of 16x16, block 32WM x 32WN) and every GEMM 2
+ * split-K factor in {1,2,4,8} on its own operands, and keeps the fastest.
+ * gapx_model_tuning lists the choices (up to max_entries; n_entries gets the
+ * count) with the measured milliseconds of one launch. */
+int gapx_model_set_autotune(gapx_model *model, int on);
+int gapx_model_tuning(gapx_model *model, int coord, int max_entries, int *bucket, int *gemm1_tile, int *gemm2_tile,
+                      int *gemm2_split, double *gemm1_ms, double *gemm2_ms, int *n_entries);
+
+/* Stage times of the calc entry points (milliseconds, HIP events on the model's
+ * stream, read back without extra synchronisation): [0] neighbour list,
+ * [1] descriptor forward, [2] GEMM 1 kernel (K with the fused pow/alpha
+ * epilogue), [3] per-centre energy rows + GEMM 2 (G), [4] descriptor adjoint,
+ * [5] distance_2b + force/virial gather + per-structure sums.
+ * gapx_model_set_timing(model, 1) starts recording and zeroes the totals;
+ * gapx_last_timings gives the last call, gapx_timing_totals the sums over all
+ * calls since timing was switched on and their number.  No counterpart in the
+ * reference (QUIP's system_timer is host wall-clock only). */
 int gapx_model_set_timing(gapx_model *model, int on);
-int gapx_last_timings(const gapx_model *model, double *ms6);
+int gapx_last_timings(gapx_model *model, double *ms6);
+int gapx_timing_totals(gapx_model *model, double *ms6, int64_t *n_calls);
 
 #ifdef __cplusplus
 }

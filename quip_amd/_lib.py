@@ -16,7 +16,11 @@ PREC = {"fp64": 0, "bf16x3": 1, "fp16x3": 2}
 
 
 class GapxError(RuntimeError):
-    pass
+    """A non-zero GAPX_E_* status; `code` holds it."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -39,6 +43,15 @@ _SIGS = [
     ("gapx_sync", C.c_int, [C.c_void_p]),
     ("gapx_calc_batch", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_double, C.c_uint] + [C.c_void_p] * 5),
+    ("gapx_calc_batch_device", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_double, C.c_uint] + [C.c_void_p] * 5),
+    ("gapx_server_create", C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),
+    ("gapx_server_submit", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint,
+                                     C.POINTER(C.c_int64)]),
+    ("gapx_server_poll", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int)]),
+    ("gapx_server_wait", C.c_int, [C.c_void_p, C.c_int64] + [C.c_void_p] * 5 + [C.POINTER(C.c_double)]),
+    ("gapx_server_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_int64)] * 3 + [C.POINTER(C.c_double)]),
+    ("gapx_server_free", C.c_int, [C.c_void_p]),
     ("gapx_connect", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                C.POINTER(C.c_void_p)]),
     ("gapx_nl_counts", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p]),
@@ -46,8 +59,11 @@ _SIGS = [
     ("gapx_nl_free", C.c_int, [C.c_void_p]),
     ("gapx_descriptors", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_double, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
+    ("gapx_model_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
+    ("gapx_model_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.POINTER(C.c_int)]),
     ("gapx_model_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_last_timings", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("gapx_timing_totals", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]
@@ -78,7 +94,7 @@ def last_error():
 
 def check(status, what=""):
     if status != 0:
-        raise GapxError("%s failed (status %d): %s" % (what, status, last_error()))
+        raise GapxError("%s failed (status %d): %s" % (what, status, last_error()), status)
 
 
 def ptr(a):

@@ -76,28 +76,38 @@ def sic32768():
     return antisite_swaps(at, 328, rng)
 
 
-def slab111(a, layers, vacuum, rng):
-    """Diamond (111)-oriented slab, periodic in all three directions with `vacuum` Angstrom of empty space."""
-    bulk = diamond(a, (3, 3, 3))
+def slab111(a, layers, vacuum, reps=(1, 1), species=(14, 14)):
+    """Diamond (111) slab of `layers` atomic layers (layers/2 bilayers), in-plane cell
+    3w*reps[0] x 2*sqrt(3)w*reps[1] with w = a/sqrt(2) (commensurate with the (111)
+    lattice: w along [1-10], sqrt(3)w along [11-2]; 24 atoms per bilayer per unit),
+    `vacuum` Angstrom of empty space along z, and z non-periodic.  species as in diamond()."""
     n = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0)
-    h = bulk.positions @ n
-    d111 = a / np.sqrt(3.0)
-    sel = h < layers * d111 / 2.0 + 1e-6
-    pos = bulk.positions[sel]
     u = np.array([1.0, -1.0, 0.0]) / np.sqrt(2.0)
     v = np.cross(n, u)
-    frame = np.stack([u, v, n])
-    p = pos @ frame.T
     w = a / np.sqrt(2.0)
-    cell = np.diag([3 * w, np.sqrt(3.0) * 1.5 * w, p[:, 2].max() - p[:, 2].min() + vacuum])
-    p = p - p.min(axis=0)
-    p[:, 0] = np.mod(p[:, 0], cell[0, 0])
-    p[:, 1] = np.mod(p[:, 1], cell[1, 1])
-    return Atoms(np.full(len(p), 14), p, cell, (True, True, True))
+    L1, L2 = 3 * w * reps[0], 2 * np.sqrt(3.0) * w * reps[1]
+    d111 = a / np.sqrt(3.0)
+    thick = layers * d111 / 2.0
+    m = int(np.ceil(1.5 * max(L1, L2, thick) / a)) + 2
+    bulk = diamond(a, (m, m, m), species)
+    p = (bulk.positions - bulk.positions.mean(axis=0)) @ np.stack([u, v, n]).T
+    sel = (p[:, 2] >= -1e-6) & (p[:, 2] < thick - 1e-6)
+    p, Z = p[sel], bulk.numbers[sel]
+    p[:, 0] = np.mod(p[:, 0], L1)
+    p[:, 1] = np.mod(p[:, 1], L2)
+    key = np.round(np.mod(p[:, :2] + 1e-6, [L1, L2]) * 1e4).astype(np.int64)
+    key = np.column_stack([key, np.round(p[:, 2] * 1e4).astype(np.int64)])
+    _, first = np.unique(key, axis=0, return_index=True)
+    p, Z = p[np.sort(first)], Z[np.sort(first)]
+    assert len(p) == 24 * reps[0] * reps[1] * (layers // 2), "slab block too small"
+    p[:, 2] += vacuum / 2.0
+    cell = np.diag([L1, L2, thick + vacuum])
+    return Atoms(Z, p, cell, (True, True, False))
 
 
 def batch512(n_struct=512, workload=3):
-    """512 Si structures, N ~ U[8, 512], mixed bulk / slab / vacancy."""
+    """512 Si structures, N ~ U[8, 512]; by seed: periodic bulk, (111) slab with 10 A
+    vacuum (z non-periodic), or bulk with 1-3 vacancies; a in [5.3, 5.6], sigma in [0.05, 0.3]."""
     rng = rng_for(workload, 1)
     out = []
     for s in range(n_struct):
@@ -105,16 +115,38 @@ def batch512(n_struct=512, workload=3):
         kind = int(rng.integers(0, 3))
         a = float(rng.uniform(5.3, 5.6))
         sig = float(rng.uniform(0.05, 0.3))
-        if kind == 1:
-            at = slab111(a, 6, 10.0, rng)
-        else:
-            reps = max(1, int(round((target / 8.0) ** (1.0 / 3.0))))
-            at = diamond(a, (reps, reps, max(1, int(np.ceil(target / (8.0 * reps * reps))))))
-        if len(at) > target:
-            at = remove_random(at, len(at) - target, rng)
-        if kind == 2 and len(at) > 8:
-            at = remove_random(at, int(rng.integers(1, 4)), rng)
-        out.append(perturb(at, sig, rng))
+        out.append(mixed_structure(rng, target, a, sig, kind))
+    return out
+
+
+def mixed_structure(rng, target, a, sigma, kind, species=(14, 14)):
+    """One structure of about `target` atoms (fewer after vacancies): kind 0 periodic
+    bulk, 1 (111) slab with 10 A vacuum (z non-periodic), 2 bulk with 1-3 vacancies."""
+    if kind == 1:
+        k = int(np.ceil(target / 72.0))
+        r1 = int(np.ceil(np.sqrt(k)))
+        at = slab111(a, 6, 10.0, (r1, int(np.ceil(k / r1))), species)
+    else:
+        reps = max(1, int(round((target / 8.0) ** (1.0 / 3.0))))
+        at = diamond(a, (reps, reps, max(1, int(np.ceil(target / (8.0 * reps * reps))))), species)
+    if len(at) > target:
+        at = remove_random(at, len(at) - target, rng)
+    if kind == 2 and len(at) > 8:
+        at = remove_random(at, int(rng.integers(1, 4)), rng)
+    return perturb(at, sigma, rng)
+
+
+def sic_requests(n_req, workload=4, n_min=16, n_max=4096):
+    """Serving stream: Si/C zincblende structures (a in [4.30, 4.42]), N log-uniform on
+    [n_min, n_max], kinds and displacements drawn as for batch512."""
+    rng = rng_for(workload, 1)
+    out = []
+    for _ in range(n_req):
+        target = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        kind = int(rng.integers(0, 3))
+        a = float(rng.uniform(4.30, 4.42))
+        sig = float(rng.uniform(0.05, 0.3))
+        out.append(mixed_structure(rng, target, a, sig, kind, species=(14, 6)))
     return out
 
 

@@ -2,20 +2,7 @@
 // reader, device-resident model state and the per-call pipeline
 //   neighbour list -> SOAP forward -> GEMM1 (K, fused GP epilogue) ->
 //   GEMM2 (G) -> SOAP adjoint -> distance_2b -> deterministic gather.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <memory>
-#include <string>
-#include <vector>
-
-#include "../../include/gapx.h"
-#include "gapx_kernels.h"
+#include "gapx_host.h"
 
 using namespace gapx;
 
@@ -24,25 +11,14 @@ using namespace gapx;
 // ---------------------------------------------------------------------------
 static thread_local char g_err[2048] = "";
 
-static int fail(int code, const char *fmt, ...) {
+int gapx::fail(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
     return code;
 }
-
-#define HIPCHK(x)                                                                                  \
-    do {                                                                                           \
-        hipError_t e_ = (x);                                                                       \
-        if (e_ != hipSuccess) return fail(GAPX_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
-    } while (0)
-
-#define CHK(x)                      \
-    do {                            \
-        int s_ = (x);               \
-        if (s_ != GAPX_OK) return s_; \
-    } while (0)
+const char *gapx::last_error_text() { return g_err; }
 
 // ---------------------------------------------------------------------------
 // minimal XML reader (elements, attributes, text, CDATA, comments)
@@ -212,83 +188,6 @@ static std::vector<int> int_list(const std::string &s) {
     }
     return v;
 }
-
-// ---------------------------------------------------------------------------
-// device buffers
-// ---------------------------------------------------------------------------
-struct DBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    int ensure(size_t bytes) {
-        if (bytes <= cap) return GAPX_OK;
-        if (p) hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = std::max<size_t>(bytes + bytes / 4, 256);
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) return fail(GAPX_E_DEVICE, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
-        cap = want;
-        return GAPX_OK;
-    }
-    template <class T> T *as() const { return (T *)p; }
-    ~DBuf() { if (p) hipFree(p); }
-};
-
-// ---------------------------------------------------------------------------
-// model
-// ---------------------------------------------------------------------------
-struct Coord {
-    int kind = 0;          // 1 soap, 2 distance_2b
-    std::string label, desc;
-    // soap
-    double rc = 0, ctw = 0.5, sigma = 0.5, cw = 1.0, bee = 10.0;
-    int n = 0, L = 0, S = 1, dim = 0;
-    std::vector<int> species, centreZ;
-    double u00 = 0;
-    std::vector<double> rb, T;
-    std::vector<int> qmap;
-    DBuf d_rb, d_T, d_qmap;
-    // d2b
-    int Z1 = 0, Z2 = 0;
-    // gp
-    int cov = 2;
-    double delta = 1, zeta = 1, signal_mean = 0;
-    int M = 0;
-    std::vector<double> Xs, alpha, scut, theta;
-    DBuf d_Xs, d_w, d_theta;
-    // mixed-precision copies of Xs (split hi/lo), built on demand
-    int split_prec = -1;
-    int dpad = 0, Mpad = 0;
-    double wscale = 1.0;
-    DBuf d_Xs_hi, d_Xs_lo, d_XsT_hi, d_XsT_lo;
-};
-
-struct Workspace {
-    DBuf pos, Z, cell_of, map_shift, cell_cnt, cell_start, cell_fill, cell_atoms, cell_unsorted, scan_part,
-        scan_part2;
-    DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k, n2_is, n2_ks;
-    DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite, centres;
-    DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws, split_ws2;
-    DBuf f, le, lv, e, vir, block_part, minmax;
-    std::vector<int> h_centres;
-};
-
-struct gapx_model {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string label;
-    double e0[119] = {0};
-    double E_scale = 1.0;
-    double cutoff = 0;
-    int precision = GAPX_PREC_FP64;
-    std::vector<std::unique_ptr<Coord>> coords;
-    DBuf d_e0;
-    Workspace ws;
-    bool timing = false;
-    hipEvent_t ev[8] = {};
-    double last_ms[6] = {0};
-    int n1_total_last = 0;
-};
 
 struct gapx_nl {
     int N = 0, cells[3] = {0, 0, 0};
@@ -527,6 +426,16 @@ static int model_load(const char *xml_text, int64_t xml_len, const char *label, 
     return GAPX_OK;
 }
 
+int gapx::new_engine(gapx_model *m, std::unique_ptr<Engine> &out) {
+    std::unique_ptr<Engine> e(new Engine());
+    hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return fail(GAPX_E_DEVICE, "hipStreamCreate: %s", hipGetErrorString(he));
+    e->ws.centres.resize(m->coords.size());
+    e->h_centres.resize(m->coords.size());
+    out = std::move(e);
+    return GAPX_OK;
+}
+
 extern "C" int gapx_model_load(const char *xml_text, int64_t xml_len, const char *label, const char *xml_dir,
                                int device, gapx_model **out) {
     if (!out) return fail(GAPX_E_ARG, "out is NULL");
@@ -538,12 +447,7 @@ extern "C" int gapx_model_load(const char *xml_text, int64_t xml_len, const char
     auto m = new gapx_model();
     m->device = device;
     int st = model_load(xml_text, xml_len, label, xml_dir, device, m);
-    if (st == GAPX_OK) {
-        hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) st = fail(GAPX_E_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
-        for (int k = 0; k < 8 && st == GAPX_OK; k++)
-            if (hipEventCreate(&m->ev[k]) != hipSuccess) st = fail(GAPX_E_DEVICE, "hipEventCreate failed");
-    }
+    if (st == GAPX_OK) st = new_engine(m, m->eng);
     if (st != GAPX_OK) {
         gapx_model_free(m);
         return st;
@@ -555,10 +459,7 @@ extern "C" int gapx_model_load(const char *xml_text, int64_t xml_len, const char
 extern "C" int gapx_model_free(gapx_model *m) {
     if (!m) return GAPX_OK;
     hipSetDevice(m->device);
-    if (m->stream) hipStreamSynchronize(m->stream);
-    for (int k = 0; k < 8; k++)
-        if (m->ev[k]) hipEventDestroy(m->ev[k]);
-    if (m->stream) hipStreamDestroy(m->stream);
+    m->eng.reset();
     delete m;
     return GAPX_OK;
 }
@@ -589,14 +490,57 @@ extern "C" int gapx_model_info(const gapx_model *m, int *n_coord, int coord, int
     }
     return GAPX_OK;
 }
-extern "C" int gapx_model_set_timing(gapx_model *m, int on) {
+extern "C" int gapx_model_set_autotune(gapx_model *m, int on) {
     if (!m) return fail(GAPX_E_ARG, "NULL model");
-    m->timing = on != 0;
+    m->autotune = on != 0;
     return GAPX_OK;
 }
-extern "C" int gapx_last_timings(const gapx_model *m, double *ms) {
+extern "C" int gapx_model_tuning(gapx_model *m, int coord, int max_entries, int *bucket, int *gemm1_tile,
+                                 int *gemm2_tile, int *gemm2_split, double *gemm1_ms, double *gemm2_ms,
+                                 int *n_entries) {
+    if (!m || !n_entries) return fail(GAPX_E_ARG, "NULL argument");
+    if (coord < 0 || coord >= (int)m->coords.size()) return fail(GAPX_E_ARG, "coordinate %d out of range", coord);
+    Coord &c = *m->coords[coord];
+    std::lock_guard<std::mutex> lk(c.tune_mu);
+    int k = 0;
+    for (auto &e : c.tune) {
+        if (k < max_entries) {
+            if (bucket) bucket[k] = e.first;
+            if (gemm1_tile) gemm1_tile[k] = e.second.g1;
+            if (gemm2_tile) gemm2_tile[k] = e.second.g2;
+            if (gemm2_split) gemm2_split[k] = e.second.parts;
+            if (gemm1_ms) gemm1_ms[k] = e.second.g1_ms;
+            if (gemm2_ms) gemm2_ms[k] = e.second.g2_ms;
+        }
+        k++;
+    }
+    *n_entries = k;
+    return GAPX_OK;
+}
+
+extern "C" int gapx_model_set_timing(gapx_model *m, int on) {
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    HIPCHK(hipSetDevice(m->device));
+    HIPCHK(hipStreamSynchronize(m->eng->stream));
+    m->eng->timer.reset();
+    m->eng->timer.on = on != 0;
+    return GAPX_OK;
+}
+extern "C" int gapx_last_timings(gapx_model *m, double *ms) {
     if (!m || !ms) return fail(GAPX_E_ARG, "NULL argument");
-    for (int k = 0; k < 6; k++) ms[k] = m->last_ms[k];
+    HIPCHK(hipSetDevice(m->device));
+    HIPCHK(hipStreamSynchronize(m->eng->stream));
+    m->eng->timer.resolve_all();
+    for (int k = 0; k < kStages; k++) ms[k] = m->eng->timer.last[k];
+    return GAPX_OK;
+}
+extern "C" int gapx_timing_totals(gapx_model *m, double *ms, int64_t *ncalls) {
+    if (!m || !ms) return fail(GAPX_E_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(m->device));
+    HIPCHK(hipStreamSynchronize(m->eng->stream));
+    m->eng->timer.resolve_all();
+    for (int k = 0; k < kStages; k++) ms[k] = m->eng->timer.total[k];
+    if (ncalls) *ncalls = m->eng->timer.ncalls;
     return GAPX_OK;
 }
 
@@ -665,34 +609,151 @@ static void prepare_geom(const double *lat, const int *pbc, double cutoff, const
 // ---------------------------------------------------------------------------
 // pipeline
 // ---------------------------------------------------------------------------
-static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat, const int *pbc, double cutoff,
-                    int *P_out) {
-    Workspace &w = m->ws;
-    hipStream_t s = m->stream;
-    if (!(cutoff > 0)) return fail(GAPX_E_GEOMETRY, "calc_connect: cutoff must be positive (got %g)", cutoff);
-    double vol = std::fabs(triple3(lat, lat + 3, lat + 6));
-    if (!(vol > 0)) return fail(GAPX_E_GEOMETRY, "lattice is singular");
-    NLGeom g;
-    bool any_np = !pbc[0] || !pbc[1] || !pbc[2];
-    double mm[6];
-    if (any_np && N > 0) {
-        double gi[9];
-        inv3(lat, gi);
-        CHK(w.minmax.ensure(16 * 8));
-        CHK(w.split_ws.ensure(9 * 8));
-        HIPCHK(hipMemcpyAsync(w.split_ws.p, gi, 9 * 8, hipMemcpyHostToDevice, s));
-        launch_minmax_frac(N, d_pos, w.split_ws.as<double>(), w.minmax.as<double>(), s);
-        HIPCHK(hipMemcpyAsync(mm, w.minmax.p, 6 * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+// fractional min/max per axis of atoms [a0, a1) from host positions, in the
+// same operation order as the device kernel (g .mult. pos)
+#pragma clang fp contract(off)
+static void host_minmax(const double *pos, int a0, int a1, const double *gi, double *mm) {
+    for (int a = 0; a < 3; a++) { mm[a] = 1e300; mm[3 + a] = -1e300; }
+    for (int n = a0; n < a1; n++) {
+        double lp[3];
+        mult3(gi, pos + 3 * (size_t)n, lp);
+        for (int a = 0; a < 3; a++) { mm[a] = std::min(mm[a], lp[a]); mm[3 + a] = std::max(mm[3 + a], lp[a]); }
     }
-    prepare_geom(lat, pbc, cutoff, any_np ? mm : nullptr, g);
-    long ncell = (long)g.cells[0] * g.cells[1] * g.cells[2];
-    if (ncell > (1L << 28)) return fail(GAPX_E_GEOMETRY, "too many cells (%ld)", ncell);
+}
+#pragma clang fp contract(on)
+
+static bool geo_equal(const NLGeom &x, const NLGeom &y) { return memcmp(&x, &y, sizeof(NLGeom)) == 0; }
+
+static bool z_in(const std::vector<int> &v, int z) {
+    for (int x : v)
+        if (x == z || x == 0) return true;
+    return false;
+}
+
+// Upload Z, the atom -> structure map and the structure offsets when they differ
+// from the previous call's; rebuild the per-coordinate centre lists with them.
+static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, const int *Z) {
+    const int N = atom_off[S];
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    bool same = eg.have_inputs && (int)eg.h_off.size() == S + 1 && (int)eg.h_Z.size() == N &&
+                std::equal(atom_off, atom_off + S + 1, eg.h_off.begin()) && std::equal(Z, Z + N, eg.h_Z.begin());
+    if (same) return GAPX_OK;
+    eg.have_inputs = false;
+    eg.h_off.assign(atom_off, atom_off + S + 1);
+    eg.h_Z.assign(Z, Z + N);
+    eg.h_sid.resize(N);
+    for (int st = 0; st < S; st++)
+        for (int i = atom_off[st]; i < atom_off[st + 1]; i++) eg.h_sid[i] = st;
+    size_t tot = (size_t)N * 8 + (size_t)(S + 1) * 4;
+    for (size_t k = 0; k < m->coords.size(); k++) {
+        auto &hc = eg.h_centres[k];
+        hc.clear();
+        const Coord &c = *m->coords[k];
+        if (c.kind == 1)
+            for (int i = 0; i < N; i++)
+                if (z_in(c.centreZ, Z[i])) hc.push_back(i);
+        tot += hc.size() * 4;
+    }
+    CHK(w.Z.ensure((size_t)N * 4 + 4));
+    CHK(w.sid.ensure((size_t)N * 4 + 4));
+    CHK(w.atom_off.ensure((size_t)(S + 1) * 4));
+    // the previous call's uploads finished before its neighbour-count synchronisation
+    CHK(eg.stage_in.ensure(tot + 64));
+    char *h = eg.stage_in.as<char>();
+    size_t o = 0;
+    auto put = [&](DBuf &dst, const void *src, size_t bytes) -> int {
+        if (!bytes) return GAPX_OK;
+        memcpy(h + o, src, bytes);
+        HIPCHK(hipMemcpyAsync(dst.p, h + o, bytes, hipMemcpyHostToDevice, s));
+        o += bytes;
+        return GAPX_OK;
+    };
+    CHK(put(w.Z, Z, (size_t)N * 4));
+    CHK(put(w.sid, eg.h_sid.data(), (size_t)N * 4));
+    CHK(put(w.atom_off, atom_off, (size_t)(S + 1) * 4));
+    for (size_t k = 0; k < m->coords.size(); k++) {
+        auto &hc = eg.h_centres[k];
+        CHK(w.centres[k].ensure(hc.size() * 4 + 4));
+        CHK(put(w.centres[k], hc.data(), hc.size() * 4));
+    }
+    eg.have_inputs = true;
+    return GAPX_OK;
+}
+
+// calc_connect for S structures at once (Connection.f95:1035-1310): per-structure
+// geometry, one set of kernels over all atoms; pairs never cross structures.
+// h_pos (nullable) lets the host compute the lat_eff extents of non-periodic axes.
+static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
+                    const double *lats, const int *pbcs, double cutoff, int *P_out) {
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    const int N = atom_off[S];
+    if (!(cutoff > 0)) return fail(GAPX_E_GEOMETRY, "calc_connect: cutoff must be positive (got %g)", cutoff);
+    std::vector<double> mm;
+    bool any_np = false;
+    for (int st = 0; st < S; st++) {
+        const double *lat = lats + 9 * (size_t)st;
+        const double vol = std::fabs(triple3(lat, lat + 3, lat + 6));
+        if (!(vol > 0)) return fail(GAPX_E_GEOMETRY, "lattice of structure %d is singular", st);
+        const int *pbc = pbcs + 3 * st;
+        if ((!pbc[0] || !pbc[1] || !pbc[2]) && atom_off[st + 1] > atom_off[st]) any_np = true;
+    }
+    if (any_np) {
+        mm.assign(6 * (size_t)S, 0.0);
+        std::vector<double> gi(9 * (size_t)S);
+        for (int st = 0; st < S; st++) inv3(lats + 9 * (size_t)st, gi.data() + 9 * st);
+        if (h_pos) {
+            for (int st = 0; st < S; st++) host_minmax(h_pos, atom_off[st], atom_off[st + 1], gi.data() + 9 * st,
+                                                       mm.data() + 6 * st);
+        } else {
+            CHK(w.ginv.ensure(gi.size() * 8));
+            CHK(w.minmax.ensure(mm.size() * 8));
+            CHK(eg.stage_geo.ensure(gi.size() * 8 + mm.size() * 8));
+            double *hg = eg.stage_geo.as<double>();
+            memcpy(hg, gi.data(), gi.size() * 8);
+            HIPCHK(hipMemcpyAsync(w.ginv.p, hg, gi.size() * 8, hipMemcpyHostToDevice, s));
+            launch_minmax_frac(S, w.atom_off.as<int>(), d_pos, w.ginv.as<double>(), w.minmax.as<double>(), s);
+            HIPCHK(hipMemcpyAsync(hg + gi.size(), w.minmax.p, mm.size() * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            memcpy(mm.data(), hg + gi.size(), mm.size() * 8);
+        }
+    }
+    std::vector<NLGeom> geo(S);
+    std::vector<int> cbase(S + 1);
+    long ncell = 0;
+    for (int st = 0; st < S; st++) {
+        const int *pbc = pbcs + 3 * st;
+        const bool np = !pbc[0] || !pbc[1] || !pbc[2];
+        memset(&geo[st], 0, sizeof(NLGeom));
+        prepare_geom(lats + 9 * (size_t)st, pbc, cutoff, np && any_np ? mm.data() + 6 * st : nullptr, geo[st]);
+        cbase[st] = (int)ncell;
+        ncell += (long)geo[st].cells[0] * geo[st].cells[1] * geo[st].cells[2];
+        if (ncell > (1L << 28)) return fail(GAPX_E_GEOMETRY, "too many cells (%ld)", ncell);
+    }
+    cbase[S] = (int)ncell;
     const int NC = (int)ncell;
+    if (!(eg.have_geo && eg.h_geo.size() == geo.size() && eg.h_cell_base == cbase &&
+          std::equal(geo.begin(), geo.end(), eg.h_geo.begin(), geo_equal))) {
+        eg.have_geo = false;
+        CHK(w.geo.ensure((size_t)S * sizeof(NLGeom) + 8));
+        CHK(w.cell_base.ensure((size_t)(S + 1) * 4));
+        const size_t gb = (size_t)S * sizeof(NLGeom), cb = (size_t)(S + 1) * 4;
+        CHK(eg.stage_geo.ensure(gb + cb + 64));   // the extents read-back above was synchronised
+        char *h = eg.stage_geo.as<char>();
+        memcpy(h, geo.data(), gb);
+        memcpy(h + gb, cbase.data(), cb);
+        HIPCHK(hipMemcpyAsync(w.geo.p, h, gb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(w.cell_base.p, h + gb, cb, hipMemcpyHostToDevice, s));
+        eg.h_geo = geo;
+        eg.h_cell_base = cbase;
+        eg.have_geo = true;
+    }
+    eg.ncell_last = NC;
     CHK(w.cell_of.ensure((size_t)N * 4 + 4));
     CHK(w.map_shift.ensure((size_t)N * 12 + 12));
-    CHK(w.cell_cnt.ensure((size_t)NC * 4));
-    CHK(w.cell_fill.ensure((size_t)NC * 4));
+    CHK(w.cell_cnt.ensure((size_t)NC * 4 + 4));
+    CHK(w.cell_fill.ensure((size_t)NC * 4 + 4));
     CHK(w.cell_start.ensure((size_t)(NC + 1) * 4));
     CHK(w.cell_atoms.ensure((size_t)N * 4 + 4));
     CHK(w.scan_part.ensure((size_t)scan_partials_size(std::max(NC, N) + 1) * 4 + 64));
@@ -704,12 +765,14 @@ static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat
     CHK(w.n2_fill.ensure((size_t)N * 4 + 4));
     CHK(w.nb_off.ensure((size_t)(N + 1) * 4));
     CHK(w.cell_unsorted.ensure((size_t)N * 4 + 4));
+    const NLGeom *dgeo = w.geo.as<NLGeom>();
+    const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
     // per-call counters start at zero
     launch_fill_i32(w.cell_cnt.as<int>(), NC, 0, s);
     launch_fill_i32(w.cell_fill.as<int>(), NC, 0, s);
     launch_fill_i32(w.n2_cnt.as<int>(), N, 0, s);
     launch_fill_i32(w.n2_fill.as<int>(), N, 0, s);
-    launch_nl_bin(N, d_pos, g, w.cell_of.as<int>(), w.map_shift.as<int>(), s);
+    launch_nl_bin(N, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(), s);
     launch_nl_cell_count(N, w.cell_of.as<int>(), w.cell_cnt.as<int>(), s);
     launch_scan_i32(w.cell_cnt.as<int>(), NC, w.cell_start.as<int>(), w.scan_part.as<int>(), s);
     launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
@@ -717,14 +780,18 @@ static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat
     launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
                         w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
     launch_nl_pairs(false, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                    w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), g, w.n1_cnt.as<int>(), nullptr, nullptr, nullptr,
-                    nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, s);
+                    w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                    nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, s);
     launch_scan_i32(w.n1_cnt.as<int>(), N, w.n1_off.as<int>(), w.scan_part.as<int>(), s);
     launch_scan_i32(w.n2_cnt.as<int>(), N, w.n2_off.as<int>(), w.scan_part2.as<int>(), s);
-    int n1_total = 0;
-    HIPCHK(hipMemcpyAsync(&n1_total, w.n1_off.as<int>() + N, 4, hipMemcpyDeviceToHost, s));
+    CHK(eg.ctl.ensure(64));
+    int *hcount = eg.ctl.as<int>();
+    hcount[0] = -1;
+    HIPCHK(hipMemcpyAsync(hcount, w.n1_off.as<int>() + N, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (n1_total < 0) return fail(GAPX_E_GEOMETRY, "neighbour count overflow");
+    const int n1_total = hcount[0];
+    if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);   // the previous call has finished
+    if (n1_total < 0 || n1_total > (INT32_MAX - 2) / 2) return fail(GAPX_E_GEOMETRY, "neighbour count overflow");
     const size_t T1 = (size_t)n1_total + 1;
     CHK(w.n1_j.ensure(T1 * 4));
     CHK(w.n1_shift.ensure(T1 * 12));
@@ -740,33 +807,28 @@ static int build_nl(gapx_model *m, int N, const double *d_pos, const double *lat
     CHK(w.nb_r.ensure(P * 8));
     CHK(w.nb_d.ensure(P * 24));
     launch_nl_pairs(true, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                    w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), g, nullptr, w.n1_off.as<int>(), w.n1_j.as<int>(),
-                    w.n1_shift.as<int>(), w.n1_r.as<double>(), nullptr, w.n2_off.as<int>(), w.n2_fill.as<int>(),
-                    w.n2_i.as<int>(), w.n2_k.as<int>(), s);
+                    w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, nullptr, w.n1_off.as<int>(),
+                    w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), nullptr, w.n2_off.as<int>(),
+                    w.n2_fill.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), s);
     launch_nl_n2_sort(N, w.n2_off.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), w.cell_of.as<int>(),
                       w.n1_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), s);
-    launch_nl_full(N, d_pos, g, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(),
-                   w.n2_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), w.nb_off.as<int>(),
-                   w.nb_j.as<int>(), w.nb_rev.as<int>(), w.nb_r.as<double>(), w.nb_d.as<double>(), s);
-    m->n1_total_last = n1_total;
+    launch_nl_full(N, d_pos, dgeo, dsid, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n1_shift.as<int>(),
+                   w.n1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(),
+                   w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_rev.as<int>(), w.nb_r.as<double>(),
+                   w.nb_d.as<double>(), s);
+    eg.n1_total_last = n1_total;
     *P_out = (int)(P - 1);
     return GAPX_OK;
 }
 
-static bool z_in(const std::vector<int> &v, int z) {
-    for (int x : v)
-        if (x == z || x == 0) return true;
-    return false;
-}
-
 // Scratch outputs.  Growing them frees the old blocks, so callers that hand
 // these buffers in as outputs size them first.
-static int ensure_outputs(Workspace &w, int N) {
+static int ensure_outputs(Workspace &w, int S, int N) {
     CHK(w.f.ensure((size_t)N * 24 + 24));
     CHK(w.le.ensure((size_t)N * 8 + 8));
     CHK(w.lv.ensure((size_t)N * 72 + 72));
-    CHK(w.e.ensure(16));
-    CHK(w.vir.ensure(80));
+    CHK(w.e.ensure((size_t)S * 8 + 8));
+    CHK(w.vir.ensure((size_t)S * 72 + 72));
     return GAPX_OK;
 }
 
@@ -779,55 +841,130 @@ static SoapDev soap_dev(const Coord &c) {
     return sp;
 }
 
-static int ensure_split(gapx_model *m, Coord &c, int prec);
 
-static void tmark(gapx_model *m, int k) {
-    if (m->timing) hipEventRecord(m->ev[k], m->stream);
-}
-
-// Device-resident evaluation of one structure.
-static int eval_device(gapx_model *m, int N, const double *d_pos, const int *Z, const double *lat, const int *pbc,
-                       double nl_cutoff, unsigned want, double *d_e, double *d_le, double *d_f, double *d_v,
-                       double *d_lv) {
-    if (N < 0) return fail(GAPX_E_ARG, "n_atoms < 0");
-    if (N > 0 && (!d_pos || !Z)) return fail(GAPX_E_ARG, "positions / Z are NULL");
-    if (!lat || !pbc) return fail(GAPX_E_ARG, "lattice / pbc are NULL");
+int gapx::check_batch(int S, const int *atom_off, const int *Z, const double *lats, const int *pbcs) {
+    if (S < 0) return fail(GAPX_E_ARG, "n_struct < 0");
+    if (!atom_off) return fail(GAPX_E_ARG, "atom_off is NULL");
+    if (atom_off[0] != 0) return fail(GAPX_E_ARG, "atom_off[0] must be 0");
+    for (int st = 0; st < S; st++)
+        if (atom_off[st + 1] < atom_off[st]) return fail(GAPX_E_ARG, "atom_off not ascending at %d", st);
+    const int N = atom_off[S];
+    if (N > 0 && !Z) return fail(GAPX_E_ARG, "Z is NULL");
+    if (S > 0 && (!lats || !pbcs)) return fail(GAPX_E_ARG, "lattice / pbc are NULL");
     for (int i = 0; i < N; i++)
         if (Z[i] < 0 || Z[i] > 118) return fail(GAPX_E_ARG, "Z[%d]=%d out of range", i, Z[i]);
-    Workspace &w = m->ws;
-    hipStream_t s = m->stream;
+    return GAPX_OK;
+}
+
+// GEMM autotuner.  For each coordinate and centre-count bucket (ceil log2 nc) the
+// first evaluation times every GEMM 1 tile and every GEMM 2 tile x split-K factor
+// once (after one warm-up launch) with HIP events on the engine's stream, on the
+// call's own operands (X is already computed), and keeps the fastest.  Later
+// calls in the bucket reuse the choice.  Without autotuning: GEMM 1 (4,2),
+// GEMM 2 (2,2) with the default split-K factor.
+static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out) {
+    out = GemmTune();
+    out.parts = gemm2_parts(nc, c.dim);
+    if (!m->autotune || nc <= 0) return GAPX_OK;
+    int bucket = 0;
+    while ((1 << bucket) < nc) bucket++;
+    {
+        std::lock_guard<std::mutex> lk(c.tune_mu);
+        auto it = c.tune.find(bucket);
+        if (it != c.tune.end()) {
+            out = it->second;
+            return GAPX_OK;
+        }
+    }
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    hipEvent_t ea, eb;
+    HIPCHK(hipEventCreate(&ea));
+    if (hipEventCreate(&eb) != hipSuccess) {
+        hipEventDestroy(ea);
+        return fail(GAPX_E_DEVICE, "hipEventCreate failed");
+    }
+    auto timed = [&](auto launch, float *ms) -> int {
+        launch();
+        hipEventRecord(ea, s);
+        launch();
+        hipEventRecord(eb, s);
+        HIPCHK(hipEventSynchronize(eb));
+        HIPCHK(hipEventElapsedTime(ms, ea, eb));
+        return GAPX_OK;
+    };
+    GemmTune best;
+    best.g1_ms = best.g2_ms = 1e30f;
+    int st = GAPX_OK;
+    for (int cfg = 0; cfg < kGemmTiles && st == GAPX_OK; cfg++) {
+        float t = 0;
+        int nt = 0;
+        st = timed([&] { launch_gemm1_f64(cfg, nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(),
+                                          c.d_w.as<double>(), c.zeta, w.W.as<double>(), w.e_part.as<double>(), &nt,
+                                          s); }, &t);
+        if (st == GAPX_OK && t < best.g1_ms) { best.g1_ms = t; best.g1 = cfg; }
+    }
+    for (int cfg = 0; cfg < kGemmTiles && st == GAPX_OK; cfg++)
+        for (int parts = 1; parts <= 8 && st == GAPX_OK; parts *= 2) {
+            float t = 0;
+            st = timed([&] { launch_gemm2_f64(cfg, parts, nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(),
+                                              w.G.as<double>(), w.Gparts.as<double>(), s); }, &t);
+            if (st == GAPX_OK && t < best.g2_ms) { best.g2_ms = t; best.g2 = cfg; best.parts = parts; }
+        }
+    hipEventDestroy(ea);
+    hipEventDestroy(eb);
+    CHK(st);
+    HIPCHK(hipGetLastError());
+    {
+        std::lock_guard<std::mutex> lk(c.tune_mu);
+        c.tune[bucket] = best;
+    }
+    out = best;
+    return GAPX_OK;
+}
+
+// Device-resident evaluation of S structures in one pass (IPModel_GAP_Calc per
+// structure).  Outputs: d_e[S], d_v[9S] and the per-atom arrays; any may be NULL,
+// in which case the engine's scratch takes it.
+static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
+                      const int *Z, const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
+                      double *d_le, double *d_f, double *d_v, double *d_lv) {
+    CHK(check_batch(S, atom_off, Z, lats, pbcs));
+    const int N = atom_off[S];
+    if (N > 0 && !d_pos) return fail(GAPX_E_ARG, "positions are NULL");
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    StageTimer &tm = eg.timer;
     HIPCHK(hipSetDevice(m->device));
     const double rc = nl_cutoff > 0 ? nl_cutoff : m->cutoff;
     if (rc < m->cutoff) return fail(GAPX_E_ARG, "nl_cutoff %g is below the model cutoff %g", rc, m->cutoff);
     const bool grad = (want & (GAPX_WANT_F | GAPX_WANT_V | GAPX_WANT_LOCAL_V)) != 0;
-    CHK(w.Z.ensure((size_t)N * 4 + 4));
-    if (N) HIPCHK(hipMemcpyAsync(w.Z.p, Z, (size_t)N * 4, hipMemcpyHostToDevice, s));
-    tmark(m, 0);
+    if (tm.on) tm.begin(s);
+    CHK(stage_inputs(m, eg, S, atom_off, Z));
+    const int t_nl0 = tm.on ? tm.mark(s) : -1;
     int P = 0;
-    CHK(build_nl(m, N, d_pos, lat, pbc, rc, &P));
-    tmark(m, 1);
+    CHK(build_nl(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, &P));
+    if (tm.on) tm.stage(0, t_nl0, tm.mark(s));
     CHK(w.pairF.ensure((size_t)(P + 1) * 24));
     CHK(w.pairE.ensure((size_t)(P + 1) * 8));
     CHK(w.esite.ensure((size_t)(N + 1) * 8));
+    CHK(ensure_outputs(w, S, N));
     launch_fill_f64(w.pairF.as<double>(), (size_t)P * 3, 0.0, s);
     launch_fill_f64(w.pairE.as<double>(), (size_t)P, 0.0, s);
     launch_fill_f64(w.esite.as<double>(), (size_t)N, 0.0, s);
-    double tsum[6] = {0, 0, 0, 0, 0, 0};
-    for (auto &cp : m->coords) {
-        Coord &c = *cp;
+    for (size_t k = 0; k < m->coords.size(); k++) {
+        Coord &c = *m->coords[k];
         if (c.kind == 2) {
+            const int ta = tm.on ? tm.mark(s) : -1;
             launch_d2b(N, w.Z.as<int>(), c.Z1, c.Z2, c.rc, c.ctw, c.M, c.d_Xs.as<double>(), c.d_w.as<double>(),
                        c.d_theta.as<double>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
                        w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), s);
+            if (tm.on) tm.stage(5, ta, tm.mark(s));
             continue;
         }
-        w.h_centres.clear();
-        for (int i = 0; i < N; i++)
-            if (z_in(c.centreZ, Z[i])) w.h_centres.push_back(i);
-        const int nc = (int)w.h_centres.size();
+        const int nc = (int)eg.h_centres[k].size();
         if (nc == 0) continue;
-        CHK(w.centres.ensure((size_t)nc * 4));
-        HIPCHK(hipMemcpyAsync(w.centres.p, w.h_centres.data(), (size_t)nc * 4, hipMemcpyHostToDevice, s));
+        const int *centres = w.centres[k].as<int>();
         const int KLM = c.S * c.n * (c.L + 1) * (c.L + 1);
         CHK(w.X.ensure((size_t)nc * c.dim * 8));
         CHK(w.norm.ensure((size_t)nc * 8));
@@ -835,74 +972,56 @@ static int eval_device(gapx_model *m, int N, const double *d_pos, const int *Z, 
         CHK(w.W.ensure((size_t)nc * c.M * 8));
         CHK(w.G.ensure((size_t)nc * c.dim * 8));
         CHK(w.e_part.ensure((size_t)nc * ((c.M + 31) / 32 + 1) * 8));
+        CHK(w.Gparts.ensure((size_t)8 * nc * c.dim * 8 + 8));
         SoapDev sp = soap_dev(c);
-        hipEvent_t e0 = m->ev[2], e1 = m->ev[3], e2 = m->ev[4], e3 = m->ev[5], e4 = m->ev[6];
-        if (m->timing) hipEventRecord(e0, s);
-        launch_soap_fwd(sp, nc, w.centres.as<int>(), w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
-                        w.nb_r.as<double>(), w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(),
-                        w.cbuf.as<double>(), s);
-        if (m->timing) hipEventRecord(e1, s);
+        const int t0 = tm.on ? tm.mark(s) : -1;
+        launch_soap_fwd(sp, nc, centres, w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
+                        w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(), w.cbuf.as<double>(), s);
+        GemmTune tu;
+        if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tu));
+        const int t1 = tm.on ? tm.mark(s) : -1;
         int ntile = 0;
         if (m->precision == GAPX_PREC_FP64) {
-            launch_gemm1_f64(nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(), c.d_w.as<double>(), c.zeta,
-                             w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
+            launch_gemm1_f64(tu.g1, nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(), c.d_w.as<double>(),
+                             c.zeta, w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
         } else {
-            CHK(ensure_split(m, c, m->precision));
+            CHK(ensure_split(m, c, m->precision, s));
             CHK(w.Wf.ensure((size_t)nc * c.Mpad * 4 + 64));
             CHK(w.split_ws.ensure((size_t)(nc + 63) / 64 * 64 * c.dpad * 4 + 64));
             launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, w.X.as<double>(), c.d_Xs_hi.p, c.d_Xs_lo.p,
                                c.d_w.as<double>(), c.zeta, c.wscale, w.Wf.as<float>(), w.e_part.as<double>(), &ntile,
                                w.split_ws.p, s);
         }
-        launch_energy_rows(nc, ntile, w.centres.as<int>(), w.e_part.as<double>(), w.esite.as<double>(), s);
-        if (m->timing) hipEventRecord(e2, s);
+        const int t2 = tm.on ? tm.mark(s) : -1;
+        launch_energy_rows(nc, ntile, centres, w.e_part.as<double>(), w.esite.as<double>(), s);
+        if (tm.on) { tm.stage(1, t0, t1); tm.stage(2, t1, t2); }
         if (grad) {
             if (m->precision == GAPX_PREC_FP64) {
-                CHK(w.Gparts.ensure((size_t)gemm2_parts(nc, c.dim) * nc * c.dim * 8 + 8));
-                launch_gemm2_f64(nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(), w.G.as<double>(),
-                                 w.Gparts.as<double>(), s);
-            }
-            else
+                launch_gemm2_f64(tu.g2, tu.parts, nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(),
+                                 w.G.as<double>(), w.Gparts.as<double>(), s);
+            } else {
                 launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.Wf.as<float>(), c.d_XsT_hi.p,
                                    c.d_XsT_lo.p, c.wscale, w.G.as<double>(), s);
-            if (m->timing) hipEventRecord(e3, s);
-            launch_soap_bwd(sp, nc, w.centres.as<int>(), w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
-                            w.nb_r.as<double>(), w.nb_d.as<double>(), w.X.as<double>(), w.G.as<double>(),
-                            w.norm.as<double>(), w.cbuf.as<double>(), w.pairF.as<double>(), s);
-            if (m->timing) hipEventRecord(e4, s);
-        }
-        if (m->timing) {
-            HIPCHK(hipStreamSynchronize(s));
-            float t;
-            hipEventElapsedTime(&t, e0, e1); tsum[1] += t;
-            hipEventElapsedTime(&t, e1, e2); tsum[2] += t;
-            if (grad) {
-                hipEventElapsedTime(&t, e2, e3); tsum[3] += t;
-                hipEventElapsedTime(&t, e3, e4); tsum[4] += t;
             }
+            const int t3 = tm.on ? tm.mark(s) : -1;
+            launch_soap_bwd(sp, nc, centres, w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
+                            w.nb_d.as<double>(), w.X.as<double>(), w.G.as<double>(), w.norm.as<double>(),
+                            w.cbuf.as<double>(), w.pairF.as<double>(), s);
+            if (tm.on) { tm.stage(3, t2, t3); tm.stage(4, t3, tm.mark(s)); }
         }
     }
-    tmark(m, 7);
-    // outputs not requested go to scratch (already sized when they are the caller's outputs)
-    CHK(ensure_outputs(w, N));
-    int nblk = 0;
-    CHK(w.block_part.ensure((size_t)((N + 3) / 4 + 1) * 80));
+    const int tg = tm.on ? tm.mark(s) : -1;
+    double *o_le = (want & GAPX_WANT_LOCAL_E) && d_le ? d_le : w.le.as<double>();
+    double *o_lv = (want & GAPX_WANT_LOCAL_V) && d_lv ? d_lv : w.lv.as<double>();
     launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
                   w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>(),
-                  (want & GAPX_WANT_F) && d_f ? d_f : w.f.as<double>(),
-                  (want & GAPX_WANT_LOCAL_E) && d_le ? d_le : w.le.as<double>(),
-                  (want & GAPX_WANT_LOCAL_V) && d_lv ? d_lv : w.lv.as<double>(), w.block_part.as<double>(), &nblk, s);
-    launch_final_reduce(nblk, w.block_part.as<double>(), m->E_scale,
-                        (want & GAPX_WANT_E) && d_e ? d_e : w.e.as<double>(),
-                        (want & GAPX_WANT_V) && d_v ? d_v : w.vir.as<double>(), s);
+                  (want & GAPX_WANT_F) && d_f ? d_f : w.f.as<double>(), o_le, o_lv, s);
+    launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, (want & GAPX_WANT_E) && d_e ? d_e : w.e.as<double>(),
+                         (want & GAPX_WANT_V) && d_v ? d_v : w.vir.as<double>(), s);
     HIPCHK(hipGetLastError());
-    if (m->timing) {
-        tmark(m, 6);
-        HIPCHK(hipStreamSynchronize(s));
-        float t;
-        hipEventElapsedTime(&t, m->ev[0], m->ev[1]); tsum[0] = t;
-        hipEventElapsedTime(&t, m->ev[7], m->ev[6]); tsum[5] = t;
-        for (int k = 0; k < 6; k++) m->last_ms[k] = tsum[k];
+    if (tm.on) {
+        tm.stage(5, tg, tm.mark(s));
+        tm.end();
     }
     return GAPX_OK;
 }
@@ -911,12 +1030,78 @@ extern "C" int gapx_calc_device(gapx_model *m, int N, const double *d_pos, const
                                 const int *pbc, double nl_cutoff, unsigned want, double *d_e, double *d_le,
                                 double *d_f, double *d_v, double *d_lv) {
     if (!m) return fail(GAPX_E_ARG, "NULL model");
-    return eval_device(m, N, d_pos, Z, lat, pbc, nl_cutoff, want, d_e, d_le, d_f, d_v, d_lv);
+    if (N < 0) return fail(GAPX_E_ARG, "n_atoms < 0");
+    const int off[2] = {0, N};
+    return eval_batch(m, *m->eng, 1, off, d_pos, nullptr, Z, lat, pbc, nl_cutoff, want, d_e, d_le, d_f, d_v, d_lv);
+}
+
+extern "C" int gapx_calc_batch_device(gapx_model *m, int n_struct, const int *atom_off, const double *d_pos,
+                                      const int *Z, const double *lattices, const int *pbcs, double nl_cutoff,
+                                      unsigned want, double *d_e, double *d_le, double *d_f, double *d_v,
+                                      double *d_lv) {
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    return eval_batch(m, *m->eng, n_struct, atom_off, d_pos, nullptr, Z, lattices, pbcs, nl_cutoff, want, d_e, d_le,
+                      d_f, d_v, d_lv);
 }
 
 extern "C" int gapx_sync(gapx_model *m) {
     if (!m) return fail(GAPX_E_ARG, "NULL model");
-    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipSetDevice(m->device));
+    HIPCHK(hipStreamSynchronize(m->eng->stream));
+    return GAPX_OK;
+}
+
+// Host-buffer evaluation of S structures on engine eg: positions go through
+// pinned staging, results come back into the caller's arrays.
+int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *pos, const int *Z,
+                     const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *e, double *le,
+                     double *f, double *v, double *lv) {
+    if ((want & GAPX_WANT_E) && !e) return fail(GAPX_E_ARG, "energy requested but e is NULL");
+    if ((want & GAPX_WANT_F) && !f) return fail(GAPX_E_ARG, "forces requested but f is NULL");
+    if ((want & GAPX_WANT_V) && !v) return fail(GAPX_E_ARG, "virial requested but virial is NULL");
+    if ((want & GAPX_WANT_LOCAL_E) && !le) return fail(GAPX_E_ARG, "local_e requested but NULL");
+    if ((want & GAPX_WANT_LOCAL_V) && !lv) return fail(GAPX_E_ARG, "local_virial requested but NULL");
+    CHK(check_batch(S, atom_off, Z, lats, pbcs));
+    const int N = atom_off[S];
+    if (N > 0 && !pos) return fail(GAPX_E_ARG, "positions are NULL");
+    HIPCHK(hipSetDevice(m->device));
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    CHK(w.pos.ensure((size_t)N * 24 + 24));
+    CHK(ensure_outputs(w, S, N));   // before taking the output pointers below
+    // one pinned block: positions in, then results out
+    const size_t bpos = (size_t)N * 24, be = (size_t)S * 8, bv = (size_t)S * 72, bf = (size_t)N * 24,
+                 ble = (size_t)N * 8, blv = (size_t)N * 72;
+    CHK(eg.stage_io.ensure(std::max(bpos, be + bv + bf + ble + blv) + 64));
+    char *h = eg.stage_io.as<char>();
+    if (N) {
+        memcpy(h, pos, bpos);
+        HIPCHK(hipMemcpyAsync(w.pos.p, h, bpos, hipMemcpyHostToDevice, s));
+    }
+    CHK(eval_batch(m, eg, S, atom_off, w.pos.as<double>(), pos, Z, lats, pbcs, nl_cutoff,
+                   want | GAPX_WANT_E | GAPX_WANT_V | GAPX_WANT_F | GAPX_WANT_LOCAL_E | GAPX_WANT_LOCAL_V,
+                   w.e.as<double>(), w.le.as<double>(), w.f.as<double>(), w.vir.as<double>(), w.lv.as<double>()));
+    char *o = h;
+    auto get = [&](bool on, const DBuf &src, size_t bytes, char **dst) -> int {
+        *dst = nullptr;
+        if (!on || !bytes) return GAPX_OK;
+        HIPCHK(hipMemcpyAsync(o, src.p, bytes, hipMemcpyDeviceToHost, s));
+        *dst = o;
+        o += bytes;
+        return GAPX_OK;
+    };
+    char *pe, *pv, *pf, *ple, *plv;
+    CHK(get(want & GAPX_WANT_E, w.e, be, &pe));
+    CHK(get(want & GAPX_WANT_V, w.vir, bv, &pv));
+    CHK(get(want & GAPX_WANT_F, w.f, bf, &pf));
+    CHK(get(want & GAPX_WANT_LOCAL_E, w.le, ble, &ple));
+    CHK(get(want & GAPX_WANT_LOCAL_V, w.lv, blv, &plv));
+    HIPCHK(hipStreamSynchronize(s));
+    if (pe) memcpy(e, pe, be);
+    if (pv) memcpy(v, pv, bv);
+    if (pf) memcpy(f, pf, bf);
+    if (ple) memcpy(le, ple, ble);
+    if (plv) memcpy(lv, plv, blv);
     return GAPX_OK;
 }
 
@@ -924,41 +1109,16 @@ extern "C" int gapx_calc(gapx_model *m, int N, const double *pos, const int *Z, 
                          double nl_cutoff, unsigned want, double *e, double *le, double *f, double *v, double *lv) {
     if (!m) return fail(GAPX_E_ARG, "NULL model");
     if (N < 0) return fail(GAPX_E_ARG, "n_atoms < 0");
-    if ((want & GAPX_WANT_E) && !e) return fail(GAPX_E_ARG, "energy requested but e is NULL");
-    if ((want & GAPX_WANT_F) && !f) return fail(GAPX_E_ARG, "forces requested but f is NULL");
-    if ((want & GAPX_WANT_V) && !v) return fail(GAPX_E_ARG, "virial requested but virial is NULL");
-    if ((want & GAPX_WANT_LOCAL_E) && !le) return fail(GAPX_E_ARG, "local_e requested but NULL");
-    if ((want & GAPX_WANT_LOCAL_V) && !lv) return fail(GAPX_E_ARG, "local_virial requested but NULL");
-    HIPCHK(hipSetDevice(m->device));
-    Workspace &w = m->ws;
-    CHK(w.pos.ensure((size_t)N * 24 + 24));
-    CHK(ensure_outputs(w, N));   // before taking the output pointers below
-    if (N > 0) HIPCHK(hipMemcpyAsync(w.pos.p, pos, (size_t)N * 24, hipMemcpyHostToDevice, m->stream));
-    CHK(eval_device(m, N, w.pos.as<double>(), Z, lat, pbc, nl_cutoff, want | GAPX_WANT_E | GAPX_WANT_V, w.e.as<double>(),
-                    w.le.as<double>(), w.f.as<double>(), w.vir.as<double>(), w.lv.as<double>()));
-    hipStream_t s = m->stream;
-    if (want & GAPX_WANT_E) HIPCHK(hipMemcpyAsync(e, w.e.p, 8, hipMemcpyDeviceToHost, s));
-    if (want & GAPX_WANT_V) HIPCHK(hipMemcpyAsync(v, w.vir.p, 72, hipMemcpyDeviceToHost, s));
-    if ((want & GAPX_WANT_F) && N) HIPCHK(hipMemcpyAsync(f, w.f.p, (size_t)N * 24, hipMemcpyDeviceToHost, s));
-    if ((want & GAPX_WANT_LOCAL_E) && N) HIPCHK(hipMemcpyAsync(le, w.le.p, (size_t)N * 8, hipMemcpyDeviceToHost, s));
-    if ((want & GAPX_WANT_LOCAL_V) && N) HIPCHK(hipMemcpyAsync(lv, w.lv.p, (size_t)N * 72, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    return GAPX_OK;
+    const int off[2] = {0, N};
+    return calc_host(m, *m->eng, 1, off, pos, Z, lat, pbc, nl_cutoff, want, e, le, f, v, lv);
 }
 
 extern "C" int gapx_calc_batch(gapx_model *m, int n_struct, const int *atom_off, const double *pos, const int *Z,
                                const double *lattices, const int *pbcs, double nl_cutoff, unsigned want,
                                double *energies, double *local_e, double *f, double *virials, double *local_virial) {
-    if (!m || !atom_off || n_struct < 0) return fail(GAPX_E_ARG, "bad batch arguments");
-    for (int s = 0; s < n_struct; s++) {
-        const int a0 = atom_off[s], n = atom_off[s + 1] - a0;
-        if (n < 0) return fail(GAPX_E_ARG, "atom_off not ascending at %d", s);
-        CHK(gapx_calc(m, n, pos + 3 * (size_t)a0, Z + a0, lattices + 9 * (size_t)s, pbcs + 3 * (size_t)s, nl_cutoff,
-                      want, energies ? energies + s : nullptr, local_e ? local_e + a0 : nullptr,
-                      f ? f + 3 * (size_t)a0 : nullptr, virials ? virials + 9 * (size_t)s : nullptr,
-                      local_virial ? local_virial + 9 * (size_t)a0 : nullptr));
-    }
-    return GAPX_OK;
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    return calc_host(m, *m->eng, n_struct, atom_off, pos, Z, lattices, pbcs, nl_cutoff, want, energies, local_e, f,
+                     virials, local_virial);
 }
 
 // ---------------------------------------------------------------------------
@@ -969,49 +1129,43 @@ extern "C" int gapx_connect(int device, int N, const double *pos, const double *
     if (!out || (N > 0 && !pos) || !lat || !pbc || N < 0) return fail(GAPX_E_ARG, "bad arguments");
     *out = nullptr;
     HIPCHK(hipSetDevice(device));
-    gapx_model tmp;
+    gapx_model tmp;   // no coordinates: only the neighbour-list part of an engine is used
     tmp.device = device;
-    HIPCHK(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
-    int st = tmp.ws.pos.ensure((size_t)N * 24 + 24);
-    if (st == GAPX_OK && N > 0) {
-        if (hipMemcpy(tmp.ws.pos.p, pos, (size_t)N * 24, hipMemcpyHostToDevice) != hipSuccess)
-            st = fail(GAPX_E_DEVICE, "copy of positions failed");
-    }
+    CHK(new_engine(&tmp, tmp.eng));
+    Engine &eg = *tmp.eng;
+    Workspace &w = eg.ws;
+    const int off[2] = {0, N};
+    std::vector<int> Z0((size_t)N, 0);
+    CHK(check_batch(1, off, Z0.data(), lat, pbc));
+    CHK(w.pos.ensure((size_t)N * 24 + 24));
+    if (N > 0) HIPCHK(hipMemcpy(w.pos.p, pos, (size_t)N * 24, hipMemcpyHostToDevice));
+    CHK(stage_inputs(&tmp, eg, 1, off, Z0.data()));
     int P = 0;
-    if (st == GAPX_OK) st = build_nl(&tmp, N, tmp.ws.pos.as<double>(), lat, pbc, cutoff, &P);
-    if (st == GAPX_OK) {
-        auto nl = new gapx_nl();
-        nl->N = N;
-        const int T1 = tmp.n1_total_last;
-        nl->n1_off.resize(N + 1); nl->n2_off.resize(N + 1); nl->map_shift.resize(3 * (size_t)N);
-        nl->n1_j.resize(T1); nl->n1_shift.resize(3 * (size_t)T1); nl->n1_r.resize(T1);
-        hipStreamSynchronize(tmp.stream);
-        hipMemcpy(nl->n1_off.data(), tmp.ws.n1_off.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost);
-        hipMemcpy(nl->n2_off.data(), tmp.ws.n2_off.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost);
-        if (N) hipMemcpy(nl->map_shift.data(), tmp.ws.map_shift.p, (size_t)N * 12, hipMemcpyDeviceToHost);
-        if (T1) {
-            hipMemcpy(nl->n1_j.data(), tmp.ws.n1_j.p, (size_t)T1 * 4, hipMemcpyDeviceToHost);
-            hipMemcpy(nl->n1_shift.data(), tmp.ws.n1_shift.p, (size_t)T1 * 12, hipMemcpyDeviceToHost);
-            hipMemcpy(nl->n1_r.data(), tmp.ws.n1_r.p, (size_t)T1 * 8, hipMemcpyDeviceToHost);
-        }
-        const int T2 = nl->n2_off[N];
-        nl->n2_j.resize(T2); nl->n2_idx.resize(T2);
-        if (T2) {
-            hipMemcpy(nl->n2_j.data(), tmp.ws.n2_is.p, (size_t)T2 * 4, hipMemcpyDeviceToHost);
-            hipMemcpy(nl->n2_idx.data(), tmp.ws.n2_ks.p, (size_t)T2 * 4, hipMemcpyDeviceToHost);
-        }
-        NLGeom g;
-        double mm[6] = {0, 0, 0, 0, 0, 0};
-        prepare_geom(lat, pbc, cutoff, nullptr, g);
-        for (int a = 0; a < 3; a++) nl->cells[a] = g.cells[a];
-        (void)mm;
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) { st = fail(GAPX_E_DEVICE, "neighbour list: %s", hipGetErrorString(e)); delete nl; }
-        else *out = nl;
+    CHK(build_nl(&tmp, eg, 1, off, w.pos.as<double>(), pos, lat, pbc, cutoff, &P));
+    HIPCHK(hipStreamSynchronize(eg.stream));
+    std::unique_ptr<gapx_nl> nl(new gapx_nl());
+    nl->N = N;
+    const int T1 = eg.n1_total_last;
+    nl->n1_off.resize(N + 1); nl->n2_off.resize(N + 1); nl->map_shift.resize(3 * (size_t)N);
+    nl->n1_j.resize(T1); nl->n1_shift.resize(3 * (size_t)T1); nl->n1_r.resize(T1);
+    HIPCHK(hipMemcpy(nl->n1_off.data(), w.n1_off.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nl->n2_off.data(), w.n2_off.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost));
+    if (N) HIPCHK(hipMemcpy(nl->map_shift.data(), w.map_shift.p, (size_t)N * 12, hipMemcpyDeviceToHost));
+    if (T1) {
+        HIPCHK(hipMemcpy(nl->n1_j.data(), w.n1_j.p, (size_t)T1 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(nl->n1_shift.data(), w.n1_shift.p, (size_t)T1 * 12, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(nl->n1_r.data(), w.n1_r.p, (size_t)T1 * 8, hipMemcpyDeviceToHost));
     }
-    hipStreamDestroy(tmp.stream);
-    tmp.stream = nullptr;
-    return st;
+    const int T2 = nl->n2_off[N];
+    nl->n2_j.resize(T2); nl->n2_idx.resize(T2);
+    if (T2) {
+        HIPCHK(hipMemcpy(nl->n2_j.data(), w.n2_is.p, (size_t)T2 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(nl->n2_idx.data(), w.n2_ks.p, (size_t)T2 * 4, hipMemcpyDeviceToHost));
+    }
+    for (int a = 0; a < 3; a++) nl->cells[a] = eg.h_geo[0].cells[a];
+    HIPCHK(hipGetLastError());
+    *out = nl.release();
+    return GAPX_OK;
 }
 extern "C" int gapx_nl_counts(const gapx_nl *nl, int *n1_total, int *n2_total, int *cells) {
     if (!nl) return fail(GAPX_E_ARG, "NULL nl");
@@ -1046,39 +1200,41 @@ extern "C" int gapx_descriptors(gapx_model *m, int coord, int N, const double *p
     Coord &c = *m->coords[coord];
     if (c.kind != 1) return fail(GAPX_E_UNSUPPORTED, "gapx_descriptors: coordinate %d is not soap", coord);
     HIPCHK(hipSetDevice(m->device));
-    Workspace &w = m->ws;
-    hipStream_t s = m->stream;
+    Engine &eg = *m->eng;
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    const int off[2] = {0, N};
+    CHK(check_batch(1, off, Z, lat, pbc));
+    if (N > 0 && !pos) return fail(GAPX_E_ARG, "positions are NULL");
     CHK(w.pos.ensure((size_t)N * 24 + 24));
-    if (N) HIPCHK(hipMemcpyAsync(w.pos.p, pos, (size_t)N * 24, hipMemcpyHostToDevice, s));
-    CHK(w.Z.ensure((size_t)N * 4 + 4));
-    if (N) HIPCHK(hipMemcpyAsync(w.Z.p, Z, (size_t)N * 4, hipMemcpyHostToDevice, s));
+    CHK(eg.stage_io.ensure((size_t)N * 24 + 64));
+    if (N) {
+        memcpy(eg.stage_io.p, pos, (size_t)N * 24);
+        HIPCHK(hipMemcpyAsync(w.pos.p, eg.stage_io.p, (size_t)N * 24, hipMemcpyHostToDevice, s));
+    }
+    CHK(stage_inputs(m, eg, 1, off, Z));
     int P = 0;
-    CHK(build_nl(m, N, w.pos.as<double>(), lat, pbc, nl_cutoff > 0 ? nl_cutoff : c.rc, &P));
-    w.h_centres.clear();
-    for (int i = 0; i < N; i++)
-        if (z_in(c.centreZ, Z[i])) w.h_centres.push_back(i);
-    const int nc = (int)w.h_centres.size();
+    CHK(build_nl(m, eg, 1, off, w.pos.as<double>(), pos, lat, pbc, nl_cutoff > 0 ? nl_cutoff : c.rc, &P));
+    const int nc = (int)eg.h_centres[coord].size();
     *n_desc = nc;
     if (nc == 0) return GAPX_OK;
-    CHK(w.centres.ensure((size_t)nc * 4));
-    HIPCHK(hipMemcpyAsync(w.centres.p, w.h_centres.data(), (size_t)nc * 4, hipMemcpyHostToDevice, s));
     const int KLM = c.S * c.n * (c.L + 1) * (c.L + 1);
     CHK(w.X.ensure((size_t)nc * c.dim * 8));
     CHK(w.norm.ensure((size_t)nc * 8));
     CHK(w.cbuf.ensure((size_t)nc * KLM * 8));
-    launch_soap_fwd(soap_dev(c), nc, w.centres.as<int>(), w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
-                    w.nb_r.as<double>(), w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(),
+    launch_soap_fwd(soap_dev(c), nc, w.centres[coord].as<int>(), w.Z.as<int>(), w.nb_off.as<int>(),
+                    w.nb_j.as<int>(), w.nb_r.as<double>(), w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(),
                     w.cbuf.as<double>(), s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(X, w.X.p, (size_t)nc * c.dim * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    memcpy(desc_atom, w.h_centres.data(), (size_t)nc * 4);
+    memcpy(desc_atom, eg.h_centres[coord].data(), (size_t)nc * 4);
     return GAPX_OK;
 }
 
 // mixed-precision sparse-point copies are built lazily the first time a model
 // runs in a split precision
-static int ensure_split(gapx_model *m, Coord &c, int prec) {
+int gapx::ensure_split(gapx_model *m, Coord &c, int prec, hipStream_t s) {
     if (c.split_prec == prec) return GAPX_OK;
     c.dpad = (c.dim + 31) / 32 * 32;
     c.Mpad = (c.M + 31) / 32 * 32;
@@ -1091,10 +1247,8 @@ static int ensure_split(gapx_model *m, Coord &c, int prec) {
     c.wscale = (prec == GAPX_PREC_FP16X3 && wmax > 0) ? std::ldexp(1.0, 8 - (int)std::ceil(std::log2(wmax))) : 1.0;
     CHK(c.d_XsT_hi.ensure((size_t)c.Mpad * c.dpad * 2));
     CHK(c.d_XsT_lo.ensure((size_t)c.Mpad * c.dpad * 2));
-    launch_split_matrix(prec, c.M, c.dim, c.dpad, c.d_Xs.as<double>(), c.dim, false, c.d_Xs_hi.p, c.d_Xs_lo.p,
-                        m->stream);
-    launch_split_matrix(prec, c.M, c.dim, c.Mpad, c.d_Xs.as<double>(), c.dim, true, c.d_XsT_hi.p, c.d_XsT_lo.p,
-                        m->stream);
+    launch_split_matrix(prec, c.M, c.dim, c.dpad, c.d_Xs.as<double>(), c.dim, false, c.d_Xs_hi.p, c.d_Xs_lo.p, s);
+    launch_split_matrix(prec, c.M, c.dim, c.Mpad, c.d_Xs.as<double>(), c.dim, true, c.d_XsT_hi.p, c.d_XsT_lo.p, s);
     HIPCHK(hipGetLastError());
     c.split_prec = prec;
     return GAPX_OK;

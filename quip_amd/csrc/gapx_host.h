@@ -1,0 +1,244 @@
+// Host-side internals of libgapx.so shared by its translation units: error
+// plumbing, device / pinned buffers, model and engine state.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gapx.h"
+#include "gapx_kernels.h"
+
+namespace gapx {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+// Records the calling thread's error message and returns `code`.
+int fail(int code, const char *fmt, ...);
+// The calling thread's last error message.
+const char *last_error_text();
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(GAPX_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHK(x)                      \
+    do {                            \
+        int s_ = (x);               \
+        if (s_ != GAPX_OK) return s_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return GAPX_OK;
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) return fail(GAPX_E_DEVICE, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+        cap = want;
+        return GAPX_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+    ~DBuf() { if (p) hipFree(p); }
+};
+
+// ---------------------------------------------------------------------------
+// model
+// ---------------------------------------------------------------------------
+struct GemmTune {
+    int g1 = 0, g2 = 1, parts = 1;   // GEMM 1 tile, GEMM 2 tile, GEMM 2 split-K factor
+    float g1_ms = 0, g2_ms = 0;      // measured times of the chosen configurations
+};
+
+struct Coord {
+    int kind = 0;          // 1 soap, 2 distance_2b
+    std::string label, desc;
+    // soap
+    double rc = 0, ctw = 0.5, sigma = 0.5, cw = 1.0, bee = 10.0;
+    int n = 0, L = 0, S = 1, dim = 0;
+    std::vector<int> species, centreZ;
+    double u00 = 0;
+    std::vector<double> rb, T;
+    std::vector<int> qmap;
+    DBuf d_rb, d_T, d_qmap;
+    // d2b
+    int Z1 = 0, Z2 = 0;
+    // gp
+    int cov = 2;
+    double delta = 1, zeta = 1, signal_mean = 0;
+    int M = 0;
+    std::vector<double> Xs, alpha, scut, theta;
+    DBuf d_Xs, d_w, d_theta;
+    // mixed-precision copies of Xs (split hi/lo), built on demand
+    int split_prec = -1;
+    int dpad = 0, Mpad = 0;
+    double wscale = 1.0;
+    DBuf d_Xs_hi, d_Xs_lo, d_XsT_hi, d_XsT_lo;
+    // FP64 GEMM tile choices per centre-count bucket (ceil log2 nc), from the autotuner
+    std::mutex tune_mu;
+    std::map<int, GemmTune> tune;
+};
+
+struct Workspace {
+    DBuf pos, Z, sid, atom_off, geo, cell_base, ginv, cell_of, map_shift, cell_cnt, cell_start, cell_fill, cell_atoms,
+        cell_unsorted, scan_part, scan_part2;
+    DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k, n2_is, n2_ks;
+    DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite;
+    DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws;
+    DBuf f, le, lv, e, vir, minmax;
+    std::vector<DBuf> centres;   // per coordinate
+};
+
+// Pinned host staging (hipHostMalloc), grow-only.
+struct HBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return GAPX_OK;
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) return fail(GAPX_E_DEVICE, "hipHostMalloc(%zu): %s", want, hipGetErrorString(e));
+        cap = want;
+        return GAPX_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+    ~HBuf() { if (p) hipHostFree(p); }
+};
+
+// Stage times from HIP events on an engine's stream.  Two event sets alternate
+// between calls; a call's set is read after the next call's neighbour-count
+// synchronisation (or at query time), so timing adds no synchronisation.
+constexpr int kStages = 6;
+struct StageTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool[2];
+    std::vector<int> marks[2];   // triples (stage, event a, event b)
+    int used[2] = {0, 0};
+    bool pending[2] = {false, false};
+    long seq[2] = {0, 0};
+    long next_seq = 0;
+    int cur = 0;
+    double last[kStages] = {0}, total[kStages] = {0};
+    long ncalls = 0;
+    void begin(hipStream_t) {
+        cur ^= 1;
+        resolve(cur);   // the call before last: complete by now
+        used[cur] = 0;
+        marks[cur].clear();
+    }
+    int mark(hipStream_t s) {
+        if (used[cur] == (int)pool[cur].size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            pool[cur].push_back(e);
+        }
+        hipEventRecord(pool[cur][used[cur]], s);
+        return used[cur]++;
+    }
+    void stage(int k, int ea, int eb) {
+        if (ea < 0 || eb < 0) return;
+        marks[cur].push_back(k); marks[cur].push_back(ea); marks[cur].push_back(eb);
+    }
+    void end() { pending[cur] = true; seq[cur] = next_seq++; }
+    void resolve(int set) {
+        if (!pending[set]) return;
+        pending[set] = false;
+        double t6[kStages] = {0};
+        for (size_t q = 0; q + 2 < marks[set].size(); q += 3) {
+            float t = 0;
+            if (hipEventSynchronize(pool[set][marks[set][q + 2]]) != hipSuccess) continue;
+            hipEventElapsedTime(&t, pool[set][marks[set][q + 1]], pool[set][marks[set][q + 2]]);
+            t6[marks[set][q]] += t;
+        }
+        for (int k = 0; k < kStages; k++) { last[k] = t6[k]; total[k] += t6[k]; }
+        ncalls++;
+    }
+    void resolve_all() {
+        int a = seq[0] <= seq[1] ? 0 : 1;
+        resolve(a);
+        resolve(a ^ 1);
+    }
+    void reset() {
+        resolve_all();
+        for (int k = 0; k < kStages; k++) total[k] = 0;
+        ncalls = 0;
+    }
+    ~StageTimer() {
+        for (auto &v : pool)
+            for (auto e : v) hipEventDestroy(e);
+    }
+};
+
+// One evaluation engine: a HIP stream, its device workspace, pinned staging and
+// the host copies of the last inputs (Z, structure layout, geometry) so that
+// unchanged inputs are not uploaded again.
+struct Engine {
+    hipStream_t stream = nullptr;
+    Workspace ws;
+    HBuf stage_in;    // Z, structure map, centre lists
+    HBuf stage_geo;   // lattice inverses, fractional extents, geometry table
+    HBuf stage_io;    // positions in, results out (host-buffer entry points)
+    HBuf ctl;         // the neighbour-count read-back
+    StageTimer timer;
+    std::vector<int> h_Z, h_off, h_sid, h_cell_base;
+    std::vector<NLGeom> h_geo;
+    std::vector<std::vector<int>> h_centres;   // per coordinate
+    bool have_inputs = false, have_geo = false;
+    int n1_total_last = 0;
+    int ncell_last = 0;
+    ~Engine() {
+        if (stream) {
+            hipStreamSynchronize(stream);
+            hipStreamDestroy(stream);
+        }
+    }
+};
+
+struct ModelState {
+    int device = 0;
+    std::string label;
+    double e0[119] = {0};
+    double E_scale = 1.0;
+    double cutoff = 0;
+    int precision = GAPX_PREC_FP64;
+    std::vector<std::unique_ptr<Coord>> coords;
+    DBuf d_e0;
+    bool autotune = true;
+    std::unique_ptr<Engine> eng;   // the engine of the plain entry points
+};
+
+// engine entry points (gapx_api.hip)
+int new_engine(gapx_model *m, std::unique_ptr<Engine> &out);
+int ensure_split(gapx_model *m, Coord &c, int prec, hipStream_t s);
+int check_batch(int S, const int *atom_off, const int *Z, const double *lats, const int *pbcs);
+// Host-buffer evaluation of S structures on engine eg (pinned staging both ways).
+int calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *pos, const int *Z,
+              const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *e, double *le, double *f,
+              double *v, double *lv);
+
+}  // namespace gapx
+
+// the opaque handle of the C ABI
+struct gapx_model : gapx::ModelState {};

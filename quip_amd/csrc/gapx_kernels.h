@@ -33,22 +33,24 @@ int scan_partials_size(int n);
 void launch_scan_i32(const int *in, int n, int *out, int *partials, hipStream_t s);
 void launch_fill_i32(int *p, int n, int v, hipStream_t s);
 void launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
-void launch_nl_bin(int N, const double *pos, const NLGeom &g, int *cell_of, int *map_shift, hipStream_t s);
+void launch_nl_bin(int N, const double *pos, const NLGeom *geo, const int *sid, const int *cell_base, int *cell_of,
+                   int *map_shift, hipStream_t s);
 void launch_nl_cell_count(int N, const int *cell_of, int *cnt, hipStream_t s);
 void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s);
 void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
                          int *atoms, hipStream_t s);
 void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
-                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom &g,
-                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt,
-                     const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, hipStream_t s);
+                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom *geo,
+                     const int *sid, const int *cell_base, int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift,
+                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k,
+                     hipStream_t s);
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
                        const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s);
-void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off, const int *n1_j,
+void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
                     hipStream_t s);
-void launch_minmax_frac(int N, const double *pos, const double *g, double *out, hipStream_t s);
+void launch_minmax_frac(int S, const int *atom_off, const double *pos, const double *g, double *out, hipStream_t s);
 
 int soap_supported(int n, int L);
 int soap_fast_supported(int n, int L);
@@ -66,13 +68,18 @@ void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z
                      const double *norm, const double *cbuf, double *pairF, hipStream_t s);
 
 // GEMMs
+// Tile configuration `cfg` in [0, kGemmTiles): (WM, WN) = (4,2), (2,2), (2,4), (4,4),
+// block tile (32 WM) x (32 WN) of 4 wavefronts.
+constexpr int kGemmTiles = 4;
 // gemm1: K = X (nc x d) . Xs^T (M x d); W = w_s zeta K^(zeta-1); e_part[i*ntile + t]
-void launch_gemm1_f64(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
-                      double *W, double *e_part, int *ntile_out, hipStream_t s);
-// gemm2: G = W (nc x M) . Xs (M x d); split-K partials in Gparts (gemm2_parts(nc,d) x nc x d)
+int gemm1_ntile(int cfg, int M);
+void launch_gemm1_f64(int cfg, int nc, int M, int d, const double *X, const double *Xs, const double *wvec,
+                      double zeta, double *W, double *e_part, int *ntile_out, hipStream_t s);
+// gemm2: G = W (nc x M) . Xs (M x d); with parts > 1 the k range is split and the
+// partials (parts x nc x d in Gparts) are summed in a fixed order
 int gemm2_parts(int nc, int d);
-void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, double *Gparts,
-                      hipStream_t s);
+void launch_gemm2_f64(int cfg, int parts, int nc, int M, int d, const double *W, const double *Xs, double *G,
+                      double *Gparts, hipStream_t s);
 // mixed precision forms: Xs pre-split into hi/lo (bf16 or fp16) copies, row- and column-major
 // W is scaled by wscale (a power of two) before its hi/lo split; GEMM2 divides it out.
 void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *X, const void *Xs_hi,
@@ -89,8 +96,8 @@ void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int 
                 const double *nb_d, double *pairF, double *pairE, hipStream_t s);
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
-                   double *local_e, double *local_virial, double *block_part, int *nblocks_out, hipStream_t s);
-void launch_final_reduce(int nblocks, const double *block_part, double E_scale, double *e_out, double *virial_out,
-                         hipStream_t s);
+                   double *local_e, double *local_virial, hipStream_t s);
+void launch_struct_reduce(int S, const int *atom_off, const double *local_e, const double *local_virial,
+                          double *e_out, double *virial_out, hipStream_t s);
 
 }  // namespace gapx

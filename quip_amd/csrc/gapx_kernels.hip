@@ -145,10 +145,14 @@ __device__ __forceinline__ void mult3_exact(const double *g, double p0, double p
     }
 }
 
-__global__ void k_nl_bin(int N, const double *__restrict__ pos, NLGeom g, int *cell_of, int *map_shift) {
+__global__ void k_nl_bin(int N, const double *__restrict__ pos, const NLGeom *__restrict__ geo,
+                         const int *__restrict__ sid, const int *__restrict__ cell_base, int *cell_of,
+                         int *map_shift) {
 #pragma clang fp contract(off)
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    const int st = sid[n];
+    const NLGeom &g = geo[st];
     double lp[3];
     mult3_exact(g.inv_eff, pos[3 * n], pos[3 * n + 1], pos[3 * n + 2], lp);
     int ci[3];
@@ -161,10 +165,13 @@ __global__ void k_nl_bin(int N, const double *__restrict__ pos, NLGeom g, int *c
         c = c < 1 ? 1 : (c > g.cells[a] ? g.cells[a] : c);
         ci[a] = c - 1;
     }
-    cell_of[n] = (ci[2] * g.cells[1] + ci[1]) * g.cells[0] + ci[0];
+    cell_of[n] = cell_base[st] + (ci[2] * g.cells[1] + ci[1]) * g.cells[0] + ci[0];
 }
-void launch_nl_bin(int N, const double *pos, const NLGeom &g, int *cell_of, int *map_shift, hipStream_t s) {
-    if (N > 0) hipLaunchKernelGGL(k_nl_bin, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pos, g, cell_of, map_shift);
+void launch_nl_bin(int N, const double *pos, const NLGeom *geo, const int *sid, const int *cell_base, int *cell_of,
+                   int *map_shift, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_nl_bin, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pos, geo, sid, cell_base, cell_of,
+                           map_shift);
 }
 
 __global__ void k_nl_cell_count(int N, const int *cell_of, int *cnt) {
@@ -221,7 +228,9 @@ template <bool FILL>
 __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restrict__ pos,
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
                                                     const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
-                                                    const int *__restrict__ cell_atoms, NLGeom g, int *n1_cnt,
+                                                    const int *__restrict__ cell_atoms,
+                                                    const NLGeom *__restrict__ geo, const int *__restrict__ sid,
+                                                    const int *__restrict__ cell_base, int *n1_cnt,
                                                     const int *n1_off, int *n1_j, int *n1_shift, double *n1_r,
                                                     int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i,
                                                     int *n2_k) {
@@ -229,7 +238,10 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
     const int lane = threadIdx.x & 63;
     const int atom1 = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (atom1 >= N) return;   // whole wavefront leaves together
-    const int lin = cell_of[atom1];
+    const int st = sid[atom1];
+    const NLGeom &g = geo[st];
+    const int cb = cell_base[st];
+    const int lin = cell_of[atom1] - cb;   // cell index within this structure
     const int cc[3] = {lin % g.cells[0] + 1, (lin / g.cells[0]) % g.cells[1] + 1, lin / (g.cells[0] * g.cells[1]) + 1};
     int lo[3], hi[3], span[3];
     for (int a = 0; a < 3; a++) {
@@ -263,6 +275,7 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
             if (k3 >= 1 && k3 <= g.cells[2] && j3 >= 1 && j3 <= g.cells[1] && i3 >= 1 && i3 <= g.cells[0]) {
                 lin2 = ((k3 - 1) * g.cells[1] + (j3 - 1)) * g.cells[0] + (i3 - 1);
                 same_cell = (lin2 == lin);
+                lin2 += cb;
             }
         }
         // two sweeps over this lane's cell: count, then (FILL) write at the scanned offset
@@ -320,17 +333,20 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
 }
 
 void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
-                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom &g,
-                     int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt,
-                     const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, hipStream_t s) {
+                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom *geo,
+                     const int *sid, const int *cell_base, int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift,
+                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k,
+                     hipStream_t s) {
     if (N <= 0) return;
     if (fill)
         hipLaunchKernelGGL(k_nl_pairs_w<true>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
-                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r, n2_cnt, n2_off,
+                           cell_start, cell_cnt, cell_atoms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r,
+                           n2_cnt, n2_off,
                            n2_fill, n2_i, n2_k);
     else
         hipLaunchKernelGGL(k_nl_pairs_w<false>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
-                           cell_start, cell_cnt, cell_atoms, g, n1_cnt, n1_off, n1_j, n1_shift, n1_r, n2_cnt, n2_off,
+                           cell_start, cell_cnt, cell_atoms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r,
+                           n2_cnt, n2_off,
                            n2_fill, n2_i, n2_k);
 }
 
@@ -370,7 +386,8 @@ void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_
 // Connection.f95:2038-2051), with diff = pos_j - pos_i + lattice.shift summed
 // m = 1..3 (:2120-2125) and the index of the mirror pair.  One wavefront per
 // atom, lanes over its entries.
-__global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict__ pos, NLGeom g,
+__global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict__ pos,
+                                                 const NLGeom *__restrict__ geo, const int *__restrict__ sid,
                                                  const int *n1_off, const int *n1_j, const int *n1_shift,
                                                  const double *n1_r, const int *n2_off, const int *n2_i,
                                                  const int *n2_k, const int *n1_rev, int *nb_off, int *nb_j,
@@ -382,6 +399,7 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
     const int base = n1_off[i] + n2_off[i];
     if (lane == 0) nb_off[i] = base;
     if (i == N) return;
+    const NLGeom &g = geo[sid[i]];
     const int n2c = n2_off[i + 1] - n2_off[i];
     const int n1c = n1_off[i + 1] - n1_off[i];
     for (int q = lane; q < n2c + n1c; q += 64) {
@@ -416,22 +434,24 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
         nb_d[3 * p] = d0; nb_d[3 * p + 1] = d1; nb_d[3 * p + 2] = d2;
     }
 }
-void launch_nl_full(int N, const double *pos, const NLGeom &g, const int *n1_off, const int *n1_j,
+void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 4)), dim3(256), 0, s, N, pos, g, n1_off, n1_j, n1_shift, n1_r,
+    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 4)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j, n1_shift, n1_r,
                        n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d);
 }
 
-// min / max of fractional coordinates (g .mult. pos) per axis, for calc_lat_eff.
-__global__ void k_minmax_frac(int N, const double *pos, const double *gmat, double *out) {
+// min / max of fractional coordinates (g .mult. pos) per axis, for calc_lat_eff;
+// one block per structure, g = the inverse of that structure's lattice.
+__global__ void k_minmax_frac(const int *atom_off, const double *pos, const double *gmat, double *out) {
 #pragma clang fp contract(off)
     __shared__ double smin[3][256], smax[3][256];
+    const int st = blockIdx.x;
     double g[9];
-    for (int k = 0; k < 9; k++) g[k] = gmat[k];
+    for (int k = 0; k < 9; k++) g[k] = gmat[9 * st + k];
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+    for (int n = atom_off[st] + threadIdx.x; n < atom_off[st + 1]; n += blockDim.x) {
         double lp[3];
         mult3_exact(g, pos[3 * n], pos[3 * n + 1], pos[3 * n + 2], lp);
         for (int a = 0; a < 3; a++) { mn[a] = fmin(mn[a], lp[a]); mx[a] = fmax(mx[a], lp[a]); }
@@ -447,10 +467,10 @@ __global__ void k_minmax_frac(int N, const double *pos, const double *gmat, doub
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int a = 0; a < 3; a++) { out[a] = smin[a][0]; out[3 + a] = smax[a][0]; }
+        for (int a = 0; a < 3; a++) { out[6 * st + a] = smin[a][0]; out[6 * st + 3 + a] = smax[a][0]; }
 }
-void launch_minmax_frac(int N, const double *pos, const double *g, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_minmax_frac, dim3(1), dim3(256), 0, s, N, pos, g, out);
+void launch_minmax_frac(int S, const int *atom_off, const double *pos, const double *g, double *out, hipStream_t s) {
+    if (S > 0) hipLaunchKernelGGL(k_minmax_frac, dim3(S), dim3(256), 0, s, atom_off, pos, g, out);
 }
 
 // ----------------------------------------------------------------------------
@@ -959,13 +979,30 @@ static int int_zeta(double z) {
     return (fabs(z - r) == 0.0 && r >= 1.0 && r <= 16.0) ? (int)r : 0;
 }
 
-void launch_gemm1_f64(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
-                      double *W, double *e_part, int *ntile_out, hipStream_t s) {
-    dim3 grid(cdiv(nc, 128), cdiv(M, 64), 1);
-    *ntile_out = grid.y;
-    if (nc <= 0) return;
-    hipLaunchKernelGGL((k_gemm_f64<true, true, 4, 2>), grid, dim3(256), 0, s, nc, M, d, d, X, Xs, W, wvec, zeta,
+// Tile shapes the autotuner chooses between: (WM, WN) per block of 4 wavefronts,
+// block tile (32 WM) x (32 WN).
+static const int kTiles[kGemmTiles][2] = {{4, 2}, {2, 2}, {2, 4}, {4, 4}};
+
+int gemm1_ntile(int cfg, int M) { return cdiv(M, 32 * kTiles[cfg][1]); }
+
+template <int WM, int WN>
+static void gemm1_go(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
+                     double *W, double *e_part, hipStream_t s) {
+    dim3 grid(cdiv(nc, 32 * WM), cdiv(M, 32 * WN), 1);
+    hipLaunchKernelGGL((k_gemm_f64<true, true, WM, WN>), grid, dim3(256), 0, s, nc, M, d, d, X, Xs, W, wvec, zeta,
                        int_zeta(zeta), e_part);
+}
+
+void launch_gemm1_f64(int cfg, int nc, int M, int d, const double *X, const double *Xs, const double *wvec,
+                      double zeta, double *W, double *e_part, int *ntile_out, hipStream_t s) {
+    *ntile_out = gemm1_ntile(cfg, M);
+    if (nc <= 0) return;
+    switch (cfg) {
+        case 0: gemm1_go<4, 2>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        case 1: gemm1_go<2, 2>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        case 2: gemm1_go<2, 4>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        default: gemm1_go<4, 4>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+    }
 }
 
 // split-K partial sums, reduced in a fixed order
@@ -977,6 +1014,7 @@ __global__ void k_sum_parts(size_t n, int parts, const double *P, double *out) {
     out[i] = s;
 }
 
+// default split-K factor: enough blocks to fill the chip
 int gemm2_parts(int nc, int d) {
     const long tiles = (long)cdiv(nc, 64) * cdiv(d, 64);
     int parts = 1;
@@ -984,14 +1022,25 @@ int gemm2_parts(int nc, int d) {
     return parts;
 }
 
-void launch_gemm2_f64(int nc, int M, int d, const double *W, const double *Xs, double *G, double *Gparts,
-                      hipStream_t s) {
-    if (nc <= 0) return;
-    const int parts = gemm2_parts(nc, d);
+template <int WM, int WN>
+static void gemm2_go(int nc, int M, int d, int parts, const double *W, const double *Xs, double *out,
+                     hipStream_t s) {
     const int kchunk = (cdiv(M, parts) + GBK - 1) / GBK * GBK;
-    dim3 grid(cdiv(nc, 64), cdiv(d, 64), parts);
-    hipLaunchKernelGGL((k_gemm_f64<false, false, 2, 2>), grid, dim3(256), 0, s, nc, d, M, kchunk, W, Xs,
-                       parts > 1 ? Gparts : G, (const double *)nullptr, 0.0, 0, (double *)nullptr);
+    dim3 grid(cdiv(nc, 32 * WM), cdiv(d, 32 * WN), parts);
+    hipLaunchKernelGGL((k_gemm_f64<false, false, WM, WN>), grid, dim3(256), 0, s, nc, d, M, kchunk, W, Xs, out,
+                       (const double *)nullptr, 0.0, 0, (double *)nullptr);
+}
+
+void launch_gemm2_f64(int cfg, int parts, int nc, int M, int d, const double *W, const double *Xs, double *G,
+                      double *Gparts, hipStream_t s) {
+    if (nc <= 0) return;
+    double *out = parts > 1 ? Gparts : G;
+    switch (cfg) {
+        case 0: gemm2_go<4, 2>(nc, M, d, parts, W, Xs, out, s); break;
+        case 1: gemm2_go<2, 2>(nc, M, d, parts, W, Xs, out, s); break;
+        case 2: gemm2_go<2, 4>(nc, M, d, parts, W, Xs, out, s); break;
+        default: gemm2_go<4, 4>(nc, M, d, parts, W, Xs, out, s); break;
+    }
     if (parts > 1) {
         const size_t n = (size_t)nc * d;
         hipLaunchKernelGGL(k_sum_parts, dim3(cdiv((long)n, 256)), dim3(256), 0, s, n, parts, Gparts, G);
@@ -1063,73 +1112,63 @@ __device__ __forceinline__ double wave_sum_g(double v) {
 __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const double *e0, double E_scale,
                                                       const int *nb_off, const int *nb_rev, const double *nb_d,
                                                       const double *pairF, const double *pairE, const double *esite,
-                                                      double *f, double *local_e, double *local_virial,
-                                                      double *block_part) {
-    __shared__ double sred[10][4];
+                                                      double *f, double *local_e, double *local_virial) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int j = blockIdx.x * 4 + w;
+    if (j >= N) return;
     double v[13];
 #pragma unroll
     for (int k = 0; k < 13; k++) v[k] = 0.0;
-    if (j < N) {
-        // lanes over the pairs of atom j; fixed lane assignment and a fixed reduction tree
-        for (int q = nb_off[j] + lane; q < nb_off[j + 1]; q += 64) {
-            const int r = nb_rev[q];
-            const double gx = pairF[3 * r], gy = pairF[3 * r + 1], gz = pairF[3 * r + 2];
-            v[0] += pairF[3 * q] - gx;
-            v[1] += pairF[3 * q + 1] - gy;
-            v[2] += pairF[3 * q + 2] - gz;
-            const double dx = nb_d[3 * r], dy = nb_d[3 * r + 1], dz = nb_d[3 * r + 2];
-            v[3] -= dx * gx; v[4] -= dy * gx; v[5] -= dz * gx;
-            v[6] -= dx * gy; v[7] -= dy * gy; v[8] -= dz * gy;
-            v[9] -= dx * gz; v[10] -= dy * gz; v[11] -= dz * gz;
-            v[12] += pairE[q] + pairE[r];
-        }
-#pragma unroll
-        for (int k = 0; k < 13; k++) v[k] = wave_sum_g(v[k]);
+    // lanes over the pairs of atom j; fixed lane assignment and a fixed reduction tree
+    for (int q = nb_off[j] + lane; q < nb_off[j + 1]; q += 64) {
+        const int r = nb_rev[q];
+        const double gx = pairF[3 * r], gy = pairF[3 * r + 1], gz = pairF[3 * r + 2];
+        v[0] += pairF[3 * q] - gx;
+        v[1] += pairF[3 * q + 1] - gy;
+        v[2] += pairF[3 * q + 2] - gz;
+        const double dx = nb_d[3 * r], dy = nb_d[3 * r + 1], dz = nb_d[3 * r + 2];
+        v[3] -= dx * gx; v[4] -= dy * gx; v[5] -= dz * gx;
+        v[6] -= dx * gy; v[7] -= dy * gy; v[8] -= dz * gy;
+        v[9] -= dx * gz; v[10] -= dy * gz; v[11] -= dz * gz;
+        v[12] += pairE[q] + pairE[r];
     }
-    double le = 0.0, lv[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) lv[k] = 0.0;
-    if (j < N) {
-        le = E_scale * (esite[j] + 0.5 * v[12] + e0[Z[j]]);
+    for (int k = 0; k < 13; k++) v[k] = wave_sum_g(v[k]);
+    if (lane < 3 && f) f[3 * j + lane] = E_scale * (lane == 0 ? v[0] : (lane == 1 ? v[1] : v[2]));
+    if (lane == 0) local_e[j] = E_scale * (esite[j] + 0.5 * v[12] + e0[Z[j]]);
+    if (lane < 9) {
+        double x = v[3];
 #pragma unroll
-        for (int k = 0; k < 9; k++) lv[k] = E_scale * v[3 + k];
-        if (lane < 3 && f) f[3 * j + lane] = E_scale * (lane == 0 ? v[0] : (lane == 1 ? v[1] : v[2]));
-        if (lane == 0 && local_e) local_e[j] = le;
-        if (lane < 9 && local_virial) {
-            double x = lv[0];
-#pragma unroll
-            for (int k = 1; k < 9; k++) x = (lane == k) ? lv[k] : x;
-            local_virial[9 * (size_t)j + lane] = x;
-        }
+        for (int k = 1; k < 9; k++) x = (lane == k) ? v[3 + k] : x;
+        local_virial[9 * (size_t)j + lane] = E_scale * x;
     }
-    if (lane == 0) {
-        sred[0][w] = le;
-#pragma unroll
-        for (int k = 0; k < 9; k++) sred[1 + k][w] = lv[k];
-    }
-    __syncthreads();
-    if (threadIdx.x < 10)
-        block_part[(size_t)blockIdx.x * 10 + threadIdx.x] =
-            (sred[threadIdx.x][0] + sred[threadIdx.x][1]) + (sred[threadIdx.x][2] + sred[threadIdx.x][3]);
 }
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
-                   double *local_e, double *local_virial, double *block_part, int *nblocks_out, hipStream_t s) {
-    int nb = cdiv(N > 0 ? N : 1, 4);
-    *nblocks_out = nb;
-    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev, nb_d, pairF,
-                       pairE, esite, f, local_e, local_virial, block_part);
+                   double *local_e, double *local_virial, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_gather, dim3(cdiv(N, 4)), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev,
+                           nb_d, pairF, pairE, esite, f, local_e, local_virial);
 }
-__global__ void k_final_reduce(int nblocks, const double *block_part, double *e_out, double *virial_out) {
+
+// Energy and virial of each structure: E = sum of local energies, V = sum of
+// local virials (IPModel_GAP.f95:524-526).  One block per structure; thread t
+// takes the structure's atoms t, t+256, ... and a fixed tree finishes the sum,
+// so a structure's totals do not depend on the other structures of a batch.
+__global__ __launch_bounds__(256) void k_struct_reduce(const int *atom_off, const double *local_e,
+                                                       const double *local_virial, double *e_out,
+                                                       double *virial_out) {
     __shared__ double sred[10][256];
+    const int st = blockIdx.x;
+    const int a0 = atom_off[st], a1 = atom_off[st + 1];
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 256)
+    for (int j = a0 + threadIdx.x; j < a1; j += 256) {
+        acc[0] += local_e[j];
 #pragma unroll
-        for (int k = 0; k < 10; k++) acc[k] += block_part[(size_t)b * 10 + k];
+        for (int k = 0; k < 9; k++) acc[1 + k] += local_virial[9 * (size_t)j + k];
+    }
 #pragma unroll
     for (int k = 0; k < 10; k++) sred[k][threadIdx.x] = acc[k];
     __syncthreads();
@@ -1139,16 +1178,17 @@ __global__ void k_final_reduce(int nblocks, const double *block_part, double *e_
             for (int k = 0; k < 10; k++) sred[k][threadIdx.x] += sred[k][threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        if (e_out) e_out[0] = sred[0][0];
-        if (virial_out)
-            for (int k = 0; k < 9; k++) virial_out[k] = sred[1 + k][0];
+    if (threadIdx.x < 10) {
+        const double x = sred[threadIdx.x][0];
+        if (threadIdx.x == 0) e_out[st] = x;
+        else virial_out[9 * (size_t)st + threadIdx.x - 1] = x;
     }
 }
-void launch_final_reduce(int nblocks, const double *block_part, double E_scale, double *e_out, double *virial_out,
-                         hipStream_t s) {
-    (void)E_scale;
-    hipLaunchKernelGGL(k_final_reduce, dim3(1), dim3(256), 0, s, nblocks, block_part, e_out, virial_out);
+void launch_struct_reduce(int S, const int *atom_off, const double *local_e, const double *local_virial,
+                          double *e_out, double *virial_out, hipStream_t s) {
+    if (S > 0)
+        hipLaunchKernelGGL(k_struct_reduce, dim3(S), dim3(256), 0, s, atom_off, local_e, local_virial, e_out,
+                           virial_out);
 }
 
 }  // namespace gapx

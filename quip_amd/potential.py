@@ -77,10 +77,20 @@ class Potential:
     def set_timing(self, on=True):
         L.check(L.lib().gapx_model_set_timing(self._h, 1 if on else 0), "set_timing")
 
+    STAGES = ("neighbour_list", "soap_forward", "gemm1", "gemm2", "soap_adjoint", "d2b_gather")
+
     def last_timings(self):
+        """Stage times (ms) of the last calc, from HIP events on the model stream."""
         t = np.zeros(6)
         L.check(L.lib().gapx_last_timings(self._h, L.ptr(t)), "last_timings")
-        return dict(zip(["neighbour_list", "soap_forward", "gemm1", "gemm2", "soap_adjoint", "d2b_gather"], t))
+        return dict(zip(self.STAGES, t))
+
+    def timing_totals(self):
+        """(stage -> summed ms, number of calls) since set_timing(True)."""
+        t = np.zeros(6)
+        n = C.c_int64()
+        L.check(L.lib().gapx_timing_totals(self._h, L.ptr(t), C.byref(n)), "timing_totals")
+        return dict(zip(self.STAGES, t)), n.value
 
     def calc(self, at, energy=None, force=None, virial=None, local_energy=None, local_virial=None, args_str="",
              cutoff=None):
@@ -156,7 +166,8 @@ class Potential:
         raise KeyError(name)
 
     def calc_batch(self, frames, want=("energy", "force", "virial"), cutoff=0.0):
-        """Many structures in one library call; returns per-structure dicts."""
+        """Many structures in one pass of the engine (gapx_calc_batch); returns
+        one dict per structure with the same keys as calc()."""
         off = np.zeros(len(frames) + 1, np.int32)
         off[1:] = np.cumsum([len(a) for a in frames])
         pos = L.f64(np.vstack([a.positions for a in frames]))
@@ -167,17 +178,25 @@ class Potential:
         E = np.zeros(len(frames))
         F = np.zeros((n, 3)) if "force" in want else None
         V = np.zeros((len(frames), 9)) if "virial" in want else None
-        w = L.WANT_E | (L.WANT_F if F is not None else 0) | (L.WANT_V if V is not None else 0)
+        LE = np.zeros(n) if "local_energy" in want else None
+        LV = np.zeros((n, 9)) if "local_virial" in want else None
+        w = (L.WANT_E | (L.WANT_F if F is not None else 0) | (L.WANT_V if V is not None else 0) |
+             (L.WANT_LOCAL_E if LE is not None else 0) | (L.WANT_LOCAL_V if LV is not None else 0))
         L.check(L.lib().gapx_calc_batch(self._h, len(frames), L.ptr(off), L.ptr(pos), L.ptr(Z), L.ptr(lats),
-                                        L.ptr(pbcs), cutoff, w, L.ptr(E), None, L.ptr(F), L.ptr(V), None),
+                                        L.ptr(pbcs), cutoff, w, L.ptr(E), L.ptr(LE), L.ptr(F), L.ptr(V), L.ptr(LV)),
                 "gapx_calc_batch")
         res = []
         for s in range(len(frames)):
+            a, b = off[s], off[s + 1]
             r = {"energy": E[s]}
             if F is not None:
-                r["force"] = F[off[s]:off[s + 1]]
+                r["force"] = F[a:b]
             if V is not None:
                 r["virial"] = V[s].reshape(3, 3).T
+            if LE is not None:
+                r["local_energy"] = LE[a:b]
+            if LV is not None:
+                r["local_virial"] = LV[a:b]
             res.append(r)
         return res
 

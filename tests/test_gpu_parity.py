@@ -173,16 +173,47 @@ def test_bitwise_repeatable(models):
     assert np.array_equal(a["local_virial"], b["local_virial"])
 
 
+def _same(rb, r, N):
+    # a batch changes GEMM 2's split-K factor (chosen from the centre count), so
+    # a structure alone and in a batch agree to rounding, not bit for bit
+    assert abs(rb["energy"] - r["energy"]) / max(N, 1) < 1e-12
+    assert np.abs(rb["force"] - r["force"]).max() < 1e-11
+    assert np.abs(rb["virial"] - r["virial"]).max() < 1e-10
+
+
 def test_batch_matches_single(models):
     m, path = models["si4096"]
     pot = Potential(param_filename=path)
     frames = configs.batch512(n_struct=12)
-    res = pot.calc_batch(frames)
+    assert any(not all(a.pbc) for a in frames)
+    res = pot.calc_batch(frames, want=("energy", "force", "virial", "local_energy", "local_virial"))
     for at, rb in zip(frames, res):
-        r = pot.calc(at.copy(), force=True, virial=True)
-        assert rb["energy"] == r["energy"]
-        assert np.array_equal(rb["force"], r["force"])
-        assert np.array_equal(rb["virial"], r["virial"])
+        r = pot.calc(at.copy(), force=True, virial=True, local_energy=True, local_virial=True)
+        _same(rb, r, len(at))
+        assert np.abs(rb["local_energy"] - r["local_energy"]).max() < 1e-12
+        assert np.abs(rb["local_virial"] - r["local_virial"]).max() < 1e-10
+        assert abs(rb["local_energy"].sum() - rb["energy"]) < 1e-9
+
+
+def test_batch512_full_size(models):
+    """The full 512-structure batch (136k atoms) in one pass: sampled structures
+    against single calls and the oracle; size-independent checks on all of them."""
+    m, path = models["si4096"]
+    pot = Potential(param_filename=path)
+    frames = configs.batch512()
+    res = pot.calc_batch(frames, want=("energy", "force", "virial", "local_energy", "local_virial"))
+    rng = np.random.default_rng(11)
+    for s in rng.choice(len(frames), 12, replace=False):
+        at = frames[s]
+        _same(res[s], pot.calc(at.copy(), force=True, virial=True), len(at))
+    for s in (0, int(np.argmax([len(a) for a in frames])), [k for k, a in enumerate(frames) if not all(a.pbc)][0]):
+        at = frames[s]
+        ro = O.gap_calc(m, at)
+        _compare(res[s], ro, len(at))
+    for at, r in zip(frames, res):
+        assert np.isfinite(r["energy"]) and np.isfinite(r["force"]).all()
+        assert np.abs(r["force"].sum(axis=0)).max() < 1e-9 * len(at)   # no net force
+        assert np.abs(r["virial"] - r["virial"].T).max() < 1e-8         # rotation invariance
 
 
 def test_finite_differences_on_gpu(models):
@@ -215,11 +246,14 @@ def _hip():
     return C.CDLL("libamdhip64.so")
 
 
-def test_device_resident_entry_point(models):
-    """gapx_calc_device with positions and outputs in HBM equals gapx_calc."""
+@pytest.mark.parametrize("case", ["si64", "slab"])
+def test_device_resident_entry_point(models, case):
+    """gapx_calc_device with positions and outputs in HBM equals gapx_calc (for the
+    slab, lat_eff's extents come from a device reduction instead of the host)."""
     m, path = models["si64"]
     pot = Potential(param_filename=path)
-    at = configs.si64()
+    at = configs.si64() if case == "si64" else configs.perturb(configs.slab111(5.43, 6, 10.0), 0.1,
+                                                               configs.rng_for(9, 1))
     ref = pot.calc(at.copy(), force=True, virial=True)
     hip = _hip()
     N = len(at)
