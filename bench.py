@@ -340,7 +340,9 @@ def main():
     roofline = dict(bound="mfma", kernel="k_gemm_f64 GEMM 1 (K = X Xs^T, fused pow/alpha epilogue; FP64 MFMA "
                     "16x16x4)", achieved=achieved / 1e12, peak=MFMA_FP64_PEAK / 1e12, unit="TFLOP/s",
                     frac=achieved / MFMA_FP64_PEAK, traffic=gemm1_traffic(), flops_per_launch=gemm_flops,
-                    avg_launch_ms=st["gemm1"])
+                    avg_launch_ms=st["gemm1"],
+                    traffic_source="HBM bytes per launch: FETCH_SIZE + WRITE_SIZE of this kernel scaled by a "
+                                   "copy-kernel calibration, profiles/k_gemm_f64_gemm1_si4096_traffic.txt")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -1,15 +1,22 @@
-"""Summaries of rocprofv3 result databases (rocpd SQLite) as short text tables.
+"""Short text summaries of rocprofv3 CSV output (``-f csv``).
 
-    python -m quip_amd.prof_summary kernels <results.db> [--match SUBSTR]
-    python -m quip_amd.prof_summary counters <results.db> [--match SUBSTR]
+    python -m quip_amd.prof_summary kernels  <run_kernel_trace.csv> [--match S] [--top N]
+    python -m quip_amd.prof_summary counters <run_counter_collection.csv> [--match S]
+    python -m quip_amd.prof_summary traffic  <fetch_counter.csv> <write_counter.csv> --kernel S --grid G
+                                            [--probe-bytes B]
 
-`kernels` prints calls, total and average duration per kernel (the --stats
-view); `counters` prints the average value per dispatch of each collected PMC
-counter (e.g. FETCH_SIZE / WRITE_SIZE in KiB) per kernel.
+kernels:  calls, total and average duration per (kernel, grid size), the
+          --stats view split by launch shape.
+counters: average counter value per dispatch per (kernel, grid size, counter).
+traffic:  HBM bytes per launch of one kernel shape from a FETCH_SIZE run and a
+          WRITE_SIZE run (KiB per dispatch), scaled by what the same counters
+          read for the device-to-device copy kernel (__amd_rocclr_copyBuffer)
+          of a probe of B bytes (B read + B written): factor = B / counter.
 """
 import argparse
+import csv
 import re
-import sqlite3
+from collections import defaultdict
 
 
 def short(name):
@@ -17,38 +24,87 @@ def short(name):
     return name.replace("void ", "").replace("gapx::", "")
 
 
-def kernels(db, match=None):
-    con = sqlite3.connect(db)
-    rows = con.execute("select name, count(*), sum(duration), avg(duration) from kernels group by name "
-                       "order by sum(duration) desc").fetchall()
-    tot = sum(r[2] for r in rows)
-    out = ["%-58s %7s %12s %10s %6s" % ("kernel", "calls", "total_us", "avg_us", "pct")]
-    for n, c, s, a in rows:
-        if match and match not in n:
+def _grid(row):
+    if "Grid_Size_X" in row:
+        return "%sx%sx%s" % (row["Grid_Size_X"], row["Grid_Size_Y"], row["Grid_Size_Z"])
+    return row.get("Grid_Size", "?")
+
+
+def kernels(path, match=None, top=40):
+    agg = defaultdict(lambda: [0, 0.0])
+    for r in csv.DictReader(open(path)):
+        k = (short(r["Kernel_Name"]), _grid(r))
+        agg[k][0] += 1
+        agg[k][1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    tot = sum(v[1] for v in agg.values())
+    out = ["%-44s %-18s %6s %11s %10s %6s" % ("kernel", "grid (threads)", "calls", "total_us", "avg_us", "pct")]
+    rows = sorted(agg.items(), key=lambda kv: -kv[1][1])
+    n = 0
+    for (name, grid), (c, s) in rows:
+        if match and match not in name:
             continue
-        out.append("%-58s %7d %12.1f %10.3f %6.2f" % (short(n)[:58], c, s / 1e3, a / 1e3, 100.0 * s / tot))
+        out.append("%-44s %-18s %6d %11.1f %10.3f %6.2f" % (name[:44], grid, c, s, s / c, 100.0 * s / tot))
+        n += 1
+        if n >= top:
+            break
     return "\n".join(out)
 
 
-def counters(db, match=None):
-    con = sqlite3.connect(db)
-    rows = con.execute("select kernel_name, counter_name, count(*), avg(value), avg(duration) from counters_collection "
-                       "group by kernel_name, counter_name order by kernel_name").fetchall()
-    out = ["%-58s %-12s %7s %14s %10s" % ("kernel", "counter", "calls", "avg_value", "avg_us")]
-    for n, cn, c, v, d in rows:
-        if match and match not in n:
+def _counter_avgs(path):
+    agg = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        agg[(short(r["Kernel_Name"]), _grid(r), r["Counter_Name"])].append(float(r["Counter_Value"]))
+    return {k: (len(v), sum(v) / len(v)) for k, v in agg.items()}
+
+
+def counters(path, match=None):
+    out = ["%-44s %-12s %-12s %6s %14s" % ("kernel", "grid", "counter", "calls", "avg_value")]
+    for (name, grid, cn), (c, v) in sorted(_counter_avgs(path).items()):
+        if match and match not in name:
             continue
-        out.append("%-58s %-12s %7d %14.1f %10.3f" % (short(n)[:58], cn, c, v, (d or 0) / 1e3))
+        out.append("%-44s %-12s %-12s %6d %14.1f" % (name[:44], grid, cn, c, v))
     return "\n".join(out)
+
+
+def traffic(fetch_csv, write_csv, kernel, grid, probe_bytes):
+    f, w = _counter_avgs(fetch_csv), _counter_avgs(write_csv)
+
+    def pick(d, name, cn, g=None):
+        vals = [(k, v) for k, v in d.items() if k[0] == name and k[2] == cn and (g is None or k[1] == g)]
+        return max(vals, key=lambda kv: kv[1][1]) if vals else None
+
+    # the probe's dispatch is the copy kernel shape with the largest counter value
+    pf, pw = pick(f, "__amd_rocclr_copyBuffer", "FETCH_SIZE"), pick(w, "__amd_rocclr_copyBuffer", "WRITE_SIZE")
+    kf, kw = pick(f, kernel, "FETCH_SIZE", grid), pick(w, kernel, "WRITE_SIZE", grid)
+    ff = probe_bytes / pf[1][1]
+    fw = probe_bytes / pw[1][1]
+    lines = ["copy probe: %d bytes read + written; FETCH_SIZE %.1f, WRITE_SIZE %.1f per dispatch"
+             % (probe_bytes, pf[1][1], pw[1][1]),
+             "scale: %.2f bytes per FETCH_SIZE unit, %.2f bytes per WRITE_SIZE unit" % (ff, fw),
+             "%s grid %s: FETCH_SIZE %.1f (%d dispatches), WRITE_SIZE %.1f (%d dispatches)"
+             % (kernel, grid, kf[1][1], kf[1][0], kw[1][1], kw[1][0]),
+             "fetch_bytes_per_launch %.0f" % (kf[1][1] * ff),
+             "write_bytes_per_launch %.0f" % (kw[1][1] * fw),
+             "hbm_bytes_per_launch %.0f" % (kf[1][1] * ff + kw[1][1] * fw)]
+    return "\n".join(lines)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["kernels", "counters"])
-    ap.add_argument("db")
+    ap.add_argument("what", choices=["kernels", "counters", "traffic"])
+    ap.add_argument("files", nargs="+")
     ap.add_argument("--match")
+    ap.add_argument("--top", type=int, default=40)
+    ap.add_argument("--kernel")
+    ap.add_argument("--grid")
+    ap.add_argument("--probe-bytes", type=int, default=256 << 20)
     a = ap.parse_args()
-    print(kernels(a.db, a.match) if a.what == "kernels" else counters(a.db, a.match))
+    if a.what == "kernels":
+        print(kernels(a.files[0], a.match, a.top))
+    elif a.what == "counters":
+        print(counters(a.files[0], a.match))
+    else:
+        print(traffic(a.files[0], a.files[1], a.kernel, a.grid, a.probe_bytes))
 
 
 if __name__ == "__main__":
