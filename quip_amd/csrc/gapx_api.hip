@@ -979,6 +979,8 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
                         w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(), w.cbuf.as<double>(), s);
         GemmTune tu;
         if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tu));
+        // split forms: one fused K -> W -> G kernel when the descriptor fits its register tile
+        const bool fused = m->precision != GAPX_PREC_FP64 && gp_fused_supported((c.dim + 31) / 32 * 32);
         const int t1 = tm.on ? tm.mark(s) : -1;
         int ntile = 0;
         if (m->precision == GAPX_PREC_FP64) {
@@ -986,11 +988,18 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
                              c.zeta, w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
         } else {
             CHK(ensure_split(m, c, m->precision, s));
-            CHK(w.Wf.ensure((size_t)nc * c.Mpad * 4 + 64));
-            CHK(w.split_ws.ensure((size_t)(nc + 63) / 64 * 64 * c.dpad * 4 + 64));
-            launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, w.X.as<double>(), c.d_Xs_hi.p, c.d_Xs_lo.p,
-                               c.d_w.as<double>(), c.zeta, c.wscale, w.Wf.as<float>(), w.e_part.as<double>(), &ntile,
-                               w.split_ws.p, s);
+            if (fused) {
+                CHK(w.Gparts.ensure((size_t)gp_fused_parts(nc, c.Mpad) * nc * c.dim * 4 + 16));
+                launch_gp_fused_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.X.as<double>(), c.d_Xs_hi.p,
+                                      c.d_Xs_lo.p, c.d_XsT_hi.p, c.d_XsT_lo.p, c.d_w.as<double>(), c.zeta, c.wscale,
+                                      w.G.as<double>(), w.Gparts.as<float>(), w.e_part.as<double>(), &ntile, s);
+            } else {
+                CHK(w.Wf.ensure((size_t)nc * c.Mpad * 4 + 64));
+                CHK(w.split_ws.ensure((size_t)(nc + 63) / 64 * 64 * c.dpad * 4 + 64));
+                launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, w.X.as<double>(), c.d_Xs_hi.p, c.d_Xs_lo.p,
+                                   c.d_w.as<double>(), c.zeta, c.wscale, w.Wf.as<float>(), w.e_part.as<double>(),
+                                   &ntile, w.split_ws.p, s);
+            }
         }
         const int t2 = tm.on ? tm.mark(s) : -1;
         launch_energy_rows(nc, ntile, centres, w.e_part.as<double>(), w.esite.as<double>(), s);
@@ -999,6 +1008,8 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
             if (m->precision == GAPX_PREC_FP64) {
                 launch_gemm2_f64(tu.g2, tu.parts, nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(),
                                  w.G.as<double>(), w.Gparts.as<double>(), s);
+            } else if (fused) {
+                launch_gp_fused_reduce(nc, c.dim, ntile, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
             } else {
                 launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.Wf.as<float>(), c.d_XsT_hi.p,
                                    c.d_XsT_lo.p, c.wscale, w.G.as<double>(), s);

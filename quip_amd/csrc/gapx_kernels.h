@@ -87,6 +87,16 @@ void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *
                         double *e_part, int *ntile_out, void *xsplit_ws, hipStream_t s);
 void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, const float *Wf, const void *XsT_hi,
                         const void *XsT_lo, double wscale, double *G, hipStream_t s);
+// fused K -> epilogue -> G for descriptors of up to 256 components (split forms):
+// one launch computes the energy partials e_part[i*parts + p] and, with parts > 1,
+// FP32 partial G tiles (parts x nc x d in Gpart) that launch_gp_fused_reduce sums;
+// with parts == 1 it writes G directly.  *ntile_out = parts.
+bool gp_fused_supported(int dpad);
+int gp_fused_parts(int nc, int Mpad);
+void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, const double *X, const void *Xs_hi,
+                           const void *Xs_lo, const void *XsT_hi, const void *XsT_lo, const double *wvec, double zeta,
+                           double wscale, double *G, float *Gpart, double *e_part, int *ntile_out, hipStream_t s);
+void launch_gp_fused_reduce(int nc, int d, int parts, const float *Gpart, double wscale, double *G, hipStream_t s);
 void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
                          void *hi, void *lo, hipStream_t s);
 

@@ -221,6 +221,172 @@ __global__ __launch_bounds__(256) void k_gemm2_split(int nc, int d, int Mpad, co
         }
 }
 
+// ---------------------------------------------------------------------------
+// Fused GP kernel for descriptors of up to 32*DK components (split forms).
+//
+// One block = 8 wavefronts = 128 centres; wave w owns centres 16w..16w+15 and
+// keeps their X rows (split hi/lo, A-operand layout) in registers.  The block
+// walks the sparse points of its part in chunks of 32.  Per chunk the chunk's
+// rows of Xs (k = descriptor index) and of Xs^T (k = sparse index) are staged
+// in LDS; each wave computes its 16 x 32 tile of K = X Xs^T (three MFMA
+// passes, FP32 accumulation), applies the FP64 epilogue (energy w K^zeta and
+// W = w zeta K^(zeta-1)), splits W into hi/lo through a per-wave LDS tile and
+// accumulates G += W Xs (16 x 32DK, in registers).  W never reaches HBM.
+// With several parts (gridDim.y) the sparse points are divided between blocks
+// and the FP32 partial G tiles are summed in a fixed order afterwards.
+// ---------------------------------------------------------------------------
+template <int PREC, int DK>
+__global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, int Mpad, int ldS, int chunks_per_part,
+                                                        const double *__restrict__ X,
+                                                        const typename Half<PREC>::T *__restrict__ Shi,
+                                                        const typename Half<PREC>::T *__restrict__ Slo,
+                                                        const typename Half<PREC>::T *__restrict__ Thi,
+                                                        const typename Half<PREC>::T *__restrict__ Tlo,
+                                                        const double *__restrict__ wvec, double zeta, int izeta,
+                                                        double wscale, double inv_wscale, double *__restrict__ G,
+                                                        float *__restrict__ Gpart, double *__restrict__ e_part) {
+    typedef Half<PREC> H;
+    typedef typename H::T T;
+    typedef typename H::V V;
+    constexpr int DP = 32 * DK;     // descriptor components handled (zero padded)
+    constexpr int XS_LD = DP + 8;   // LDS row strides (elements): 16-byte reads of 16 rows hit distinct banks
+    constexpr int XT_LD = 40;
+    __shared__ __attribute__((aligned(16))) T xs_l[2][32 * XS_LD];
+    __shared__ __attribute__((aligned(16))) T xt_l[2][DP * XT_LD];
+    __shared__ __attribute__((aligned(16))) T w_l[8][2][16 * XT_LD];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
+    const int row0 = blockIdx.x * 128 + wv * 16;
+    const int part = blockIdx.y, nparts = gridDim.y;
+    const int c_beg = part * chunks_per_part, c_end = min(Mpad / 32, c_beg + chunks_per_part);
+
+    V xh[DK], xl[DK];
+    {
+        const int row = row0 + r16;
+#pragma unroll
+        for (int ks = 0; ks < DK; ks++) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k = 32 * ks + kg + e;
+                const double x = (row < nc && k < d) ? X[(size_t)row * d + k] : 0.0;
+                const T h = H::from(x);
+                xh[ks][e] = h;
+                xl[ks][e] = H::from(x - H::to(h));
+            }
+        }
+    }
+    float4_t gacc[2 * DK];
+#pragma unroll
+    for (int q = 0; q < 2 * DK; q++) gacc[q] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+
+    for (int ch = c_beg; ch < c_end; ch++) {
+        const int s0 = 32 * ch;
+        __syncthreads();   // the previous chunk's LDS reads are done
+        for (int idx = t; idx < 32 * (DP / 8); idx += 512) {
+            const int r = idx / (DP / 8), c8 = (idx % (DP / 8)) * 8;
+            V h = {}, l = {};
+            if (c8 < ldS) {
+                h = *(const V *)(Shi + (size_t)(s0 + r) * ldS + c8);
+                l = *(const V *)(Slo + (size_t)(s0 + r) * ldS + c8);
+            }
+            *(V *)(xs_l[0] + r * XS_LD + c8) = h;
+            *(V *)(xs_l[1] + r * XS_LD + c8) = l;
+        }
+        for (int idx = t; idx < DP * 4; idx += 512) {
+            const int q = idx >> 2, c8 = (idx & 3) * 8;
+            V h = {}, l = {};
+            if (q < d) {
+                h = *(const V *)(Thi + (size_t)q * Mpad + s0 + c8);
+                l = *(const V *)(Tlo + (size_t)q * Mpad + s0 + c8);
+            }
+            *(V *)(xt_l[0] + q * XT_LD + c8) = h;
+            *(V *)(xt_l[1] + q * XT_LD + c8) = l;
+        }
+        __syncthreads();
+        // K tile: 16 centres x 32 sparse points
+        float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int ks = 0; ks < DK; ks++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int o = (16 * j + r16) * XS_LD + 32 * ks + kg;
+                const V bh = *(const V *)(xs_l[0] + o), bl = *(const V *)(xs_l[1] + o);
+                kacc[j] = H::mfma(xl[ks], bh, kacc[j]);
+                kacc[j] = H::mfma(xh[ks], bl, kacc[j]);
+                kacc[j] = H::mfma(xh[ks], bh, kacc[j]);
+            }
+        // FP64 epilogue; accumulator element v of lane l is centre rq + v, sparse point 16 j + r16
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int sp = s0 + 16 * j + r16;
+            const double wsv = sp < M ? wvec[sp] : 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double kv = (double)kacc[j][v];
+                double kz1, kz;
+                if (izeta > 0) {
+                    kz1 = 1.0;
+                    for (int q = 1; q < izeta; q++) kz1 *= kv;
+                    kz = kz1 * kv;
+                } else {
+                    kz1 = pow(kv, zeta - 1.0);
+                    kz = pow(kv, zeta);
+                }
+                rs[v] += wsv * kz;
+                const double wk = wsv * zeta * kz1 * wscale;
+                const T h = H::from(wk);
+                w_l[wv][0][(rq + v) * XT_LD + 16 * j + r16] = h;
+                w_l[wv][1][(rq + v) * XT_LD + 16 * j + r16] = H::from(wk - H::to(h));
+            }
+        }
+        __syncthreads();   // W tile visible in A-operand layout
+        const V wh = *(const V *)(w_l[wv][0] + r16 * XT_LD + kg), wl = *(const V *)(w_l[wv][1] + r16 * XT_LD + kg);
+#pragma unroll
+        for (int q = 0; q < 2 * DK; q++) {
+            const int o = (16 * q + r16) * XT_LD + kg;
+            const V bh = *(const V *)(xt_l[0] + o), bl = *(const V *)(xt_l[1] + o);
+            gacc[q] = H::mfma(wl, bh, gacc[q]);
+            gacc[q] = H::mfma(wh, bl, gacc[q]);
+            gacc[q] = H::mfma(wh, bh, gacc[q]);
+        }
+    }
+    // energy partials of this part: lanes sharing lane>>4 hold the same centres
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        double x = rs[v];
+        for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        rs[v] = x;
+    }
+    if (r16 == 0)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = row0 + rq + v;
+            if (row < nc) e_part[(size_t)row * nparts + part] = rs[v];
+        }
+#pragma unroll
+    for (int q = 0; q < 2 * DK; q++) {
+        const int col = 16 * q + r16;
+        if (col >= d) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = row0 + rq + v;
+            if (row >= nc) continue;
+            if (nparts == 1) G[(size_t)row * d + col] = (double)gacc[q][v] * inv_wscale;
+            else Gpart[((size_t)part * nc + row) * d + col] = gacc[q][v];
+        }
+    }
+}
+
+// G = inv_wscale * sum over parts (fixed order) of the FP32 partials
+__global__ void k_sum_parts_f32(size_t n, int parts, const float *P, double inv_wscale, double *G) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int p = 0; p < parts; p++) s += (double)P[(size_t)p * n + i];
+    G[i] = s * inv_wscale;
+}
+
 static int int_zeta_m(double z) {
     double r = floor(z + 0.5);
     return (fabs(z - r) == 0.0 && r >= 1.0 && r <= 16.0) ? (int)r : 0;
@@ -248,6 +414,49 @@ void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *
         hipLaunchKernelGGL(k_gemm1_split<2>, grid, dim3(256), 0, s, nc, M, dpad, Mpad, (const _Float16 *)Xhi,
                            (const _Float16 *)Xlo, (const _Float16 *)Xs_hi, (const _Float16 *)Xs_lo, wvec, zeta,
                            int_zeta_m(zeta), wscale, (_Float16 *)Whi, (_Float16 *)Wlo, e_part);
+}
+
+bool gp_fused_supported(int dpad) { return dpad <= 256; }
+
+int gp_fused_parts(int nc, int Mpad) {
+    const int row_blocks = cdivm(nc, 128), chunks = Mpad / 32;
+    int parts = std::min(chunks, std::max(1, cdivm(512, row_blocks)));
+    const int per = cdivm(chunks, parts);
+    return cdivm(chunks, per);
+}
+
+template <int PREC, int DK>
+static void gp_fused_go(int nc, int M, int d, int dpad, int Mpad, int parts, const double *X, const void *Shi,
+                        const void *Slo, const void *Thi, const void *Tlo, const double *wvec, double zeta,
+                        double wscale, double *G, float *Gpart, double *e_part, hipStream_t s) {
+    typedef typename Half<PREC>::T T;
+    const int per = cdivm(Mpad / 32, parts);
+    dim3 grid(cdivm(nc, 128), parts);
+    hipLaunchKernelGGL((k_gp_fused_split<PREC, DK>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per, X,
+                       (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta, int_zeta_m(zeta),
+                       wscale, 1.0 / wscale, G, Gpart, e_part);
+}
+
+void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, const double *X, const void *Xs_hi,
+                           const void *Xs_lo, const void *XsT_hi, const void *XsT_lo, const double *wvec, double zeta,
+                           double wscale, double *G, float *Gpart, double *e_part, int *ntile_out, hipStream_t s) {
+    const int parts = gp_fused_parts(nc, Mpad);
+    *ntile_out = parts;
+    if (nc <= 0) return;
+    const int dk = dpad <= 64 ? 2 : (dpad <= 128 ? 4 : 8);
+#define GO(P, D) gp_fused_go<P, D>(nc, M, d, dpad, Mpad, parts, X, Xs_hi, Xs_lo, XsT_hi, XsT_lo, wvec, zeta, wscale,                                    G, Gpart, e_part, s)
+    if (prec == 1) {
+        if (dk == 2) GO(1, 2); else if (dk == 4) GO(1, 4); else GO(1, 8);
+    } else {
+        if (dk == 2) GO(2, 2); else if (dk == 4) GO(2, 4); else GO(2, 8);
+    }
+#undef GO
+}
+
+void launch_gp_fused_reduce(int nc, int d, int parts, const float *Gpart, double wscale, double *G, hipStream_t s) {
+    if (parts <= 1 || nc <= 0) return;
+    const size_t n = (size_t)nc * d;
+    hipLaunchKernelGGL(k_sum_parts_f32, dim3(cdivm((long)n, 256)), dim3(256), 0, s, n, parts, Gpart, 1.0 / wscale, G);
 }
 
 void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, const float *Wf, const void *XsT_hi,

@@ -214,6 +214,13 @@ def test_batch512_full_size(models):
         assert np.isfinite(r["energy"]) and np.isfinite(r["force"]).all()
         assert np.abs(r["force"].sum(axis=0)).max() < 1e-9 * len(at)   # no net force
         assert np.abs(r["virial"] - r["virial"].T).max() < 1e-8         # rotation invariance
+    # split precision over the whole batch (one sparse-point part per centre block)
+    # against the FP64 batch: 1 meV/atom, 10 meV/A force RMSE
+    mixed = Potential(param_filename=path, precision="bf16x3").calc_batch(frames)
+    de = max(abs(a["energy"] - b["energy"]) / len(at) for at, a, b in zip(frames, mixed, res))
+    fr = np.sqrt(np.mean(np.concatenate([(a["force"] - b["force"]).ravel() for a, b in zip(mixed, res)]) ** 2))
+    print("batch512 bf16x3 max dE/atom", de, "force rmse", fr)
+    assert de < 1e-3 and fr < 1e-2
 
 
 def test_finite_differences_on_gpu(models):
