@@ -7,11 +7,12 @@
 // per-lane recursion state (radial: F_l, F_(l+1), 1/x per grid point;
 // Legendre: two previous levels) stays in registers with no scratch.
 //
-//  forward: lanes = neighbours of a chunk write [g_(b,l) | Y_lm] rows into LDS,
-//           lanes = (species, b, lm) accumulate c, lanes = power-spectrum
-//           entries form p, a wave reduction gives |p|.
-//  adjoint: dE/dp -> dE/dc (LDS), then lanes = neighbours re-evaluate
-//           g, dg/dr, Y and grad Y and contract them with dE/dc.
+//  forward: per l, lanes = neighbours stage fc F_l(a) and Y_l,m in LDS, lanes
+//           = (a, m) accumulate c' in the primitive radial basis; c = T^T c'
+//           once per centre; lanes = power-spectrum entries form p, a wave
+//           reduction gives |p|.
+//  adjoint: dE/dp -> dE/dc -> dE/dc' = T dE/dc (LDS), then lanes = neighbours
+//           re-evaluate F_l, Y and grad Y and contract them with dE/dc'.
 #include "gapx_kernels.h"
 
 #include <algorithm>
@@ -23,6 +24,7 @@ namespace gapx {
 // N_lm of the real spherical harmonics (x sqrt 2 for m > 0) and (2m-1)!!
 static __constant__ double c_ynorm[(kMaxL + 1) * (kMaxL + 2) / 2];
 static __constant__ double c_dfact[kMaxL + 2];
+static __constant__ double c_rinv[2 * kMaxL + 4];   // 1/k (k = 0 unused)
 
 static bool g_soap_const_ready = false;
 static void ensure_soap_constants() {
@@ -38,12 +40,19 @@ static void ensure_soap_constants() {
         }
     df[0] = 1.0;
     for (int m = 1; m < kMaxL + 2; m++) df[m] = df[m - 1] * (2.0 * m - 1.0);
+    double ri[2 * kMaxL + 4];
+    ri[0] = 0.0;
+    for (int k = 1; k < 2 * kMaxL + 4; k++) ri[k] = 1.0 / (double)k;
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_rinv), ri, sizeof(ri));
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn));
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df));
     g_soap_const_ready = true;
 }
 
 #define RELOAD_BARRIER() __asm__ volatile("" ::: "memory")
+
+// neighbours staged per pass of the forward kernel (one lane each)
+constexpr int kSoapChunk = 32;
 
 __device__ __forceinline__ double fc_s(double r, double rc, double w) {
     if (r > rc) return 0.0;
@@ -63,6 +72,18 @@ __device__ __forceinline__ int species_s(const SoapDev &sp, int z) {
     for (int s = 0; s < sp.S; s++)
         if (sp.species_Z[s] == z || sp.species_Z[s] == 0) return s;
     return -1;
+}
+
+// Associated Legendre values P_l^m (m = 0..l) of one neighbour, level by level,
+// in two LDS rows: on entry cur holds P_(l-2), prev P_(l-1); on exit cur holds P_l.
+__device__ __forceinline__ void legendre_level(int l, double uz, const double *prev, double *cur) {
+    for (int m = 0; m <= l; m++) {
+        double pc;
+        if (m == l) pc = c_dfact[m];
+        else if (m == l - 1) pc = (2.0 * l - 1.0) * uz * prev[m];
+        else pc = ((2.0 * l - 1.0) * uz * prev[m] - (double)(l + m - 1) * cur[m]) * c_rinv[l - m];
+        cur[m] = pc;
+    }
 }
 
 // i_l recursion state for all grid points: F0 = f_l, F1 = f_(l+1)
@@ -85,6 +106,7 @@ struct Rad {
                 F1[a] = 0.5 * (em + ep) * iv - F0[a] * iv;
                 ix[a] = iv;
             }
+            __builtin_amdgcn_sched_barrier(0);   // one grid point's exponentials at a time
         }
     }
     __device__ __forceinline__ void step(int l) {
@@ -97,163 +119,168 @@ struct Rad {
     }
 };
 
+// Forward.  With g_b(r) = fc(r) sum_a F_l(a, r) T[a][b] (F = the i_l recursion
+// values of grid point a), c[b][lm] = sum_a T[a][b] c'[a][lm] where
+// c'[a][lm] = sum_k fc F_l(a, r_k) Y_lm(k) is accumulated in the primitive
+// (Gaussian) radial basis and transformed once per centre.
+// Per neighbour chunk and per l: lanes = neighbours stage fc F_l(a) and Y_l,m
+// in LDS; lanes = (a, m) accumulate c' for each species.
 template <int N, int L>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))) void k_soap_fwd_x(SoapDev sp, const int *__restrict__ centres,
-                                                   const int *__restrict__ Z, const int *__restrict__ nb_off,
-                                                   const int *__restrict__ nb_j, const double *__restrict__ nb_r,
-                                                   const double *__restrict__ nb_d, double *X, double *norm_out,
-                                                   double *cbuf, int NB, int rows_cap) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))) void k_soap_fwd_x(
+    SoapDev sp, const int *__restrict__ centres, const int *__restrict__ Z, const int *__restrict__ nb_off,
+    const int *__restrict__ nb_j, const double *__restrict__ nb_r, const double *__restrict__ nb_d, double *X,
+    double *norm_out, double *cbuf) {
     extern __shared__ double lds[];
     constexpr int LM = (L + 1) * (L + 1);
-    constexpr int GL = N * (L + 1);
-    constexpr int RL = GL + LM;
-    const int K = sp.K, KLM = K * LM;
-    double *Ts = lds;                 // N*N
-    double *rbs = Ts + N * N;         // N
-    double *cL = rbs + N;             // K*LM
-    double *rows = cL + KLM;          // rows_cap
-    int *spc = (int *)(rows + rows_cap);
+    constexpr int YW = 2 * L + 1;
+    constexpr int NB = kSoapChunk;
+    const int K = sp.K, KLM = K * LM, S = sp.S;
+    double *Ts = lds;              // N*N
+    double *rbs = Ts + N * N;      // N
+    double *cP = rbs + N;          // K*LM: c' during accumulation, c after the transform
+    double *Rl = cP + KLM;         // S x NB x N: fc F_l(a) of neighbour k, zero unless k has species s
+    double *Yl = Rl + S * NB * N;  // NB x YW
     const int lane = threadIdx.x;
     const int desc = blockIdx.x;
     const int ci = centres[desc];
     for (int e = lane; e < N * N; e += 64) Ts[e] = sp.T[e];
     if (lane < N) rbs[lane] = sp.rb[lane];
-    for (int e = lane; e < KLM; e += 64) cL[e] = 0.0;
+    for (int e = lane; e < KLM; e += 64) cP[e] = 0.0;
     __syncthreads();
-    if (lane == 0) {
-        const int sc = species_s(sp, Z[ci]);
-        if (sc >= 0) cL[(sc * N) * LM] += sp.cw * sp.u00 * c_ynorm[0];
-    }
     const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
     for (int c0 = 0; c0 < np; c0 += NB) {
         const int nk = min(NB, np - c0);
-        __syncthreads();
+        int s = -1;
+        double fc = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+        Rad<N> rs;
         if (lane < nk) {
             const int p = p0 + c0 + lane;
             const double r = nb_r[p];
-            int s = species_s(sp, Z[nb_j[p]]);
-            double *row = rows + (size_t)lane * RL;
+            s = species_s(sp, Z[nb_j[p]]);
             if (r > sp.rc) s = -1;
             if (s >= 0) {
-                const double fc = fc_s(r, sp.rc, sp.ctw);
-                Rad<N> rs;
+                fc = fc_s(r, sp.rc, sp.ctw);
                 rs.init(rbs, sp.alpha, r);
-#pragma unroll
-                for (int l = 0; l <= L; l++) {
-#pragma unroll
-                    for (int b = 0; b < N; b++) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int a = 0; a < N; a++) acc += rs.F0[a] * Ts[a * N + b];
-                        row[b * (L + 1) + l] = fc * acc;
-                    }
-                    rs.step(l);
-                    RELOAD_BARRIER();
-                }
                 const double ir = 1.0 / r;
-                const double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
-                double *Yr = row + GL;
-                double Pm1[L + 1], Pm2[L + 1];
-#pragma unroll
-                for (int l = 0; l <= L; l++) {
-                    double Pc[L + 1];
-#pragma unroll
-                    for (int m = 0; m <= l; m++) {
-                        if (m == l) Pc[m] = c_dfact[m];
-                        else if (m == l - 1) Pc[m] = (2.0 * l - 1.0) * uz * Pm1[m];
-                        else Pc[m] = ((2.0 * l - 1.0) * uz * Pm1[m] - (double)(l + m - 1) * Pm2[m]) * (1.0 / (double)(l - m));
-                    }
-                    double A = 1.0, B = 0.0;
-#pragma unroll
-                    for (int m = 0; m <= l; m++) {
-                        if (m > 0) {
-                            const double An = A * ux - B * uy;
-                            B = A * uy + B * ux;
-                            A = An;
-                        }
-                        const double Nn = c_ynorm[l * (l + 1) / 2 + m] * Pc[m];
-                        if (m == 0) Yr[l * l + l] = Nn;
-                        else {
-                            Yr[l * l + l + m] = Nn * A;
-                            Yr[l * l + l - m] = Nn * B;
-                        }
-                    }
-#pragma unroll
-                    for (int m = 0; m <= l; m++) {
-                        Pm2[m] = Pm1[m];
-                        Pm1[m] = Pc[m];
-                    }
-                }
+                ux = nb_d[3 * p] * ir; uy = nb_d[3 * p + 1] * ir; uz = nb_d[3 * p + 2] * ir;
             }
-            spc[lane] = s;
         }
-        __syncthreads();
-        // lanes over lm; each lane accumulates all radial channels of one species at a time.
-        // spc[k] is the same for every lane, so the species test is a uniform branch.
-        for (int lm = lane; lm < LM; lm += 64) {
-            int l = (int)sqrt((double)lm + 0.5);
-            if (l * l > lm) l--;
-            for (int sp_i = 0; sp_i < K / N; sp_i++) {
-                double acc[N];
+        double *Pq = Yl + NB * YW + lane * 2 * (L + 2);   // this lane's two Legendre rows
+#pragma unroll 1
+        for (int l = 0; l <= L; l++) {
+            __syncthreads();   // the previous level's reads of Rl / Yl are done
+            if (lane < NB) {
+                for (int si = 0; si < S; si++)
 #pragma unroll
-                for (int b = 0; b < N; b++) acc[b] = 0.0;
-                bool any = false;
-                for (int k = 0; k < nk; k++) {
-                    if (spc[k] != sp_i) continue;
-                    any = true;
-                    const double *row = rows + (size_t)k * RL;
-                    const double y = row[GL + lm];
-#pragma unroll
-                    for (int b = 0; b < N; b++) acc[b] += row[b * (L + 1) + l] * y;
+                    for (int a = 0; a < N; a++) Rl[(si * NB + lane) * N + a] = (si == s) ? fc * rs.F0[a] : 0.0;
+            }
+            if (s >= 0) {
+                rs.step(l);
+                double *cur = Pq + (l & 1) * (L + 2);
+                legendre_level(l, uz, Pq + ((l + 1) & 1) * (L + 2), cur);
+                double A = 1.0, B = 0.0;
+                double *Yr = Yl + lane * YW;
+                for (int m = 0; m <= l; m++) {
+                    if (m > 0) {
+                        const double An = A * ux - B * uy;
+                        B = A * uy + B * ux;
+                        A = An;
+                    }
+                    const double Nn = c_ynorm[l * (l + 1) / 2 + m] * cur[m];
+                    if (m == 0) Yr[l] = Nn;
+                    else {
+                        Yr[l + m] = Nn * A;
+                        Yr[l - m] = Nn * B;
+                    }
                 }
-                if (any) {
-#pragma unroll
-                    for (int b = 0; b < N; b++) cL[(sp_i * N + b) * LM + lm] += acc[b];
+            } else if (lane < NB) {
+                for (int m = 0; m < 2 * l + 1; m++) Yl[lane * YW + m] = 0.0;
+            }
+            __syncthreads();
+            const int W = 2 * l + 1;
+            for (int it = lane; it < N * W; it += 64) {
+                const int a = it / W, mi = it - a * W;
+                for (int si = 0; si < S; si++) {
+                    const double *Rs = Rl + si * NB * N + a;
+                    double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll 4
+                    for (int k = 0; k < NB; k += 2) {
+                        acc0 += Rs[k * N] * Yl[k * YW + mi];
+                        acc1 += Rs[(k + 1) * N] * Yl[(k + 1) * YW + mi];
+                    }
+                    cP[(si * N + a) * LM + l * l + mi] += acc0 + acc1;
                 }
             }
         }
     }
     __syncthreads();
-    double *pb = rows;
+    // c = T^T c', one (species, lm) column per lane, in place
+    for (int it = lane; it < S * LM; it += 64) {
+        const int si = it / LM, lm = it - si * LM;
+        RELOAD_BARRIER();   // T is re-read from LDS for every column
+        double cp[N];
+#pragma unroll
+        for (int a = 0; a < N; a++) cp[a] = cP[(si * N + a) * LM + lm];
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; a++) acc += Ts[a * N + b] * cp[a];
+            cP[(si * N + b) * LM + lm] = acc;
+            RELOAD_BARRIER();   // one column of T in registers at a time
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {   // centre atom term
+        const int sc = species_s(sp, Z[ci]);
+        if (sc >= 0) cP[(sc * N) * LM] += sp.cw * sp.u00 * c_ynorm[0];
+    }
+    __syncthreads();
+    // power spectrum, written unnormalised and rescaled in place by the same lanes
     const int d = sp.dim;
+    double *xo = X + (size_t)desc * d;
     double ss = 0.0;
     for (int q = lane; q < d - 1; q += 64) {
         const int code = sp.qmap[q];
         const int ia = code >> 16, jb = (code >> 8) & 0xff, l = code & 0xff;
         double v = 0.0;
-        for (int m = -l; m <= l; m++) v += cL[ia * LM + l * l + l + m] * cL[jb * LM + l * l + l + m];
+        for (int m = -l; m <= l; m++) v += cP[ia * LM + l * l + l + m] * cP[jb * LM + l * l + l + m];
         v *= (ia != jb ? 1.4142135623730950488 : 1.0) / sqrt(2.0 * l + 1.0);
-        pb[q] = v;
+        xo[q] = v;
         ss += v * v;
     }
     ss = wave_sum_s(ss);
     const double nrm = sqrt(ss);
     const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-    double *xo = X + (size_t)desc * d;
-    for (int q = lane; q < d - 1; q += 64) xo[q] = pb[q] * inv;
+    for (int q = lane; q < d - 1; q += 64) xo[q] = xo[q] * inv;
     if (lane == 0) {
         xo[d - 1] = 0.0;
         norm_out[desc] = nrm;
     }
     double *co = cbuf + (size_t)desc * KLM;
-    for (int e = lane; e < KLM; e += 64) co[e] = cL[e];
+    for (int e = lane; e < KLM; e += 64) co[e] = cP[e];
 }
 
+// Adjoint.  dE/dc in the orthonormal basis is transformed once per centre to
+// the primitive basis, dc'[a][lm] = sum_b T[a][b] dc[b][lm], so that per
+// neighbour and (l, m): sum_b dc g_b = fc S0 and sum_b dc dg_b/dr =
+// fc (cl S0 + 2 alpha S1) + dfc S0 with S0 = sum_a F_l(a) dc'[a], S1 =
+// sum_a r_a F_(l+1)(a) dc'[a] and cl = -2 alpha r + l / r.
 template <int N, int L>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))) void k_soap_bwd_x(
-    SoapDev sp, int nc, int cpb, const int *__restrict__ centres, const int *__restrict__ Z, const int *__restrict__ nb_off,
-    const int *__restrict__ nb_j, const double *__restrict__ nb_r, const double *__restrict__ nb_d,
-    const double *__restrict__ X, const double *__restrict__ G, const double *__restrict__ norm,
-    const double *__restrict__ cbuf, double *pairF) {
-    // Two centres per wavefront: a centre has ~30 neighbours, so one lane per
-    // neighbour over 32 lanes keeps the whole wavefront busy.
+    SoapDev sp, int nc, int cpb, const int *__restrict__ centres, const int *__restrict__ Z,
+    const int *__restrict__ nb_off, const int *__restrict__ nb_j, const double *__restrict__ nb_r,
+    const double *__restrict__ nb_d, const double *__restrict__ X, const double *__restrict__ G,
+    const double *__restrict__ norm, const double *__restrict__ cbuf, double *pairF, int pq_own) {
+    // cpb centres per wavefront: a centre has ~30 neighbours, so one lane per
+    // neighbour over 64/cpb lanes keeps the wavefront busy.
     extern __shared__ double lds[];
     constexpr int LM = (L + 1) * (L + 1);
-    const int K = sp.K, KLM = K * LM, d = sp.dim;
+    const int K = sp.K, KLM = K * LM, d = sp.dim, S = sp.S;
     const int per = 2 * KLM + d;
     double *Ts = lds;              // N*N
     double *rbs = Ts + N * N;      // N
-    double *cen = rbs + N;         // 2 x [cL (K*LM) | dcL (K*LM) | dpL (d)]
+    double *cen = rbs + N;         // cpb x [cL (K*LM) | dc (K*LM) | dpL (d)]
     const int lane = threadIdx.x;
     const int d0 = cpb * blockIdx.x;
     const int nh = min(cpb, nc - d0);
@@ -288,12 +315,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
         dcL[e] = acc / sqrt(2.0 * l + 1.0);
     }
     __syncthreads();
+    // dc -> dc' in place, one (centre, species, lm) column per lane
+    for (int it = lane; it < nh * S * LM; it += 64) {
+        const int h = it / (S * LM), rem = it - h * S * LM, si = rem / LM, lm = rem - si * LM;
+        double *col = cen + h * per + KLM + si * N * LM + lm;
+        RELOAD_BARRIER();   // T is re-read from LDS for every column
+        double v[N];
+#pragma unroll
+        for (int b = 0; b < N; b++) v[b] = col[b * LM];
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int b = 0; b < N; b++) acc += Ts[a * N + b] * v[b];
+            col[a * LM] = acc;
+            RELOAD_BARRIER();   // one row of T in registers at a time
+        }
+    }
+    __syncthreads();
     const int lanes_per = 64 / cpb;
     const int h = lane / lanes_per, sub = lane - h * lanes_per;
     if (h >= nh) return;
     const int ci = centres[d0 + h];
-    const double *dcL = cen + h * per + KLM;
+    const double *dcP = cen + h * per + KLM;
+    // this lane's two Legendre rows: whole rows in the centre's c and dE/dp
+    // regions, which are no longer needed, or in a region of their own
+    constexpr int PR = 2 * (L + 2);
+    const int in_c = KLM / PR;
+    double *Pq = pq_own ? cen + cpb * per + lane * PR
+                        : (sub < in_c ? cen + h * per + sub * PR : cen + h * per + 2 * KLM + (sub - in_c) * PR);
     const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
+    const double al2 = 2.0 * sp.alpha;
     for (int k = sub; k < np; k += lanes_per) {
         const int p = p0 + k;
         const double r = nb_r[p];
@@ -302,38 +354,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
         const double fc = fc_s(r, sp.rc, sp.ctw), dfc = dfc_s(r, sp.rc, sp.ctw);
         const double ir = 1.0 / r;
         const double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
-        const double *dcs = dcL + s * N * LM;
+        const double *dcs = dcP + s * N * LM;
         Rad<N> rs;
         rs.init(rbs, sp.alpha, r);
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        double Pm1[L + 1], Pm2[L + 1];
-#pragma unroll
+#pragma unroll 1
         for (int l = 0; l <= L; l++) {
-            double gb[N], dgb[N];
-            const double cl = -2.0 * sp.alpha * r + (double)l * ir;
-#pragma unroll
-            for (int b = 0; b < N; b++) {
-                double g0 = 0.0, q1 = 0.0;
-#pragma unroll
-                for (int a = 0; a < N; a++) {
-                    const double t = Ts[a * N + b];
-                    g0 += rs.F0[a] * t;
-                    q1 += rbs[a] * rs.F1[a] * t;
-                }
-                gb[b] = fc * g0;
-                dgb[b] = fc * (cl * g0 + 2.0 * sp.alpha * q1) + dfc * g0;
-            }
-            rs.step(l);
-            double Pc[L + 2];
-#pragma unroll
-            for (int m = 0; m <= l; m++) {
-                if (m == l) Pc[m] = c_dfact[m];
-                else if (m == l - 1) Pc[m] = (2.0 * l - 1.0) * uz * Pm1[m];
-                else Pc[m] = ((2.0 * l - 1.0) * uz * Pm1[m] - (double)(l + m - 1) * Pm2[m]) * (1.0 / (double)(l - m));
-            }
-            Pc[l + 1] = 0.0;
+            const double cl = -al2 * r + (double)l * ir;
+            double *cur = Pq + (l & 1) * (L + 2);
+            legendre_level(l, uz, Pq + ((l + 1) & 1) * (L + 2), cur);
+            cur[l + 1] = 0.0;
             double A = 1.0, B = 0.0, Am = 0.0, Bm = 0.0;
-#pragma unroll
             for (int m = 0; m <= l; m++) {
                 if (m > 0) {
                     Am = A;
@@ -343,7 +374,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
                     A = An;
                 }
                 const double Nn = c_ynorm[l * (l + 1) / 2 + m];
-                const double P = Pc[m], Pz = Pc[m + 1];
+                const double P = cur[m], Pz = cur[m + 1];
 #pragma unroll
                 for (int sg = 0; sg < 2; sg++) {
                     if (m == 0 && sg == 1) continue;
@@ -352,13 +383,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
                     else if (sg == 0) { Y = Nn * P * A; Fx = Nn * P * m * Am; Fy = -Nn * P * m * Bm; Fz = Nn * Pz * A; }
                     else { Y = Nn * P * B; Fx = Nn * P * m * Bm; Fy = Nn * P * m * Am; Fz = Nn * Pz * B; }
                     const int lm = l * l + l + (sg ? -m : m);
-                    double s1 = 0.0, s2 = 0.0;
+                    double S0 = 0.0, S1 = 0.0;
 #pragma unroll
-                    for (int b = 0; b < N; b++) {
-                        const double dc = dcs[b * LM + lm];
-                        s1 += dc * dgb[b];
-                        s2 += dc * gb[b];
+                    for (int a = 0; a < N; a++) {
+                        const double dc = dcs[a * LM + lm];
+                        S0 += rs.F0[a] * dc;
+                        S1 += rbs[a] * rs.F1[a] * dc;
                     }
+                    const double s2 = fc * S0, s1 = fc * (cl * S0 + al2 * S1) + dfc * S0;
                     const double ud = ux * Fx + uy * Fy + uz * Fz;
                     const double a1 = s1 * Y - s2 * ud * ir, a2 = s2 * ir;
                     fx += a1 * ux + a2 * Fx;
@@ -366,12 +398,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
                     fz += a1 * uz + a2 * Fz;
                 }
             }
-#pragma unroll
-            for (int m = 0; m <= l; m++) {
-                Pm2[m] = Pm1[m];
-                Pm1[m] = Pc[m];
-            }
-            RELOAD_BARRIER();
+            rs.step(l);
         }
         pairF[3 * p] += fx;
         pairF[3 * p + 1] += fy;
@@ -395,18 +422,15 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     ensure_soap_constants();
     if (nc <= 0) return true;
     const int LM = (sp.L + 1) * (sp.L + 1);
-    const int RL = sp.n * (sp.L + 1) + LM;
     const int KLM = sp.K * LM;
-    const size_t fixed = (size_t)(sp.n * sp.n + sp.n + KLM) * 8;
-    auto need = [&](int nb) { return fixed + std::max((size_t)nb * RL, (size_t)sp.dim) * 8 + nb * 4; };
-    int NB = 64;
-    while (NB > 8 && need(NB) > 40 * 1024) NB /= 2;
-    const int rows_cap = (int)std::max((size_t)NB * RL, (size_t)sp.dim);
-    const size_t sh = need(NB);
+    // neighbour chunk of 32: covers a centre's neighbours within 5 A in one pass
+    // for covalent solids and keeps the block's LDS small
+    const int NB = kSoapChunk;
+    const size_t sh = (size_t)(sp.n * sp.n + sp.n + KLM + NB * (sp.S * sp.n + 2 * sp.L + 1) + NB * 2 * (sp.L + 2)) * 8;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
         hipLaunchKernelGGL((k_soap_fwd_x<NN, LL>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j,     \
-                           nb_r, nb_d, X, norm, cbuf, NB, rows_cap);                                            \
+                           nb_r, nb_d, X, norm, cbuf);                                                          \
         return true;                                                                                            \
     }
     SOAP_FAST(LAUNCH)
@@ -421,14 +445,17 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     ensure_soap_constants();
     if (nc <= 0) return true;
     const int LM = (sp.L + 1) * (sp.L + 1);
-    const size_t one = (size_t)(2 * sp.K * LM + sp.dim) * 8;
+    const int KLM = sp.K * LM, PR = 2 * (sp.L + 2);
+    const size_t one = (size_t)(2 * KLM + sp.dim) * 8;
     const int cpb = (sp.n * sp.n + sp.n) * 8 + 2 * one <= 48 * 1024 ? 2 : 1;   // two centres per wave when LDS allows
-    const size_t sh = (size_t)(sp.n * sp.n + sp.n) * 8 + cpb * one;
+    // Legendre rows of the 64/cpb lanes of a centre fit in its dead c and dE/dp regions?
+    const int lanes_per = 64 / cpb, in_c = KLM / PR;
+    const int pq_own = (lanes_per > in_c && (lanes_per - in_c) * PR > sp.dim) ? 1 : 0;
+    const size_t sh = (size_t)(sp.n * sp.n + sp.n) * 8 + cpb * one + (pq_own ? (size_t)64 * PR * 8 : 0);
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
         hipLaunchKernelGGL((k_soap_bwd_x<NN, LL>), dim3((nc + cpb - 1) / cpb), dim3(64), sh, s, sp, nc, cpb,     \
-                           centres, Z,                                                                          \
-                           nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF);                                  \
+                           centres, Z, nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, pq_own);              \
         return true;                                                                                            \
     }
     SOAP_FAST(LAUNCH)
