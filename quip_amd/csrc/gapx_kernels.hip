@@ -224,6 +224,12 @@ __device__ __forceinline__ int wave_excl_scan_i(int v, int *total) {
     return x - v;
 }
 
+// One wavefront per atom1.  Candidates are flattened: the offset cells are
+// taken in the reference's loop order (k2 slowest, i2 fastest), and in each
+// cell's descending list the atoms with index >= atom1 form a prefix (found by
+// binary search).  An exclusive wave scan of the prefix lengths numbers the
+// candidates in append order; each lane then tests one candidate, and a ballot
+// gives every accepted pair its slot in neighbour1(atom1).
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restrict__ pos,
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
@@ -235,99 +241,124 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
                                                     int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i,
                                                     int *n2_k) {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
-    const int atom1 = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ int s_pref[4][65];
+    __shared__ int s_start[4][64];
+    __shared__ int s_sh[4][64];   // i4, j4, k4 packed (each in [-128, 127]) and the same-cell flag
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int atom1 = blockIdx.x * 4 + wv;
     if (atom1 >= N) return;   // whole wavefront leaves together
     const int st = sid[atom1];
     const NLGeom &g = geo[st];
     const int cb = cell_base[st];
     const int lin = cell_of[atom1] - cb;   // cell index within this structure
     const int cc[3] = {lin % g.cells[0] + 1, (lin / g.cells[0]) % g.cells[1] + 1, lin / (g.cells[0] * g.cells[1]) + 1};
-    int lo[3], hi[3], span[3];
+    int lo[3], span[3];
     for (int a = 0; a < 3; a++) {
-        if (g.pbc[a]) { lo[a] = -g.img[a]; hi[a] = g.img[a]; }
+        int hi;
+        if (g.pbc[a]) { lo[a] = -g.img[a]; hi = g.img[a]; }
         else {
             lo[a] = (g.img[a] < cc[a]) ? -g.img[a] : -cc[a] + 1;
-            hi[a] = (g.img[a] < g.cells[a] - cc[a]) ? g.img[a] : g.cells[a] - cc[a];
+            hi = (g.img[a] < g.cells[a] - cc[a]) ? g.img[a] : g.cells[a] - cc[a];
         }
-        span[a] = hi[a] - lo[a] + 1;
+        span[a] = hi - lo[a] + 1;
     }
     const int noff = span[0] * span[1] * span[2];
     const double p1x = pos[3 * atom1], p1y = pos[3 * atom1 + 1], p1z = pos[3 * atom1 + 2];
     const int ms1x = map_shift[3 * atom1], ms1y = map_shift[3 * atom1 + 1], ms1z = map_shift[3 * atom1 + 2];
     const double rc = g.cutoff;
+    const unsigned long long below = (1ull << lane) - 1ull;
     int base = FILL ? n1_off[atom1] : 0;
     int total_all = 0;
     for (int o0 = 0; o0 < noff; o0 += 64) {
+        // this lane's offset cell: list start, shift, and the length of its atom >= atom1 prefix
         const int o = o0 + lane;
-        int cnt = 0;
-        int lin2 = -1, i4 = 0, j4 = 0, k4 = 0;
-        bool same_cell = false;
+        int cnt = 0, start = 0, packed = 0;
         if (o < noff) {
             const int i2 = lo[0] + o % span[0], j2 = lo[1] + (o / span[0]) % span[1], k2 = lo[2] + o / (span[0] * span[1]);
             int k3 = 1, j3 = 1, i3 = 1;
             if (g.cells[2] > 1) k3 = (cc[2] + k2 - 1 + g.cells[2]) % g.cells[2] + 1;
             if (g.cells[1] > 1) j3 = (cc[1] + j2 - 1 + g.cells[1]) % g.cells[1] + 1;
             if (g.cells[0] > 1) i3 = (cc[0] + i2 - 1 + g.cells[0]) % g.cells[0] + 1;
-            k4 = (cc[2] + k2 - k3) / g.cells[2];
-            j4 = (cc[1] + j2 - j3) / g.cells[1];
-            i4 = (cc[0] + i2 - i3) / g.cells[0];
+            const int k4 = (cc[2] + k2 - k3) / g.cells[2];
+            const int j4 = (cc[1] + j2 - j3) / g.cells[1];
+            const int i4 = (cc[0] + i2 - i3) / g.cells[0];
             if (k3 >= 1 && k3 <= g.cells[2] && j3 >= 1 && j3 <= g.cells[1] && i3 >= 1 && i3 <= g.cells[0]) {
-                lin2 = ((k3 - 1) * g.cells[1] + (j3 - 1)) * g.cells[0] + (i3 - 1);
-                same_cell = (lin2 == lin);
-                lin2 += cb;
+                const int lin2 = ((k3 - 1) * g.cells[1] + (j3 - 1)) * g.cells[0] + (i3 - 1);
+                const int same = (lin2 == lin) ? 1 : 0;
+                start = cell_start[lin2 + cb];
+                int lo_q = 0, hi_q = cell_cnt[lin2 + cb];
+                while (lo_q < hi_q) {   // first position holding an atom < atom1
+                    const int mid = (lo_q + hi_q) >> 1;
+                    if (cell_atoms[start + mid] >= atom1) lo_q = mid + 1;
+                    else hi_q = mid;
+                }
+                cnt = lo_q;
+                packed = ((i4 & 255) | ((j4 & 255) << 8) | ((k4 & 255) << 16)) | (same << 24);
             }
         }
-        // two sweeps over this lane's cell: count, then (FILL) write at the scanned offset
-        for (int pass = 0; pass < (FILL ? 2 : 1); pass++) {
-            int w = 0, out0 = 0;
-            if (pass == 1) {
-                int tot;
-                out0 = base + wave_excl_scan_i(cnt, &tot);
-                base += tot;
-            }
-            if (lin2 >= 0) {
-                const int st = cell_start[lin2], nc = cell_cnt[lin2];
-                for (int q = 0; q < nc; q++) {
-                    const int atom2 = cell_atoms[st + q];
-                    if (atom2 < atom1) break;
-                    if (atom2 == atom1 && i4 == 0 && j4 == 0 && k4 == 0 && same_cell) continue;
-                    const int sh0 = i4 - ms1x + map_shift[3 * atom2];
-                    const int sh1 = j4 - ms1y + map_shift[3 * atom2 + 1];
-                    const int sh2 = k4 - ms1z + map_shift[3 * atom2 + 2];
+        int tot;
+        const int pre = wave_excl_scan_i(cnt, &tot);
+        s_pref[wv][lane] = pre;
+        s_start[wv][lane] = start;
+        s_sh[wv][lane] = packed;
+        if (lane == 0) s_pref[wv][64] = tot;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int c0 = 0; c0 < tot; c0 += 64) {
+            const int c = c0 + lane;
+            bool acc = false;
+            int atom2 = 0, sh0 = 0, sh1 = 0, sh2 = 0;
+            double d = 0.0;
+            if (c < tot) {
+                // offset cell of candidate c: the last lane whose prefix start is <= c
+                int lo_o = 0, hi_o = 63;
+                while (lo_o < hi_o) {   // prefix starts are non-decreasing over the 64 lanes
+                    const int mid = (lo_o + hi_o + 1) >> 1;
+                    if (s_pref[wv][mid] <= c) lo_o = mid;
+                    else hi_o = mid - 1;
+                }
+                const int q = c - s_pref[wv][lo_o];
+                atom2 = cell_atoms[s_start[wv][lo_o] + q];
+                const int pk = s_sh[wv][lo_o];
+                const int i4 = (int)(signed char)(pk & 255), j4 = (int)(signed char)((pk >> 8) & 255),
+                          k4 = (int)(signed char)((pk >> 16) & 255);
+                const bool same_cell = (pk >> 24) & 1;
+                if (!(atom2 == atom1 && i4 == 0 && j4 == 0 && k4 == 0 && same_cell)) {
+                    sh0 = i4 - ms1x + map_shift[3 * atom2];
+                    sh1 = j4 - ms1y + map_shift[3 * atom2 + 1];
+                    sh2 = k4 - ms1z + map_shift[3 * atom2 + 2];
                     double d0 = pos[3 * atom2] - p1x, d1 = pos[3 * atom2 + 1] - p1y, d2 = pos[3 * atom2 + 2] - p1z;
                     d0 = d0 + g.lat[0] * (double)sh0; d1 = d1 + g.lat[1] * (double)sh0; d2 = d2 + g.lat[2] * (double)sh0;
                     d0 = d0 + g.lat[3] * (double)sh1; d1 = d1 + g.lat[4] * (double)sh1; d2 = d2 + g.lat[5] * (double)sh1;
                     d0 = d0 + g.lat[6] * (double)sh2; d1 = d1 + g.lat[7] * (double)sh2; d2 = d2 + g.lat[8] * (double)sh2;
-                    if (d0 > rc || d1 > rc || d2 > rc) continue;
-                    double dsq = d0 * d0;
-                    dsq = dsq + d1 * d1;
-                    dsq = dsq + d2 * d2;
-                    const double d = sqrt(dsq);
-                    if (!(d < rc)) continue;
-                    if (pass == 1) {
-                        const int e = out0 + w;
-                        n1_j[e] = atom2;
-                        n1_shift[3 * e] = sh0; n1_shift[3 * e + 1] = sh1; n1_shift[3 * e + 2] = sh2;
-                        n1_r[e] = d;
-                        if (atom2 != atom1) {      // mirror entry in neighbour2(atom2), ordered later
-                            const int slot = atomicAdd(&n2_fill[atom2], 1);
-                            n2_i[n2_off[atom2] + slot] = atom1;
-                            n2_k[n2_off[atom2] + slot] = e - n1_off[atom1];
-                        }
-                    } else if (!FILL && atom2 != atom1) {
-                        atomicAdd(&n2_cnt[atom2], 1);
+                    if (!(d0 > rc || d1 > rc || d2 > rc)) {
+                        double dsq = d0 * d0;
+                        dsq = dsq + d1 * d1;
+                        dsq = dsq + d2 * d2;
+                        d = sqrt(dsq);
+                        acc = d < rc;
                     }
-                    w++;
                 }
             }
-            if (pass == 0) cnt = w;
+            const unsigned long long mask = __ballot(acc);
+            if (FILL && acc) {
+                const int e = base + __popcll(mask & below);
+                n1_j[e] = atom2;
+                n1_shift[3 * e] = sh0; n1_shift[3 * e + 1] = sh1; n1_shift[3 * e + 2] = sh2;
+                n1_r[e] = d;
+                if (atom2 != atom1) {      // mirror entry in neighbour2(atom2), ordered later
+                    const int slot = atomicAdd(&n2_fill[atom2], 1);
+                    n2_i[n2_off[atom2] + slot] = atom1;
+                    n2_k[n2_off[atom2] + slot] = e - n1_off[atom1];
+                }
+            } else if (!FILL && acc && atom2 != atom1) {
+                atomicAdd(&n2_cnt[atom2], 1);
+            }
+            base += __popcll(mask);
+            total_all += __popcll(mask);
         }
-        if (!FILL) {
-            int tot;
-            wave_excl_scan_i(cnt, &tot);
-            total_all += tot;
-        }
+        __builtin_amdgcn_wave_barrier();   // this group's LDS rows are read before the next group writes them
     }
     if (!FILL && lane == 0) n1_cnt[atom1] = total_all;
 }
