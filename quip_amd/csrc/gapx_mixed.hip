@@ -40,8 +40,9 @@ template <>
 struct Half<2> {   // fp16
     typedef _Float16 T;
     typedef f16x8_t V;
-    static __device__ __forceinline__ T from(double x) { return (T)x; }
-    static __device__ __forceinline__ double to(T x) { return (double)x; }
+    // through float: the hardware converts float to half; the lo part keeps the residual
+    static __device__ __forceinline__ T from(double x) { return (T)(float)x; }
+    static __device__ __forceinline__ double to(T x) { return (double)(float)x; }
     static __device__ __forceinline__ float4_t mfma(V a, V b, float4_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
@@ -232,6 +233,7 @@ __global__ __launch_bounds__(256) void k_gemm2_split(int nc, int d, int Mpad, co
 // passes, FP32 accumulation), applies the FP64 epilogue (energy w K^zeta and
 // W = w zeta K^(zeta-1)), splits W into hi/lo through a per-wave LDS tile and
 // accumulates G += W Xs (16 x 32DK, in registers).  W never reaches HBM.
+// The next chunk is fetched into registers while the current one is used.
 // With several parts (gridDim.y) the sparse points are divided between blocks
 // and the FP32 partial G tiles are summed in a fixed order afterwards.
 // ---------------------------------------------------------------------------
@@ -280,30 +282,58 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     for (int q = 0; q < 2 * DK; q++) gacc[q] = (float4_t){0.f, 0.f, 0.f, 0.f};
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
 
-    for (int ch = c_beg; ch < c_end; ch++) {
-        const int s0 = 32 * ch;
-        __syncthreads();   // the previous chunk's LDS reads are done
-        for (int idx = t; idx < 32 * (DP / 8); idx += 512) {
+    // Staging of a chunk: thread t owns 16-byte vectors of the Xs rows and of the
+    // Xs^T rows (hi and lo each); the next chunk is fetched into registers while
+    // the current one is used.
+    constexpr int XS_N = 32 * (DP / 8), XT_N = DP * 4;
+    constexpr int XS_V = (XS_N + 511) / 512, XT_V = (XT_N + 511) / 512;
+    V ps_h[XS_V], ps_l[XS_V], pt_h[XT_V], pt_l[XT_V];
+    auto fetch = [&](int s0) {
+#pragma unroll
+        for (int u = 0; u < XS_V; u++) {
+            const int idx = t + 512 * u;
             const int r = idx / (DP / 8), c8 = (idx % (DP / 8)) * 8;
-            V h = {}, l = {};
-            if (c8 < ldS) {
-                h = *(const V *)(Shi + (size_t)(s0 + r) * ldS + c8);
-                l = *(const V *)(Slo + (size_t)(s0 + r) * ldS + c8);
+            ps_h[u] = (V){}; ps_l[u] = (V){};
+            if (idx < XS_N && c8 < ldS) {
+                ps_h[u] = *(const V *)(Shi + (size_t)(s0 + r) * ldS + c8);
+                ps_l[u] = *(const V *)(Slo + (size_t)(s0 + r) * ldS + c8);
             }
-            *(V *)(xs_l[0] + r * XS_LD + c8) = h;
-            *(V *)(xs_l[1] + r * XS_LD + c8) = l;
         }
-        for (int idx = t; idx < DP * 4; idx += 512) {
+#pragma unroll
+        for (int u = 0; u < XT_V; u++) {
+            const int idx = t + 512 * u;
             const int q = idx >> 2, c8 = (idx & 3) * 8;
-            V h = {}, l = {};
-            if (q < d) {
-                h = *(const V *)(Thi + (size_t)q * Mpad + s0 + c8);
-                l = *(const V *)(Tlo + (size_t)q * Mpad + s0 + c8);
+            pt_h[u] = (V){}; pt_l[u] = (V){};
+            if (idx < XT_N && q < d) {
+                pt_h[u] = *(const V *)(Thi + (size_t)q * Mpad + s0 + c8);
+                pt_l[u] = *(const V *)(Tlo + (size_t)q * Mpad + s0 + c8);
             }
-            *(V *)(xt_l[0] + q * XT_LD + c8) = h;
-            *(V *)(xt_l[1] + q * XT_LD + c8) = l;
+        }
+    };
+    if (c_beg < c_end) fetch(32 * c_beg);
+    for (int ch = c_beg; ch < c_end; ch++) {
+        __syncthreads();   // the previous chunk's LDS reads are done
+#pragma unroll
+        for (int u = 0; u < XS_V; u++) {
+            const int idx = t + 512 * u;
+            const int r = idx / (DP / 8), c8 = (idx % (DP / 8)) * 8;
+            if (idx < XS_N) {
+                *(V *)(xs_l[0] + r * XS_LD + c8) = ps_h[u];
+                *(V *)(xs_l[1] + r * XS_LD + c8) = ps_l[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < XT_V; u++) {
+            const int idx = t + 512 * u;
+            const int q = idx >> 2, c8 = (idx & 3) * 8;
+            if (idx < XT_N) {
+                *(V *)(xt_l[0] + q * XT_LD + c8) = pt_h[u];
+                *(V *)(xt_l[1] + q * XT_LD + c8) = pt_l[u];
+            }
         }
         __syncthreads();
+        const int s0 = 32 * ch;
+        if (ch + 1 < c_end) fetch(s0 + 32);
         // K tile: 16 centres x 32 sparse points
         float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
@@ -340,7 +370,10 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
                 w_l[wv][1][(rq + v) * XT_LD + 16 * j + r16] = H::from(wk - H::to(h));
             }
         }
-        __syncthreads();   // W tile visible in A-operand layout
+        // the W tile is this wave's own: a wavefront barrier makes it visible in A-operand layout
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const V wh = *(const V *)(w_l[wv][0] + r16 * XT_LD + kg), wl = *(const V *)(w_l[wv][1] + r16 * XT_LD + kg);
 #pragma unroll
         for (int q = 0; q < 2 * DK; q++) {
