@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_gemm2_split(int nc, int d, int Mpad, co
 // With several parts (gridDim.y) the sparse points are divided between blocks
 // and the FP32 partial G tiles are summed in a fixed order afterwards.
 // ---------------------------------------------------------------------------
-template <int PREC, int DK>
+template <int PREC, int DK, int ZI>
 __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, int Mpad, int ldS, int chunks_per_part,
                                                         const double *__restrict__ X,
                                                         const typename Half<PREC>::T *__restrict__ Shi,
@@ -346,28 +346,41 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
                 kacc[j] = H::mfma(xh[ks], bl, kacc[j]);
                 kacc[j] = H::mfma(xh[ks], bh, kacc[j]);
             }
-        // FP64 epilogue; accumulator element v of lane l is centre rq + v, sparse point 16 j + r16
+        // epilogue; accumulator element v of lane l is centre rq + v, sparse point 16 j + r16.
+        // The energy sum is FP64; K^zeta and W are formed from the FP32 K in FP32 for the
+        // common zeta = 4 (W is then split into 16-bit halves), in FP64 otherwise.
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int sp = s0 + 16 * j + r16;
             const double wsv = sp < M ? wvec[sp] : 0.0;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const double kv = (double)kacc[j][v];
-                double kz1, kz;
-                if (izeta > 0) {
-                    kz1 = 1.0;
-                    for (int q = 1; q < izeta; q++) kz1 *= kv;
-                    kz = kz1 * kv;
+                T h, lo;
+                if constexpr (ZI == 4) {
+                    const float kf = kacc[j][v];
+                    const float k2 = kf * kf;
+                    rs[v] += wsv * (double)(k2 * k2);
+                    const float wk = (float)(wsv * 4.0 * wscale) * (k2 * kf);
+                    h = (T)wk;
+                    lo = (T)(wk - (float)h);
                 } else {
-                    kz1 = pow(kv, zeta - 1.0);
-                    kz = pow(kv, zeta);
+                    const double kv = (double)kacc[j][v];
+                    double kz1, kz;
+                    if (izeta > 0) {
+                        kz1 = 1.0;
+                        for (int q = 1; q < izeta; q++) kz1 *= kv;
+                        kz = kz1 * kv;
+                    } else {
+                        kz1 = pow(kv, zeta - 1.0);
+                        kz = pow(kv, zeta);
+                    }
+                    rs[v] += wsv * kz;
+                    const double wk = wsv * zeta * kz1 * wscale;
+                    h = H::from(wk);
+                    lo = H::from(wk - H::to(h));
                 }
-                rs[v] += wsv * kz;
-                const double wk = wsv * zeta * kz1 * wscale;
-                const T h = H::from(wk);
                 w_l[wv][0][(rq + v) * XT_LD + 16 * j + r16] = h;
-                w_l[wv][1][(rq + v) * XT_LD + 16 * j + r16] = H::from(wk - H::to(h));
+                w_l[wv][1][(rq + v) * XT_LD + 16 * j + r16] = lo;
             }
         }
         // the W tile is this wave's own: a wavefront barrier makes it visible in A-operand layout
@@ -465,9 +478,15 @@ static void gp_fused_go(int nc, int M, int d, int dpad, int Mpad, int parts, con
     typedef typename Half<PREC>::T T;
     const int per = cdivm(Mpad / 32, parts);
     dim3 grid(cdivm(nc, 128), parts);
-    hipLaunchKernelGGL((k_gp_fused_split<PREC, DK>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per, X,
-                       (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta, int_zeta_m(zeta),
-                       wscale, 1.0 / wscale, G, Gpart, e_part);
+    const int iz = int_zeta_m(zeta);
+    if (iz == 4)
+        hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 4>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per, X,
+                           (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta, iz, wscale,
+                           1.0 / wscale, G, Gpart, e_part);
+    else
+        hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 0>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per, X,
+                           (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta, iz, wscale,
+                           1.0 / wscale, G, Gpart, e_part);
 }
 
 void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, const double *X, const void *Xs_hi,
