@@ -249,6 +249,9 @@ def main():
     ap.add_argument("--batch", default="fp64,bf16x3",
                     help="precision forms of the 512-structure batching form ('' to skip)")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--sic", default="fp64,bf16x3,fp16x3",
+                    help="precision forms timed on the 32768-atom SiC structure ('' to skip)")
+    ap.add_argument("--sic-steps", type=int, default=5)
     ap.add_argument("--serve", default="bf16x3,fp64",
                     help="precision forms of the serving form ('' to skip); the first gets the three arrival rates")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of each Poisson stream")
@@ -320,11 +323,36 @@ def main():
             forms["batch512_" + prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - rb["force"]) ** 2)))
     serving = None
     serve_list = [f for f in args.serve.split(",") if f]
-    if serve_list:
-        log("building the two-species serving model from GPU descriptors")
+    sic_list = [f for f in args.sic.split(",") if f]
+    if serve_list or sic_list:
+        log("building the two-species SiC model from GPU descriptors")
         smodel = configs.model_sic(gpu_descriptor_fn(tmp))
         spath = os.path.join(tmp, "sic.xml")
         write_gap_xml(spath, smodel)
+    if sic_list:
+        if "fp64" not in sic_list:
+            sic_list.insert(0, "fp64")
+        sat = configs.sic32768()
+        NS = len(sat)
+        sb = dict(pos=dev.alloc(NS * 24), e=dev.alloc(8), f=dev.alloc(NS * 24), v=dev.alloc(72))
+        dev.h2d(sb["pos"], L.f64(sat.positions))
+        sres = {}
+        for prec in sic_list:
+            pot = Potential(param_filename=spath, precision=prec)
+            log("timing sic32768", prec)
+            dt, res, stages = run_form(pot, sat, dev, sb, args.sic_steps, 1)
+            sres[prec] = res
+            forms["sic32768_" + prec] = dict(atoms_per_s=NS * args.sic_steps / dt,
+                                             ms_per_step=1e3 * dt / args.sic_steps,
+                                             stage_ms={k: round(v, 4) for k, v in stages.items()})
+            del pot
+        for prec in sic_list[1:]:
+            r, r0 = sres[prec], sres["fp64"]
+            f = forms["sic32768_" + prec]
+            f["dE_per_atom_vs_fp64"] = abs(r["energy"] - r0["energy"]) / NS
+            f["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - r0["force"]) ** 2)))
+            f["speedup_vs_fp64"] = f["atoms_per_s"] / forms["sic32768_fp64"]["atoms_per_s"]
+    if serve_list:
         serving = {"model": "Si/C two coordinates (centre Si, centre C), SOAP n_max 12 l_max 8 cutoff 5.0 (d=%d), "
                             "2 x 2000 sparse points, zeta 4" % smodel.coords[0].descriptor.dim,
                    "requests": "Si/C zincblende, N log-uniform on [16, 4096]; bulk / (111) slab / vacancies",

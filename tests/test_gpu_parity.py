@@ -223,6 +223,45 @@ def test_batch512_full_size(models):
     assert de < 1e-3 and fr < 1e-2
 
 
+@pytest.fixture(scope="module")
+def sic_model(tmp_path_factory):
+    """The two-species precision-form model: SOAP n_max 12, l_max 8 (d = 2701), 2 x 2000 sparse points."""
+    m = configs.model_sic(_desc_fn)
+    path = str(tmp_path_factory.mktemp("sic") / "sic.xml")
+    write_gap_xml(path, m)
+    return m, path
+
+
+def test_sic_model_matches_oracle(sic_model):
+    """Full two-species model on a 512-atom SiC cell with antisite swaps, FP64 against the oracle."""
+    m, path = sic_model
+    rng = configs.rng_for(2, 31)
+    at = configs.antisite_swaps(configs.perturb(configs.diamond(4.36, (4, 4, 4), species=(14, 6)), 0.1, rng), 6, rng)
+    r = Potential(param_filename=path).calc(at.copy(), force=True, virial=True, local_energy=True, local_virial=True)
+    _compare(r, O.gap_calc(m, at), len(at))
+
+
+def test_sic32768_precision_forms(sic_model):
+    """The 32768-atom SiC precision form: split forms against FP64 within 1 meV/atom and
+    10 meV/A force RMSE; size-independent invariants of the FP64 result."""
+    m, path = sic_model
+    at = configs.sic32768()
+    r64 = Potential(param_filename=path).calc(at.copy(), force=True, virial=True)
+    assert np.abs(r64["force"].sum(axis=0)).max() < 1e-8 * len(at)
+    assert np.abs(r64["virial"] - r64["virial"].T).max() < 1e-7
+    shifted = at.copy()
+    shifted.positions = shifted.positions + at.cell[0]          # a whole lattice vector: same structure
+    rs = Potential(param_filename=path).calc(shifted, force=True)
+    assert abs(rs["energy"] - r64["energy"]) / len(at) < 1e-12
+    assert np.abs(rs["force"] - r64["force"]).max() < 1e-10
+    for prec in ("bf16x3", "fp16x3"):
+        r = Potential(param_filename=path, precision=prec).calc(at.copy(), force=True)
+        de = abs(r["energy"] - r64["energy"]) / len(at)
+        fr = float(np.sqrt(np.mean((r["force"] - r64["force"]) ** 2)))
+        print("sic32768", prec, "dE/atom", de, "force rmse", fr)
+        assert de < 1e-3 and fr < 1e-2
+
+
 def test_finite_differences_on_gpu(models):
     m, path = models["si64"]
     pot = Potential(param_filename=path)
