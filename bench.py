@@ -79,9 +79,11 @@ class DeviceArrays:
         self.ptrs = []
 
 
-def run_form(pot, at, dev, bufs, steps, warmup):
+def run_form(pot, at, dev, bufs, steps, warmup, graphs=False):
     """FP64/split plain form on one structure; stage times from HIP events recorded
-    during the timed steps themselves (no extra synchronisation)."""
+    during the timed steps themselves (no extra synchronisation).  With graphs the
+    launch sequence after the neighbour count is replayed as a HIP graph and no
+    stage events are recorded."""
     lib = L.lib()
     N = len(at)
     Z, lat, pbc = L.i32(at.numbers), L.f64(at.cell), L.i32(at.pbc)
@@ -91,10 +93,11 @@ def run_form(pot, at, dev, bufs, steps, warmup):
         L.check(lib.gapx_calc_device(pot.handle, N, bufs["pos"], L.ptr(Z), L.ptr(lat), L.ptr(pbc), 0.0, want,
                                      bufs["e"], None, bufs["f"], bufs["v"], None), "gapx_calc_device")
 
+    pot.set_graphs(graphs)
     for _ in range(warmup):
         step()
     L.check(lib.gapx_sync(pot.handle), "sync")
-    pot.set_timing(True)
+    pot.set_timing(not graphs)   # stage events are recorded in eager runs only
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
@@ -110,7 +113,7 @@ def run_form(pot, at, dev, bufs, steps, warmup):
     return dt, dict(energy=e[0], force=f, virial=v.reshape(3, 3).T), stages
 
 
-def run_batch(pot, frames, dev, steps, warmup):
+def run_batch(pot, frames, dev, steps, warmup, graphs=False):
     """Batching form: every structure of `frames` in one pass (gapx_calc_batch_device)."""
     lib = L.lib()
     S = len(frames)
@@ -129,10 +132,11 @@ def run_batch(pot, frames, dev, steps, warmup):
         L.check(lib.gapx_calc_batch_device(pot.handle, S, L.ptr(off), b["pos"], L.ptr(Z), L.ptr(lats), L.ptr(pbcs),
                                            0.0, want, b["e"], None, b["f"], b["v"], None), "gapx_calc_batch_device")
 
+    pot.set_graphs(graphs)
     for _ in range(warmup):
         step()
     L.check(lib.gapx_sync(pot.handle), "sync")
-    pot.set_timing(True)
+    pot.set_timing(not graphs)
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
@@ -293,6 +297,13 @@ def main():
                            stage_ms={k: round(v, 5) for k, v in stages.items()})
         results[prec] = res
         del pot
+    for prec in [f for f in form_list if f in ("fp64", "bf16x3")]:
+        pot = Potential(param_filename=path, precision=prec)
+        log("timing", prec, "with graph replay")
+        dt, res, stages = run_form(pot, at, dev, bufs, args.steps, args.warmup, graphs=True)
+        forms[prec + "_graph"] = dict(atoms_per_s=N * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
+                                      max_dF_vs_eager=float(np.abs(res["force"] - results[prec]["force"]).max()))
+        del pot
     ref = results["fp64"]
     for prec in [f for f in form_list if f != "fp64"]:
         r = results[prec]
@@ -314,6 +325,15 @@ def main():
                 atoms_per_s=NB * args.batch_steps / dt, ms_per_step=1e3 * dt / args.batch_steps,
                 structures=len(frames), atoms=NB, stage_ms={k: round(v, 4) for k, v in stages.items()},
                 speedup_vs_fp64_plain=NB * args.batch_steps / dt / forms["fp64"]["atoms_per_s"])
+            del pot
+        if "bf16x3" in batch_list:
+            pot = Potential(param_filename=path, precision="bf16x3")
+            log("timing batch bf16x3 with graph replay")
+            dt, NB, r, stages = run_batch(pot, frames, dev, args.batch_steps, 2, graphs=True)
+            forms["batch512_bf16x3_graph"] = dict(
+                atoms_per_s=NB * args.batch_steps / dt, ms_per_step=1e3 * dt / args.batch_steps,
+                speedup_vs_fp64_plain=NB * args.batch_steps / dt / forms["fp64"]["atoms_per_s"],
+                max_dF_vs_eager=float(np.abs(r["force"] - bres["bf16x3"]["force"]).max()))
             del pot
         rb = bres["fp64"]
         for prec in batch_list[1:]:
@@ -379,7 +399,7 @@ def main():
         forms["fp64"]["max_dF_vs_oracle"] = float(np.abs(ref["force"] - ro["forces"]).max())
     dev.free()
     fp = forms["fp64"]
-    mixed = [k for k in forms if k.startswith("batch512_") and not k.endswith("fp64")]
+    mixed = [k for k in forms if k.startswith("batch512_") and "fp64" not in k]
     line = {
         "metric": "SOAP-GAP atoms/s (E+F+virial), mixed vs FP64 GPU; p50/p99 served latency",
         "value": fp["atoms_per_s"], "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -162,6 +162,12 @@ int gapx_descriptors(gapx_model *model, int coord, int n_atoms, const double *po
  * gapx_model_tuning lists the choices (up to max_entries; n_entries gets the
  * count) with the measured milliseconds of one launch. */
 int gapx_model_set_autotune(gapx_model *model, int on);
+
+/* Replay of the launch sequence after the neighbour count as a HIP graph (off by
+ * default).  The sequence is captured on the first call and re-captured when a
+ * size, a buffer, an output pointer or E_scale changes; calls with stage timing
+ * on run it eagerly.  No counterpart in the reference. */
+int gapx_model_set_graphs(gapx_model *model, int on);
 int gapx_model_tuning(gapx_model *model, int coord, int max_entries, int *bucket, int *gemm1_tile, int *gemm2_tile,
                       int *gemm2_split, double *gemm1_ms, double *gemm2_ms, int *n_entries);
 

@@ -447,7 +447,10 @@ extern "C" int gapx_model_load(const char *xml_text, int64_t xml_len, const char
     auto m = new gapx_model();
     m->device = device;
     int st = model_load(xml_text, xml_len, label, xml_dir, device, m);
-    if (st == GAPX_OK) st = new_engine(m, m->eng);
+    if (st == GAPX_OK) {
+        soap_constants_init();
+        st = new_engine(m, m->eng);
+    }
     if (st != GAPX_OK) {
         gapx_model_free(m);
         return st;
@@ -488,6 +491,11 @@ extern "C" int gapx_model_info(const gapx_model *m, int *n_coord, int coord, int
         if (dim) *dim = m->coords[coord]->dim;
         if (n_sparse) *n_sparse = m->coords[coord]->M;
     }
+    return GAPX_OK;
+}
+extern "C" int gapx_model_set_graphs(gapx_model *m, int on) {
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    m->graphs = on != 0;
     return GAPX_OK;
 }
 extern "C" int gapx_model_set_autotune(gapx_model *m, int on) {
@@ -945,95 +953,155 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     int P = 0;
     CHK(build_nl(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, &P));
     if (tm.on) tm.stage(0, t_nl0, tm.mark(s));
+
+    // Prepare: every buffer the launches use is sized here (to the largest
+    // coordinate), tiles are chosen and split copies made, so that the launch
+    // sequence below allocates nothing and can be captured as a graph.
+    const size_t NCOORD = m->coords.size();
+    std::vector<GemmTune> tunes(NCOORD);
+    std::vector<char> fused(NCOORD, 0);
+    size_t mX = 8, mN = 8, mC = 8, mW = 8, mE = 8, mGp = 8, mWf = 8, mSw = 8;
+    for (size_t k = 0; k < NCOORD; k++) {
+        const Coord &c = *m->coords[k];
+        const int nc = (int)eg.h_centres[k].size();
+        if (c.kind != 1 || nc == 0) continue;
+        const size_t KLM = (size_t)c.S * c.n * (c.L + 1) * (c.L + 1);
+        const int dpad = (c.dim + 31) / 32 * 32, Mpad = (c.M + 31) / 32 * 32;
+        fused[k] = m->precision != GAPX_PREC_FP64 && gp_fused_supported(dpad);
+        mX = std::max(mX, (size_t)nc * c.dim * 8);
+        mN = std::max(mN, (size_t)nc * 8);
+        mC = std::max(mC, (size_t)nc * KLM * 8);
+        mE = std::max(mE, (size_t)nc * ((c.M + 31) / 32 + 1) * 8);
+        if (m->precision == GAPX_PREC_FP64) {
+            mW = std::max(mW, (size_t)nc * c.M * 8);
+            mGp = std::max(mGp, (size_t)8 * nc * c.dim * 8);
+        } else if (fused[k]) {
+            mGp = std::max(mGp, (size_t)gp_fused_parts(nc, Mpad) * nc * c.dim * 4);
+        } else {
+            mWf = std::max(mWf, (size_t)nc * Mpad * 4);
+            mSw = std::max(mSw, (size_t)(nc + 63) / 64 * 64 * dpad * 4);
+        }
+    }
+    CHK(w.X.ensure(mX)); CHK(w.G.ensure(mX)); CHK(w.norm.ensure(mN)); CHK(w.cbuf.ensure(mC));
+    CHK(w.e_part.ensure(mE)); CHK(w.W.ensure(mW)); CHK(w.Gparts.ensure(mGp + 16));
+    CHK(w.Wf.ensure(mWf + 64)); CHK(w.split_ws.ensure(mSw + 64));
     CHK(w.pairF.ensure((size_t)(P + 1) * 24));
     CHK(w.pairE.ensure((size_t)(P + 1) * 8));
     CHK(w.esite.ensure((size_t)(N + 1) * 8));
     CHK(ensure_outputs(w, S, N));
-    launch_fill_f64(w.pairF.as<double>(), (size_t)P * 3, 0.0, s);
-    launch_fill_f64(w.pairE.as<double>(), (size_t)P, 0.0, s);
-    launch_fill_f64(w.esite.as<double>(), (size_t)N, 0.0, s);
-    for (size_t k = 0; k < m->coords.size(); k++) {
+    for (size_t k = 0; k < NCOORD; k++) {
         Coord &c = *m->coords[k];
-        if (c.kind == 2) {
-            const int ta = tm.on ? tm.mark(s) : -1;
-            launch_d2b(N, w.Z.as<int>(), c.Z1, c.Z2, c.rc, c.ctw, c.M, c.d_Xs.as<double>(), c.d_w.as<double>(),
-                       c.d_theta.as<double>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
-                       w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), s);
-            if (tm.on) tm.stage(5, ta, tm.mark(s));
-            continue;
-        }
         const int nc = (int)eg.h_centres[k].size();
-        if (nc == 0) continue;
-        const int *centres = w.centres[k].as<int>();
-        const int KLM = c.S * c.n * (c.L + 1) * (c.L + 1);
-        CHK(w.X.ensure((size_t)nc * c.dim * 8));
-        CHK(w.norm.ensure((size_t)nc * 8));
-        CHK(w.cbuf.ensure((size_t)nc * KLM * 8));
-        CHK(w.W.ensure((size_t)nc * c.M * 8));
-        CHK(w.G.ensure((size_t)nc * c.dim * 8));
-        CHK(w.e_part.ensure((size_t)nc * ((c.M + 31) / 32 + 1) * 8));
-        CHK(w.Gparts.ensure((size_t)8 * nc * c.dim * 8 + 8));
-        SoapDev sp = soap_dev(c);
-        const int t0 = tm.on ? tm.mark(s) : -1;
-        launch_soap_fwd(sp, nc, centres, w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
-                        w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(), w.cbuf.as<double>(), s);
-        GemmTune tu;
-        if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tu));
-        // split forms: one fused K -> W -> G kernel when the descriptor fits its register tile
-        const bool fused = m->precision != GAPX_PREC_FP64 && gp_fused_supported((c.dim + 31) / 32 * 32);
-        const int t1 = tm.on ? tm.mark(s) : -1;
-        int ntile = 0;
-        if (m->precision == GAPX_PREC_FP64) {
-            launch_gemm1_f64(tu.g1, nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(), c.d_w.as<double>(),
-                             c.zeta, w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
-        } else {
-            CHK(ensure_split(m, c, m->precision, s));
-            if (fused) {
-                CHK(w.Gparts.ensure((size_t)gp_fused_parts(nc, c.Mpad) * nc * c.dim * 4 + 16));
+        if (c.kind != 1 || nc == 0) continue;
+        if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tunes[k]));
+        else CHK(ensure_split(m, c, m->precision, s));
+    }
+    double *o_e = (want & GAPX_WANT_E) && d_e ? d_e : w.e.as<double>();
+    double *o_v = (want & GAPX_WANT_V) && d_v ? d_v : w.vir.as<double>();
+    double *o_f = (want & GAPX_WANT_F) && d_f ? d_f : w.f.as<double>();
+    double *o_le = (want & GAPX_WANT_LOCAL_E) && d_le ? d_le : w.le.as<double>();
+    double *o_lv = (want & GAPX_WANT_LOCAL_V) && d_lv ? d_lv : w.lv.as<double>();
+
+    // The launch sequence (stage events only when timing, never inside a graph).
+    auto launch_all = [&](bool marks) {
+        launch_fill_f64(w.pairF.as<double>(), (size_t)P * 3, 0.0, s);
+        launch_fill_f64(w.pairE.as<double>(), (size_t)P, 0.0, s);
+        launch_fill_f64(w.esite.as<double>(), (size_t)N, 0.0, s);
+        for (size_t k = 0; k < NCOORD; k++) {
+            Coord &c = *m->coords[k];
+            if (c.kind == 2) {
+                const int ta = marks ? tm.mark(s) : -1;
+                launch_d2b(N, w.Z.as<int>(), c.Z1, c.Z2, c.rc, c.ctw, c.M, c.d_Xs.as<double>(), c.d_w.as<double>(),
+                           c.d_theta.as<double>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
+                           w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), s);
+                if (marks) tm.stage(5, ta, tm.mark(s));
+                continue;
+            }
+            const int nc = (int)eg.h_centres[k].size();
+            if (nc == 0) continue;
+            const int *centres = w.centres[k].as<int>();
+            const GemmTune &tu = tunes[k];
+            SoapDev sp = soap_dev(c);
+            const int t0 = marks ? tm.mark(s) : -1;
+            launch_soap_fwd(sp, nc, centres, w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
+                            w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(), w.cbuf.as<double>(), s);
+            const int t1 = marks ? tm.mark(s) : -1;
+            int ntile = 0;
+            if (m->precision == GAPX_PREC_FP64) {
+                launch_gemm1_f64(tu.g1, nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(), c.d_w.as<double>(),
+                                 c.zeta, w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
+            } else if (fused[k]) {   // split forms: K -> W -> G in one kernel
                 launch_gp_fused_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.X.as<double>(), c.d_Xs_hi.p,
                                       c.d_Xs_lo.p, c.d_XsT_hi.p, c.d_XsT_lo.p, c.d_w.as<double>(), c.zeta, c.wscale,
                                       w.G.as<double>(), w.Gparts.as<float>(), w.e_part.as<double>(), &ntile, s);
             } else {
-                CHK(w.Wf.ensure((size_t)nc * c.Mpad * 4 + 64));
-                CHK(w.split_ws.ensure((size_t)(nc + 63) / 64 * 64 * c.dpad * 4 + 64));
                 launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, w.X.as<double>(), c.d_Xs_hi.p, c.d_Xs_lo.p,
                                    c.d_w.as<double>(), c.zeta, c.wscale, w.Wf.as<float>(), w.e_part.as<double>(),
                                    &ntile, w.split_ws.p, s);
             }
-        }
-        const int t2 = tm.on ? tm.mark(s) : -1;
-        launch_energy_rows(nc, ntile, centres, w.e_part.as<double>(), w.esite.as<double>(), s);
-        if (tm.on) { tm.stage(1, t0, t1); tm.stage(2, t1, t2); }
-        if (grad) {
-            if (m->precision == GAPX_PREC_FP64) {
-                launch_gemm2_f64(tu.g2, tu.parts, nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(),
-                                 w.G.as<double>(), w.Gparts.as<double>(), s);
-            } else if (fused) {
-                launch_gp_fused_reduce(nc, c.dim, ntile, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
-            } else {
-                launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.Wf.as<float>(), c.d_XsT_hi.p,
-                                   c.d_XsT_lo.p, c.wscale, w.G.as<double>(), s);
+            const int t2 = marks ? tm.mark(s) : -1;
+            launch_energy_rows(nc, ntile, centres, w.e_part.as<double>(), w.esite.as<double>(), s);
+            if (marks) { tm.stage(1, t0, t1); tm.stage(2, t1, t2); }
+            if (grad) {
+                if (m->precision == GAPX_PREC_FP64) {
+                    launch_gemm2_f64(tu.g2, tu.parts, nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(),
+                                     w.G.as<double>(), w.Gparts.as<double>(), s);
+                } else if (fused[k]) {
+                    launch_gp_fused_reduce(nc, c.dim, ntile, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
+                } else {
+                    launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.Wf.as<float>(), c.d_XsT_hi.p,
+                                       c.d_XsT_lo.p, c.wscale, w.G.as<double>(), s);
+                }
+                const int t3 = marks ? tm.mark(s) : -1;
+                launch_soap_bwd(sp, nc, centres, w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
+                                w.nb_r.as<double>(), w.nb_d.as<double>(), w.X.as<double>(), w.G.as<double>(),
+                                w.norm.as<double>(), w.cbuf.as<double>(), w.pairF.as<double>(), s);
+                if (marks) { tm.stage(3, t2, t3); tm.stage(4, t3, tm.mark(s)); }
             }
-            const int t3 = tm.on ? tm.mark(s) : -1;
-            launch_soap_bwd(sp, nc, centres, w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
-                            w.nb_d.as<double>(), w.X.as<double>(), w.G.as<double>(), w.norm.as<double>(),
-                            w.cbuf.as<double>(), w.pairF.as<double>(), s);
-            if (tm.on) { tm.stage(3, t2, t3); tm.stage(4, t3, tm.mark(s)); }
         }
+        const int tg = marks ? tm.mark(s) : -1;
+        launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
+                      w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>(), o_f, o_le,
+                      o_lv, s);
+        launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s);
+        if (marks) tm.stage(5, tg, tm.mark(s));
+    };
+
+    if (m->graphs && !tm.on) {
+        // replay the captured sequence when nothing it depends on has changed
+        std::vector<long long> key = {S, N, P, m->precision, (long long)want, (long long)(uintptr_t)o_e,
+                                      (long long)(uintptr_t)o_v, (long long)(uintptr_t)o_f, (long long)(uintptr_t)o_le,
+                                      (long long)(uintptr_t)o_lv, alloc_generation()};
+        long long es;
+        memcpy(&es, &m->E_scale, 8);
+        key.push_back(es);
+        for (size_t k = 0; k < NCOORD; k++)
+            key.insert(key.end(), {(long long)eg.h_centres[k].size(), tunes[k].g1, tunes[k].g2, tunes[k].parts});
+        if (!eg.gexec || key != eg.gkey) {
+            if (eg.gexec) {
+                hipGraphExecDestroy(eg.gexec);
+                eg.gexec = nullptr;
+            }
+            eg.gkey.clear();
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            launch_all(false);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(s, &g);
+            if (ce != hipSuccess || !g) return fail(GAPX_E_DEVICE, "graph capture: %s", hipGetErrorString(ce));
+            const hipError_t ie = hipGraphInstantiate(&eg.gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                eg.gexec = nullptr;
+                return fail(GAPX_E_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+            }
+            eg.gkey = key;
+        }
+        HIPCHK(hipGraphLaunch(eg.gexec, s));
+    } else {
+        launch_all(tm.on);
     }
-    const int tg = tm.on ? tm.mark(s) : -1;
-    double *o_le = (want & GAPX_WANT_LOCAL_E) && d_le ? d_le : w.le.as<double>();
-    double *o_lv = (want & GAPX_WANT_LOCAL_V) && d_lv ? d_lv : w.lv.as<double>();
-    launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
-                  w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>(),
-                  (want & GAPX_WANT_F) && d_f ? d_f : w.f.as<double>(), o_le, o_lv, s);
-    launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, (want & GAPX_WANT_E) && d_e ? d_e : w.e.as<double>(),
-                         (want & GAPX_WANT_V) && d_v ? d_v : w.vir.as<double>(), s);
     HIPCHK(hipGetLastError());
-    if (tm.on) {
-        tm.stage(5, tg, tm.mark(s));
-        tm.end();
-    }
+    if (tm.on) tm.end();
     return GAPX_OK;
 }
 

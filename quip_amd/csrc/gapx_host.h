@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -43,6 +44,13 @@ const char *last_error_text();
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
+// Incremented whenever a device buffer is (re)allocated: captured launch
+// sequences hold buffer addresses and are replayed only while it is unchanged.
+inline long long alloc_generation(bool bump = false) {
+    static std::atomic<long long> gen{0};
+    return bump ? ++gen : gen.load();
+}
+
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
@@ -55,6 +63,7 @@ struct DBuf {
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) return fail(GAPX_E_DEVICE, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
         cap = want;
+        alloc_generation(true);
         return GAPX_OK;
     }
     template <class T> T *as() const { return (T *)p; }
@@ -162,16 +171,19 @@ struct StageTimer {
         marks[cur].push_back(k); marks[cur].push_back(ea); marks[cur].push_back(eb);
     }
     void end() { pending[cur] = true; seq[cur] = next_seq++; }
+    static void add(std::vector<hipEvent_t> &pl, const std::vector<int> &mk, double *t6) {
+        for (size_t q = 0; q + 2 < mk.size(); q += 3) {
+            float t = 0;
+            if (hipEventSynchronize(pl[mk[q + 2]]) != hipSuccess) continue;
+            if (hipEventElapsedTime(&t, pl[mk[q + 1]], pl[mk[q + 2]]) != hipSuccess) continue;
+            t6[mk[q]] += t;
+        }
+    }
     void resolve(int set) {
         if (!pending[set]) return;
         pending[set] = false;
         double t6[kStages] = {0};
-        for (size_t q = 0; q + 2 < marks[set].size(); q += 3) {
-            float t = 0;
-            if (hipEventSynchronize(pool[set][marks[set][q + 2]]) != hipSuccess) continue;
-            hipEventElapsedTime(&t, pool[set][marks[set][q + 1]], pool[set][marks[set][q + 2]]);
-            t6[marks[set][q]] += t;
-        }
+        add(pool[set], marks[set], t6);
         for (int k = 0; k < kStages; k++) { last[k] = t6[k]; total[k] += t6[k]; }
         ncalls++;
     }
@@ -206,9 +218,12 @@ struct Engine {
     std::vector<NLGeom> h_geo;
     std::vector<std::vector<int>> h_centres;   // per coordinate
     bool have_inputs = false, have_geo = false;
+    hipGraphExec_t gexec = nullptr;   // the captured launch sequence after the neighbour count
+    std::vector<long long> gkey;
     int n1_total_last = 0;
     int ncell_last = 0;
     ~Engine() {
+        if (gexec) hipGraphExecDestroy(gexec);
         if (stream) {
             hipStreamSynchronize(stream);
             hipStreamDestroy(stream);
@@ -226,6 +241,7 @@ struct ModelState {
     std::vector<std::unique_ptr<Coord>> coords;
     DBuf d_e0;
     bool autotune = true;
+    bool graphs = false;
     std::unique_ptr<Engine> eng;   // the engine of the plain entry points
 };
 

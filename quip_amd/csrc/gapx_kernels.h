@@ -52,6 +52,10 @@ void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid,
                     hipStream_t s);
 void launch_minmax_frac(int S, const int *atom_off, const double *pos, const double *g, double *out, hipStream_t s);
 
+// upload the constant tables of the SOAP kernels to the current device (once per
+// device; done at model load so that no launch sequence contains an upload)
+void soap_constants_init();
+void soap_fast_constants_init();
 int soap_supported(int n, int L);
 int soap_fast_supported(int n, int L);
 bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,

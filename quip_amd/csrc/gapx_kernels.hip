@@ -26,9 +26,11 @@ namespace gapx {
 __constant__ double c_ynorm[(kMaxL + 1) * (kMaxL + 2) / 2];   // N_lm (x sqrt 2 for m>0)
 __constant__ double c_dfact[kMaxL + 2];                       // (2m-1)!!
 
-static bool g_const_ready = false;
+// constant tables are uploaded once per device (each device has its own copy)
+static bool g_const_ready[64] = {};
 static void ensure_constants() {
-    if (g_const_ready) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_const_ready[dev]) return;
     double yn[(kMaxL + 1) * (kMaxL + 2) / 2], df[kMaxL + 2];
     for (int l = 0; l <= kMaxL; l++)
         for (int m = 0; m <= l; m++) {
@@ -42,7 +44,12 @@ static void ensure_constants() {
     for (int m = 1; m < kMaxL + 2; m++) df[m] = df[m - 1] * (2.0 * m - 1.0);
     hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn));
     hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df));
-    g_const_ready = true;
+    g_const_ready[dev] = true;
+}
+
+void soap_constants_init() {
+    ensure_constants();
+    soap_fast_constants_init();
 }
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }

@@ -26,9 +26,10 @@ static __constant__ double c_ynorm[(kMaxL + 1) * (kMaxL + 2) / 2];
 static __constant__ double c_dfact[kMaxL + 2];
 static __constant__ double c_rinv[2 * kMaxL + 4];   // 1/k (k = 0 unused)
 
-static bool g_soap_const_ready = false;
+static bool g_soap_const_ready[64] = {};   // per device
 static void ensure_soap_constants() {
-    if (g_soap_const_ready) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_soap_const_ready[dev]) return;
     double yn[(kMaxL + 1) * (kMaxL + 2) / 2], df[kMaxL + 2];
     for (int l = 0; l <= kMaxL; l++)
         for (int m = 0; m <= l; m++) {
@@ -46,8 +47,10 @@ static void ensure_soap_constants() {
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_rinv), ri, sizeof(ri));
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn));
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df));
-    g_soap_const_ready = true;
+    g_soap_const_ready[dev] = true;
 }
+
+void soap_fast_constants_init() { ensure_soap_constants(); }
 
 #define RELOAD_BARRIER() __asm__ volatile("" ::: "memory")
 

@@ -74,6 +74,10 @@ class Potential:
         L.check(L.lib().gapx_model_info(self._h, C.byref(n), -1, None, None), "gapx_model_info")
         return n.value
 
+    def set_graphs(self, on=True):
+        """Replay the launch sequence after the neighbour count as a HIP graph."""
+        L.check(L.lib().gapx_model_set_graphs(self._h, 1 if on else 0), "set_graphs")
+
     def set_timing(self, on=True):
         L.check(L.lib().gapx_model_set_timing(self._h, 1 if on else 0), "set_timing")
 
