@@ -138,6 +138,16 @@ int gapx_server_stats(gapx_server *server, int64_t *n_done, int64_t *n_batches, 
                       double *busy_s);
 int gapx_server_free(gapx_server *server);
 
+/* Skin reuse of the neighbour list (calc_connect with cutoff_skin,
+ * Connection.f95:1085-1127; on by default).  When a calc's nl_cutoff exceeds the
+ * model cutoff by a skin, the list built by the previous call on the same atoms
+ * (same Z, structure layout, lattices, pbc and nl_cutoff) is kept as long as no
+ * atom has moved by half the skin since it was built; only its distances are
+ * recomputed (connection_calc_dists, Connection.f95:1321-1445).
+ * gapx_last_nl_rebuilt tells whether the last calc rebuilt the list. */
+int gapx_model_set_nl_reuse(gapx_model *model, int on);
+int gapx_last_nl_rebuilt(const gapx_model *model, int *rebuilt);
+
 /* Neighbour tables exactly as QUIP's calc_connect builds them
  * (Connection.f95:1035-1310): neighbour1(i) = (j >= i, shift, r) in append
  * order, neighbour2(i) = (j < i, index into neighbour1(j)).  Built on the GPU. */

@@ -61,6 +61,8 @@ _SIGS = [
                                    C.c_double, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
     ("gapx_model_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_model_set_graphs", C.c_int, [C.c_void_p, C.c_int]),
+    ("gapx_model_set_nl_reuse", C.c_int, [C.c_void_p, C.c_int]),
+    ("gapx_last_nl_rebuilt", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("gapx_model_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.POINTER(C.c_int)]),
     ("gapx_model_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_last_timings", C.c_int, [C.c_void_p, C.c_void_p]),

@@ -493,6 +493,16 @@ extern "C" int gapx_model_info(const gapx_model *m, int *n_coord, int coord, int
     }
     return GAPX_OK;
 }
+extern "C" int gapx_model_set_nl_reuse(gapx_model *m, int on) {
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    m->nl_reuse = on != 0;
+    return GAPX_OK;
+}
+extern "C" int gapx_last_nl_rebuilt(const gapx_model *m, int *rebuilt) {
+    if (!m || !rebuilt) return fail(GAPX_E_ARG, "NULL argument");
+    *rebuilt = m->eng->last_rebuilt ? 1 : 0;
+    return GAPX_OK;
+}
 extern "C" int gapx_model_set_graphs(gapx_model *m, int on) {
     if (!m) return fail(GAPX_E_ARG, "NULL model");
     m->graphs = on != 0;
@@ -601,6 +611,7 @@ static void prepare_geom(const double *lat, const int *pbc, double cutoff, const
         g.cells[a] = std::max(1, (int)(vol / (cutoff * norm3(bc))));
     }
     inv3(lat, gi);   // fit_box_in_cell on the true lattice
+    memcpy(g.ginv, gi, sizeof(gi));
     double mxs[3] = {0, 0, 0}, lc[3], corner[3];
     for (int k = -1; k <= 1; k += 2)
         for (int j = -1; j <= 1; j += 2)
@@ -648,6 +659,7 @@ static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, c
                 std::equal(atom_off, atom_off + S + 1, eg.h_off.begin()) && std::equal(Z, Z + N, eg.h_Z.begin());
     if (same) return GAPX_OK;
     eg.have_inputs = false;
+    eg.nl_valid = false;   // a stored neighbour list belongs to other atoms
     eg.h_off.assign(atom_off, atom_off + S + 1);
     eg.h_Z.assign(Z, Z + N);
     eg.h_sid.resize(N);
@@ -829,6 +841,59 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     return GAPX_OK;
 }
 
+// calc_connect with skin reuse (Connection.f95:1085-1127): with a cutoff above the
+// model's (a skin), a list built for the same structures, cutoff and lattices is
+// kept while no atom has moved by half the skin since that build; its distances
+// are then recomputed as calc_dists does (Connection.f95:1321-1445) and the full
+// per-atom view re-formed.  The displacement test and the distances run in one
+// pass and are read back with one synchronisation, as the rebuild's count is.
+static int connect(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
+                   const double *lats, const int *pbcs, double cutoff, int *P_out) {
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    const int N = atom_off[S];
+    const double skin = cutoff - m->cutoff;
+    eg.last_rebuilt = true;
+    if (skin > 0 && m->nl_reuse && eg.nl_valid && eg.nl_cut == cutoff && (int)eg.nl_pbcs.size() == 3 * S &&
+        std::equal(lats, lats + 9 * S, eg.nl_lats.begin()) && std::equal(pbcs, pbcs + 3 * S, eg.nl_pbcs.begin())) {
+        CHK(w.skin_flags.ensure(64));
+        launch_fill_i32(w.skin_flags.as<int>(), 4, 0, s);
+        unsigned long long *md = (unsigned long long *)w.skin_flags.p;
+        int *ov = w.skin_flags.as<int>() + 2;
+        launch_nl_skin_check(N, d_pos, w.last_pos.as<double>(), w.geo.as<NLGeom>(), w.sid.as<int>(),
+                             w.n1_off.as<int>(), w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), md, ov,
+                             s);
+        CHK(eg.ctl.ensure(64));
+        HIPCHK(hipMemcpyAsync(eg.ctl.p, w.skin_flags.p, 16, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);   // the previous call has finished
+        double maxd;
+        memcpy(&maxd, eg.ctl.p, 8);
+        const int overlap = eg.ctl.as<int>()[2];
+        if (maxd < 0.5 * skin) {
+            if (overlap) return fail(GAPX_E_GEOMETRY, "calc_dists: two atoms overlap exactly");
+            launch_nl_full(N, d_pos, w.geo.as<NLGeom>(), w.sid.as<int>(), w.n1_off.as<int>(), w.n1_j.as<int>(),
+                           w.n1_shift.as<int>(), w.n1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(),
+                           w.n2_ks.as<int>(), w.n1_rev.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
+                           w.nb_rev.as<int>(), w.nb_r.as<double>(), w.nb_d.as<double>(), s);
+            eg.last_rebuilt = false;
+            *P_out = 2 * eg.n1_total_last;
+            return GAPX_OK;
+        }
+    }
+    eg.nl_valid = false;
+    CHK(build_nl(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, cutoff, P_out));
+    if (skin > 0 && m->nl_reuse) {   // remember this build for the next call
+        CHK(w.last_pos.ensure((size_t)N * 24 + 24));
+        if (N) HIPCHK(hipMemcpyAsync(w.last_pos.p, d_pos, (size_t)N * 24, hipMemcpyDeviceToDevice, s));
+        eg.nl_cut = cutoff;
+        eg.nl_lats.assign(lats, lats + 9 * S);
+        eg.nl_pbcs.assign(pbcs, pbcs + 3 * S);
+        eg.nl_valid = true;
+    }
+    return GAPX_OK;
+}
+
 // Scratch outputs.  Growing them frees the old blocks, so callers that hand
 // these buffers in as outputs size them first.
 static int ensure_outputs(Workspace &w, int S, int N) {
@@ -951,7 +1016,7 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     CHK(stage_inputs(m, eg, S, atom_off, Z));
     const int t_nl0 = tm.on ? tm.mark(s) : -1;
     int P = 0;
-    CHK(build_nl(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, &P));
+    CHK(connect(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, &P));
     if (tm.on) tm.stage(0, t_nl0, tm.mark(s));
 
     // Prepare: every buffer the launches use is sized here (to the largest

@@ -114,6 +114,7 @@ struct Workspace {
     DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite;
     DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws;
     DBuf f, le, lv, e, vir, minmax;
+    DBuf last_pos, skin_flags;   // positions of the last build (skin reuse), reuse test results
     std::vector<DBuf> centres;   // per coordinate
 };
 
@@ -218,6 +219,11 @@ struct Engine {
     std::vector<NLGeom> h_geo;
     std::vector<std::vector<int>> h_centres;   // per coordinate
     bool have_inputs = false, have_geo = false;
+    // skin reuse: the last build's cutoff and lattices; valid while the atoms are the same
+    bool nl_valid = false, last_rebuilt = true;
+    double nl_cut = 0;
+    std::vector<double> nl_lats;
+    std::vector<int> nl_pbcs;
     hipGraphExec_t gexec = nullptr;   // the captured launch sequence after the neighbour count
     std::vector<long long> gkey;
     int n1_total_last = 0;
@@ -242,6 +248,7 @@ struct ModelState {
     DBuf d_e0;
     bool autotune = true;
     bool graphs = false;
+    bool nl_reuse = true;   // keep a neighbour list within its skin (calc_connect with cutoff_skin)
     std::unique_ptr<Engine> eng;   // the engine of the plain entry points
 };
 

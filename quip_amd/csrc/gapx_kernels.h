@@ -12,6 +12,7 @@ constexpr int kMaxL = 12;
 struct NLGeom {
     double lat[9];       // lattice(k,m) at lat[3*m+k]: column m is lattice vector m
     double inv_eff[9];   // inverse of the effective lattice, column-major g(i,j) at [3*j+i]
+    double ginv[9];      // inverse of the true lattice, same layout
     double offset[3];    // lat_offset of non-periodic axes
     int cells[3];        // cellsNa, cellsNb, cellsNc
     int img[3];          // cell_image_N{a,b,c} after halving
@@ -50,6 +51,11 @@ void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
                     hipStream_t s);
+// skin reuse: max displacement since the last build (bits of a double, atomicMax into a
+// zeroed word) and the stored neighbour1 distances recomputed as calc_dists does
+void launch_nl_skin_check(int N, const double *pos, const double *last, const NLGeom *geo, const int *sid,
+                          const int *n1_off, const int *n1_j, const int *n1_shift, double *n1_r,
+                          unsigned long long *max_disp, int *overlap, hipStream_t s);
 void launch_minmax_frac(int S, const int *atom_off, const double *pos, const double *g, double *out, hipStream_t s);
 
 // upload the constant tables of the SOAP kernels to the current device (once per

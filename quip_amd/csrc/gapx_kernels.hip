@@ -480,6 +480,80 @@ void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid,
                        n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d);
 }
 
+// Skin reuse (calc_connect with cutoff_skin, Connection.f95:1085-1127).
+// Largest displacement of any atom since the positions `last` of the last build,
+// minimum image over periodic axes (fractional displacement wrapped to the
+// nearest lattice vector), as the bits of a non-negative double in *out (atomicMax).
+__global__ void k_nl_max_disp(int N, const double *__restrict__ pos, const double *__restrict__ last,
+                              const NLGeom *__restrict__ geo, const int *__restrict__ sid,
+                              unsigned long long *out) {
+    __shared__ double red[256];
+    double mx = 0.0;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+        const NLGeom &g = geo[sid[n]];
+        const double dx = pos[3 * n] - last[3 * n], dy = pos[3 * n + 1] - last[3 * n + 1],
+                     dz = pos[3 * n + 2] - last[3 * n + 2];
+        // fractional displacement through the inverse of the true lattice (kept in lat_inv order)
+        double f[3];
+        for (int a = 0; a < 3; a++) f[a] = g.ginv[a] * dx + g.ginv[3 + a] * dy + g.ginv[6 + a] * dz;
+        for (int a = 0; a < 3; a++)
+            if (g.pbc[a]) f[a] -= rint(f[a]);
+        double c[3];
+        for (int k = 0; k < 3; k++) c[k] = g.lat[k] * f[0] + g.lat[3 + k] * f[1] + g.lat[6 + k] * f[2];
+        mx = fmax(mx, sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]));
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// connection_calc_dists (Connection.f95:1321-1445) on the stored neighbour1
+// entries: r_ij = norm(pos_j + lattice.shift - pos_i) with j_pos formed as
+// pos_j + ((L1 s1 + L2 s2) + L3 s3).  The reference visits every pair from both
+// ends and the later visit wins, so for i < j the stored value is computed from
+// j's side (shift negated); self images from i's side.  r < 1e-10 sets *overlap.
+__global__ void k_nl_dists(int N, const double *__restrict__ pos, const NLGeom *__restrict__ geo,
+                           const int *__restrict__ sid, const int *__restrict__ n1_off, const int *__restrict__ n1_j,
+                           const int *__restrict__ n1_shift, double *n1_r, int *overlap) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const NLGeom &g = geo[sid[i]];
+    for (int e = n1_off[i]; e < n1_off[i + 1]; e++) {
+        const int j = n1_j[e];
+        int s0 = n1_shift[3 * e], s1 = n1_shift[3 * e + 1], s2 = n1_shift[3 * e + 2];
+        int from = i, to = j;          // r = norm(pos(to) + L.s - pos(from))
+        if (j != i) { from = j; to = i; s0 = -s0; s1 = -s1; s2 = -s2; }
+        double v[3];
+        for (int k = 0; k < 3; k++) {
+            double t = g.lat[k] * (double)s0;
+            t = t + g.lat[3 + k] * (double)s1;
+            t = t + g.lat[6 + k] * (double)s2;
+            v[k] = (pos[3 * to + k] + t) - pos[3 * from + k];
+        }
+        double dsq = v[0] * v[0];
+        dsq = dsq + v[1] * v[1];
+        dsq = dsq + v[2] * v[2];
+        const double r = sqrt(dsq);
+        if (r < 1e-10) atomicOr(overlap, 1);
+        n1_r[e] = r;
+    }
+}
+
+void launch_nl_skin_check(int N, const double *pos, const double *last, const NLGeom *geo, const int *sid,
+                          const int *n1_off, const int *n1_j, const int *n1_shift, double *n1_r,
+                          unsigned long long *max_disp, int *overlap, hipStream_t s) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_nl_max_disp, dim3(std::min(cdiv(N, 256), 1024)), dim3(256), 0, s, N, pos, last, geo, sid,
+                       max_disp);
+    hipLaunchKernelGGL(k_nl_dists, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j, n1_shift,
+                       n1_r, overlap);
+}
+
 // min / max of fractional coordinates (g .mult. pos) per axis, for calc_lat_eff;
 // one block per structure, g = the inverse of that structure's lattice.
 __global__ void k_minmax_frac(const int *atom_off, const double *pos, const double *gmat, double *out) {

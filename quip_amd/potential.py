@@ -74,6 +74,16 @@ class Potential:
         L.check(L.lib().gapx_model_info(self._h, C.byref(n), -1, None, None), "gapx_model_info")
         return n.value
 
+    def set_nl_reuse(self, on=True):
+        """Keep the neighbour list within its skin (calc_connect with cutoff_skin)."""
+        L.check(L.lib().gapx_model_set_nl_reuse(self._h, 1 if on else 0), "set_nl_reuse")
+
+    def nl_rebuilt(self):
+        """Whether the last calc rebuilt the neighbour list (calc_connect's did_rebuild)."""
+        r = C.c_int()
+        L.check(L.lib().gapx_last_nl_rebuilt(self._h, C.byref(r)), "last_nl_rebuilt")
+        return bool(r.value)
+
     def set_graphs(self, on=True):
         """Replay the launch sequence after the neighbour count as a HIP graph."""
         L.check(L.lib().gapx_model_set_graphs(self._h, 1 if on else 0), "set_graphs")

@@ -262,6 +262,33 @@ def test_sic32768_precision_forms(sic_model):
         assert de < 1e-3 and fr < 1e-2
 
 
+def test_skin_reuse(models):
+    """calc_connect with a skin: small moves keep the list (distances recomputed as
+    calc_dists does) and match the oracle's fresh build; a move of half the skin
+    or more rebuilds it."""
+    m, path = models["si64"]
+    pot = Potential(param_filename=path)
+    at = configs.si64(3)
+    rc = pot.cutoff() + 0.6
+    rng = np.random.default_rng(21)
+    pot.calc(at.copy(), force=True, cutoff=rc)
+    assert pot.nl_rebuilt()
+    for step in range(4):
+        at.positions = at.positions + rng.normal(0.0, 0.01, at.positions.shape)
+        r = pot.calc(at.copy(), force=True, virial=True, local_energy=True, local_virial=True, cutoff=rc)
+        assert not pot.nl_rebuilt()
+        _compare(r, O.gap_calc(m, at, nl_cutoff=rc), len(at))
+    at.positions[5] += np.array([0.4, 0.0, 0.0])
+    r = pot.calc(at.copy(), force=True, virial=True, local_energy=True, local_virial=True, cutoff=rc)
+    assert pot.nl_rebuilt()
+    _compare(r, O.gap_calc(m, at, nl_cutoff=rc), len(at))
+    # no skin: every call builds
+    pot.calc(at.copy(), force=True)
+    assert pot.nl_rebuilt()
+    pot.calc(at.copy(), force=True)
+    assert pot.nl_rebuilt()
+
+
 def _eq(a, b):
     assert a["energy"] == b["energy"]
     assert np.array_equal(a["force"], b["force"])
