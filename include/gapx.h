@@ -166,9 +166,10 @@ int gapx_descriptors(gapx_model *model, int coord, int n_atoms, const double *po
 
 /* FP64 GEMM autotuner (on by default).  No counterpart in the reference.  For
  * each gpCoordinate and centre-count bucket b (2^(b-1) < n_centres <= 2^b) the
- * first evaluation times every GEMM tile shape (tile t: (WM,WN) = (4,2), (2,2),
- * (2,4), (4,4) wavefront tiles of 16x16, block 32WM x 32WN) and every GEMM 2
- * split-K factor in {1,2,4,8} on its own operands, and keeps the fastest.
+ * first evaluation times every GEMM tile shape (tile t: (WM,WN,k tile) =
+ * (4,2,16), (2,2,16), (2,4,16), (4,2,32), (2,2,32), (2,4,32); wavefront tiles of
+ * 16x16, block 32WM x 32WN) and every GEMM 2 split-K factor in {1,2,4,8} on its
+ * own operands, and keeps the fastest.
  * gapx_model_tuning lists the choices (up to max_entries; n_entries gets the
  * count) with the measured milliseconds of one launch. */
 int gapx_model_set_autotune(gapx_model *model, int on);

@@ -931,7 +931,7 @@ int gapx::check_batch(int S, const int *atom_off, const int *Z, const double *la
 
 // GEMM autotuner.  For each coordinate and centre-count bucket (ceil log2 nc) the
 // first evaluation times every GEMM 1 tile and every GEMM 2 tile x split-K factor
-// once (after one warm-up launch) with HIP events on the engine's stream, on the
+// (fastest of three launches after a warm-up) with HIP events on the engine's stream, on the
 // call's own operands (X is already computed), and keeps the fastest.  Later
 // calls in the bucket reuse the choice.  Without autotuning: GEMM 1 (4,2),
 // GEMM 2 (2,2) with the default split-K factor.
@@ -957,13 +957,18 @@ static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out)
         hipEventDestroy(ea);
         return fail(GAPX_E_DEVICE, "hipEventCreate failed");
     }
-    auto timed = [&](auto launch, float *ms) -> int {
+    auto timed = [&](auto launch, float *ms) -> int {   // one warm-up, then the fastest of three launches
         launch();
-        hipEventRecord(ea, s);
-        launch();
-        hipEventRecord(eb, s);
-        HIPCHK(hipEventSynchronize(eb));
-        HIPCHK(hipEventElapsedTime(ms, ea, eb));
+        *ms = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            hipEventRecord(ea, s);
+            launch();
+            hipEventRecord(eb, s);
+            HIPCHK(hipEventSynchronize(eb));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ea, eb));
+            *ms = std::min(*ms, t);
+        }
         return GAPX_OK;
     };
     GemmTune best;

@@ -78,9 +78,9 @@ void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z
                      const double *norm, const double *cbuf, double *pairF, hipStream_t s);
 
 // GEMMs
-// Tile configuration `cfg` in [0, kGemmTiles): (WM, WN) = (4,2), (2,2), (2,4), (4,4),
-// block tile (32 WM) x (32 WN) of 4 wavefronts.
-constexpr int kGemmTiles = 4;
+// Tile configuration `cfg` in [0, kGemmTiles): (WM, WN, k tile) = (4,2,16), (2,2,16),
+// (2,4,16), (4,2,32), (2,2,32), (2,4,32); block tile (32 WM) x (32 WN) of 4 wavefronts.
+constexpr int kGemmTiles = 6;
 // gemm1: K = X (nc x d) . Xs^T (M x d); W = w_s zeta K^(zeta-1); e_part[i*ntile + t]
 int gemm1_ntile(int cfg, int M);
 void launch_gemm1_f64(int cfg, int nc, int M, int d, const double *X, const double *Xs, const double *wvec,

@@ -947,7 +947,7 @@ void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z
 // ----------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int GBK = 16, GPAD = 4;
+constexpr int GPAD = 4;
 
 // C = A . B (+ fused GP epilogue).  A is row-major (rows x K, k contiguous);
 // B is row-major (cols x K) when BT, else (K x cols).  4 wavefronts in a 2x2
@@ -955,7 +955,7 @@ constexpr int GBK = 16, GPAD = 4;
 // The next k tile is fetched into registers while the current one feeds the
 // MFMAs from LDS.  blockIdx.z selects a k range (split-K) whose partial result
 // goes to C + z * rows * cols.
-template <bool BT, bool EPI, int WM, int WN>
+template <bool BT, bool EPI, int WM, int WN, int GBK>
 __global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, int kchunk, const double *__restrict__ A,
                                                   const double *__restrict__ B, double *__restrict__ C,
                                                   const double *__restrict__ wvec, double zeta, int izeta,
@@ -979,9 +979,9 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, int kc
     double ra[AL], rb[BL];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < AL; e++) {      // A: row = (t + 256 e) / 16 ... k contiguous pairs of 4
+        for (int e = 0; e < AL; e++) {      // A: GBK consecutive k of one row per GBK threads
             const int idx = t + 256 * e;
-            const int row = idx >> 4, kk = idx & 15;
+            const int row = idx / GBK, kk = idx % GBK;
             const int gr = m0 + row, gk = k0 + kk;
             ra[e] = (gr < Mr && gk < kend) ? A[(size_t)gr * Kd + gk] : 0.0;
         }
@@ -989,7 +989,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, int kc
         for (int e = 0; e < BL; e++) {
             const int idx = t + 256 * e;
             if (BT) {
-                const int col = idx >> 4, kk = idx & 15;
+                const int col = idx / GBK, kk = idx % GBK;
                 const int gc = n0 + col, gk = k0 + kk;
                 rb[e] = (gc < Nc && gk < kend) ? B[(size_t)gc * Kd + gk] : 0.0;
             } else {
@@ -1003,12 +1003,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, int kc
 #pragma unroll
         for (int e = 0; e < AL; e++) {
             const int idx = t + 256 * e;
-            As[idx & 15][idx >> 4] = ra[e];
+            As[idx % GBK][idx / GBK] = ra[e];
         }
 #pragma unroll
         for (int e = 0; e < BL; e++) {
             const int idx = t + 256 * e;
-            if (BT) Bs[idx & 15][idx >> 4] = rb[e];
+            if (BT) Bs[idx % GBK][idx / GBK] = rb[e];
             else Bs[idx / BN][idx % BN] = rb[e];
         }
     };
@@ -1091,18 +1091,18 @@ static int int_zeta(double z) {
     return (fabs(z - r) == 0.0 && r >= 1.0 && r <= 16.0) ? (int)r : 0;
 }
 
-// Tile shapes the autotuner chooses between: (WM, WN) per block of 4 wavefronts,
-// block tile (32 WM) x (32 WN).
-static const int kTiles[kGemmTiles][2] = {{4, 2}, {2, 2}, {2, 4}, {4, 4}};
+// Tile shapes the autotuner chooses between: (WM, WN, k tile) per block of 4
+// wavefronts, block tile (32 WM) x (32 WN).
+static const int kTiles[kGemmTiles][3] = {{4, 2, 16}, {2, 2, 16}, {2, 4, 16}, {4, 2, 32}, {2, 2, 32}, {2, 4, 32}};
 
 int gemm1_ntile(int cfg, int M) { return cdiv(M, 32 * kTiles[cfg][1]); }
 
-template <int WM, int WN>
+template <int WM, int WN, int BK>
 static void gemm1_go(int nc, int M, int d, const double *X, const double *Xs, const double *wvec, double zeta,
                      double *W, double *e_part, hipStream_t s) {
     dim3 grid(cdiv(nc, 32 * WM), cdiv(M, 32 * WN), 1);
-    hipLaunchKernelGGL((k_gemm_f64<true, true, WM, WN>), grid, dim3(256), 0, s, nc, M, d, d, X, Xs, W, wvec, zeta,
-                       int_zeta(zeta), e_part);
+    hipLaunchKernelGGL((k_gemm_f64<true, true, WM, WN, BK>), grid, dim3(256), 0, s, nc, M, d, d, X, Xs, W, wvec,
+                       zeta, int_zeta(zeta), e_part);
 }
 
 void launch_gemm1_f64(int cfg, int nc, int M, int d, const double *X, const double *Xs, const double *wvec,
@@ -1110,10 +1110,12 @@ void launch_gemm1_f64(int cfg, int nc, int M, int d, const double *X, const doub
     *ntile_out = gemm1_ntile(cfg, M);
     if (nc <= 0) return;
     switch (cfg) {
-        case 0: gemm1_go<4, 2>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
-        case 1: gemm1_go<2, 2>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
-        case 2: gemm1_go<2, 4>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
-        default: gemm1_go<4, 4>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        case 0: gemm1_go<4, 2, 16>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        case 1: gemm1_go<2, 2, 16>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        case 2: gemm1_go<2, 4, 16>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        case 3: gemm1_go<4, 2, 32>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        case 4: gemm1_go<2, 2, 32>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
+        default: gemm1_go<2, 4, 32>(nc, M, d, X, Xs, wvec, zeta, W, e_part, s); break;
     }
 }
 
@@ -1134,12 +1136,12 @@ int gemm2_parts(int nc, int d) {
     return parts;
 }
 
-template <int WM, int WN>
+template <int WM, int WN, int BK>
 static void gemm2_go(int nc, int M, int d, int parts, const double *W, const double *Xs, double *out,
                      hipStream_t s) {
-    const int kchunk = (cdiv(M, parts) + GBK - 1) / GBK * GBK;
+    const int kchunk = (cdiv(M, parts) + BK - 1) / BK * BK;
     dim3 grid(cdiv(nc, 32 * WM), cdiv(d, 32 * WN), parts);
-    hipLaunchKernelGGL((k_gemm_f64<false, false, WM, WN>), grid, dim3(256), 0, s, nc, d, M, kchunk, W, Xs, out,
+    hipLaunchKernelGGL((k_gemm_f64<false, false, WM, WN, BK>), grid, dim3(256), 0, s, nc, d, M, kchunk, W, Xs, out,
                        (const double *)nullptr, 0.0, 0, (double *)nullptr);
 }
 
@@ -1148,10 +1150,12 @@ void launch_gemm2_f64(int cfg, int parts, int nc, int M, int d, const double *W,
     if (nc <= 0) return;
     double *out = parts > 1 ? Gparts : G;
     switch (cfg) {
-        case 0: gemm2_go<4, 2>(nc, M, d, parts, W, Xs, out, s); break;
-        case 1: gemm2_go<2, 2>(nc, M, d, parts, W, Xs, out, s); break;
-        case 2: gemm2_go<2, 4>(nc, M, d, parts, W, Xs, out, s); break;
-        default: gemm2_go<4, 4>(nc, M, d, parts, W, Xs, out, s); break;
+        case 0: gemm2_go<4, 2, 16>(nc, M, d, parts, W, Xs, out, s); break;
+        case 1: gemm2_go<2, 2, 16>(nc, M, d, parts, W, Xs, out, s); break;
+        case 2: gemm2_go<2, 4, 16>(nc, M, d, parts, W, Xs, out, s); break;
+        case 3: gemm2_go<4, 2, 32>(nc, M, d, parts, W, Xs, out, s); break;
+        case 4: gemm2_go<2, 2, 32>(nc, M, d, parts, W, Xs, out, s); break;
+        default: gemm2_go<2, 4, 32>(nc, M, d, parts, W, Xs, out, s); break;
     }
     if (parts > 1) {
         const size_t n = (size_t)nc * d;
