@@ -11,8 +11,9 @@
 //           = (a, m) accumulate c' in the primitive radial basis; c = T^T c'
 //           once per centre; lanes = power-spectrum entries form p, a wave
 //           reduction gives |p|.
-//  adjoint: dE/dp -> dE/dc -> dE/dc' = T dE/dc (LDS), then lanes = neighbours
-//           re-evaluate F_l, Y and grad Y and contract them with dE/dc'.
+//  adjoint: dE/dp -> dE/dc per level as a small product staged in LDS ->
+//           dE/dc' = T dE/dc, then lanes = neighbours re-evaluate F_l, Y and
+//           grad Y and contract them with dE/dc'.
 #include "gapx_kernels.h"
 
 #include <algorithm>
@@ -93,7 +94,34 @@ __device__ __forceinline__ void legendre_level(int l, double uz, const double *p
 template <int N>
 struct Rad {
     double F0[N], F1[N], ix[N];
-    __device__ __forceinline__ void init(const double *rbs, double alpha, double r) {
+    // rbs: grid r_a = a * dr; gA[a] = exp(-alpha r_a^2); iA[a] = 1/a.
+    // exp(-alpha (r -+ r_a)^2) = exp(-alpha r^2) gA[a] q^(+-a) with q = exp(2 alpha r dr) and
+    // 1/(2 alpha r r_a) = iA[a] / (2 alpha r dr): two exponentials and one division per
+    // neighbour.  Outside the range where the powers stay finite, every point is direct.
+    __device__ __forceinline__ void init(const double *rbs, const double *gA, const double *iA, double alpha,
+                                         double r) {
+        const double xq = N > 1 ? 2.0 * alpha * r * rbs[1] : 0.0;
+        const double rmax = rbs[N - 1];
+        if (xq > 0.0 && xq * (N - 1) < 600.0 && alpha * (r * r + rmax * rmax) < 600.0) {
+            const double e0 = exp(-alpha * r * r);
+            const double q = exp(xq), qi = 1.0 / q, ixq = 1.0 / xq;
+            double qa = 1.0, qia = 1.0;
+            F0[0] = e0;
+            F1[0] = 0.0;
+            ix[0] = 0.0;
+#pragma unroll
+            for (int a = 1; a < N; a++) {
+                qa *= q;
+                qia *= qi;
+                const double eg = e0 * gA[a];
+                const double em = eg * qa, ep = eg * qia;
+                const double iv = ixq * iA[a];
+                F0[a] = 0.5 * (em - ep) * iv;
+                F1[a] = 0.5 * (em + ep) * iv - F0[a] * iv;
+                ix[a] = iv;
+            }
+            return;
+        }
 #pragma unroll
         for (int a = 0; a < N; a++) {
             const double ra = rbs[a];
@@ -139,15 +167,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
     constexpr int NB = kSoapChunk;
     const int K = sp.K, KLM = K * LM, S = sp.S;
     double *Ts = lds;              // N*N
-    double *rbs = Ts + N * N;      // N
-    double *cP = rbs + N;          // K*LM: c' during accumulation, c after the transform
+    double *rbs = Ts + N * N;      // N grid points, then exp(-alpha r_a^2) and 1/a (N each)
+    double *gA = rbs + N, *iA = gA + N;
+    double *cP = rbs + 3 * N;      // K*LM: c' during accumulation, c after the transform
     double *Rl = cP + KLM;         // S x NB x N: fc F_l(a) of neighbour k, zero unless k has species s
     double *Yl = Rl + S * NB * N;  // NB x YW
     const int lane = threadIdx.x;
     const int desc = blockIdx.x;
     const int ci = centres[desc];
     for (int e = lane; e < N * N; e += 64) Ts[e] = sp.T[e];
-    if (lane < N) rbs[lane] = sp.rb[lane];
+    if (lane < N) {
+        rbs[lane] = sp.rb[lane];
+        gA[lane] = exp(-sp.alpha * sp.rb[lane] * sp.rb[lane]);
+        iA[lane] = lane > 0 ? 1.0 / (double)lane : 0.0;
+    }
     for (int e = lane; e < KLM; e += 64) cP[e] = 0.0;
     __syncthreads();
     const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
@@ -163,7 +196,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
             if (r > sp.rc) s = -1;
             if (s >= 0) {
                 fc = fc_s(r, sp.rc, sp.ctw);
-                rs.init(rbs, sp.alpha, r);
+                rs.init(rbs, gA, iA, sp.alpha, r);
                 const double ir = 1.0 / r;
                 ux = nb_d[3 * p] * ir; uy = nb_d[3 * p + 1] * ir; uz = nb_d[3 * p + 2] * ir;
             }
@@ -274,54 +307,105 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
     SoapDev sp, int nc, int cpb, const int *__restrict__ centres, const int *__restrict__ Z,
     const int *__restrict__ nb_off, const int *__restrict__ nb_j, const double *__restrict__ nb_r,
     const double *__restrict__ nb_d, const double *__restrict__ X, const double *__restrict__ G,
-    const double *__restrict__ norm, const double *__restrict__ cbuf, double *pairF, int pq_own) {
+    const double *__restrict__ norm, const double *__restrict__ cbuf, double *pairF, int whole) {
     // cpb centres per wavefront: a centre has ~30 neighbours, so one lane per
     // neighbour over 64/cpb lanes keeps the wavefront busy.
+    // LDS: T, the grid tables, dE/dc (then dE/dc') of each centre, the centre
+    // scalars, and one staging region used first for the per-l blocks of the
+    // dE/dc product and then for the lanes' Legendre rows.
     extern __shared__ double lds[];
     constexpr int LM = (L + 1) * (L + 1);
+    constexpr int PR = 2 * (L + 2);
     const int K = sp.K, KLM = K * LM, d = sp.dim, S = sp.S;
-    const int per = 2 * KLM + d;
     double *Ts = lds;              // N*N
-    double *rbs = Ts + N * N;      // N
-    double *cen = rbs + N;         // cpb x [cL (K*LM) | dc (K*LM) | dpL (d)]
+    double *rbs = Ts + N * N;      // N grid points, then exp(-alpha r_a^2) and 1/a (N each)
+    double *gA = rbs + N, *iA = gA + N;
+    double *dcs_all = rbs + 3 * N; // cpb x K*LM
+    double *csc = dcs_all + cpb * KLM;   // cpb x (dot, 1/|p|)
+    double *stg = csc + 2 * cpb;   // max(K*K + K*(2L+1), 64*PR) or, whole, cpb*(K*LM + d)
     const int lane = threadIdx.x;
     const int d0 = cpb * blockIdx.x;
     const int nh = min(cpb, nc - d0);
     for (int e = lane; e < N * N; e += 64) Ts[e] = sp.T[e];
-    if (lane < N) rbs[lane] = sp.rb[lane];
+    if (lane < N) {
+        rbs[lane] = sp.rb[lane];
+        gA[lane] = exp(-sp.alpha * sp.rb[lane] * sp.rb[lane]);
+        iA[lane] = lane > 0 ? 1.0 / (double)lane : 0.0;
+    }
     for (int h = 0; h < nh; h++) {
         const int desc = d0 + h;
-        double *cL = cen + h * per, *dpL = cL + 2 * KLM;
         const double *x = X + (size_t)desc * d, *gg = G + (size_t)desc * d;
         double dot = 0.0;
         for (int q = lane; q < d; q += 64) dot += x[q] * gg[q];
         dot = wave_sum_s(dot);
-        const double nrm = norm[desc];
-        const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-        for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
-        for (int e = lane; e < KLM; e += 64) cL[e] = cbuf[(size_t)desc * KLM + e];
-    }
-    __syncthreads();
-    for (int it = lane; it < nh * KLM; it += 64) {
-        const int h = it / KLM, e = it - h * KLM;
-        const double *cL = cen + h * per, *dpL = cL + 2 * KLM;
-        double *dcL = cen + h * per + KLM;
-        const int ia = e / LM, lm = e - ia * LM;
-        int l = (int)sqrt((double)lm + 0.5);
-        if (l * l > lm) l--;
-        double acc = 0.0;
-        for (int jb = 0; jb < K; jb++) {
-            const int hi = ia > jb ? ia : jb, lo = ia > jb ? jb : ia;
-            const int q = (hi * (hi + 1) / 2 + lo) * (L + 1) + l;
-            acc += dpL[q] * (ia == jb ? 2.0 : 1.4142135623730950488) * cL[jb * LM + lm];
+        if (lane == 0) {
+            const double nrm = norm[desc];
+            csc[2 * h] = dot;
+            csc[2 * h + 1] = nrm > 0.0 ? 1.0 / nrm : 0.0;
         }
-        dcL[e] = acc / sqrt(2.0 * l + 1.0);
+        if (whole) {   // c and dE/dp of every centre fit in the staging region together
+            const double nrm = norm[desc];
+            const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+            double *cL = stg + h * (KLM + d), *dpL = cL + KLM;
+            for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
+            for (int e = lane; e < KLM; e += 64) cL[e] = cbuf[(size_t)desc * KLM + e];
+        }
     }
     __syncthreads();
+    if (whole) {   // dE/dc in one pass over (centre, ia, lm)
+        for (int it = lane; it < nh * KLM; it += 64) {
+            const int h = it / KLM, e = it - h * KLM;
+            const double *cL = stg + h * (KLM + d), *dpL = cL + KLM;
+            const int ia = e / LM, lm = e - ia * LM;
+            int l = (int)sqrt((double)lm + 0.5);
+            if (l * l > lm) l--;
+            double acc = 0.0;
+            for (int jb = 0; jb < K; jb++) {
+                const int hi = ia > jb ? ia : jb, lo = ia > jb ? jb : ia;
+                const int q = (hi * (hi + 1) / 2 + lo) * (L + 1) + l;
+                acc += dpL[q] * (ia == jb ? 2.0 : 1.4142135623730950488) * cL[jb * LM + lm];
+            }
+            dcs_all[h * KLM + e] = acc / sqrt(2.0 * l + 1.0);
+        }
+        __syncthreads();
+    }
+    // otherwise dE/dc per centre and level l: dc_l (K x 2l+1) = P_l (K x K) c_l (K x 2l+1) with
+    // P_l[ia][jb] = dE/dp_(ia,jb),l x (2 if ia = jb else sqrt 2) / sqrt(2l+1) and
+    // dE/dp = (G - x (x.G)) / |p|; both blocks are staged in LDS one level at a time.
+    for (int h = 0; h < (whole ? 0 : nh); h++) {
+        const int desc = d0 + h;
+        const double *x = X + (size_t)desc * d, *gg = G + (size_t)desc * d;
+        const double *cg = cbuf + (size_t)desc * KLM;
+        const double dot = csc[2 * h], inv = csc[2 * h + 1];
+        double *dcL = dcs_all + h * KLM;
+        for (int l = 0; l <= L; l++) {
+            const int W = 2 * l + 1;
+            const double il = 1.0 / sqrt(2.0 * l + 1.0);
+            double *Pl = stg, *Cl = stg + K * K;
+            for (int e = lane; e < K * K; e += 64) {
+                const int ia = e / K, jb = e - ia * K;
+                const int hi = ia > jb ? ia : jb, lo = ia > jb ? jb : ia;
+                const int q = (hi * (hi + 1) / 2 + lo) * (L + 1) + l;
+                Pl[e] = (gg[q] - x[q] * dot) * inv * (ia == jb ? 2.0 : 1.4142135623730950488) * il;
+            }
+            for (int e = lane; e < K * W; e += 64) {
+                const int jb = e / W, m = e - jb * W;
+                Cl[e] = cg[jb * LM + l * l + m];
+            }
+            __syncthreads();
+            for (int e = lane; e < K * W; e += 64) {
+                const int ia = e / W, m = e - ia * W;
+                double acc = 0.0;
+                for (int jb = 0; jb < K; jb++) acc += Pl[ia * K + jb] * Cl[jb * W + m];
+                dcL[ia * LM + l * l + m] = acc;
+            }
+            __syncthreads();
+        }
+    }
     // dc -> dc' in place, one (centre, species, lm) column per lane
     for (int it = lane; it < nh * S * LM; it += 64) {
         const int h = it / (S * LM), rem = it - h * S * LM, si = rem / LM, lm = rem - si * LM;
-        double *col = cen + h * per + KLM + si * N * LM + lm;
+        double *col = dcs_all + h * KLM + si * N * LM + lm;
         RELOAD_BARRIER();   // T is re-read from LDS for every column
         double v[N];
 #pragma unroll
@@ -340,13 +424,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
     const int h = lane / lanes_per, sub = lane - h * lanes_per;
     if (h >= nh) return;
     const int ci = centres[d0 + h];
-    const double *dcP = cen + h * per + KLM;
-    // this lane's two Legendre rows: whole rows in the centre's c and dE/dp
-    // regions, which are no longer needed, or in a region of their own
-    constexpr int PR = 2 * (L + 2);
-    const int in_c = KLM / PR;
-    double *Pq = pq_own ? cen + cpb * per + lane * PR
-                        : (sub < in_c ? cen + h * per + sub * PR : cen + h * per + 2 * KLM + (sub - in_c) * PR);
+    const double *dcP = dcs_all + h * KLM;
+    double *Pq = stg + lane * PR;   // this lane's two Legendre rows
     const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
     const double al2 = 2.0 * sp.alpha;
     for (int k = sub; k < np; k += lanes_per) {
@@ -359,7 +438,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
         const double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
         const double *dcs = dcP + s * N * LM;
         Rad<N> rs;
-        rs.init(rbs, sp.alpha, r);
+        rs.init(rbs, gA, iA, sp.alpha, r);
         double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll 1
         for (int l = 0; l <= L; l++) {
@@ -429,7 +508,7 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     // neighbour chunk of 32: covers a centre's neighbours within 5 A in one pass
     // for covalent solids and keeps the block's LDS small
     const int NB = kSoapChunk;
-    const size_t sh = (size_t)(sp.n * sp.n + sp.n + KLM + NB * (sp.S * sp.n + 2 * sp.L + 1) + NB * 2 * (sp.L + 2)) * 8;
+    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + NB * (sp.S * sp.n + 2 * sp.L + 1) + NB * 2 * (sp.L + 2)) * 8;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
         hipLaunchKernelGGL((k_soap_fwd_x<NN, LL>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j,     \
@@ -449,16 +528,19 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     if (nc <= 0) return true;
     const int LM = (sp.L + 1) * (sp.L + 1);
     const int KLM = sp.K * LM, PR = 2 * (sp.L + 2);
-    const size_t one = (size_t)(2 * KLM + sp.dim) * 8;
-    const int cpb = (sp.n * sp.n + sp.n) * 8 + 2 * one <= 48 * 1024 ? 2 : 1;   // two centres per wave when LDS allows
-    // Legendre rows of the 64/cpb lanes of a centre fit in its dead c and dE/dp regions?
-    const int lanes_per = 64 / cpb, in_c = KLM / PR;
-    const int pq_own = (lanes_per > in_c && (lanes_per - in_c) * PR > sp.dim) ? 1 : 0;
-    const size_t sh = (size_t)(sp.n * sp.n + sp.n) * 8 + cpb * one + (pq_own ? (size_t)64 * PR * 8 : 0);
+    const size_t stg0 = std::max((size_t)sp.K * sp.K + (size_t)sp.K * (2 * sp.L + 1), (size_t)64 * PR);
+    // whole: c and dE/dp of all centres of the block staged at once (small descriptors)
+    auto stg = [&](int cpb, int whole) { return whole ? std::max(stg0, (size_t)cpb * (KLM + sp.dim)) : stg0; };
+    auto need = [&](int cpb, int whole) {
+        return ((size_t)sp.n * sp.n + 3 * sp.n + (size_t)cpb * (KLM + 2) + stg(cpb, whole)) * 8;
+    };
+    const int whole = need(2, 1) <= 24 * 1024 ? 1 : 0;
+    const int cpb = need(2, whole) <= 24 * 1024 ? 2 : 1;   // two centres per wave when LDS stays small
+    const size_t sh = need(cpb, whole);
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
         hipLaunchKernelGGL((k_soap_bwd_x<NN, LL>), dim3((nc + cpb - 1) / cpb), dim3(64), sh, s, sp, nc, cpb,     \
-                           centres, Z, nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, pq_own);              \
+                           centres, Z, nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, whole);               \
         return true;                                                                                            \
     }
     SOAP_FAST(LAUNCH)
