@@ -170,8 +170,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
     double *rbs = Ts + N * N;      // N grid points, then exp(-alpha r_a^2) and 1/a (N each)
     double *gA = rbs + N, *iA = gA + N;
     double *cP = rbs + 3 * N;      // K*LM: c' during accumulation, c after the transform
-    double *Rl = cP + KLM;         // S x NB x N: fc F_l(a) of neighbour k, zero unless k has species s
-    double *Yl = Rl + S * NB * N;  // NB x YW
+    // Per-lane rows use odd strides (in doubles): 32 lanes of an 8-byte access then
+    // fall on distinct bank pairs.
+    constexpr int RW = N | 1;      // row stride of Rl
+    constexpr int PR = (2 * (L + 2)) | 1;   // row stride of a lane's two Legendre rows
+    double *Rl = cP + KLM;         // S x NB x RW: fc F_l(a) of neighbour k, zero unless k has species s
+    double *Yl = Rl + S * NB * RW; // NB x YW
     const int lane = threadIdx.x;
     const int desc = blockIdx.x;
     const int ci = centres[desc];
@@ -201,14 +205,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
                 ux = nb_d[3 * p] * ir; uy = nb_d[3 * p + 1] * ir; uz = nb_d[3 * p + 2] * ir;
             }
         }
-        double *Pq = Yl + NB * YW + lane * 2 * (L + 2);   // this lane's two Legendre rows
+        double *Pq = Yl + NB * YW + lane * PR;   // this lane's two Legendre rows
 #pragma unroll 1
         for (int l = 0; l <= L; l++) {
             __syncthreads();   // the previous level's reads of Rl / Yl are done
             if (lane < NB) {
                 for (int si = 0; si < S; si++)
 #pragma unroll
-                    for (int a = 0; a < N; a++) Rl[(si * NB + lane) * N + a] = (si == s) ? fc * rs.F0[a] : 0.0;
+                    for (int a = 0; a < N; a++) Rl[(si * NB + lane) * RW + a] = (si == s) ? fc * rs.F0[a] : 0.0;
             }
             if (s >= 0) {
                 rs.step(l);
@@ -237,12 +241,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
             for (int it = lane; it < N * W; it += 64) {
                 const int a = it / W, mi = it - a * W;
                 for (int si = 0; si < S; si++) {
-                    const double *Rs = Rl + si * NB * N + a;
+                    const double *Rs = Rl + si * NB * RW + a;
                     double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll 4
                     for (int k = 0; k < NB; k += 2) {
-                        acc0 += Rs[k * N] * Yl[k * YW + mi];
-                        acc1 += Rs[(k + 1) * N] * Yl[(k + 1) * YW + mi];
+                        acc0 += Rs[k * RW] * Yl[k * YW + mi];
+                        acc1 += Rs[(k + 1) * RW] * Yl[(k + 1) * YW + mi];
                     }
                     cP[(si * N + a) * LM + l * l + mi] += acc0 + acc1;
                 }
@@ -315,7 +319,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
     // dE/dc product and then for the lanes' Legendre rows.
     extern __shared__ double lds[];
     constexpr int LM = (L + 1) * (L + 1);
-    constexpr int PR = 2 * (L + 2);
+    constexpr int PR = (2 * (L + 2)) | 1;   // odd: a lane's rows on distinct banks
     const int K = sp.K, KLM = K * LM, d = sp.dim, S = sp.S;
     double *Ts = lds;              // N*N
     double *rbs = Ts + N * N;      // N grid points, then exp(-alpha r_a^2) and 1/a (N each)
@@ -508,7 +512,8 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     // neighbour chunk of 32: covers a centre's neighbours within 5 A in one pass
     // for covalent solids and keeps the block's LDS small
     const int NB = kSoapChunk;
-    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + NB * (sp.S * sp.n + 2 * sp.L + 1) + NB * 2 * (sp.L + 2)) * 8;
+    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + NB * (sp.S * (sp.n | 1) + 2 * sp.L + 1) +
+                               NB * ((2 * (sp.L + 2)) | 1)) * 8;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
         hipLaunchKernelGGL((k_soap_fwd_x<NN, LL>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j,     \
@@ -527,7 +532,7 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     ensure_soap_constants();
     if (nc <= 0) return true;
     const int LM = (sp.L + 1) * (sp.L + 1);
-    const int KLM = sp.K * LM, PR = 2 * (sp.L + 2);
+    const int KLM = sp.K * LM, PR = (2 * (sp.L + 2)) | 1;
     const size_t stg0 = std::max((size_t)sp.K * sp.K + (size_t)sp.K * (2 * sp.L + 1), (size_t)64 * PR);
     // whole: c and dE/dp of all centres of the block staged at once (small descriptors)
     auto stg = [&](int cpb, int whole) { return whole ? std::max(stg0, (size_t)cpb * (KLM + sp.dim)) : stg0; };
