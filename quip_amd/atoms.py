@@ -67,10 +67,14 @@ def _parse_kv(line):
 def read_xyz(path, index=None):
     """Read an extended-XYZ file (the format QUIP's xyz.c / CInOutput.f95 read).
 
-    Returns a list of Atoms (or one Atoms when ``index`` is an int)."""
+    ``path`` is a file name or an open text stream.  Returns a list of Atoms
+    (or one Atoms when ``index`` is an int)."""
     frames = []
-    with open(path) as fh:
-        lines = fh.read().splitlines()
+    if hasattr(path, "read"):
+        lines = path.read().splitlines()
+    else:
+        with open(path) as fh:
+            lines = fh.read().splitlines()
     i = 0
     while i < len(lines):
         if not lines[i].strip():
@@ -119,23 +123,42 @@ def read_xyz(path, index=None):
     return frames
 
 
-def write_xyz(path, frames, mode="w"):
-    """Write frames as extended XYZ with species, pos and any float per-atom arrays."""
+def _info_value(v):
+    """extxyz value: arrays as quoted numbers in column-major order (QUIP's reshape(m, (/n/)))."""
+    a = np.asarray(v)
+    if a.ndim == 0:
+        return str(v)
+    if a.dtype.kind == "b":
+        return '"%s"' % " ".join("T" if x else "F" for x in a.ravel(order="F"))
+    return '"%s"' % " ".join("%.16g" % x for x in a.ravel(order="F"))
+
+
+def write_xyz(path, frames, mode="w", real_format="%18.10f"):
+    """Write frames as extended XYZ with species, pos and any float per-atom arrays.
+    `path` is a file name or an open text stream; `real_format` formats per-atom reals."""
     if isinstance(frames, Atoms):
         frames = [frames]
+    if hasattr(path, "write"):
+        _write_frames(path, frames, real_format)
+        return
     with open(path, mode) as fh:
-        for at in frames:
-            extra = [(k, v) for k, v in at.arrays.items() if np.asarray(v).dtype.kind == "f"]
-            props = "species:S:1:pos:R:3" + "".join(
-                ":%s:R:%d" % (k, 1 if np.ndim(v) == 1 else np.shape(v)[1]) for k, v in extra)
-            lat = " ".join("%.10f" % x for x in at.cell.ravel())
-            info = " ".join("%s=%s" % (k, v) for k, v in at.info.items() if not isinstance(v, str) or " " not in v)
-            fh.write("%d\n" % len(at))
-            fh.write('Lattice="%s" Properties=%s %s pbc="%s"\n' % (
-                lat, props, info, " ".join("T" if p else "F" for p in at.pbc)))
-            for a in range(len(at)):
-                row = "%-2s %18.10f %18.10f %18.10f" % ((_SYMBOLS[at.numbers[a]],) + tuple(at.positions[a]))
-                for k, v in extra:
-                    vv = np.atleast_1d(np.asarray(v)[a])
-                    row += "".join(" %18.10f" % x for x in vv)
-                fh.write(row + "\n")
+        _write_frames(fh, frames, real_format)
+
+
+def _write_frames(fh, frames, rf):
+    for at in frames:
+        extra = [(k, v) for k, v in at.arrays.items() if np.asarray(v).dtype.kind == "f"]
+        props = "species:S:1:pos:R:3" + "".join(
+            ":%s:R:%d" % (k, 1 if np.ndim(v) == 1 else np.shape(v)[1]) for k, v in extra)
+        lat = " ".join("%.10f" % x for x in at.cell.ravel())
+        info = " ".join("%s=%s" % (k, _info_value(v)) for k, v in at.info.items()
+                        if not isinstance(v, str) or " " not in v)
+        fh.write("%d\n" % len(at))
+        fh.write('Lattice="%s" Properties=%s %s pbc="%s"\n' % (
+            lat, props, info, " ".join("T" if p else "F" for p in at.pbc)))
+        for a in range(len(at)):
+            row = "%-2s" % _SYMBOLS[at.numbers[a]] + "".join(" " + rf % x for x in at.positions[a])
+            for k, v in extra:
+                vv = np.atleast_1d(np.asarray(v)[a])
+                row += "".join(" " + rf % x for x in vv)
+            fh.write(row + "\n")

@@ -289,6 +289,36 @@ def test_skin_reuse(models):
     assert pot.nl_rebuilt()
 
 
+def test_quip_cli_trajectory(models, tmp_path):
+    """The quip-style front end over a three-frame trajectory (E F V L with the
+    default 0.5 A skin): every printed energy and every AT frame's forces and
+    local energies match the oracle."""
+    import io
+    from quip_amd import quip as Q
+    from quip_amd.atoms import write_xyz
+    m, path = models["si64"]
+    rng = np.random.default_rng(31)
+    frames = [configs.si64(3)]
+    for k in range(2):
+        nxt = frames[-1].copy()
+        nxt.positions = nxt.positions + rng.normal(0.0, 0.01, nxt.positions.shape)
+        frames.append(nxt)
+    src = str(tmp_path / "traj.xyz")
+    write_xyz(src, frames)
+    out = io.StringIO()
+    Q.run(["atoms_filename=" + src, "param_filename=" + path, "E", "F", "V", "L"], stdout=out)
+    text = out.getvalue().splitlines()
+    energies = [float(ln.split("=")[1]) for ln in text if ln.startswith("Energy=")]
+    body = "\n".join(ln[3:] for ln in text if ln.startswith("AT "))
+    back = read_xyz(io.StringIO(body))
+    assert len(energies) == 3 and len(back) == 3
+    for at, e, b in zip(frames, energies, back):
+        ro = O.gap_calc(m, at)
+        assert abs(e - ro["energy"]) / len(at) < E_TOL
+        assert np.abs(b.arrays["force"] - ro["forces"]).max() < 1e-7      # printed with %16.8f
+        assert np.abs(b.arrays["local_energy"] - ro["local_energy"]).max() < 1e-7
+
+
 def _eq(a, b):
     assert a["energy"] == b["energy"]
     assert np.array_equal(a["force"], b["force"])

@@ -57,3 +57,43 @@ def test_workload_generators_are_seeded():
     assert len(s) == 32768 and (s.numbers == 14).sum() == 16384
     frames = configs.batch512(n_struct=16)
     assert all(8 <= len(f) <= 512 for f in frames)
+
+
+def test_quip_cli_arguments_and_report():
+    """The quip-style front end: argument grammar, calc_args it builds and the
+    printed report (quip.f95:144-234, 698-750, 822) for a frame whose results
+    are already on it (no evaluation here)."""
+    import io
+    from quip_amd import quip as Q
+    from quip_amd.atoms import read_xyz
+    opt = Q.parse_cli(["atoms_filename=a.xyz", "param_filename=gp.xml", "E", "forces", "V=T", "L",
+                       "calc_args=add_noise=F", "cutoff_skin=0.3"])
+    assert opt["E"] and opt["F"] and opt["V"] and opt["L"]
+    assert opt["atoms_filename"] == "a.xyz" and opt["cutoff_skin"] == "0.3"
+    assert Q.calc_args_for(opt) == "add_noise=F energy force virial local_energy local_virial"
+    assert Q.calc_args_for(Q.parse_cli(["F"])) == "force"
+    with pytest.raises(SystemExit):
+        Q.parse_cli(["bogus_key=1"])
+    at = configs.diamond(5.43, (1, 1, 1))
+    at.info["energy"] = -42.5
+    V = np.array([[1.0, 0.5, 0.0], [0.5, 2.0, 0.0], [0.0, 0.0, 3.0]])
+    at.info["virial"] = V
+    at.arrays["force"] = np.arange(24.0).reshape(8, 3)
+    out = io.StringIO()
+    Q.report(at, opt, out)
+    Q.write_xyz(Q._Prefixed(out, "AT"), at, real_format=opt["real_format"])
+    lines = out.getvalue().splitlines()
+    vol = 5.43 ** 3
+    assert lines[0] == "Energy=-42.5"
+    assert [float(x) for x in lines[1].split()[1:]] == [1.0, 0.5, 0.0]
+    p = lines[4].split()
+    assert p[:2] == ["Pressure", "eV/A^3"] and abs(float(p[2]) - 1.0 / vol) < 1e-15
+    assert abs(float(p[6]) - 1.0 / vol * 160.21766339999996) < 1e-9
+    assert lines[7].startswith("Cell Volume:") and abs(float(lines[7].split()[2]) - vol) < 1e-9
+    at_lines = lines[8:]
+    assert all(ln.startswith("AT ") for ln in at_lines) and len(at_lines) == 10
+    back = read_xyz(io.StringIO("\n".join(ln[3:] for ln in at_lines)), 0)
+    assert np.allclose(back.positions, at.positions, atol=1e-8)
+    assert np.allclose(back.arrays["force"], at.arrays["force"])
+    assert back.info["Energy"] == -42.5
+    assert np.allclose(np.array(back.info["virial"].split(), float).reshape(3, 3, order="F"), V)
