@@ -223,6 +223,205 @@ __global__ __launch_bounds__(256) void k_gemm2_split(int nc, int d, int Mpad, co
 }
 
 // ---------------------------------------------------------------------------
+// LDS-tiled split GEMMs for large problems (descriptors beyond the fused
+// kernel's range, many centres).
+//
+// Block tile 128 x 128 of C[row, col] = sum_k A[row, k] B[col, k] (A, B hi/lo,
+// k-contiguous, leading dimension ldk, a multiple of 32).  256 threads = 2 x 2
+// wavefronts of 64 x 64 (4 x 4 MFMA tiles).  Per k step of 32 the block's
+// 128 x 32 slices of A and B (hi and lo) are written to LDS from registers
+// fetched during the previous step, so every 16-byte fragment read from LDS
+// feeds 12 MFMAs.  The three passes per k step run in the order of mma_tile
+// (lo*hi, hi*lo, hi*hi), so an element's FP32 result is bitwise the same as
+// from the 64 x 64 kernels: a row's result does not depend on which kernel the
+// problem size selects.  Tiles are renumbered so that each XCD (blocks are
+// dealt to the 8 XCDs round robin) works on a contiguous range of tiles that
+// share their A rows in its L2.
+// ---------------------------------------------------------------------------
+constexpr int kLT = 128, kLLD = 40;   // tile edge; LDS row stride in elements (16-byte reads conflict-free)
+
+__device__ __forceinline__ void lds_tile_coords(int ntn, int &tm, int &tn) {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int tile = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    tm = tile / ntn;
+    tn = tile - tm * ntn;
+}
+
+template <int PREC>
+__device__ __forceinline__ void lds_gemm_main(const typename Half<PREC>::T *__restrict__ Ahi,
+                                              const typename Half<PREC>::T *__restrict__ Alo,
+                                              const typename Half<PREC>::T *__restrict__ Bhi,
+                                              const typename Half<PREC>::T *__restrict__ Blo, int ldk, int m0,
+                                              int amax, int n0, int bmax, typename Half<PREC>::T *As,
+                                              typename Half<PREC>::T *Bs, float4_t acc[4][4]) {
+    typedef Half<PREC> H;
+    typedef typename H::V V;
+    constexpr int TS = kLT * kLLD;   // elements of one staged slice
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
+    const int r16 = lane & 15, kg = (lane >> 4) * 8;
+    // thread t stages 16-byte vectors t and t + 256 of each 128 x 32 slice (row = idx / 4)
+    V pa[2][2], pb[2][2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int idx = t + 256 * u, row = idx >> 2, c8 = (idx & 3) * 8;
+            const int ar = m0 + row, br = n0 + row;
+            pa[u][0] = pa[u][1] = pb[u][0] = pb[u][1] = (V){};
+            if (ar < amax) {
+                pa[u][0] = *(const V *)(Ahi + (size_t)ar * ldk + k0 + c8);
+                pa[u][1] = *(const V *)(Alo + (size_t)ar * ldk + k0 + c8);
+            }
+            if (br < bmax) {
+                pb[u][0] = *(const V *)(Bhi + (size_t)br * ldk + k0 + c8);
+                pb[u][1] = *(const V *)(Blo + (size_t)br * ldk + k0 + c8);
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    fetch(0);
+    for (int k0 = 0; k0 < ldk; k0 += 32) {
+        __syncthreads();   // the previous step's LDS reads are done
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int idx = t + 256 * u, o = (idx >> 2) * kLLD + (idx & 3) * 8;
+            *(V *)(As + o) = pa[u][0];
+            *(V *)(As + TS + o) = pa[u][1];
+            *(V *)(Bs + o) = pb[u][0];
+            *(V *)(Bs + TS + o) = pb[u][1];
+        }
+        __syncthreads();
+        if (k0 + 32 < ldk) fetch(k0 + 32);
+        V ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = (wm * 64 + 16 * i + r16) * kLLD + kg;
+            ah[i] = *(const V *)(As + o);
+            al[i] = *(const V *)(As + TS + o);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int o = (wn * 64 + 16 * j + r16) * kLLD + kg;
+            const V bh = *(const V *)(Bs + o), bl = *(const V *)(Bs + TS + o);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                acc[i][j] = H::mfma(al[i], bh, acc[i][j]);
+                acc[i][j] = H::mfma(ah[i], bl, acc[i][j]);
+                acc[i][j] = H::mfma(ah[i], bh, acc[i][j]);
+            }
+        }
+    }
+}
+
+// GEMM1 on 128 x 128 tiles: K = X Xs^T with the epilogue of k_gemm1_split
+// (e_part has one column per tile column, ntn in all).  INTZ: zeta is a small
+// integer (powers by multiplication); otherwise pow().
+template <int PREC, bool INTZ>
+__global__ __launch_bounds__(256) void k_gemm1_split_lds(int nc, int M, int dpad, int Mpad, int ntn,
+                                                          const typename Half<PREC>::T *Xhi,
+                                                          const typename Half<PREC>::T *Xlo,
+                                                          const typename Half<PREC>::T *Shi,
+                                                          const typename Half<PREC>::T *Slo, const double *wvec,
+                                                          double zeta, int izeta, double wscale,
+                                                          typename Half<PREC>::T *Whi, typename Half<PREC>::T *Wlo,
+                                                          double *e_part) {
+    typedef Half<PREC> H;
+    __shared__ __attribute__((aligned(16))) typename H::T As[2 * kLT * kLLD];
+    __shared__ __attribute__((aligned(16))) typename H::T Bs[2 * kLT * kLLD];
+    __shared__ double red[2][kLT];
+    int tm, tn;
+    lds_tile_coords(ntn, tm, tn);
+    const int m0 = tm * kLT, n0 = tn * kLT;
+    float4_t acc[4][4];
+    lds_gemm_main<PREC>(Xhi, Xlo, Shi, Slo, dpad, m0, nc, n0, M, As, Bs, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+    double wv[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        wv[j] = col < M ? wvec[col] : 0.0;
+    }
+    // one row tile at a time keeps the FP64 row sums of four rows live
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double rowsum[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
+                const double kv = (double)acc[i][j][v];
+                double kz1, kz;
+                if constexpr (INTZ) {
+                    kz1 = 1.0;
+                    for (int q = 1; q < izeta; q++) kz1 *= kv;
+                    kz = kz1 * kv;
+                } else {
+                    kz1 = pow(kv, zeta - 1.0);
+                    kz = pow(kv, zeta);
+                }
+                rowsum[v] += wv[j] * kz;
+                if (row < nc && col < Mpad) {
+                    const double wk = col < M ? wv[j] * zeta * kz1 * wscale : 0.0;
+                    typename H::T h = H::from(wk);
+                    Whi[(size_t)row * Mpad + col] = h;
+                    Wlo[(size_t)row * Mpad + col] = H::from(wk - H::to(h));
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double x = rowsum[v];
+            for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if ((lane & 15) == 0) red[wn][wm * 64 + i * 16 + 4 * (lane >> 4) + v] = x;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kLT) {
+        const int row = m0 + threadIdx.x;
+        if (row < nc) e_part[(size_t)row * ntn + tn] = red[0][threadIdx.x] + red[1][threadIdx.x];
+    }
+}
+
+// GEMM2 on 128 x 128 tiles: G = W Xs (FP64 output, unscaled)
+template <int PREC>
+__global__ __launch_bounds__(256) void k_gemm2_split_lds(int nc, int d, int Mpad, int ntn,
+                                                          const typename Half<PREC>::T *Whi,
+                                                          const typename Half<PREC>::T *Wlo,
+                                                          const typename Half<PREC>::T *Thi,
+                                                          const typename Half<PREC>::T *Tlo, double inv_wscale,
+                                                          double *G) {
+    typedef Half<PREC> H;
+    __shared__ __attribute__((aligned(16))) typename H::T As[2 * kLT * kLLD];
+    __shared__ __attribute__((aligned(16))) typename H::T Bs[2 * kLT * kLLD];
+    int tm, tn;
+    lds_tile_coords(ntn, tm, tn);
+    const int m0 = tm * kLT, n0 = tn * kLT;
+    float4_t acc[4][4];
+    lds_gemm_main<PREC>(Whi, Wlo, Thi, Tlo, Mpad, m0, nc, n0, d, As, Bs, acc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
+                if (row < nc && col < d) G[(size_t)row * d + col] = (double)acc[i][j][v] * inv_wscale;
+            }
+    }
+}
+
+// 128 x 128 tiles when they alone fill the GPU (two per CU); 64 x 64 tiles otherwise
+static bool use_lds_tiles(int rows, int cols) {
+    return (long)cdivm(rows, kLT) * cdivm(cols, kLT) >= 512;
+}
+
+// ---------------------------------------------------------------------------
 // Fused GP kernel for descriptors of up to 32*DK components (split forms).
 //
 // One block = 8 wavefronts = 128 centres; wave w owns centres 16w..16w+15 and
@@ -444,7 +643,9 @@ void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *
                         double *e_part, int *ntile_out, void *xsplit_ws, hipStream_t s) {
     const int Mpad = (M + 31) / 32 * 32;
     dim3 grid(cdivm(nc, 64), cdivm(M, 64));
-    *ntile_out = grid.y;
+    const bool lds = use_lds_tiles(nc, M);
+    const int ntn = cdivm(M, kLT), nblk = cdivm(nc, kLT) * ntn;
+    *ntile_out = lds ? ntn : (int)grid.y;
     if (nc <= 0) return;
     const size_t xsz = (size_t)((nc + 63) / 64 * 64) * dpad;
     char *xs = (char *)xsplit_ws;
@@ -452,7 +653,16 @@ void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *
     launch_split_matrix(prec, nc, d, dpad, X, d, false, Xhi, Xlo, s);
     char *wf = (char *)Wf;
     void *Whi = wf, *Wlo = wf + (size_t)nc * Mpad * 2;
-    if (prec == 1)
+    const int iz = int_zeta_m(zeta);
+#define G1L(P, T, IZ)                                                                                            \
+    hipLaunchKernelGGL((k_gemm1_split_lds<P, IZ>), dim3(nblk), dim3(256), 0, s, nc, M, dpad, Mpad, ntn,          \
+                       (const T *)Xhi, (const T *)Xlo, (const T *)Xs_hi, (const T *)Xs_lo, wvec, zeta, iz, wscale, \
+                       (T *)Whi, (T *)Wlo, e_part)
+    if (lds && prec == 1) {
+        if (iz > 0) G1L(1, __bf16, true); else G1L(1, __bf16, false);
+    } else if (lds) {
+        if (iz > 0) G1L(2, _Float16, true); else G1L(2, _Float16, false);
+    } else if (prec == 1)
         hipLaunchKernelGGL(k_gemm1_split<1>, grid, dim3(256), 0, s, nc, M, dpad, Mpad, (const __bf16 *)Xhi,
                            (const __bf16 *)Xlo, (const __bf16 *)Xs_hi, (const __bf16 *)Xs_lo, wvec, zeta,
                            int_zeta_m(zeta), wscale, (__bf16 *)Whi, (__bf16 *)Wlo, e_part);
@@ -460,6 +670,7 @@ void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *
         hipLaunchKernelGGL(k_gemm1_split<2>, grid, dim3(256), 0, s, nc, M, dpad, Mpad, (const _Float16 *)Xhi,
                            (const _Float16 *)Xlo, (const _Float16 *)Xs_hi, (const _Float16 *)Xs_lo, wvec, zeta,
                            int_zeta_m(zeta), wscale, (_Float16 *)Whi, (_Float16 *)Wlo, e_part);
+#undef G1L
 }
 
 bool gp_fused_supported(int dpad) { return dpad <= 256; }
@@ -519,6 +730,18 @@ void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, cons
     const char *wf = (const char *)Wf;
     const void *Whi = wf, *Wlo = wf + (size_t)nc * Mpad * 2;
     const double inv = 1.0 / wscale;
+    if (use_lds_tiles(nc, d)) {
+        const int ntn = cdivm(d, kLT), nblk = cdivm(nc, kLT) * ntn;
+        if (prec == 1)
+            hipLaunchKernelGGL(k_gemm2_split_lds<1>, dim3(nblk), dim3(256), 0, s, nc, d, Mpad, ntn,
+                               (const __bf16 *)Whi, (const __bf16 *)Wlo, (const __bf16 *)XsT_hi,
+                               (const __bf16 *)XsT_lo, inv, G);
+        else
+            hipLaunchKernelGGL(k_gemm2_split_lds<2>, dim3(nblk), dim3(256), 0, s, nc, d, Mpad, ntn,
+                               (const _Float16 *)Whi, (const _Float16 *)Wlo, (const _Float16 *)XsT_hi,
+                               (const _Float16 *)XsT_lo, inv, G);
+        return;
+    }
     if (prec == 1)
         hipLaunchKernelGGL(k_gemm2_split<1>, grid, dim3(256), 0, s, nc, d, Mpad, (const __bf16 *)Whi,
                            (const __bf16 *)Wlo, (const __bf16 *)XsT_hi, (const __bf16 *)XsT_lo, inv, G);

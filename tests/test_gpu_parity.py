@@ -262,6 +262,25 @@ def test_sic32768_precision_forms(sic_model):
         assert de < 1e-3 and fr < 1e-2
 
 
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+def test_split_gemm_tile_shapes_agree(sic_model, prec):
+    """A 64-atom SiC cell alone (its split GEMMs run on 64 x 64 tiles) and batched with
+    an 8192-atom one (the batch's GEMMs run on 128 x 128 LDS tiles): both kernels issue
+    the MFMA passes of an element in the same order, so the small cell's results agree
+    to the rounding of the FP64 sums."""
+    m, path = sic_model
+    small = configs.antisite_swaps(configs.perturb(configs.diamond(4.36, (2, 2, 2), species=(14, 6)), 0.1,
+                                                   configs.rng_for(2, 41)), 3, configs.rng_for(2, 42))
+    big = configs.perturb(configs.diamond(4.36, (8, 8, 16), species=(14, 6)), 0.1, configs.rng_for(2, 43))
+    pot = Potential(param_filename=path, precision=prec)
+    alone = pot.calc(small.copy(), force=True, virial=True)
+    rb = pot.calc_batch([small, big])[0]
+    print("tile shapes", prec, abs(rb["energy"] - alone["energy"]), np.abs(rb["force"] - alone["force"]).max())
+    assert abs(rb["energy"] - alone["energy"]) / len(small) < 1e-12
+    assert np.abs(rb["force"] - alone["force"]).max() < 1e-11
+    assert np.abs(rb["virial"] - alone["virial"]).max() < 1e-10
+
+
 def test_skin_reuse(models):
     """calc_connect with a skin: small moves keep the list (distances recomputed as
     calc_dists does) and match the oracle's fresh build; a move of half the skin
