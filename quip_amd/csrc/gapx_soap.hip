@@ -154,8 +154,9 @@ struct Rad {
 // values of grid point a), c[b][lm] = sum_a T[a][b] c'[a][lm] where
 // c'[a][lm] = sum_k fc F_l(a, r_k) Y_lm(k) is accumulated in the primitive
 // (Gaussian) radial basis and transformed once per centre.
-// Per neighbour chunk and per l: lanes = neighbours stage fc F_l(a) and Y_l,m
-// in LDS; lanes = (a, m) accumulate c' for each species.
+// Per neighbour chunk: the chunk's neighbours are listed per species (wave
+// ballot); per l, lanes = neighbours stage fc F_l(a) and Y_l,m in LDS and
+// lanes = (a, m) accumulate c' of each species over that species' list only.
 template <int N, int L>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))) void k_soap_fwd_x(
     SoapDev sp, const int *__restrict__ centres, const int *__restrict__ Z, const int *__restrict__ nb_off,
@@ -174,8 +175,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
     // fall on distinct bank pairs.
     constexpr int RW = N | 1;      // row stride of Rl
     constexpr int PR = (2 * (L + 2)) | 1;   // row stride of a lane's two Legendre rows
-    double *Rl = cP + KLM;         // S x NB x RW: fc F_l(a) of neighbour k, zero unless k has species s
-    double *Yl = Rl + S * NB * RW; // NB x YW
+    double *Rl = cP + KLM;         // NB x RW: fc F_l(a) of neighbour k
+    double *Yl = Rl + NB * RW;     // NB x YW
+    double *Pq0 = Yl + NB * YW;    // NB x PR: the lanes' Legendre rows
+    int *kidx = (int *)(Pq0 + NB * PR);   // S x NB: the chunk's neighbours of each species
+    int *ns = kidx + S * NB;       // their counts
     const int lane = threadIdx.x;
     const int desc = blockIdx.x;
     const int ci = centres[desc];
@@ -190,6 +194,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
     const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
     for (int c0 = 0; c0 < np; c0 += NB) {
         const int nk = min(NB, np - c0);
+        __syncthreads();   // the previous chunk's reads of the lists are done
         int s = -1;
         double fc = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
         Rad<N> rs;
@@ -205,14 +210,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
                 ux = nb_d[3 * p] * ir; uy = nb_d[3 * p + 1] * ir; uz = nb_d[3 * p + 2] * ir;
             }
         }
-        double *Pq = Yl + NB * YW + lane * PR;   // this lane's two Legendre rows
+        for (int si = 0; S > 1 && si < S; si++) {   // neighbours beyond the cutoff are in no list
+            const bool mine = lane < nk && s == si;
+            const unsigned long long mk = __ballot(mine);
+            if (mine) kidx[si * NB + __popcll(mk & ((1ull << lane) - 1ull))] = lane;
+            if (lane == 0) ns[si] = __popcll(mk);
+        }
+        double *Pq = Pq0 + lane * PR;   // this lane's two Legendre rows
 #pragma unroll 1
         for (int l = 0; l <= L; l++) {
             __syncthreads();   // the previous level's reads of Rl / Yl are done
-            if (lane < NB) {
-                for (int si = 0; si < S; si++)
+            if (lane < NB) {   // rows of neighbours outside every list stay zero for the dense loop
 #pragma unroll
-                    for (int a = 0; a < N; a++) Rl[(si * NB + lane) * RW + a] = (si == s) ? fc * rs.F0[a] : 0.0;
+                for (int a = 0; a < N; a++) Rl[lane * RW + a] = s >= 0 ? fc * rs.F0[a] : 0.0;
             }
             if (s >= 0) {
                 rs.step(l);
@@ -240,13 +250,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
             const int W = 2 * l + 1;
             for (int it = lane; it < N * W; it += 64) {
                 const int a = it / W, mi = it - a * W;
-                for (int si = 0; si < S; si++) {
-                    const double *Rs = Rl + si * NB * RW + a;
+                if (S == 1) {   // one species: every row of the chunk, zero rows included
                     double acc0 = 0.0, acc1 = 0.0;
 #pragma unroll 4
                     for (int k = 0; k < NB; k += 2) {
-                        acc0 += Rs[k * RW] * Yl[k * YW + mi];
-                        acc1 += Rs[(k + 1) * RW] * Yl[(k + 1) * YW + mi];
+                        acc0 += Rl[k * RW + a] * Yl[k * YW + mi];
+                        acc1 += Rl[(k + 1) * RW + a] * Yl[(k + 1) * YW + mi];
+                    }
+                    cP[a * LM + l * l + mi] += acc0 + acc1;
+                    continue;
+                }
+                for (int si = 0; si < S; si++) {
+                    const int n = ns[si];
+                    const int *ki = kidx + si * NB;
+                    double acc0 = 0.0, acc1 = 0.0;
+                    int q = 0;
+#pragma unroll 2
+                    for (; q + 1 < n; q += 2) {
+                        const int k0 = ki[q], k1 = ki[q + 1];
+                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];
+                        acc1 += Rl[k1 * RW + a] * Yl[k1 * YW + mi];
+                    }
+                    if (q < n) {
+                        const int k0 = ki[q];
+                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];
                     }
                     cP[(si * N + a) * LM + l * l + mi] += acc0 + acc1;
                 }
@@ -512,8 +539,8 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     // neighbour chunk of 32: covers a centre's neighbours within 5 A in one pass
     // for covalent solids and keeps the block's LDS small
     const int NB = kSoapChunk;
-    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + NB * (sp.S * (sp.n | 1) + 2 * sp.L + 1) +
-                               NB * ((2 * (sp.L + 2)) | 1)) * 8;
+    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + NB * ((sp.n | 1) + 2 * sp.L + 1) +
+                               NB * ((2 * (sp.L + 2)) | 1)) * 8 + (size_t)sp.S * (NB + 1) * 4;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
         hipLaunchKernelGGL((k_soap_fwd_x<NN, LL>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j,     \
