@@ -113,6 +113,27 @@ int gapx_calc_batch_device(gapx_model *model, int n_struct, const int *atom_off,
                            const double *lattices, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
                            double *d_local_e, double *d_f, double *d_virials, double *d_local_virial);
 
+/* LAMMPS pair style entry.  Replaces quip_lammps_wrapper (quip_lammps_wrapper.f95:30-156)
+ * with the model handle in place of the transfer()-encoded Potential
+ * (quip_potential, n_quip_potential); the other arguments keep their names,
+ * order and meaning.  The n_all = nlocal + nghost atoms are LAMMPS's local atoms
+ * followed by its ghosts (periodic images appear as ghosts, so no shifts are
+ * used); quip_x is 3 x n_all, atomic_numbers n_all.  ilist (0-based, inum
+ * entries) names the local atoms whose full neighbour lists are packed in
+ * quip_neigh (1-based atom numbers, quip_num_neigh[ni] of them for ilist[ni],
+ * sum_num_neigh in all).  As the reference, the connection keeps a pair once,
+ * in neighbour1 of the lower index (:104-144); descriptors are centred on the
+ * ilist atoms only (atom_mask_name=local, :147); forces, local energies and
+ * local virials are returned for all n_all atoms (ghost entries carry the
+ * contributions LAMMPS folds back onto their owners); quip_e and quip_virial
+ * are the sums over all of them.  lattice (3 x 3) is accepted for the
+ * reference's signature and not used (:111-114).  lmptag may be NULL.
+ * Two atoms closer than 1e-10 A are an error, as in the reference. */
+int gapx_calc_lammps(gapx_model *model, int nlocal, int nghost, const int *atomic_numbers, const int *lmptag,
+                     int inum, int sum_num_neigh, const int *ilist, const int *quip_num_neigh,
+                     const int *quip_neigh, const double *lattice, const double *quip_x, double *quip_e,
+                     double *quip_local_e, double *quip_virial, double *quip_local_virial, double *quip_force);
+
 /* Serving loop (continuous batching).  No counterpart in the reference, whose
  * IPModel_GAP_Calc serves one structure per call; each request is evaluated
  * with the physics of gapx_calc.

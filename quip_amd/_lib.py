@@ -38,6 +38,8 @@ _SIGS = [
     ("gapx_model_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("gapx_calc", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                             C.c_uint] + [C.c_void_p] * 5),
+    ("gapx_calc_lammps", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+     + [C.c_void_p] * 10),
     ("gapx_calc_device", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                    C.c_uint] + [C.c_void_p] * 5),
     ("gapx_sync", C.c_int, [C.c_void_p]),

@@ -651,11 +651,14 @@ static bool z_in(const std::vector<int> &v, int z) {
 
 // Upload Z, the atom -> structure map and the structure offsets when they differ
 // from the previous call's; rebuild the per-coordinate centre lists with them.
-static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, const int *Z) {
+// local (nullable): only atoms with local[i] != 0 are descriptor centres
+// (IPModel_GAP's atom_mask_name); the mask is uploaded too and never cached.
+static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, const int *Z,
+                        const unsigned char *local = nullptr) {
     const int N = atom_off[S];
     Workspace &w = eg.ws;
     hipStream_t s = eg.stream;
-    bool same = eg.have_inputs && (int)eg.h_off.size() == S + 1 && (int)eg.h_Z.size() == N &&
+    bool same = !local && eg.have_inputs && (int)eg.h_off.size() == S + 1 && (int)eg.h_Z.size() == N &&
                 std::equal(atom_off, atom_off + S + 1, eg.h_off.begin()) && std::equal(Z, Z + N, eg.h_Z.begin());
     if (same) return GAPX_OK;
     eg.have_inputs = false;
@@ -672,9 +675,10 @@ static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, c
         const Coord &c = *m->coords[k];
         if (c.kind == 1)
             for (int i = 0; i < N; i++)
-                if (z_in(c.centreZ, Z[i])) hc.push_back(i);
+                if (z_in(c.centreZ, Z[i]) && (!local || local[i])) hc.push_back(i);
         tot += hc.size() * 4;
     }
+    if (local) tot += (size_t)N + 8;
     CHK(w.Z.ensure((size_t)N * 4 + 4));
     CHK(w.sid.ensure((size_t)N * 4 + 4));
     CHK(w.atom_off.ensure((size_t)(S + 1) * 4));
@@ -697,7 +701,11 @@ static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, c
         CHK(w.centres[k].ensure(hc.size() * 4 + 4));
         CHK(put(w.centres[k], hc.data(), hc.size() * 4));
     }
-    eg.have_inputs = true;
+    if (local) {
+        CHK(w.cmask.ensure((size_t)N + 8));
+        CHK(put(w.cmask, local, (size_t)N));
+    }
+    eg.have_inputs = !local;   // a masked call's centre lists are not reused
     return GAPX_OK;
 }
 
@@ -1004,9 +1012,60 @@ static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out)
 // Device-resident evaluation of S structures in one pass (IPModel_GAP_Calc per
 // structure).  Outputs: d_e[S], d_v[9S] and the per-atom arrays; any may be NULL,
 // in which case the engine's scratch takes it.
+// The connection of a LAMMPS pair style (quip_lammps_wrapper.f95:104-144):
+// neighbour1/neighbour2 built on the host from LAMMPS's lists, all shifts zero
+// (periodic images are explicit ghost atoms).  The per-atom full view is formed
+// on the GPU by the kernel calc_connect uses.  The list is not kept for skin
+// reuse.
+static int connect_external(Engine &eg, const ExtNL &x, int N, const double *d_pos, const double *lat, int *P_out) {
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    const int n1_total = x.n1_off[N];
+    const size_t T1 = (size_t)n1_total + 1, P = 2 * (size_t)n1_total + 1;
+    eg.nl_valid = false;
+    eg.have_geo = false;
+    NLGeom g;
+    memset(&g, 0, sizeof(NLGeom));
+    for (int k = 0; k < 9; k++) g.lat[k] = lat[k];
+    CHK(w.geo.ensure(sizeof(NLGeom) + 8));
+    CHK(w.n1_off.ensure((size_t)(N + 1) * 4));
+    CHK(w.n2_off.ensure((size_t)(N + 1) * 4));
+    CHK(w.nb_off.ensure((size_t)(N + 1) * 4));
+    CHK(w.n1_j.ensure(T1 * 4));
+    CHK(w.n1_shift.ensure(T1 * 12));
+    CHK(w.n1_r.ensure(T1 * 8));
+    CHK(w.n1_rev.ensure(T1 * 4));
+    CHK(w.n2_is.ensure(T1 * 4));
+    CHK(w.n2_ks.ensure(T1 * 4));
+    CHK(w.nb_j.ensure(P * 4));
+    CHK(w.nb_rev.ensure(P * 4));
+    CHK(w.nb_r.ensure(P * 8));
+    CHK(w.nb_d.ensure(P * 24));
+    // pageable sources, read before this function returns (synchronised below)
+    HIPCHK(hipMemcpyAsync(w.geo.p, &g, sizeof(NLGeom), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.n1_off.p, x.n1_off.data(), (size_t)(N + 1) * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(w.n2_off.p, x.n2_off.data(), (size_t)(N + 1) * 4, hipMemcpyHostToDevice, s));
+    if (n1_total > 0) {
+        HIPCHK(hipMemcpyAsync(w.n1_j.p, x.n1_j.data(), (size_t)n1_total * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(w.n1_r.p, x.n1_r.data(), (size_t)n1_total * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(w.n1_rev.p, x.n1_rev.data(), (size_t)n1_total * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(w.n2_is.p, x.n2_i.data(), (size_t)n1_total * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(w.n2_ks.p, x.n2_k.data(), (size_t)n1_total * 4, hipMemcpyHostToDevice, s));
+        launch_fill_i32(w.n1_shift.as<int>(), 3 * n1_total, 0, s);
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    launch_nl_full(N, d_pos, w.geo.as<NLGeom>(), w.sid.as<int>(), w.n1_off.as<int>(), w.n1_j.as<int>(),
+                   w.n1_shift.as<int>(), w.n1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(),
+                   w.n2_ks.as<int>(), w.n1_rev.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_rev.as<int>(),
+                   w.nb_r.as<double>(), w.nb_d.as<double>(), s);
+    eg.n1_total_last = n1_total;
+    *P_out = (int)(P - 1);
+    return GAPX_OK;
+}
+
 static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
                       const int *Z, const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
-                      double *d_le, double *d_f, double *d_v, double *d_lv) {
+                      double *d_le, double *d_f, double *d_v, double *d_lv, const ExtNL *ext = nullptr) {
     CHK(check_batch(S, atom_off, Z, lats, pbcs));
     const int N = atom_off[S];
     if (N > 0 && !d_pos) return fail(GAPX_E_ARG, "positions are NULL");
@@ -1018,10 +1077,13 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     if (rc < m->cutoff) return fail(GAPX_E_ARG, "nl_cutoff %g is below the model cutoff %g", rc, m->cutoff);
     const bool grad = (want & (GAPX_WANT_F | GAPX_WANT_V | GAPX_WANT_LOCAL_V)) != 0;
     if (tm.on) tm.begin(s);
-    CHK(stage_inputs(m, eg, S, atom_off, Z));
+    if (ext && S != 1) return fail(GAPX_E_ARG, "a supplied connection covers one structure");
+    CHK(stage_inputs(m, eg, S, atom_off, Z, ext ? ext->local.data() : nullptr));
+    const unsigned char *cmask = ext ? w.cmask.as<unsigned char>() : nullptr;
     const int t_nl0 = tm.on ? tm.mark(s) : -1;
     int P = 0;
-    CHK(connect(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, &P));
+    if (ext) CHK(connect_external(eg, *ext, N, d_pos, lats, &P));
+    else CHK(connect(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, &P));
     if (tm.on) tm.stage(0, t_nl0, tm.mark(s));
 
     // Prepare: every buffer the launches use is sized here (to the largest
@@ -1083,7 +1145,7 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
                 const int ta = marks ? tm.mark(s) : -1;
                 launch_d2b(N, w.Z.as<int>(), c.Z1, c.Z2, c.rc, c.ctw, c.M, c.d_Xs.as<double>(), c.d_w.as<double>(),
                            c.d_theta.as<double>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
-                           w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), s);
+                           w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), cmask, s);
                 if (marks) tm.stage(5, ta, tm.mark(s));
                 continue;
             }
@@ -1132,12 +1194,12 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
         const int tg = marks ? tm.mark(s) : -1;
         launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
                       w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>(), o_f, o_le,
-                      o_lv, s);
+                      o_lv, cmask, s);
         launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s);
         if (marks) tm.stage(5, tg, tm.mark(s));
     };
 
-    if (m->graphs && !tm.on) {
+    if (m->graphs && !tm.on && !ext) {
         // replay the captured sequence when nothing it depends on has changed
         std::vector<long long> key = {S, N, P, m->precision, (long long)want, (long long)(uintptr_t)o_e,
                                       (long long)(uintptr_t)o_v, (long long)(uintptr_t)o_f, (long long)(uintptr_t)o_le,
@@ -1204,7 +1266,7 @@ extern "C" int gapx_sync(gapx_model *m) {
 // pinned staging, results come back into the caller's arrays.
 int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *pos, const int *Z,
                      const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *e, double *le,
-                     double *f, double *v, double *lv) {
+                     double *f, double *v, double *lv, const ExtNL *ext) {
     if ((want & GAPX_WANT_E) && !e) return fail(GAPX_E_ARG, "energy requested but e is NULL");
     if ((want & GAPX_WANT_F) && !f) return fail(GAPX_E_ARG, "forces requested but f is NULL");
     if ((want & GAPX_WANT_V) && !v) return fail(GAPX_E_ARG, "virial requested but virial is NULL");
@@ -1229,7 +1291,8 @@ int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     }
     CHK(eval_batch(m, eg, S, atom_off, w.pos.as<double>(), pos, Z, lats, pbcs, nl_cutoff,
                    want | GAPX_WANT_E | GAPX_WANT_V | GAPX_WANT_F | GAPX_WANT_LOCAL_E | GAPX_WANT_LOCAL_V,
-                   w.e.as<double>(), w.le.as<double>(), w.f.as<double>(), w.vir.as<double>(), w.lv.as<double>()));
+                   w.e.as<double>(), w.le.as<double>(), w.f.as<double>(), w.vir.as<double>(), w.lv.as<double>(),
+                   ext));
     char *o = h;
     auto get = [&](bool on, const DBuf &src, size_t bytes, char **dst) -> int {
         *dst = nullptr;
@@ -1260,6 +1323,98 @@ extern "C" int gapx_calc(gapx_model *m, int N, const double *pos, const int *Z, 
     if (N < 0) return fail(GAPX_E_ARG, "n_atoms < 0");
     const int off[2] = {0, N};
     return calc_host(m, *m->eng, 1, off, pos, Z, lat, pbc, nl_cutoff, want, e, le, f, v, lv);
+}
+
+// LAMMPS pair style entry (quip_lammps_wrapper.f95:30-156).  The connection is
+// filled as the reference fills at%connect (:104-144): for each listed local
+// atom i and each LAMMPS neighbour j (1-based in quip_neigh) with i <= j, j is
+// appended to neighbour1(i) with zero shift and r = |x_i - x_j|, and (i, that
+// index) to neighbour2(j); descriptors are centred on the ilist atoms only
+// (atom_mask_name=local, :147), and e0 is added for those atoms only
+// (IPModel_GAP.f95:577-589).
+extern "C" int gapx_calc_lammps(gapx_model *m, int nlocal, int nghost, const int *Zs, const int *lmptag, int inum,
+                                int sum_num_neigh, const int *ilist, const int *num_neigh, const int *neigh,
+                                const double *lattice, const double *x, double *e, double *le, double *vir,
+                                double *lv, double *f) {
+    (void)lmptag;
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    if (nlocal < 0 || nghost < 0 || inum < 0 || sum_num_neigh < 0)
+        return fail(GAPX_E_ARG, "negative count (nlocal %d nghost %d inum %d sum_num_neigh %d)", nlocal, nghost, inum,
+                    sum_num_neigh);
+    const long Nl = (long)nlocal + nghost;
+    if (Nl > INT32_MAX / 16) return fail(GAPX_E_ARG, "too many atoms (%ld)", Nl);
+    const int N = (int)Nl;
+    if (!e || !le || !vir || !lv || !f) return fail(GAPX_E_ARG, "an output array is NULL");
+    if (nlocal == 0) {   // this region is vacuum (:150-155)
+        *e = 0.0;
+        for (int k = 0; k < 9; k++) vir[k] = 0.0;
+        for (long i = 0; i < Nl; i++) {
+            le[i] = 0.0;
+            for (int k = 0; k < 3; k++) f[3 * i + k] = 0.0;
+            for (int k = 0; k < 9; k++) lv[9 * i + k] = 0.0;
+        }
+        return GAPX_OK;
+    }
+    if (!Zs || !x) return fail(GAPX_E_ARG, "atomic_numbers / quip_x are NULL");
+    if (inum > 0 && (!ilist || !num_neigh)) return fail(GAPX_E_ARG, "ilist / quip_num_neigh are NULL");
+    if (sum_num_neigh > 0 && !neigh) return fail(GAPX_E_ARG, "quip_neigh is NULL");
+    ExtNL c;
+    c.local.assign(N, 0);
+    std::vector<int> n1c(N, 0), n2c(N, 0);
+    long nn = 0;
+    for (int ni = 0; ni < inum; ni++) {   // validate and count
+        const int i = ilist[ni];
+        if (i < 0 || i >= N) return fail(GAPX_E_ARG, "ilist[%d] = %d outside [0, %d)", ni, i, N);
+        if (c.local[i]) return fail(GAPX_E_ARG, "atom %d listed twice in ilist", i);
+        c.local[i] = 1;
+        if (num_neigh[ni] < 0) return fail(GAPX_E_ARG, "quip_num_neigh[%d] < 0", ni);
+        for (int n = 0; n < num_neigh[ni]; n++, nn++) {
+            if (nn >= sum_num_neigh) return fail(GAPX_E_ARG, "quip_num_neigh adds up to more than sum_num_neigh");
+            const int j = neigh[nn] - 1;
+            if (j < 0 || j >= N) return fail(GAPX_E_ARG, "quip_neigh[%ld] = %d outside [1, %d]", nn, j + 1, N);
+            if (i <= j) {
+                n1c[i]++;
+                n2c[j]++;
+            }
+        }
+    }
+    c.n1_off.assign(N + 1, 0);
+    c.n2_off.assign(N + 1, 0);
+    for (int i = 0; i < N; i++) {
+        c.n1_off[i + 1] = c.n1_off[i] + n1c[i];
+        c.n2_off[i + 1] = c.n2_off[i] + n2c[i];
+    }
+    const int T = c.n1_off[N];
+    c.n1_j.resize(T);
+    c.n1_r.resize(T);
+    c.n1_rev.resize(T);
+    c.n2_i.resize(T);
+    c.n2_k.resize(T);
+    std::fill(n1c.begin(), n1c.end(), 0);
+    std::fill(n2c.begin(), n2c.end(), 0);
+    nn = 0;
+    for (int ni = 0; ni < inum; ni++) {   // fill in the reference's append order
+        const int i = ilist[ni];
+        for (int n = 0; n < num_neigh[ni]; n++, nn++) {
+            const int j = neigh[nn] - 1;
+            if (i > j) continue;
+            const double dx = x[3 * (size_t)i] - x[3 * (size_t)j], dy = x[3 * (size_t)i + 1] - x[3 * (size_t)j + 1],
+                         dz = x[3 * (size_t)i + 2] - x[3 * (size_t)j + 2];
+            const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (r < 1.0e-10) return fail(GAPX_E_GEOMETRY, "atoms %d and %d overlap exactly", i + 1, j + 1);
+            const int k1 = n1c[i]++, e1 = c.n1_off[i] + k1;
+            c.n1_j[e1] = j;
+            c.n1_r[e1] = r;
+            const int k2 = n2c[j]++;
+            c.n2_i[c.n2_off[j] + k2] = i;
+            c.n2_k[c.n2_off[j] + k2] = k1;
+            c.n1_rev[e1] = k2;
+        }
+    }
+    static const double eye[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const int off[2] = {0, N}, pbc[3] = {0, 0, 0};
+    const unsigned all = GAPX_WANT_E | GAPX_WANT_F | GAPX_WANT_V | GAPX_WANT_LOCAL_E | GAPX_WANT_LOCAL_V;
+    return calc_host(m, *m->eng, 1, off, x, Zs, lattice ? lattice : eye, pbc, 0.0, all, e, le, f, vir, lv, &c);
 }
 
 extern "C" int gapx_calc_batch(gapx_model *m, int n_struct, const int *atom_off, const double *pos, const int *Z,

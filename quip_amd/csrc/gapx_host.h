@@ -115,7 +115,17 @@ struct Workspace {
     DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws;
     DBuf f, le, lv, e, vir, minmax;
     DBuf last_pos, skin_flags;   // positions of the last build (skin reuse), reuse test results
+    DBuf cmask;                  // descriptor-centre mask of a caller-supplied connection
     std::vector<DBuf> centres;   // per coordinate
+};
+
+// A connection handed in by the caller instead of calc_connect (a LAMMPS pair
+// style): QUIP's neighbour1/neighbour2 tables with zero shifts, built on the
+// host, and the atoms that may be descriptor centres.
+struct ExtNL {
+    std::vector<int> n1_off, n1_j, n1_rev, n2_off, n2_i, n2_k;
+    std::vector<double> n1_r;
+    std::vector<unsigned char> local;
 };
 
 // Pinned host staging (hipHostMalloc), grow-only.
@@ -257,9 +267,10 @@ int new_engine(gapx_model *m, std::unique_ptr<Engine> &out);
 int ensure_split(gapx_model *m, Coord &c, int prec, hipStream_t s);
 int check_batch(int S, const int *atom_off, const int *Z, const double *lats, const int *pbcs);
 // Host-buffer evaluation of S structures on engine eg (pinned staging both ways).
+// ext: a caller-supplied connection (one structure), else calc_connect.
 int calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *pos, const int *Z,
               const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *e, double *le, double *f,
-              double *v, double *lv);
+              double *v, double *lv, const ExtNL *ext = nullptr);
 
 }  // namespace gapx
 

@@ -111,12 +111,14 @@ void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double 
                          void *hi, void *lo, hipStream_t s);
 
 void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_part, double *esite, hipStream_t s);
+// mask (nullable): only atoms with mask[i] != 0 are the first atom of a pair descriptor
 void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
                 const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
-                const double *nb_d, double *pairF, double *pairE, hipStream_t s);
+                const double *nb_d, double *pairF, double *pairE, const unsigned char *mask, hipStream_t s);
+// mask (nullable): e0 is added to the local energy of masked atoms only
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
-                   double *local_e, double *local_virial, hipStream_t s);
+                   double *local_e, double *local_virial, const unsigned char *mask, hipStream_t s);
 void launch_struct_reduce(int S, const int *atom_off, const double *local_e, const double *local_virial,
                           double *e_out, double *virial_out, hipStream_t s);
 

@@ -1180,9 +1180,9 @@ void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_p
 // ----------------------------------------------------------------------------
 __global__ void k_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
                       const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
-                      const double *nb_d, double *pairF, double *pairE) {
+                      const double *nb_d, double *pairF, double *pairE, const unsigned char *mask) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i >= N || (mask && !mask[i])) return;
     const bool zi1 = (Z1 == 0) || (Z[i] == Z1), zi2 = (Z2 == 0) || (Z[i] == Z2);
     const double it2 = 1.0 / (theta[0] * theta[0]);
     for (int p = nb_off[i]; p < nb_off[i + 1]; p++) {
@@ -1209,10 +1209,10 @@ __global__ void k_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw
 }
 void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
                 const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
-                const double *nb_d, double *pairF, double *pairE, hipStream_t s) {
+                const double *nb_d, double *pairF, double *pairE, const unsigned char *mask, hipStream_t s) {
     if (N > 0)
         hipLaunchKernelGGL(k_d2b, dim3(cdiv(N, 128)), dim3(128), 0, s, N, Z, Z1, Z2, rc, ctw, M, xs, wvec, theta,
-                           nb_off, nb_j, nb_r, nb_d, pairF, pairE);
+                           nb_off, nb_j, nb_r, nb_d, pairF, pairE, mask);
 }
 
 // ----------------------------------------------------------------------------
@@ -1228,7 +1228,8 @@ __device__ __forceinline__ double wave_sum_g(double v) {
 __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const double *e0, double E_scale,
                                                       const int *nb_off, const int *nb_rev, const double *nb_d,
                                                       const double *pairF, const double *pairE, const double *esite,
-                                                      double *f, double *local_e, double *local_virial) {
+                                                      double *f, double *local_e, double *local_virial,
+                                                      const unsigned char *mask) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int j = blockIdx.x * 4 + w;
     if (j >= N) return;
@@ -1251,7 +1252,7 @@ __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const
 #pragma unroll
     for (int k = 0; k < 13; k++) v[k] = wave_sum_g(v[k]);
     if (lane < 3 && f) f[3 * j + lane] = E_scale * (lane == 0 ? v[0] : (lane == 1 ? v[1] : v[2]));
-    if (lane == 0) local_e[j] = E_scale * (esite[j] + 0.5 * v[12] + e0[Z[j]]);
+    if (lane == 0) local_e[j] = E_scale * (esite[j] + 0.5 * v[12] + ((mask && !mask[j]) ? 0.0 : e0[Z[j]]));
     if (lane < 9) {
         double x = v[3];
 #pragma unroll
@@ -1261,10 +1262,10 @@ __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const
 }
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
-                   double *local_e, double *local_virial, hipStream_t s) {
+                   double *local_e, double *local_virial, const unsigned char *mask, hipStream_t s) {
     if (N > 0)
         hipLaunchKernelGGL(k_gather, dim3(cdiv(N, 4)), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev,
-                           nb_d, pairF, pairE, esite, f, local_e, local_virial);
+                           nb_d, pairF, pairE, esite, f, local_e, local_virial, mask);
 }
 
 // Energy and virial of each structure: E = sum of local energies, V = sum of

@@ -179,6 +179,25 @@ class Potential:
             return np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])
         raise KeyError(name)
 
+    def calc_lammps(self, nlocal, nghost, numbers, ilist, num_neigh, neigh, positions, lattice=None, tags=None):
+        """The LAMMPS pair style call (quip_lammps_wrapper.f95:30-156, gapx_calc_lammps):
+        `numbers` and `positions` (n_all x 3) cover the local atoms followed by the
+        ghosts; `ilist` (0-based) the local atoms whose full neighbour lists are
+        packed in `neigh` (1-based, `num_neigh[k]` for ilist[k]).  Returns energy,
+        local_energy (n_all), virial (3x3), local_virial (n_all x 9) and force
+        (n_all x 3), ghost entries included."""
+        n = nlocal + nghost
+        Z, pos = L.i32(numbers), L.f64(np.asarray(positions).reshape(n, 3))
+        il, nn, ng = L.i32(ilist), L.i32(num_neigh), L.i32(neigh)
+        lat = L.f64(np.eye(3) if lattice is None else lattice)
+        tg = L.i32(np.arange(1, n + 1) if tags is None else tags)
+        e, le, v, lv, f = np.zeros(1), np.zeros(n), np.zeros(9), np.zeros((n, 9)), np.zeros((n, 3))
+        L.check(L.lib().gapx_calc_lammps(self._h, nlocal, nghost, L.ptr(Z), L.ptr(tg), len(il), len(ng), L.ptr(il),
+                                         L.ptr(nn), L.ptr(ng), L.ptr(lat), L.ptr(pos), L.ptr(e), L.ptr(le), L.ptr(v),
+                                         L.ptr(lv), L.ptr(f)), "gapx_calc_lammps")
+        return {"energy": float(e[0]), "local_energy": le, "virial": v.reshape(3, 3).T, "local_virial": lv,
+                "force": f}
+
     def calc_batch(self, frames, want=("energy", "force", "virial"), cutoff=0.0):
         """Many structures in one pass of the engine (gapx_calc_batch); returns
         one dict per structure with the same keys as calc()."""

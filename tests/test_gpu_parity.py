@@ -338,6 +338,77 @@ def test_quip_cli_trajectory(models, tmp_path):
         assert np.abs(b.arrays["local_energy"] - ro["local_energy"]).max() < 1e-7
 
 
+def _lammps_view(at, rc, rng):
+    """A periodic cell as a LAMMPS pair style sees it: the cell's atoms as local
+    atoms, every periodic image within rc of one of them as a ghost, and each local
+    atom's full neighbour list (local and ghost atoms within rc, 1-based), with the
+    local atoms listed in a shuffled order."""
+    import itertools
+    n = len(at)
+    gpos, gZ, gown = [], [], []
+    for sh in itertools.product((-1, 0, 1), repeat=3):
+        if sh == (0, 0, 0):
+            continue
+        img = at.positions + np.array(sh, float) @ at.cell
+        dmin = np.linalg.norm(img[:, None, :] - at.positions[None, :, :], axis=2).min(axis=1)
+        keep = np.nonzero(dmin < rc)[0]
+        gpos.append(img[keep]); gZ.append(at.numbers[keep]); gown.append(keep)
+    x = np.vstack([at.positions] + gpos)
+    Z = np.concatenate([at.numbers] + gZ)
+    owner = np.concatenate([np.arange(n)] + gown)
+    ilist = rng.permutation(n)
+    num, neigh = [], []
+    for i in ilist:
+        d = np.linalg.norm(x - x[i], axis=1)
+        js = np.nonzero((d < rc) & (np.arange(len(x)) != i))[0]
+        num.append(len(js))
+        neigh.extend(rng.permutation(js) + 1)
+    return x, Z, owner, ilist, np.array(num), np.array(neigh)
+
+
+def test_lammps_entry_matches_periodic(models):
+    """gapx_calc_lammps on the ghost-atom view of a periodic cell (SOAP + distance_2b
+    model): with the ghosts' forces, local energies and local virials folded onto
+    their owners (LAMMPS's reverse communication) the results are the periodic
+    ones of the oracle; a vacuum region returns zeros; overlapping atoms and bad
+    indices are errors."""
+    m, path = models["si64"]
+    pot = Potential(param_filename=path)
+    at = configs.si64()
+    rc = pot.cutoff() + 0.5            # the pair style's cutoff plus its neighbour skin
+    rng = np.random.default_rng(51)
+    x, Z, owner, ilist, num, neigh = _lammps_view(at, rc, rng)
+    nl, ng = len(at), len(x) - len(at)
+    print("lammps view", nl, ng, len(neigh), flush=True)
+    r = pot.calc_lammps(nl, ng, Z, ilist, num, neigh, x, lattice=at.cell)
+    ro = O.gap_calc(m, at)
+    F = np.zeros((nl, 3)); np.add.at(F, owner, r["force"])
+    LE = np.zeros(nl); np.add.at(LE, owner, r["local_energy"])
+    LV = np.zeros((nl, 9)); np.add.at(LV, owner, r["local_virial"])
+    assert abs(r["energy"] - ro["energy"]) / nl < E_TOL
+    assert abs(r["local_energy"].sum() - r["energy"]) < 1e-9
+    assert np.abs(F - ro["forces"]).max() < F_TOL
+    assert np.abs(r["virial"] - ro["virial"]).max() < F_TOL * 10
+    assert np.abs(LE - ro["local_energy"]).max() < E_TOL * 10
+    assert np.abs(LV - ro["local_virial"]).max() < F_TOL * 10
+    assert np.abs(r["local_energy"][nl:]).max() > 0     # ghosts carry their share of the pair terms
+    # the calc_connect path afterwards is unaffected
+    rp = pot.calc(at.copy(), force=True)
+    assert abs(rp["energy"] - ro["energy"]) / nl < E_TOL
+    # vacuum region
+    rv = pot.calc_lammps(0, 3, Z[:3], [], [], [], x[:3])
+    assert rv["energy"] == 0.0 and not rv["force"].any()
+    # errors: two atoms on top of each other, a neighbour index out of range
+    x2 = x.copy(); x2[ilist[0]] = x2[neigh[0] - 1]
+    with pytest.raises(L.GapxError) as e:
+        pot.calc_lammps(nl, ng, Z, ilist, num, neigh, x2)
+    assert e.value.code == 6
+    bad = neigh.copy(); bad[3] = len(x) + 1
+    with pytest.raises(L.GapxError) as e:
+        pot.calc_lammps(nl, ng, Z, ilist, num, bad, x)
+    assert e.value.code == 1
+
+
 def _eq(a, b):
     assert a["energy"] == b["energy"]
     assert np.array_equal(a["force"], b["force"])
