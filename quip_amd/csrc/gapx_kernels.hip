@@ -391,15 +391,43 @@ void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, 
 // neighbour2(j) in the reference's append order: the global loop visits atom1
 // by (cell linear index, position in its descending cell list), then by its
 // neighbour1 entry index.  One wavefront per atom ranks its entries by that key
-// (all keys distinct) and writes them in order; it also records, for every
-// neighbour1 entry, the slot of its mirror in neighbour2.
+// (all keys distinct; up to 64 entries are staged in LDS first) and writes them
+// in order; it also records, for every neighbour1 entry, the slot of its mirror
+// in neighbour2.
 __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k,
                                                     const int *cell_of, const int *n1_off, int *n2_is, int *n2_ks,
                                                     int *n1_rev) {
-    const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (j >= N) return;
+    __shared__ int s_i[4][64], s_k[4][64], s_c[4][64];   // the wave's entries when they fit one row
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = blockIdx.x * 4 + wv;
+    if (j >= N) return;   // whole wavefront leaves together
     const int b = n2_off[j], n = n2_off[j + 1] - b;
+    if (n <= 64) {
+        int vi = 0, vk = 0, vc = 0;
+        if (lane < n) {
+            vi = n2_i[b + lane];
+            vk = n2_k[b + lane];
+            vc = cell_of[vi];
+            s_i[wv][lane] = vi;
+            s_k[wv][lane] = vk;
+            s_c[wv][lane] = vc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < n) {
+            int rank = 0;
+            for (int y = 0; y < n; y++) {
+                const int yi = s_i[wv][y], yk = s_k[wv][y], yc = s_c[wv][y];
+                const bool before = (yc < vc) || (yc == vc && (yi > vi || (yi == vi && yk < vk)));
+                rank += before ? 1 : 0;
+            }
+            n2_is[b + rank] = vi;
+            n2_ks[b + rank] = vk;
+            n1_rev[n1_off[vi] + vk] = rank;
+        }
+        return;
+    }
     for (int x = lane; x < n; x += 64) {
         const int vi = n2_i[b + x], vk = n2_k[b + x], vc = cell_of[vi];
         int rank = 0;
@@ -1220,9 +1248,9 @@ void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int 
 // local_virial_j(a,b) = -sum_q d_(rev q),a pairF_(rev q),b,
 // local_e_j = e_site + (pairE[q] + pairE[rev q])/2 + e0(Z_j); block partial sums.
 // ----------------------------------------------------------------------------
-constexpr int GATHER_BS = 256;   // 4 wavefronts, one atom each
-__device__ __forceinline__ double wave_sum_g(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+constexpr int GATHER_BS = 256;   // 4 wavefronts, two atoms each (a half-wave per atom)
+__device__ __forceinline__ double half_sum_g(double v) {   // sum over the 32 lanes of a half-wave
+    for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
 __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const double *e0, double E_scale,
@@ -1230,14 +1258,15 @@ __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const
                                                       const double *pairF, const double *pairE, const double *esite,
                                                       double *f, double *local_e, double *local_virial,
                                                       const unsigned char *mask) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int j = blockIdx.x * 4 + w;
-    if (j >= N) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, hl = lane & 31;
+    const int j = (blockIdx.x * 4 + w) * 2 + (lane >> 5);
+    const bool live = j < N;
     double v[13];
 #pragma unroll
     for (int k = 0; k < 13; k++) v[k] = 0.0;
-    // lanes over the pairs of atom j; fixed lane assignment and a fixed reduction tree
-    for (int q = nb_off[j] + lane; q < nb_off[j + 1]; q += 64) {
+    // a half-wave over the pairs of atom j; fixed lane assignment and a fixed reduction tree
+    const int q0 = live ? nb_off[j] : 0, q1 = live ? nb_off[j + 1] : 0;
+    for (int q = q0 + hl; q < q1; q += 32) {
         const int r = nb_rev[q];
         const double gx = pairF[3 * r], gy = pairF[3 * r + 1], gz = pairF[3 * r + 2];
         v[0] += pairF[3 * q] - gx;
@@ -1250,21 +1279,22 @@ __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const
         v[12] += pairE[q] + pairE[r];
     }
 #pragma unroll
-    for (int k = 0; k < 13; k++) v[k] = wave_sum_g(v[k]);
-    if (lane < 3 && f) f[3 * j + lane] = E_scale * (lane == 0 ? v[0] : (lane == 1 ? v[1] : v[2]));
-    if (lane == 0) local_e[j] = E_scale * (esite[j] + 0.5 * v[12] + ((mask && !mask[j]) ? 0.0 : e0[Z[j]]));
-    if (lane < 9) {
+    for (int k = 0; k < 13; k++) v[k] = half_sum_g(v[k]);
+    if (!live) return;
+    if (hl < 3 && f) f[3 * j + hl] = E_scale * (hl == 0 ? v[0] : (hl == 1 ? v[1] : v[2]));
+    if (hl == 0) local_e[j] = E_scale * (esite[j] + 0.5 * v[12] + ((mask && !mask[j]) ? 0.0 : e0[Z[j]]));
+    if (hl < 9) {
         double x = v[3];
 #pragma unroll
-        for (int k = 1; k < 9; k++) x = (lane == k) ? v[3 + k] : x;
-        local_virial[9 * (size_t)j + lane] = E_scale * x;
+        for (int k = 1; k < 9; k++) x = (hl == k) ? v[3 + k] : x;
+        local_virial[9 * (size_t)j + hl] = E_scale * x;
     }
 }
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
                    double *local_e, double *local_virial, const unsigned char *mask, hipStream_t s) {
     if (N > 0)
-        hipLaunchKernelGGL(k_gather, dim3(cdiv(N, 4)), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev,
+        hipLaunchKernelGGL(k_gather, dim3(cdiv(N, 8)), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev,
                            nb_d, pairF, pairE, esite, f, local_e, local_virial, mask);
 }
 

@@ -90,25 +90,31 @@ __device__ __forceinline__ void legendre_level(int l, double uz, const double *p
     }
 }
 
-// i_l recursion state for all grid points: F0 = f_l, F1 = f_(l+1)
-template <int N>
+// i_l recursion state for all grid points: F0 = f_l, F1 = f_(l+1).  The
+// recursion factor 1/x_a = 1/(2 alpha r r_a) is ixq / a with ixq = 1/(2 alpha r dr)
+// (r_a = a dr).  KEEP_IX stores it per point (more registers; measured faster in
+// launches too small to fill the GPU), otherwise it is formed from ixq when used
+// (fewer registers: more wavefronts per SIMD in large launches).
+// rbs: grid r_a = a * dr; gA[a] = exp(-alpha r_a^2); iA[a] = 1/a.
+// exp(-alpha (r -+ r_a)^2) = exp(-alpha r^2) gA[a] q^(+-a) with q = exp(2 alpha r dr):
+// two exponentials and one division per neighbour.  Outside the range where the
+// powers stay finite, every point is direct.
+template <int N, bool KEEP_IX = false>
 struct Rad {
-    double F0[N], F1[N], ix[N];
-    // rbs: grid r_a = a * dr; gA[a] = exp(-alpha r_a^2); iA[a] = 1/a.
-    // exp(-alpha (r -+ r_a)^2) = exp(-alpha r^2) gA[a] q^(+-a) with q = exp(2 alpha r dr) and
-    // 1/(2 alpha r r_a) = iA[a] / (2 alpha r dr): two exponentials and one division per
-    // neighbour.  Outside the range where the powers stay finite, every point is direct.
+    double F0[N], F1[N], ixq;
+    double ix[KEEP_IX ? N : 1];
     __device__ __forceinline__ void init(const double *rbs, const double *gA, const double *iA, double alpha,
                                          double r) {
         const double xq = N > 1 ? 2.0 * alpha * r * rbs[1] : 0.0;
         const double rmax = rbs[N - 1];
+        ixq = 1.0 / xq;
         if (xq > 0.0 && xq * (N - 1) < 600.0 && alpha * (r * r + rmax * rmax) < 600.0) {
             const double e0 = exp(-alpha * r * r);
-            const double q = exp(xq), qi = 1.0 / q, ixq = 1.0 / xq;
+            const double q = exp(xq), qi = 1.0 / q;
             double qa = 1.0, qia = 1.0;
             F0[0] = e0;
             F1[0] = 0.0;
-            ix[0] = 0.0;
+            if constexpr (KEEP_IX) ix[0] = 0.0;
 #pragma unroll
             for (int a = 1; a < N; a++) {
                 qa *= q;
@@ -118,34 +124,55 @@ struct Rad {
                 const double iv = ixq * iA[a];
                 F0[a] = 0.5 * (em - ep) * iv;
                 F1[a] = 0.5 * (em + ep) * iv - F0[a] * iv;
-                ix[a] = iv;
+                if constexpr (KEEP_IX) ix[a] = iv;
             }
             return;
         }
 #pragma unroll
         for (int a = 0; a < N; a++) {
             const double ra = rbs[a];
-            const double x = 2.0 * alpha * r * ra;
-            if (x == 0.0) {
+            if constexpr (KEEP_IX) {
+                const double x = 2.0 * alpha * r * ra;
+                if (x == 0.0) {
+                    F0[a] = exp(-alpha * (ra * ra + r * r));
+                    F1[a] = 0.0;
+                    ix[a] = 0.0;
+                } else {
+                    const double ep = exp(-alpha * (r + ra) * (r + ra)), em = exp(-alpha * (r - ra) * (r - ra));
+                    const double iv = 1.0 / x;
+                    F0[a] = 0.5 * (em - ep) * iv;
+                    F1[a] = 0.5 * (em + ep) * iv - F0[a] * iv;
+                    ix[a] = iv;
+                }
+            } else if (a == 0) {
                 F0[a] = exp(-alpha * (ra * ra + r * r));
                 F1[a] = 0.0;
-                ix[a] = 0.0;
             } else {
                 const double ep = exp(-alpha * (r + ra) * (r + ra)), em = exp(-alpha * (r - ra) * (r - ra));
-                const double iv = 1.0 / x;
+                const double iv = ixq * (1.0 / (double)a);
                 F0[a] = 0.5 * (em - ep) * iv;
                 F1[a] = 0.5 * (em + ep) * iv - F0[a] * iv;
-                ix[a] = iv;
             }
             __builtin_amdgcn_sched_barrier(0);   // one grid point's exponentials at a time
         }
     }
     __device__ __forceinline__ void step(int l) {
+        if constexpr (KEEP_IX) {
 #pragma unroll
-        for (int a = 0; a < N; a++) {
-            const double n2 = F0[a] - (2.0 * l + 3.0) * ix[a] * F1[a];
-            F0[a] = F1[a];
-            F1[a] = (ix[a] == 0.0) ? 0.0 : n2;
+            for (int a = 0; a < N; a++) {
+                const double n2 = F0[a] - (2.0 * l + 3.0) * ix[a] * F1[a];
+                F0[a] = F1[a];
+                F1[a] = (ix[a] == 0.0) ? 0.0 : n2;
+            }
+        } else {
+            F0[0] = F1[0];   // grid point 0 (r_a = 0): f_(l+1) = 0 for every l
+            F1[0] = 0.0;
+#pragma unroll
+            for (int a = 1; a < N; a++) {
+                const double n2 = F0[a] - (2.0 * l + 3.0) * (ixq * (1.0 / (double)a)) * F1[a];
+                F0[a] = F1[a];
+                F1[a] = n2;
+            }
         }
     }
 };
@@ -333,8 +360,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
 // neighbour and (l, m): sum_b dc g_b = fc S0 and sum_b dc dg_b/dr =
 // fc (cl S0 + 2 alpha S1) + dfc S0 with S0 = sum_a F_l(a) dc'[a], S1 =
 // sum_a r_a F_(l+1)(a) dc'[a] and cl = -2 alpha r + l / r.
-template <int N, int L>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))) void k_soap_bwd_x(
+// OCC: the occupancy the register budget is cut for.  The higher one pays off when
+// there are many more wavefronts than SIMDs; a small launch runs faster with the
+// larger budget.
+template <int N, int L, int OCC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) void k_soap_bwd_x(
     SoapDev sp, int nc, int cpb, const int *__restrict__ centres, const int *__restrict__ Z,
     const int *__restrict__ nb_off, const int *__restrict__ nb_j, const double *__restrict__ nb_r,
     const double *__restrict__ nb_d, const double *__restrict__ X, const double *__restrict__ G,
@@ -468,7 +498,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_BWD, 8))
         const double ir = 1.0 / r;
         const double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
         const double *dcs = dcP + s * N * LM;
-        Rad<N> rs;
+        Rad<N, (OCC <= WPE_BWD)> rs;
         rs.init(rbs, gA, iA, sp.alpha, r);
         double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll 1
@@ -552,6 +582,22 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     return false;
 }
 
+template <int N, int L>
+static void bwd_go(bool occ3, int nblk, size_t sh, hipStream_t s, const SoapDev &sp, int nc, int cpb,
+                   const int *centres, const int *Z, const int *nb_off, const int *nb_j, const double *nb_r,
+                   const double *nb_d, const double *X, const double *G, const double *norm, const double *cbuf,
+                   double *pairF, int whole) {
+    if constexpr (N <= 8) {
+        if (occ3) {
+            hipLaunchKernelGGL((k_soap_bwd_x<N, L, WPE_BWD + 1>), dim3(nblk), dim3(64), sh, s, sp, nc, cpb, centres,
+                               Z, nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, whole);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_soap_bwd_x<N, L, WPE_BWD>), dim3(nblk), dim3(64), sh, s, sp, nc, cpb, centres, Z, nb_off,
+                       nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, whole);
+}
+
 bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                           const int *nb_j, const double *nb_r, const double *nb_d, const double *X, const double *G,
                           const double *norm, const double *cbuf, double *pairF, hipStream_t s) {
@@ -569,10 +615,14 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     const int whole = need(2, 1) <= 24 * 1024 ? 1 : 0;
     const int cpb = need(2, whole) <= 24 * 1024 ? 2 : 1;   // two centres per wave when LDS stays small
     const size_t sh = need(cpb, whole);
+    const int nblk = (nc + cpb - 1) / cpb;
+    // three wavefronts per SIMD for n <= 8 (161 VGPRs without spills) once the launch has
+    // at least four per SIMD to schedule
+    const bool occ3 = sp.n <= 8 && nblk >= 4 * 1024;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
-        hipLaunchKernelGGL((k_soap_bwd_x<NN, LL>), dim3((nc + cpb - 1) / cpb), dim3(64), sh, s, sp, nc, cpb,     \
-                           centres, Z, nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, whole);               \
+        bwd_go<NN, LL>(occ3, nblk, sh, s, sp, nc, cpb, centres, Z, nb_off, nb_j, nb_r, nb_d, X, G, norm,   \
+                       cbuf, pairF, whole);                                                                     \
         return true;                                                                                            \
     }
     SOAP_FAST(LAUNCH)

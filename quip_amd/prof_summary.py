@@ -2,7 +2,7 @@
 
     python -m quip_amd.prof_summary kernels  <run_kernel_trace.csv> [--match S] [--top N]
     python -m quip_amd.prof_summary counters <run_counter_collection.csv> [--match S]
-    python -m quip_amd.prof_summary traffic  <fetch_counter.csv> <write_counter.csv> --kernel S --grid G
+    python -m quip_amd.prof_summary traffic  <fetch_counter.csv> <write_counter.csv> --kernel S [--grid G]
                                             [--probe-bytes B]
 
 kernels:  calls, total and average duration per (kernel, grid size), the
@@ -12,6 +12,9 @@ traffic:  HBM bytes per launch of one kernel shape from a FETCH_SIZE run and a
           WRITE_SIZE run (KiB per dispatch), scaled by what the same counters
           read for the device-to-device copy kernel (__amd_rocclr_copyBuffer)
           of a probe of B bytes (B read + B written): factor = B / counter.
+          Without --grid, --kernel is a name prefix and the (kernel, grid) shape
+          with the most dispatches in the FETCH_SIZE run is taken (the shape an
+          autotuner settled on after timing its candidates once each).
 """
 import argparse
 import csv
@@ -75,6 +78,11 @@ def traffic(fetch_csv, write_csv, kernel, grid, probe_bytes):
 
     # the probe's dispatch is the copy kernel shape with the largest counter value
     pf, pw = pick(f, "__amd_rocclr_copyBuffer", "FETCH_SIZE"), pick(w, "__amd_rocclr_copyBuffer", "WRITE_SIZE")
+    if grid is None:   # the most-dispatched shape whose name starts with `kernel`
+        cand = [(v[0], k) for k, v in f.items() if k[0].startswith(kernel) and k[2] == "FETCH_SIZE"]
+        if not cand:
+            raise SystemExit("no FETCH_SIZE dispatches of %s" % kernel)
+        _, (kernel, grid, _) = max(cand)
     kf, kw = pick(f, kernel, "FETCH_SIZE", grid), pick(w, kernel, "WRITE_SIZE", grid)
     ff = probe_bytes / pf[1][1]
     fw = probe_bytes / pw[1][1]
