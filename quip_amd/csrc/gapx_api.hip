@@ -807,17 +807,45 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
                         w.cell_unsorted.as<int>(), s);
     launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
                         w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
-    launch_nl_pairs(false, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                    w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
-                    nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, s);
+    // The candidate search runs once, into per-atom slots sized from the largest counts
+    // of the previous build; its counts are exact either way.  The first build counts
+    // first, and a build whose slots overflowed fills the exact tables in a second
+    // pass, as the first does.
+    const int cap1 = eg.nl_cap1, cap2 = eg.nl_cap2;
+    const bool one_pass = cap1 > 0 && cap2 > 0 && (size_t)N * std::max(cap1, cap2) < (size_t)INT32_MAX / 4;
+    CHK(w.nl_flags.ensure(64));
+    int *flags = w.nl_flags.as<int>();   // [0] overflow, [1] max neighbour1 count, [2] max neighbour2 count
+    launch_fill_i32(flags, 4, 0, s);
+    if (one_pass) {
+        CHK(w.t1_j.ensure((size_t)N * cap1 * 4 + 4));
+        CHK(w.t1_shift.ensure((size_t)N * cap1 * 12 + 12));
+        CHK(w.t1_r.ensure((size_t)N * cap1 * 8 + 8));
+        CHK(w.t2_i.ensure((size_t)N * cap2 * 4 + 4));
+        CHK(w.t2_k.ensure((size_t)N * cap2 * 4 + 4));
+        launch_nl_pairs(2, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        w.t1_j.as<int>(), w.t1_shift.as<int>(), w.t1_r.as<double>(), w.n2_cnt.as<int>(), nullptr,
+                        nullptr, w.t2_i.as<int>(), w.t2_k.as<int>(), cap1, cap2, flags, s);
+    } else {
+        launch_nl_pairs(0, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, 0, 0,
+                        nullptr, s);
+    }
     launch_scan_i32(w.n1_cnt.as<int>(), N, w.n1_off.as<int>(), w.scan_part.as<int>(), s);
     launch_scan_i32(w.n2_cnt.as<int>(), N, w.n2_off.as<int>(), w.scan_part2.as<int>(), s);
+    launch_count_max(N, w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), flags + 1, s);
     CHK(eg.ctl.ensure(64));
     int *hcount = eg.ctl.as<int>();
     hcount[0] = -1;
     HIPCHK(hipMemcpyAsync(hcount, w.n1_off.as<int>() + N, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hcount + 1, flags, 12, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const int n1_total = hcount[0];
+    const bool overflow = hcount[1] != 0;
+    // next build's slots: the largest counts of this one plus a quarter and 8
+    eg.nl_cap1 = (hcount[2] + hcount[2] / 4 + 8 + 3) & ~3;
+    eg.nl_cap2 = (hcount[3] + hcount[3] / 4 + 8 + 3) & ~3;
     if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);   // the previous call has finished
     if (n1_total < 0 || n1_total > (INT32_MAX - 2) / 2) return fail(GAPX_E_GEOMETRY, "neighbour count overflow");
     const size_t T1 = (size_t)n1_total + 1;
@@ -834,10 +862,17 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     CHK(w.nb_rev.ensure(P * 4));
     CHK(w.nb_r.ensure(P * 8));
     CHK(w.nb_d.ensure(P * 24));
-    launch_nl_pairs(true, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                    w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, nullptr, w.n1_off.as<int>(),
-                    w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), nullptr, w.n2_off.as<int>(),
-                    w.n2_fill.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), s);
+    if (one_pass && !overflow) {
+        launch_nl_compact(N, cap1, cap2, w.n1_off.as<int>(), w.n2_off.as<int>(), w.t1_j.as<int>(),
+                          w.t1_shift.as<int>(), w.t1_r.as<double>(), w.t2_i.as<int>(), w.t2_k.as<int>(),
+                          w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), w.n2_i.as<int>(),
+                          w.n2_k.as<int>(), s);
+    } else {
+        launch_nl_pairs(1, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, nullptr, w.n1_off.as<int>(),
+                        w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), nullptr, w.n2_off.as<int>(),
+                        w.n2_fill.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), 0, 0, nullptr, s);
+    }
     launch_nl_n2_sort(N, w.n2_off.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), w.cell_of.as<int>(),
                       w.n1_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), s);
     launch_nl_full(N, d_pos, dgeo, dsid, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n1_shift.as<int>(),

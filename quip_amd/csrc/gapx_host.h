@@ -116,6 +116,7 @@ struct Workspace {
     DBuf f, le, lv, e, vir, minmax;
     DBuf last_pos, skin_flags;   // positions of the last build (skin reuse), reuse test results
     DBuf cmask;                  // descriptor-centre mask of a caller-supplied connection
+    DBuf t1_j, t1_shift, t1_r, t2_i, t2_k, nl_flags;   // per-atom slots of the one-pass neighbour search
     std::vector<DBuf> centres;   // per coordinate
 };
 
@@ -237,6 +238,7 @@ struct Engine {
     hipGraphExec_t gexec = nullptr;   // the captured launch sequence after the neighbour count
     std::vector<long long> gkey;
     int n1_total_last = 0;
+    int nl_cap1 = 0, nl_cap2 = 0;   // slot sizes of the one-pass neighbour search (0: count first)
     int ncell_last = 0;
     ~Engine() {
         if (gexec) hipGraphExecDestroy(gexec);

@@ -40,11 +40,17 @@ void launch_nl_cell_count(int N, const int *cell_of, int *cnt, hipStream_t s);
 void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s);
 void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
                          int *atoms, hipStream_t s);
-void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
+// mode 0: count; 1: fill the CSR tables; 2: one pass into per-atom slots of cap1 / cap2
+// (counts in n1_cnt / n2_cnt, *ovf = 1 when a slot range overflows)
+void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, const int *cell_of,
                      const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom *geo,
                      const int *sid, const int *cell_base, int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift,
-                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k,
-                     hipStream_t s);
+                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, int cap1,
+                     int cap2, int *ovf, hipStream_t s);
+void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off, const int *t1_j,
+                       const int *t1_shift, const double *t1_r, const int *t2_i, const int *t2_k, int *n1_j,
+                       int *n1_shift, double *n1_r, int *n2_i, int *n2_k, hipStream_t s);
+void launch_count_max(int N, const int *c1, const int *c2, int *mx, hipStream_t s);
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
                        const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s);
 void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,

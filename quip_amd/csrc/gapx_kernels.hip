@@ -237,7 +237,11 @@ __device__ __forceinline__ int wave_excl_scan_i(int v, int *total) {
 // binary search).  An exclusive wave scan of the prefix lengths numbers the
 // candidates in append order; each lane then tests one candidate, and a ballot
 // gives every accepted pair its slot in neighbour1(atom1).
-template <bool FILL>
+// MODE 0 counts (neighbour1 per atom1, neighbour2 per atom2); MODE 1 fills the
+// exact CSR tables; MODE 2 fills per-atom slots of cap1 / cap2 entries
+// (neighbour1 of atom a at a*cap1, neighbour2 of atom a at a*cap2), counts both,
+// and raises *ovf when a count exceeds its capacity.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restrict__ pos,
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
                                                     const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
@@ -246,8 +250,9 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
                                                     const int *__restrict__ cell_base, int *n1_cnt,
                                                     const int *n1_off, int *n1_j, int *n1_shift, double *n1_r,
                                                     int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i,
-                                                    int *n2_k) {
+                                                    int *n2_k, int cap1, int cap2, int *ovf) {
 #pragma clang fp contract(off)
+    constexpr bool FILL = MODE == 1;
     __shared__ int s_pref[4][65];
     __shared__ int s_start[4][64];
     __shared__ int s_sh[4][64];   // i4, j4, k4 packed (each in [-128, 127]) and the same-cell flag
@@ -274,7 +279,8 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
     const int ms1x = map_shift[3 * atom1], ms1y = map_shift[3 * atom1 + 1], ms1z = map_shift[3 * atom1 + 2];
     const double rc = g.cutoff;
     const unsigned long long below = (1ull << lane) - 1ull;
-    int base = FILL ? n1_off[atom1] : 0;
+    const int base1 = MODE == 1 ? n1_off[atom1] : (MODE == 2 ? atom1 * cap1 : 0);
+    int base = base1;
     int total_all = 0;
     for (int o0 = 0; o0 < noff; o0 += 64) {
         // this lane's offset cell: list start, shift, and the length of its atom >= atom1 prefix
@@ -359,7 +365,23 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
                     n2_i[n2_off[atom2] + slot] = atom1;
                     n2_k[n2_off[atom2] + slot] = e - n1_off[atom1];
                 }
-            } else if (!FILL && acc && atom2 != atom1) {
+            } else if (MODE == 2 && acc) {
+                const int e = base + __popcll(mask & below), k1 = e - base1;
+                if (k1 < cap1) {
+                    n1_j[e] = atom2;
+                    n1_shift[3 * e] = sh0; n1_shift[3 * e + 1] = sh1; n1_shift[3 * e + 2] = sh2;
+                    n1_r[e] = d;
+                }
+                if (atom2 != atom1) {
+                    const int slot = atomicAdd(&n2_cnt[atom2], 1);
+                    if (slot < cap2 && k1 < cap1) {
+                        n2_i[atom2 * cap2 + slot] = atom1;
+                        n2_k[atom2 * cap2 + slot] = k1;
+                    } else {
+                        *ovf = 1;
+                    }
+                }
+            } else if (MODE == 0 && acc && atom2 != atom1) {
                 atomicAdd(&n2_cnt[atom2], 1);
             }
             base += __popcll(mask);
@@ -367,25 +389,73 @@ __global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restr
         }
         __builtin_amdgcn_wave_barrier();   // this group's LDS rows are read before the next group writes them
     }
-    if (!FILL && lane == 0) n1_cnt[atom1] = total_all;
+    if (MODE != 1 && lane == 0) n1_cnt[atom1] = total_all;
+    if (MODE == 2 && lane == 0 && total_all > cap1) *ovf = 1;
 }
 
-void launch_nl_pairs(bool fill, int N, const double *pos, const int *map_shift, const int *cell_of,
+void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, const int *cell_of,
                      const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom *geo,
                      const int *sid, const int *cell_base, int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift,
-                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k,
-                     hipStream_t s) {
+                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, int cap1,
+                     int cap2, int *ovf, hipStream_t s) {
     if (N <= 0) return;
-    if (fill)
-        hipLaunchKernelGGL(k_nl_pairs_w<true>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
-                           cell_start, cell_cnt, cell_atoms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r,
-                           n2_cnt, n2_off,
-                           n2_fill, n2_i, n2_k);
-    else
-        hipLaunchKernelGGL(k_nl_pairs_w<false>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,
-                           cell_start, cell_cnt, cell_atoms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r,
-                           n2_cnt, n2_off,
-                           n2_fill, n2_i, n2_k);
+#define NLP(M)                                                                                                  \
+    hipLaunchKernelGGL(k_nl_pairs_w<M>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of, cell_start, \
+                       cell_cnt, cell_atoms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r, n2_cnt,   \
+                       n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
+    if (mode == 1) NLP(1);
+    else if (mode == 2) NLP(2);
+    else NLP(0);
+#undef NLP
+}
+
+// Per-atom slots of the one-pass search -> the CSR tables (one wavefront per atom).
+__global__ __launch_bounds__(256) void k_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off,
+                                                    const int *t1_j, const int *t1_shift, const double *t1_r,
+                                                    const int *t2_i, const int *t2_k, int *n1_j, int *n1_shift,
+                                                    double *n1_r, int *n2_i, int *n2_k) {
+    const int lane = threadIdx.x & 63;
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= N) return;
+    const int b1 = n1_off[a], c1 = n1_off[a + 1] - b1;
+    for (int q = lane; q < c1; q += 64) {
+        const size_t src = (size_t)a * cap1 + q;
+        n1_j[b1 + q] = t1_j[src];
+        n1_shift[3 * (b1 + q)] = t1_shift[3 * src];
+        n1_shift[3 * (b1 + q) + 1] = t1_shift[3 * src + 1];
+        n1_shift[3 * (b1 + q) + 2] = t1_shift[3 * src + 2];
+        n1_r[b1 + q] = t1_r[src];
+    }
+    const int b2 = n2_off[a], c2 = n2_off[a + 1] - b2;
+    for (int q = lane; q < c2; q += 64) {
+        const size_t src = (size_t)a * cap2 + q;
+        n2_i[b2 + q] = t2_i[src];
+        n2_k[b2 + q] = t2_k[src];
+    }
+}
+void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off, const int *t1_j,
+                       const int *t1_shift, const double *t1_r, const int *t2_i, const int *t2_k, int *n1_j,
+                       int *n1_shift, double *n1_r, int *n2_i, int *n2_k, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_nl_compact, dim3(cdiv(N, 4)), dim3(256), 0, s, N, cap1, cap2, n1_off, n2_off, t1_j,
+                           t1_shift, t1_r, t2_i, t2_k, n1_j, n1_shift, n1_r, n2_i, n2_k);
+}
+
+// Largest neighbour1 and neighbour2 counts (atomic max of block maxima into mx[0], mx[1]).
+__global__ __launch_bounds__(256) void k_count_max(int N, const int *c1, const int *c2, int *mx) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int a = i < N ? c1[i] : 0, b = i < N ? c2[i] : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        a = max(a, __shfl_xor(a, off, 64));
+        b = max(b, __shfl_xor(b, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&mx[0], a);
+        atomicMax(&mx[1], b);
+    }
+}
+void launch_count_max(int N, const int *c1, const int *c2, int *mx, hipStream_t s) {
+    if (N > 0) hipLaunchKernelGGL(k_count_max, dim3(cdiv(N, 256)), dim3(256), 0, s, N, c1, c2, mx);
 }
 
 // neighbour2(j) in the reference's append order: the global loop visits atom1

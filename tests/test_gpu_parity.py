@@ -173,6 +173,31 @@ def test_bitwise_repeatable(models):
     assert np.array_equal(a["local_virial"], b["local_virial"])
 
 
+def test_one_pass_neighbour_search(models):
+    """The neighbour search runs once into per-atom slots sized by the previous build.
+    A first build (exact count and fill), a repeat (one pass), a denser structure
+    that overflows the slots (second pass fills the exact tables) and a repeat of
+    that one all give the results of a fresh model bit for bit."""
+    m, path = models["si64"]
+    pot = Potential(param_filename=path)
+    sparse = configs.si64()
+    sparse.positions = sparse.positions * 1.3
+    sparse.cell = sparse.cell * 1.3
+    dense = configs.si64()
+
+    def fresh(at):
+        return Potential(param_filename=path).calc(at.copy(), force=True, virial=True, local_virial=True)
+
+    def same(a, b):
+        assert a["energy"] == b["energy"]
+        assert np.array_equal(a["force"], b["force"]) and np.array_equal(a["virial"], b["virial"])
+        assert np.array_equal(a["local_virial"], b["local_virial"])
+
+    for at in (sparse, sparse, dense, dense):
+        print("one-pass search", len(at), at.cell[0, 0], flush=True)
+        same(pot.calc(at.copy(), force=True, virial=True, local_virial=True), fresh(at))
+
+
 def _same(rb, r, N):
     # a batch changes GEMM 2's split-K factor (chosen from the centre count), so
     # a structure alone and in a batch agree to rounding, not bit for bit
