@@ -796,12 +796,14 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     const NLGeom *dgeo = w.geo.as<NLGeom>();
     const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
     // per-call counters start at zero
-    launch_fill_i32(w.cell_cnt.as<int>(), NC, 0, s);
-    launch_fill_i32(w.cell_fill.as<int>(), NC, 0, s);
-    launch_fill_i32(w.n2_cnt.as<int>(), N, 0, s);
-    launch_fill_i32(w.n2_fill.as<int>(), N, 0, s);
-    launch_nl_bin(N, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(), s);
-    launch_nl_cell_count(N, w.cell_of.as<int>(), w.cell_cnt.as<int>(), s);
+    CHK(w.nl_flags.ensure(64));
+    int *flags = w.nl_flags.as<int>();   // [0] slot overflow, [1] max neighbour1 count, [2] max neighbour2 count
+    {
+        int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags};
+        const int zn[5] = {NC, NC, N, N, 4};
+        launch_zero_i32_multi(5, zp, zn, s);
+    }
+    launch_nl_bin(N, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(), w.cell_cnt.as<int>(), s);
     launch_scan_i32(w.cell_cnt.as<int>(), NC, w.cell_start.as<int>(), w.scan_part.as<int>(), s);
     launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
                         w.cell_unsorted.as<int>(), s);
@@ -813,9 +815,6 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     // pass, as the first does.
     const int cap1 = eg.nl_cap1, cap2 = eg.nl_cap2;
     const bool one_pass = cap1 > 0 && cap2 > 0 && (size_t)N * std::max(cap1, cap2) < (size_t)INT32_MAX / 4;
-    CHK(w.nl_flags.ensure(64));
-    int *flags = w.nl_flags.as<int>();   // [0] overflow, [1] max neighbour1 count, [2] max neighbour2 count
-    launch_fill_i32(flags, 4, 0, s);
     if (one_pass) {
         CHK(w.t1_j.ensure((size_t)N * cap1 * 4 + 4));
         CHK(w.t1_shift.ensure((size_t)N * cap1 * 12 + 12));
@@ -832,9 +831,8 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
                         nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, 0, 0,
                         nullptr, s);
     }
-    launch_scan_i32(w.n1_cnt.as<int>(), N, w.n1_off.as<int>(), w.scan_part.as<int>(), s);
-    launch_scan_i32(w.n2_cnt.as<int>(), N, w.n2_off.as<int>(), w.scan_part2.as<int>(), s);
-    launch_count_max(N, w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), flags + 1, s);
+    launch_scan2_max(w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), N, w.n1_off.as<int>(), w.n2_off.as<int>(), flags + 1,
+                     w.scan_part.as<int>(), w.scan_part2.as<int>(), s);
     CHK(eg.ctl.ensure(64));
     int *hcount = eg.ctl.as<int>();
     hcount[0] = -1;
@@ -1171,9 +1169,11 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
 
     // The launch sequence (stage events only when timing, never inside a graph).
     auto launch_all = [&](bool marks) {
-        launch_fill_f64(w.pairF.as<double>(), (size_t)P * 3, 0.0, s);
-        launch_fill_f64(w.pairE.as<double>(), (size_t)P, 0.0, s);
-        launch_fill_f64(w.esite.as<double>(), (size_t)N, 0.0, s);
+        {
+            double *zp[3] = {w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>()};
+            const size_t zn[3] = {(size_t)P * 3, (size_t)P, (size_t)N};
+            launch_zero_f64_multi(3, zp, zn, s);
+        }
         for (size_t k = 0; k < NCOORD; k++) {
             Coord &c = *m->coords[k];
             if (c.kind == 2) {

@@ -34,9 +34,14 @@ int scan_partials_size(int n);
 void launch_scan_i32(const int *in, int n, int *out, int *partials, hipStream_t s);
 void launch_fill_i32(int *p, int n, int v, hipStream_t s);
 void launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
+constexpr int kMultiFill = 6;   // arrays one zero-fill launch covers
+void launch_zero_i32_multi(int k, int *const *p, const int *n, hipStream_t s);
+void launch_zero_f64_multi(int k, double *const *p, const size_t *n, hipStream_t s);
+void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out2, int *mx, int *part1,
+                      int *part2, hipStream_t s);
+// bins the atoms and counts each cell's atoms (cell_cnt zeroed by the caller)
 void launch_nl_bin(int N, const double *pos, const NLGeom *geo, const int *sid, const int *cell_base, int *cell_of,
-                   int *map_shift, hipStream_t s);
-void launch_nl_cell_count(int N, const int *cell_of, int *cnt, hipStream_t s);
+                   int *map_shift, int *cell_cnt, hipStream_t s);
 void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s);
 void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
                          int *atoms, hipStream_t s);

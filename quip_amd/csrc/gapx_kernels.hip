@@ -72,6 +72,35 @@ void launch_fill_f64(double *p, size_t n, double v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(cdiv((long)n, 256)), dim3(256), 0, s, p, n, v);
 }
 
+// Zero up to kMultiFill arrays in one launch (blockIdx.y picks the array).
+struct ZeroI32Set { int *p[kMultiFill]; int n[kMultiFill]; };
+struct ZeroF64Set { double *p[kMultiFill]; size_t n[kMultiFill]; };
+__global__ void k_zero_i32_multi(ZeroI32Set z) {
+    const int a = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < z.n[a]; i += gridDim.x * blockDim.x) z.p[a][i] = 0;
+}
+__global__ void k_zero_f64_multi(ZeroF64Set z) {
+    const int a = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.n[a]; i += (size_t)gridDim.x * blockDim.x)
+        z.p[a][i] = 0.0;
+}
+void launch_zero_i32_multi(int k, int *const *p, const int *n, hipStream_t s) {
+    ZeroI32Set z = {};
+    int mx = 0;
+    for (int a = 0; a < k && a < kMultiFill; a++) { z.p[a] = p[a]; z.n[a] = p[a] ? n[a] : 0; mx = std::max(mx, z.n[a]); }
+    if (mx > 0)
+        hipLaunchKernelGGL(k_zero_i32_multi, dim3(std::min(cdiv(mx, 256), 4096), std::min(k, kMultiFill)), dim3(256), 0,
+                           s, z);
+}
+void launch_zero_f64_multi(int k, double *const *p, const size_t *n, hipStream_t s) {
+    ZeroF64Set z = {};
+    size_t mx = 0;
+    for (int a = 0; a < k && a < kMultiFill; a++) { z.p[a] = p[a]; z.n[a] = p[a] ? n[a] : 0; mx = std::max(mx, z.n[a]); }
+    if (mx > 0)
+        hipLaunchKernelGGL(k_zero_f64_multi, dim3((int)std::min<size_t>((mx + 255) / 256, 8192), std::min(k, kMultiFill)),
+                           dim3(256), 0, s, z);
+}
+
 
 // ----------------------------------------------------------------------------
 // exclusive prefix sum of n ints into out[0..n] (out[n] = total), three passes:
@@ -127,12 +156,74 @@ __global__ void k_scan_apply(const int *in, int n, const int *partials, int *out
         ex += v[k];
     }
 }
+// One block of 1024 threads for n <= kScanOne: thread t scans its contiguous run of
+// ceil(n / 1024) items; a wave scan and a scan of the 16 wave totals join the runs.
+// Block b of the launch scans array b; mx (nullable) gets each array's largest item.
+constexpr int kScanOne = 65536;
+struct ScanSet { const int *in[2]; int *out[2]; };
+__global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx) {
+    __shared__ int wsum[16], wmax[16];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int *in = ss.in[b];
+    int *out = ss.out[b];
+    const int per = (n + 1023) / 1024, i0 = t * per, i1 = min(n, i0 + per);
+    int s = 0, m = 0;
+    for (int i = i0; i < i1; i++) {
+        const int v = in[i];
+        s += v;
+        m = max(m, v);
+    }
+    int x = s;   // inclusive wave scan of the run sums
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    int wm = m;
+    for (int off = 32; off > 0; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
+    if (lane == 63) wsum[wv] = x;
+    if (lane == 0) wmax[wv] = wm;
+    __syncthreads();
+    int before = 0, total = 0, big = 0;
+    for (int k = 0; k < 16; k++) {
+        before += k < wv ? wsum[k] : 0;
+        total += wsum[k];
+        big = max(big, wmax[k]);
+    }
+    int ex = before + x - s;
+    for (int i = i0; i < i1; i++) {
+        const int v = in[i];
+        out[i] = ex;
+        ex += v;
+    }
+    if (t == 0) {
+        out[n] = total;
+        if (mx) mx[b] = big;
+    }
+}
 int scan_partials_size(int n) { return cdiv(n > 0 ? n : 1, SCAN_TILE); }
 void launch_scan_i32(const int *in, int n, int *out, int *partials, hipStream_t s) {
+    if (n <= kScanOne) {
+        ScanSet ss = {{in, in}, {out, out}};
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, s, ss, n, nullptr);
+        return;
+    }
     const int nb = scan_partials_size(n);
     hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BS), 0, s, in, n, partials);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BS), 0, s, partials, nb, out + n);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BS), 0, s, in, n, partials, out);
+}
+// Both neighbour counts scanned, and their largest values written to mx[0], mx[1]
+// (mx zeroed by the caller), in one launch when n <= kScanOne.
+void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out2, int *mx, int *part1,
+                      int *part2, hipStream_t s) {
+    if (n <= kScanOne) {
+        ScanSet ss = {{in1, in2}, {out1, out2}};
+        hipLaunchKernelGGL(k_scan_one, dim3(2), dim3(1024), 0, s, ss, n, mx);
+        return;
+    }
+    launch_scan_i32(in1, n, out1, part1, s);
+    launch_scan_i32(in2, n, out2, part2, s);
+    launch_count_max(n, in1, in2, mx, s);
 }
 
 // ----------------------------------------------------------------------------
@@ -154,7 +245,7 @@ __device__ __forceinline__ void mult3_exact(const double *g, double p0, double p
 
 __global__ void k_nl_bin(int N, const double *__restrict__ pos, const NLGeom *__restrict__ geo,
                          const int *__restrict__ sid, const int *__restrict__ cell_base, int *cell_of,
-                         int *map_shift) {
+                         int *map_shift, int *cell_cnt) {
 #pragma clang fp contract(off)
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -172,21 +263,15 @@ __global__ void k_nl_bin(int N, const double *__restrict__ pos, const NLGeom *__
         c = c < 1 ? 1 : (c > g.cells[a] ? g.cells[a] : c);
         ci[a] = c - 1;
     }
-    cell_of[n] = cell_base[st] + (ci[2] * g.cells[1] + ci[1]) * g.cells[0] + ci[0];
+    const int cell = cell_base[st] + (ci[2] * g.cells[1] + ci[1]) * g.cells[0] + ci[0];
+    cell_of[n] = cell;
+    atomicAdd(&cell_cnt[cell], 1);
 }
 void launch_nl_bin(int N, const double *pos, const NLGeom *geo, const int *sid, const int *cell_base, int *cell_of,
-                   int *map_shift, hipStream_t s) {
+                   int *map_shift, int *cell_cnt, hipStream_t s) {
     if (N > 0)
         hipLaunchKernelGGL(k_nl_bin, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pos, geo, sid, cell_base, cell_of,
-                           map_shift);
-}
-
-__global__ void k_nl_cell_count(int N, const int *cell_of, int *cnt) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < N) atomicAdd(&cnt[cell_of[n]], 1);
-}
-void launch_nl_cell_count(int N, const int *cell_of, int *cnt, hipStream_t s) {
-    if (N > 0) hipLaunchKernelGGL(k_nl_cell_count, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, cnt);
+                           map_shift, cell_cnt);
 }
 __global__ void k_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
