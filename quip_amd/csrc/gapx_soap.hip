@@ -184,8 +184,75 @@ struct Rad {
 // Per neighbour chunk: the chunk's neighbours are listed per species (wave
 // ballot); per l, lanes = neighbours stage fc F_l(a) and Y_l,m in LDS and
 // lanes = (a, m) accumulate c' of each species over that species' list only.
-template <int N, int L>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))) void k_soap_fwd_x(
+// One level of a chunk: stage fc F_l and Y_l of the neighbours, then accumulate
+// c' of level l.  A macro so that both level loops (rolled and unrolled) inline it.
+#define SOAP_FWD_LEVEL(l, cur, prv)                                                                                  \
+            __syncthreads();   /* the previous level's reads of Rl / Yl are done */                                  \
+            if (lane < NB) {   /* rows of neighbours outside every list stay zero for the dense loop */              \
+_Pragma("unroll")                                                                                                    \
+                for (int a = 0; a < N; a++) Rl[lane * RW + a] = s >= 0 ? fc * rs.F0[a] : 0.0;                        \
+            }                                                                                                        \
+            if (s >= 0) {                                                                                            \
+                rs.step(l);                                                                                          \
+                legendre_level(l, uz, prv, cur);                                                                     \
+                double A = 1.0, B = 0.0;                                                                             \
+                double *Yr = Yl + lane * YW;                                                                         \
+                for (int m = 0; m <= l; m++) {                                                                       \
+                    if (m > 0) {                                                                                     \
+                        const double An = A * ux - B * uy;                                                           \
+                        B = A * uy + B * ux;                                                                         \
+                        A = An;                                                                                      \
+                    }                                                                                                \
+                    const double Nn = c_ynorm[l * (l + 1) / 2 + m] * cur[m];                                         \
+                    if (m == 0) Yr[l] = Nn;                                                                          \
+                    else {                                                                                           \
+                        Yr[l + m] = Nn * A;                                                                          \
+                        Yr[l - m] = Nn * B;                                                                          \
+                    }                                                                                                \
+                }                                                                                                    \
+            } else if (lane < NB) {                                                                                  \
+                for (int m = 0; m < 2 * l + 1; m++) Yl[lane * YW + m] = 0.0;                                         \
+            }                                                                                                        \
+            __syncthreads();                                                                                         \
+            const int W = 2 * l + 1;                                                                                 \
+            for (int it = lane; it < N * W; it += 64) {                                                              \
+                const int a = it / W, mi = it - a * W;                                                               \
+                if (S == 1) {   /* one species: every row of the chunk, zero rows included */                        \
+                    double acc0 = 0.0, acc1 = 0.0;                                                                   \
+_Pragma("unroll 4")                                                                                                  \
+                    for (int k = 0; k < NB; k += 2) {                                                                \
+                        acc0 += Rl[k * RW + a] * Yl[k * YW + mi];                                                    \
+                        acc1 += Rl[(k + 1) * RW + a] * Yl[(k + 1) * YW + mi];                                        \
+                    }                                                                                                \
+                    cP[a * LM + l * l + mi] += acc0 + acc1;                                                          \
+                    continue;                                                                                        \
+                }                                                                                                    \
+                for (int si = 0; si < S; si++) {                                                                     \
+                    const int n = ns[si];                                                                            \
+                    const int *ki = kidx + si * NB;                                                                  \
+                    double acc0 = 0.0, acc1 = 0.0;                                                                   \
+                    int q = 0;                                                                                       \
+_Pragma("unroll 2")                                                                                                  \
+                    for (; q + 1 < n; q += 2) {                                                                      \
+                        const int k0 = ki[q], k1 = ki[q + 1];                                                        \
+                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];                                                  \
+                        acc1 += Rl[k1 * RW + a] * Yl[k1 * YW + mi];                                                  \
+                    }                                                                                                \
+                    if (q < n) {                                                                                     \
+                        const int k0 = ki[q];                                                                        \
+                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];                                                  \
+                    }                                                                                                \
+                    cP[(si * N + a) * LM + l * l + mi] += acc0 + acc1;                                               \
+                }                                                                                                    \
+            }
+
+// PREG: each lane's Legendre levels in registers, with the level loop unrolled (no
+// LDS rows, more registers: two wavefronts per SIMD); otherwise two LDS rows per
+// lane and a rolled level loop (three wavefronts per SIMD).  Measured: registers are
+// faster for launches below 16K centres and for n >= 10 or l >= 8 (whose LDS
+// already limits the occupancy), the LDS rows for large single-species batches.
+template <int N, int L, bool PREG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREG ? 2 : WPE_FWD, 8))) void k_soap_fwd_x(
     SoapDev sp, const int *__restrict__ centres, const int *__restrict__ Z, const int *__restrict__ nb_off,
     const int *__restrict__ nb_j, const double *__restrict__ nb_r, const double *__restrict__ nb_d, double *X,
     double *norm_out, double *cbuf) {
@@ -204,8 +271,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
     constexpr int PR = (2 * (L + 2)) | 1;   // row stride of a lane's two Legendre rows
     double *Rl = cP + KLM;         // NB x RW: fc F_l(a) of neighbour k
     double *Yl = Rl + NB * RW;     // NB x YW
-    double *Pq0 = Yl + NB * YW;    // NB x PR: the lanes' Legendre rows
-    int *kidx = (int *)(Pq0 + NB * PR);   // S x NB: the chunk's neighbours of each species
+    double *Pq0 = Yl + NB * YW;    // NB x PR: the lanes' Legendre rows (not PREG)
+    int *kidx = (int *)(Pq0 + (PREG ? 0 : NB * PR));   // S x NB: the chunk's neighbours of each species
     int *ns = kidx + S * NB;       // their counts
     const int lane = threadIdx.x;
     const int desc = blockIdx.x;
@@ -243,67 +310,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE_FWD, 8))
             if (mine) kidx[si * NB + __popcll(mk & ((1ull << lane) - 1ull))] = lane;
             if (lane == 0) ns[si] = __popcll(mk);
         }
-        double *Pq = Pq0 + lane * PR;   // this lane's two Legendre rows
-#pragma unroll 1
-        for (int l = 0; l <= L; l++) {
-            __syncthreads();   // the previous level's reads of Rl / Yl are done
-            if (lane < NB) {   // rows of neighbours outside every list stay zero for the dense loop
+        if constexpr (PREG) {
+            double P0[L + 2], P1[L + 2];
 #pragma unroll
-                for (int a = 0; a < N; a++) Rl[lane * RW + a] = s >= 0 ? fc * rs.F0[a] : 0.0;
+            for (int l = 0; l <= L; l++) {
+                double *cur = (l & 1) ? P1 : P0;
+                const double *prv = (l & 1) ? P0 : P1;
+                SOAP_FWD_LEVEL(l, cur, prv);
             }
-            if (s >= 0) {
-                rs.step(l);
+        } else {
+            double *Pq = Pq0 + lane * PR;   // this lane's two Legendre rows
+#pragma unroll 1
+            for (int l = 0; l <= L; l++) {
                 double *cur = Pq + (l & 1) * (L + 2);
-                legendre_level(l, uz, Pq + ((l + 1) & 1) * (L + 2), cur);
-                double A = 1.0, B = 0.0;
-                double *Yr = Yl + lane * YW;
-                for (int m = 0; m <= l; m++) {
-                    if (m > 0) {
-                        const double An = A * ux - B * uy;
-                        B = A * uy + B * ux;
-                        A = An;
-                    }
-                    const double Nn = c_ynorm[l * (l + 1) / 2 + m] * cur[m];
-                    if (m == 0) Yr[l] = Nn;
-                    else {
-                        Yr[l + m] = Nn * A;
-                        Yr[l - m] = Nn * B;
-                    }
-                }
-            } else if (lane < NB) {
-                for (int m = 0; m < 2 * l + 1; m++) Yl[lane * YW + m] = 0.0;
-            }
-            __syncthreads();
-            const int W = 2 * l + 1;
-            for (int it = lane; it < N * W; it += 64) {
-                const int a = it / W, mi = it - a * W;
-                if (S == 1) {   // one species: every row of the chunk, zero rows included
-                    double acc0 = 0.0, acc1 = 0.0;
-#pragma unroll 4
-                    for (int k = 0; k < NB; k += 2) {
-                        acc0 += Rl[k * RW + a] * Yl[k * YW + mi];
-                        acc1 += Rl[(k + 1) * RW + a] * Yl[(k + 1) * YW + mi];
-                    }
-                    cP[a * LM + l * l + mi] += acc0 + acc1;
-                    continue;
-                }
-                for (int si = 0; si < S; si++) {
-                    const int n = ns[si];
-                    const int *ki = kidx + si * NB;
-                    double acc0 = 0.0, acc1 = 0.0;
-                    int q = 0;
-#pragma unroll 2
-                    for (; q + 1 < n; q += 2) {
-                        const int k0 = ki[q], k1 = ki[q + 1];
-                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];
-                        acc1 += Rl[k1 * RW + a] * Yl[k1 * YW + mi];
-                    }
-                    if (q < n) {
-                        const int k0 = ki[q];
-                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];
-                    }
-                    cP[(si * N + a) * LM + l * l + mi] += acc0 + acc1;
-                }
+                const double *prv = Pq + ((l + 1) & 1) * (L + 2);
+                SOAP_FWD_LEVEL(l, cur, prv);
             }
         }
     }
@@ -569,18 +590,24 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     // neighbour chunk of 32: covers a centre's neighbours within 5 A in one pass
     // for covalent solids and keeps the block's LDS small
     const int NB = kSoapChunk;
+    const bool preg = sp.n >= 10 || sp.L >= 8 || nc < 16384;
     const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + NB * ((sp.n | 1) + 2 * sp.L + 1) +
-                               NB * ((2 * (sp.L + 2)) | 1)) * 8 + (size_t)sp.S * (NB + 1) * 4;
+                               (preg ? 0 : NB * ((2 * (sp.L + 2)) | 1))) * 8 + (size_t)sp.S * (NB + 1) * 4;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
-        hipLaunchKernelGGL((k_soap_fwd_x<NN, LL>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, nb_j,     \
-                           nb_r, nb_d, X, norm, cbuf);                                                          \
+        if (preg)                                                                                               \
+            hipLaunchKernelGGL((k_soap_fwd_x<NN, LL, true>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off,  \
+                               nb_j, nb_r, nb_d, X, norm, cbuf);                                                \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_soap_fwd_x<NN, LL, false>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, \
+                               nb_j, nb_r, nb_d, X, norm, cbuf);                                                \
         return true;                                                                                            \
     }
     SOAP_FAST(LAUNCH)
 #undef LAUNCH
     return false;
 }
+#undef SOAP_FWD_LEVEL
 
 template <int N, int L>
 static void bwd_go(bool occ3, int nblk, size_t sh, hipStream_t s, const SoapDev &sp, int nc, int cpb,
