@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
     double *gA = rbs + N, *iA = gA + N;
     double *dcs_all = rbs + 3 * N; // cpb x K*LM
     double *csc = dcs_all + cpb * KLM;   // cpb x (dot, 1/|p|)
-    double *stg = csc + 2 * cpb;   // max(K*K + K*(2L+1), 64*PR) or, whole, cpb*(K*LM + d)
+    double *stg = csc + 2 * cpb;   // max(K*K + K*(2L+1), 64*PR), or max(cpb*d, 64*PR) when whole
     const int lane = threadIdx.x;
     const int d0 = cpb * blockIdx.x;
     const int nh = min(cpb, nc - d0);
@@ -425,19 +425,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
             csc[2 * h] = dot;
             csc[2 * h + 1] = nrm > 0.0 ? 1.0 / nrm : 0.0;
         }
-        if (whole) {   // c and dE/dp of every centre fit in the staging region together
+        if (whole) {   // dE/dp of every centre staged together; c is read from cbuf
             const double nrm = norm[desc];
             const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-            double *cL = stg + h * (KLM + d), *dpL = cL + KLM;
+            double *dpL = stg + h * d;
             for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
-            for (int e = lane; e < KLM; e += 64) cL[e] = cbuf[(size_t)desc * KLM + e];
         }
     }
     __syncthreads();
     if (whole) {   // dE/dc in one pass over (centre, ia, lm)
         for (int it = lane; it < nh * KLM; it += 64) {
             const int h = it / KLM, e = it - h * KLM;
-            const double *cL = stg + h * (KLM + d), *dpL = cL + KLM;
+            const double *dpL = stg + h * d, *cL = cbuf + (size_t)(d0 + h) * KLM;
             const int ia = e / LM, lm = e - ia * LM;
             int l = (int)sqrt((double)lm + 0.5);
             if (l * l > lm) l--;
@@ -634,8 +633,8 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     const int LM = (sp.L + 1) * (sp.L + 1);
     const int KLM = sp.K * LM, PR = (2 * (sp.L + 2)) | 1;
     const size_t stg0 = std::max((size_t)sp.K * sp.K + (size_t)sp.K * (2 * sp.L + 1), (size_t)64 * PR);
-    // whole: c and dE/dp of all centres of the block staged at once (small descriptors)
-    auto stg = [&](int cpb, int whole) { return whole ? std::max(stg0, (size_t)cpb * (KLM + sp.dim)) : stg0; };
+    // whole: dE/dp of all centres of the block staged at once (small descriptors)
+    auto stg = [&](int cpb, int whole) { return whole ? std::max(stg0, (size_t)cpb * sp.dim) : stg0; };
     auto need = [&](int cpb, int whole) {
         return ((size_t)sp.n * sp.n + 3 * sp.n + (size_t)cpb * (KLM + 2) + stg(cpb, whole)) * 8;
     };
