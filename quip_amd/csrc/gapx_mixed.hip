@@ -429,10 +429,11 @@ static bool use_lds_tiles(int rows, int cols) {
 // walks the sparse points of its part in chunks of 32.  Per chunk the chunk's
 // rows of Xs (k = descriptor index) and of Xs^T (k = sparse index) are staged
 // in LDS; each wave computes its 16 x 32 tile of K = X Xs^T (three MFMA
-// passes, FP32 accumulation), applies the FP64 epilogue (energy w K^zeta and
-// W = w zeta K^(zeta-1)), splits W into hi/lo through a per-wave LDS tile and
-// accumulates G += W Xs (16 x 32DK, in registers).  W never reaches HBM.
-// The next chunk is fetched into registers while the current one is used.
+// passes, FP32 accumulation), applies the epilogue (energy w K^zeta summed in
+// FP64, W = w zeta K^(zeta-1)), splits W into hi/lo through a per-wave LDS tile
+// and accumulates G += W Xs (16 x 32DK, in registers).  W never reaches HBM.
+// The next chunk is fetched into registers while the current one is used and
+// written to the other of two LDS buffers after it, one barrier per chunk.
 // With several parts (gridDim.y) the sparse points are divided between blocks
 // and the FP32 partial G tiles are summed in a fixed order afterwards.
 // ---------------------------------------------------------------------------
@@ -450,11 +451,19 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     typedef typename H::T T;
     typedef typename H::V V;
     constexpr int DP = 32 * DK;     // descriptor components handled (zero padded)
-    constexpr int XS_LD = DP + 8;   // LDS row strides (elements): 16-byte reads of 16 rows hit distinct banks
-    constexpr int XT_LD = 40;
-    __shared__ __attribute__((aligned(16))) T xs_l[2][32 * XS_LD];
-    __shared__ __attribute__((aligned(16))) T xt_l[2][DP * XT_LD];
-    __shared__ __attribute__((aligned(16))) T w_l[8][2][16 * XT_LD];
+    // LDS rows without padding; 16-byte vectors XOR-swizzled so that the 16 rows one
+    // fragment read touches fall on 16 distinct 16-byte bank groups:
+    //   Xs rows (XSV vectors):                     vector c of row r at c ^ (r & XSM)
+    //   Xs^T and W rows (4 vectors of 8 sparse points): vector c of row q at c ^ ((q >> 2) & 3)
+    // Two chunk buffers: the next chunk is written while the current one is read, so
+    // each chunk needs one barrier.
+    constexpr int XSV = DP / 8;                     // 8, 16 or 32
+    constexpr int XSM = XSV >= 16 ? 15 : XSV - 1;
+    __shared__ __attribute__((aligned(16))) T xs_l[2][2][32 * DP];   // [buffer][hi, lo]
+    __shared__ __attribute__((aligned(16))) T xt_l[2][2][DP * 32];
+    __shared__ __attribute__((aligned(16))) T w_l[8][2][16 * 32];
+    auto xs_at = [](int r, int c) { return r * DP + ((c ^ (r & XSM)) << 3); };
+    auto xt_at = [](int q, int c) { return q * 32 + ((c ^ ((q >> 2) & 3)) << 3); };
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
     const int row0 = blockIdx.x * 128 + wv * 16;
@@ -509,28 +518,33 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
             }
         }
     };
-    if (c_beg < c_end) fetch(32 * c_beg);
-    for (int ch = c_beg; ch < c_end; ch++) {
-        __syncthreads();   // the previous chunk's LDS reads are done
+    auto put = [&](int b) {   // the fetched chunk -> LDS buffer b
 #pragma unroll
         for (int u = 0; u < XS_V; u++) {
             const int idx = t + 512 * u;
-            const int r = idx / (DP / 8), c8 = (idx % (DP / 8)) * 8;
             if (idx < XS_N) {
-                *(V *)(xs_l[0] + r * XS_LD + c8) = ps_h[u];
-                *(V *)(xs_l[1] + r * XS_LD + c8) = ps_l[u];
+                const int o = xs_at(idx / XSV, idx % XSV);
+                *(V *)(xs_l[b][0] + o) = ps_h[u];
+                *(V *)(xs_l[b][1] + o) = ps_l[u];
             }
         }
 #pragma unroll
         for (int u = 0; u < XT_V; u++) {
             const int idx = t + 512 * u;
-            const int q = idx >> 2, c8 = (idx & 3) * 8;
             if (idx < XT_N) {
-                *(V *)(xt_l[0] + q * XT_LD + c8) = pt_h[u];
-                *(V *)(xt_l[1] + q * XT_LD + c8) = pt_l[u];
+                const int o = xt_at(idx >> 2, idx & 3);
+                *(V *)(xt_l[b][0] + o) = pt_h[u];
+                *(V *)(xt_l[b][1] + o) = pt_l[u];
             }
         }
-        __syncthreads();
+    };
+    if (c_beg < c_end) {
+        fetch(32 * c_beg);
+        put(0);
+    }
+    __syncthreads();
+    for (int ch = c_beg; ch < c_end; ch++) {
+        const int b = (ch - c_beg) & 1;
         const int s0 = 32 * ch;
         if (ch + 1 < c_end) fetch(s0 + 32);
         // K tile: 16 centres x 32 sparse points
@@ -539,8 +553,8 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
         for (int ks = 0; ks < DK; ks++)
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                const int o = (16 * j + r16) * XS_LD + 32 * ks + kg;
-                const V bh = *(const V *)(xs_l[0] + o), bl = *(const V *)(xs_l[1] + o);
+                const int o = xs_at(16 * j + r16, 4 * ks + (kg >> 3));
+                const V bh = *(const V *)(xs_l[b][0] + o), bl = *(const V *)(xs_l[b][1] + o);
                 kacc[j] = H::mfma(xl[ks], bh, kacc[j]);
                 kacc[j] = H::mfma(xh[ks], bl, kacc[j]);
                 kacc[j] = H::mfma(xh[ks], bh, kacc[j]);
@@ -578,23 +592,28 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
                     h = H::from(wk);
                     lo = H::from(wk - H::to(h));
                 }
-                w_l[wv][0][(rq + v) * XT_LD + 16 * j + r16] = h;
-                w_l[wv][1][(rq + v) * XT_LD + 16 * j + r16] = lo;
+                const int row = rq + v, e = 16 * j + r16;
+                const int o = xt_at(row, e >> 3) + (e & 7);
+                w_l[wv][0][o] = h;
+                w_l[wv][1][o] = lo;
             }
         }
         // the W tile is this wave's own: a wavefront barrier makes it visible in A-operand layout
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const V wh = *(const V *)(w_l[wv][0] + r16 * XT_LD + kg), wl = *(const V *)(w_l[wv][1] + r16 * XT_LD + kg);
+        const int ow = xt_at(r16, kg >> 3);
+        const V wh = *(const V *)(w_l[wv][0] + ow), wl = *(const V *)(w_l[wv][1] + ow);
 #pragma unroll
         for (int q = 0; q < 2 * DK; q++) {
-            const int o = (16 * q + r16) * XT_LD + kg;
-            const V bh = *(const V *)(xt_l[0] + o), bl = *(const V *)(xt_l[1] + o);
+            const int o = xt_at(16 * q + r16, kg >> 3);
+            const V bh = *(const V *)(xt_l[b][0] + o), bl = *(const V *)(xt_l[b][1] + o);
             gacc[q] = H::mfma(wl, bh, gacc[q]);
             gacc[q] = H::mfma(wh, bl, gacc[q]);
             gacc[q] = H::mfma(wh, bh, gacc[q]);
         }
+        if (ch + 1 < c_end) put(b ^ 1);   // the other buffer was last read before the previous barrier
+        __syncthreads();
     }
     // energy partials of this part: lanes sharing lane>>4 hold the same centres
 #pragma unroll
