@@ -190,13 +190,13 @@ struct Rad {
             __syncthreads();   /* the previous level's reads of Rl / Yl are done */                                  \
             if (lane < NB) {   /* rows of neighbours outside every list stay zero for the dense loop */              \
 _Pragma("unroll")                                                                                                    \
-                for (int a = 0; a < N; a++) Rl[lane * RW + a] = s >= 0 ? fc * rs.F0[a] : 0.0;                        \
+                for (int a = 0; a < N; a++) Rl[a * RT + lane] = s >= 0 ? fc * rs.F0[a] : 0.0;                        \
             }                                                                                                        \
             if (s >= 0) {                                                                                            \
                 rs.step(l);                                                                                          \
                 legendre_level(l, uz, prv, cur);                                                                     \
                 double A = 1.0, B = 0.0;                                                                             \
-                double *Yr = Yl + lane * YW;                                                                         \
+                double *Yr = Yl + lane;   /* slot m of this neighbour at Yr[m * RT] */                               \
                 for (int m = 0; m <= l; m++) {                                                                       \
                     if (m > 0) {                                                                                     \
                         const double An = A * ux - B * uy;                                                           \
@@ -204,25 +204,27 @@ _Pragma("unroll")                                                               
                         A = An;                                                                                      \
                     }                                                                                                \
                     const double Nn = c_ynorm[l * (l + 1) / 2 + m] * cur[m];                                         \
-                    if (m == 0) Yr[l] = Nn;                                                                          \
+                    if (m == 0) Yr[l * RT] = Nn;                                                                     \
                     else {                                                                                           \
-                        Yr[l + m] = Nn * A;                                                                          \
-                        Yr[l - m] = Nn * B;                                                                          \
+                        Yr[(l + m) * RT] = Nn * A;                                                                   \
+                        Yr[(l - m) * RT] = Nn * B;                                                                   \
                     }                                                                                                \
                 }                                                                                                    \
             } else if (lane < NB) {                                                                                  \
-                for (int m = 0; m < 2 * l + 1; m++) Yl[lane * YW + m] = 0.0;                                         \
+                for (int m = 0; m < 2 * l + 1; m++) Yl[m * RT + lane] = 0.0;                                         \
             }                                                                                                        \
             __syncthreads();                                                                                         \
             const int W = 2 * l + 1;                                                                                 \
             for (int it = lane; it < N * W; it += 64) {                                                              \
                 const int a = it / W, mi = it - a * W;                                                               \
-                if (S == 1) {   /* one species: every row of the chunk, zero rows included */                        \
+                if (S == 1) {   /* one species: every row of the chunk in pairs, zero rows included */            \
+                    const double2 *R2 = (const double2 *)(Rl + a * RT), *Y2 = (const double2 *)(Yl + mi * RT);      \
                     double acc0 = 0.0, acc1 = 0.0;                                                                   \
 _Pragma("unroll 4")                                                                                                  \
-                    for (int k = 0; k < NB; k += 2) {                                                                \
-                        acc0 += Rl[k * RW + a] * Yl[k * YW + mi];                                                    \
-                        acc1 += Rl[(k + 1) * RW + a] * Yl[(k + 1) * YW + mi];                                        \
+                    for (int k2 = 0; k2 < NB / 2; k2++) {                                                               \
+                        const double2 rv = R2[k2], yv = Y2[k2];                                                      \
+                        acc0 += rv.x * yv.x;                                                                         \
+                        acc1 += rv.y * yv.y;                                                                         \
                     }                                                                                                \
                     cP[a * LM + l * l + mi] += acc0 + acc1;                                                          \
                     continue;                                                                                        \
@@ -235,12 +237,12 @@ _Pragma("unroll 4")                                                             
 _Pragma("unroll 2")                                                                                                  \
                     for (; q + 1 < n; q += 2) {                                                                      \
                         const int k0 = ki[q], k1 = ki[q + 1];                                                        \
-                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];                                                  \
-                        acc1 += Rl[k1 * RW + a] * Yl[k1 * YW + mi];                                                  \
+                        acc0 += Rl[a * RT + k0] * Yl[mi * RT + k0];                                                  \
+                        acc1 += Rl[a * RT + k1] * Yl[mi * RT + k1];                                                  \
                     }                                                                                                \
                     if (q < n) {                                                                                     \
                         const int k0 = ki[q];                                                                        \
-                        acc0 += Rl[k0 * RW + a] * Yl[k0 * YW + mi];                                                  \
+                        acc0 += Rl[a * RT + k0] * Yl[mi * RT + k0];                                                  \
                     }                                                                                                \
                     cP[(si * N + a) * LM + l * l + mi] += acc0 + acc1;                                               \
                 }                                                                                                    \
@@ -265,13 +267,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREG ? 2 : W
     double *rbs = Ts + N * N;      // N grid points, then exp(-alpha r_a^2) and 1/a (N each)
     double *gA = rbs + N, *iA = gA + N;
     double *cP = rbs + 3 * N;      // K*LM: c' during accumulation, c after the transform
-    // Per-lane rows use odd strides (in doubles): 32 lanes of an 8-byte access then
-    // fall on distinct bank pairs.
-    constexpr int RW = N | 1;      // row stride of Rl
-    constexpr int PR = (2 * (L + 2)) | 1;   // row stride of a lane's two Legendre rows
-    double *Rl = cP + KLM;         // NB x RW: fc F_l(a) of neighbour k
-    double *Yl = Rl + NB * RW;     // NB x YW
-    double *Pq0 = Yl + NB * YW;    // NB x PR: the lanes' Legendre rows (not PREG)
+    // The staged levels are stored one row per radial point a (Rl) and per m slot (Yl),
+    // neighbours along the row: the writes of 32 lanes are consecutive, and the dense
+    // accumulation reads two neighbours per 16-byte access.  The row stride RT = NB + 2
+    // keeps rows 16-byte aligned (N (N + 3) is even, and Rl's offset is rounded up to even)
+    // and puts successive rows 16 bytes apart in the bank cycle.
+    constexpr int RT = NB + 2;     // row stride of Rl and Yl
+    constexpr int PR = (2 * (L + 2)) | 1;   // row stride of a lane's two Legendre rows (odd: distinct banks)
+    double *Rl = cP + ((KLM + 1) & ~1);   // N x RT: fc F_l(a) of neighbour k at Rl[a * RT + k]
+    double *Yl = Rl + N * RT;      // YW x RT: Y_l,m of neighbour k at Yl[slot * RT + k]
+    double *Pq0 = Yl + YW * RT;    // NB x PR: the lanes' Legendre rows (not PREG)
     int *kidx = (int *)(Pq0 + (PREG ? 0 : NB * PR));   // S x NB: the chunk's neighbours of each species
     int *ns = kidx + S * NB;       // their counts
     const int lane = threadIdx.x;
@@ -590,7 +595,7 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     // for covalent solids and keeps the block's LDS small
     const int NB = kSoapChunk;
     const bool preg = sp.n >= 10 || sp.L >= 8 || nc < 16384;
-    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + NB * ((sp.n | 1) + 2 * sp.L + 1) +
+    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + 1 + (NB + 2) * (sp.n + 2 * sp.L + 1) +
                                (preg ? 0 : NB * ((2 * (sp.L + 2)) | 1))) * 8 + (size_t)sp.S * (NB + 1) * 4;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
