@@ -92,17 +92,14 @@ __device__ __forceinline__ void legendre_level(int l, double uz, const double *p
 
 // i_l recursion state for all grid points: F0 = f_l, F1 = f_(l+1).  The
 // recursion factor 1/x_a = 1/(2 alpha r r_a) is ixq / a with ixq = 1/(2 alpha r dr)
-// (r_a = a dr).  KEEP_IX stores it per point (more registers; measured faster in
-// launches too small to fill the GPU), otherwise it is formed from ixq when used
-// (fewer registers: more wavefronts per SIMD in large launches).
+// (r_a = a dr), formed from ixq when used (no per-point array: fewer registers).
 // rbs: grid r_a = a * dr; gA[a] = exp(-alpha r_a^2); iA[a] = 1/a.
 // exp(-alpha (r -+ r_a)^2) = exp(-alpha r^2) gA[a] q^(+-a) with q = exp(2 alpha r dr):
 // two exponentials and one division per neighbour.  Outside the range where the
 // powers stay finite, every point is direct.
-template <int N, bool KEEP_IX = false>
+template <int N>
 struct Rad {
     double F0[N], F1[N], ixq;
-    double ix[KEEP_IX ? N : 1];
     __device__ __forceinline__ void init(const double *rbs, const double *gA, const double *iA, double alpha,
                                          double r) {
         const double xq = N > 1 ? 2.0 * alpha * r * rbs[1] : 0.0;
@@ -114,7 +111,6 @@ struct Rad {
             double qa = 1.0, qia = 1.0;
             F0[0] = e0;
             F1[0] = 0.0;
-            if constexpr (KEEP_IX) ix[0] = 0.0;
 #pragma unroll
             for (int a = 1; a < N; a++) {
                 qa *= q;
@@ -124,27 +120,13 @@ struct Rad {
                 const double iv = ixq * iA[a];
                 F0[a] = 0.5 * (em - ep) * iv;
                 F1[a] = 0.5 * (em + ep) * iv - F0[a] * iv;
-                if constexpr (KEEP_IX) ix[a] = iv;
             }
             return;
         }
 #pragma unroll
         for (int a = 0; a < N; a++) {
             const double ra = rbs[a];
-            if constexpr (KEEP_IX) {
-                const double x = 2.0 * alpha * r * ra;
-                if (x == 0.0) {
-                    F0[a] = exp(-alpha * (ra * ra + r * r));
-                    F1[a] = 0.0;
-                    ix[a] = 0.0;
-                } else {
-                    const double ep = exp(-alpha * (r + ra) * (r + ra)), em = exp(-alpha * (r - ra) * (r - ra));
-                    const double iv = 1.0 / x;
-                    F0[a] = 0.5 * (em - ep) * iv;
-                    F1[a] = 0.5 * (em + ep) * iv - F0[a] * iv;
-                    ix[a] = iv;
-                }
-            } else if (a == 0) {
+            if (a == 0) {
                 F0[a] = exp(-alpha * (ra * ra + r * r));
                 F1[a] = 0.0;
             } else {
@@ -156,23 +138,32 @@ struct Rad {
             __builtin_amdgcn_sched_barrier(0);   // one grid point's exponentials at a time
         }
     }
+    // Scaled form (the adjoint): F1 holds a f_(l+1)(a), so that the sum over grid points
+    // of r_a f_(l+1)(a) v_a is dr times the sum of F1[a] v_a, with no product by r_a.
+    // Starting from the state init leaves, step_scaled keeps this form:
+    // a f_(l+2) = a f_l - (2l+3) ixq f_(l+1) and f_(l+1) = F1[a] / a.
+    __device__ __forceinline__ void scale_f1() {
+#pragma unroll
+        for (int a = 0; a < N; a++) F1[a] *= (double)a;
+    }
+    __device__ __forceinline__ void step_scaled(int l) {
+        const double c = (2.0 * l + 3.0) * ixq;
+        F0[0] = 0.0;   // grid point 0 (r_a = 0): f_(l+1) = 0 for every l
+#pragma unroll
+        for (int a = 1; a < N; a++) {
+            const double t = F1[a] * (1.0 / (double)a);
+            F1[a] = (double)a * F0[a] - c * t;
+            F0[a] = t;
+        }
+    }
     __device__ __forceinline__ void step(int l) {
-        if constexpr (KEEP_IX) {
+        F0[0] = F1[0];   // grid point 0 (r_a = 0): f_(l+1) = 0 for every l
+        F1[0] = 0.0;
 #pragma unroll
-            for (int a = 0; a < N; a++) {
-                const double n2 = F0[a] - (2.0 * l + 3.0) * ix[a] * F1[a];
-                F0[a] = F1[a];
-                F1[a] = (ix[a] == 0.0) ? 0.0 : n2;
-            }
-        } else {
-            F0[0] = F1[0];   // grid point 0 (r_a = 0): f_(l+1) = 0 for every l
-            F1[0] = 0.0;
-#pragma unroll
-            for (int a = 1; a < N; a++) {
-                const double n2 = F0[a] - (2.0 * l + 3.0) * (ixq * (1.0 / (double)a)) * F1[a];
-                F0[a] = F1[a];
-                F1[a] = n2;
-            }
+        for (int a = 1; a < N; a++) {
+            const double n2 = F0[a] - (2.0 * l + 3.0) * (ixq * (1.0 / (double)a)) * F1[a];
+            F0[a] = F1[a];
+            F1[a] = n2;
         }
     }
 };
@@ -514,6 +505,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
     double *Pq = stg + lane * PR;   // this lane's two Legendre rows
     const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
     const double al2 = 2.0 * sp.alpha;
+    const double dr = rbs[1];   // grid spacing: r_a = a dr
     for (int k = sub; k < np; k += lanes_per) {
         const int p = p0 + k;
         const double r = nb_r[p];
@@ -523,16 +515,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
         const double ir = 1.0 / r;
         const double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
         const double *dcs = dcP + s * N * LM;
-        Rad<N, (OCC <= WPE_BWD)> rs;
+        Rad<N> rs;
         rs.init(rbs, gA, iA, sp.alpha, r);
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        rs.scale_f1();
+        // Per (l, m): S0 = sum_a f_l(a) dc'[a], S1 = sum_a r_a f_(l+1)(a) dc'[a] (dr times the
+        // scaled sum), s2 = fc S0 and s1 = fc (cl S0 + 2 alpha S1) + dfc S0 with cl = -2 alpha r + l / r;
+        // the force on the pair is sum (s1 Y - s2 (u.gY) / r) u + sum s2 gY / r over (l, m), gY the
+        // angular gradient of Y.  Only the sums T0 = sum S0 Y, T0L = sum l S0 Y, T1 = sum S1 Y and
+        // V = sum S0 gY are kept; a pair of real harmonics (m and -m) is handled together.
+        double T0 = 0.0, T0L = 0.0, T1 = 0.0, Vx = 0.0, Vy = 0.0, Vz = 0.0;
 #pragma unroll 1
         for (int l = 0; l <= L; l++) {
-            const double cl = -al2 * r + (double)l * ir;
             double *cur = Pq + (l & 1) * (L + 2);
             legendre_level(l, uz, Pq + ((l + 1) & 1) * (L + 2), cur);
             cur[l + 1] = 0.0;
             double A = 1.0, B = 0.0, Am = 0.0, Bm = 0.0;
+            double t0 = 0.0;
             for (int m = 0; m <= l; m++) {
                 if (m > 0) {
                     Am = A;
@@ -542,32 +540,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
                     A = An;
                 }
                 const double Nn = c_ynorm[l * (l + 1) / 2 + m];
-                const double P = cur[m], Pz = cur[m + 1];
+                const double NnP = Nn * cur[m], NnPz = Nn * cur[m + 1];
+                const double *dca = dcs + l * l + l + m, *dcb = dcs + l * l + l - m;
+                double S0a = 0.0, S1a = 0.0, S0b = 0.0, S1b = 0.0;
 #pragma unroll
-                for (int sg = 0; sg < 2; sg++) {
-                    if (m == 0 && sg == 1) continue;
-                    double Y, Fx, Fy, Fz;
-                    if (m == 0) { Y = Nn * P; Fx = 0.0; Fy = 0.0; Fz = Nn * Pz; }
-                    else if (sg == 0) { Y = Nn * P * A; Fx = Nn * P * m * Am; Fy = -Nn * P * m * Bm; Fz = Nn * Pz * A; }
-                    else { Y = Nn * P * B; Fx = Nn * P * m * Bm; Fy = Nn * P * m * Am; Fz = Nn * Pz * B; }
-                    const int lm = l * l + l + (sg ? -m : m);
-                    double S0 = 0.0, S1 = 0.0;
+                for (int a = 0; a < N; a++) {
+                    const double x = dca[a * LM];
+                    S0a += rs.F0[a] * x;
+                    S1a += rs.F1[a] * x;
+                }
+                if (m > 0) {
 #pragma unroll
                     for (int a = 0; a < N; a++) {
-                        const double dc = dcs[a * LM + lm];
-                        S0 += rs.F0[a] * dc;
-                        S1 += rbs[a] * rs.F1[a] * dc;
+                        const double x = dcb[a * LM];
+                        S0b += rs.F0[a] * x;
+                        S1b += rs.F1[a] * x;
                     }
-                    const double s2 = fc * S0, s1 = fc * (cl * S0 + al2 * S1) + dfc * S0;
-                    const double ud = ux * Fx + uy * Fy + uz * Fz;
-                    const double a1 = s1 * Y - s2 * ud * ir, a2 = s2 * ir;
-                    fx += a1 * ux + a2 * Fx;
-                    fy += a1 * uy + a2 * Fy;
-                    fz += a1 * uz + a2 * Fz;
+                }
+                // Y = NnP (A, B), gY_z = NnPz (A, B), (gY_x, gY_y) = m NnP (Am, -Bm) and (Bm, Am)
+                const double u0 = S0a * A + S0b * B, u1 = S1a * A + S1b * B;
+                t0 += NnP * u0;
+                T1 += NnP * u1;
+                Vz += NnPz * u0;
+                if (m > 0) {
+                    const double NnPm = NnP * (double)m;
+                    Vx += NnPm * (S0a * Am + S0b * Bm);
+                    Vy += NnPm * (S0b * Am - S0a * Bm);
                 }
             }
-            rs.step(l);
+            T0 += t0;
+            T0L += (double)l * t0;
+            rs.step_scaled(l);
         }
+        const double sY = (dfc - al2 * r * fc) * T0 + fc * ir * T0L + al2 * fc * dr * T1;
+        const double gx = fc * Vx, gy = fc * Vy, gz = fc * Vz;
+        const double a1 = sY - (ux * gx + uy * gy + uz * gz) * ir;
+        const double fx = a1 * ux + gx * ir, fy = a1 * uy + gy * ir, fz = a1 * uz + gz * ir;
         pairF[3 * p] += fx;
         pairF[3 * p + 1] += fy;
         pairF[3 * p + 2] += fz;
