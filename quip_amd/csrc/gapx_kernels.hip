@@ -156,47 +156,69 @@ __global__ void k_scan_apply(const int *in, int n, const int *partials, int *out
         ex += v[k];
     }
 }
-// One block of 1024 threads for n <= kScanOne: thread t scans its contiguous run of
-// ceil(n / 1024) items; a wave scan and a scan of the 16 wave totals join the runs.
-// Block b of the launch scans array b; mx (nullable) gets each array's largest item.
-constexpr int kScanOne = 65536;
+// One block of 1024 threads for n <= kScanOne.  The array passes through LDS in chunks
+// of kScanChunk items: loaded and stored coalesced, scanned as 16 contiguous items per
+// thread (item i sits at i + i/16, so a wave's 16-item runs fall on distinct banks),
+// with a wave scan and a scan of the 16 wave totals joining the runs and a carry
+// joining the chunks.  Block b of the launch scans array b; mx (nullable) gets each
+// array's largest item.
+constexpr int kScanOne = 1 << 18;
+constexpr int kScanChunk = 16 * 1024;
 struct ScanSet { const int *in[2]; int *out[2]; };
 __global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx) {
+    __shared__ int buf[kScanChunk + kScanChunk / 16];
     __shared__ int wsum[16], wmax[16];
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int *in = ss.in[b];
     int *out = ss.out[b];
-    const int per = (n + 1023) / 1024, i0 = t * per, i1 = min(n, i0 + per);
-    int s = 0, m = 0;
-    for (int i = i0; i < i1; i++) {
-        const int v = in[i];
-        s += v;
-        m = max(m, v);
-    }
-    int x = s;   // inclusive wave scan of the run sums
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    int wm = m;
-    for (int off = 32; off > 0; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
-    if (lane == 63) wsum[wv] = x;
-    if (lane == 0) wmax[wv] = wm;
-    __syncthreads();
-    int before = 0, total = 0, big = 0;
-    for (int k = 0; k < 16; k++) {
-        before += k < wv ? wsum[k] : 0;
-        total += wsum[k];
-        big = max(big, wmax[k]);
-    }
-    int ex = before + x - s;
-    for (int i = i0; i < i1; i++) {
-        const int v = in[i];
-        out[i] = ex;
-        ex += v;
+    int carry = 0, big = 0;
+    for (int c0 = 0; c0 < n; c0 += kScanChunk) {
+        const int cn = min(kScanChunk, n - c0);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = t + 1024 * k;
+            buf[i + (i >> 4)] = i < cn ? in[c0 + i] : 0;
+        }
+        __syncthreads();
+        int v[16], s = 0, m = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            v[k] = buf[17 * t + k];
+            s += v[k];
+            m = max(m, v[k]);
+        }
+        int x = s;   // inclusive wave scan of the run sums
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+        if (lane == 63) wsum[wv] = x;
+        if (lane == 0) wmax[wv] = m;
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int q = 0; q < 16; q++) {
+            before += q < wv ? wsum[q] : 0;
+            total += wsum[q];
+            big = max(big, wmax[q]);
+        }
+        int ex = carry + before + x - s;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            buf[17 * t + k] = ex;
+            ex += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int i = t + 1024 * k;
+            if (i < cn) out[c0 + i] = buf[i + (i >> 4)];
+        }
+        carry += total;
+        __syncthreads();   // buf and the wave totals are rewritten by the next chunk
     }
     if (t == 0) {
-        out[n] = total;
+        out[n] = carry;
         if (mx) mx[b] = big;
     }
 }
@@ -494,70 +516,87 @@ void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, c
 #undef NLP
 }
 
-// Per-atom slots of the one-pass search -> the CSR tables (one wavefront per atom).
+// Per-atom slots of the one-pass search -> the CSR tables (one thread per slot).
 __global__ __launch_bounds__(256) void k_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off,
                                                     const int *t1_j, const int *t1_shift, const double *t1_r,
                                                     const int *t2_i, const int *t2_k, int *n1_j, int *n1_shift,
                                                     double *n1_r, int *n2_i, int *n2_k) {
-    const int lane = threadIdx.x & 63;
-    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (a >= N) return;
-    const int b1 = n1_off[a], c1 = n1_off[a + 1] - b1;
-    for (int q = lane; q < c1; q += 64) {
-        const size_t src = (size_t)a * cap1 + q;
-        n1_j[b1 + q] = t1_j[src];
-        n1_shift[3 * (b1 + q)] = t1_shift[3 * src];
-        n1_shift[3 * (b1 + q) + 1] = t1_shift[3 * src + 1];
-        n1_shift[3 * (b1 + q) + 2] = t1_shift[3 * src + 2];
-        n1_r[b1 + q] = t1_r[src];
+    // one thread per slot: the N * cap1 neighbour1 slots, then the N * cap2 neighbour2 slots
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x, n1s = (size_t)N * cap1;
+    if (idx < n1s) {
+        const int a = (int)(idx / cap1), q = (int)(idx - (size_t)a * cap1);
+        const int b1 = n1_off[a];
+        if (q >= n1_off[a + 1] - b1) return;
+        n1_j[b1 + q] = t1_j[idx];
+        n1_shift[3 * (b1 + q)] = t1_shift[3 * idx];
+        n1_shift[3 * (b1 + q) + 1] = t1_shift[3 * idx + 1];
+        n1_shift[3 * (b1 + q) + 2] = t1_shift[3 * idx + 2];
+        n1_r[b1 + q] = t1_r[idx];
+        return;
     }
-    const int b2 = n2_off[a], c2 = n2_off[a + 1] - b2;
-    for (int q = lane; q < c2; q += 64) {
-        const size_t src = (size_t)a * cap2 + q;
-        n2_i[b2 + q] = t2_i[src];
-        n2_k[b2 + q] = t2_k[src];
-    }
+    const size_t r = idx - n1s;
+    if (r >= (size_t)N * cap2) return;
+    const int a = (int)(r / cap2), q = (int)(r - (size_t)a * cap2);
+    const int b2 = n2_off[a];
+    if (q >= n2_off[a + 1] - b2) return;
+    n2_i[b2 + q] = t2_i[r];
+    n2_k[b2 + q] = t2_k[r];
 }
 void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off, const int *t1_j,
                        const int *t1_shift, const double *t1_r, const int *t2_i, const int *t2_k, int *n1_j,
                        int *n1_shift, double *n1_r, int *n2_i, int *n2_k, hipStream_t s) {
     if (N > 0)
-        hipLaunchKernelGGL(k_nl_compact, dim3(cdiv(N, 4)), dim3(256), 0, s, N, cap1, cap2, n1_off, n2_off, t1_j,
+        hipLaunchKernelGGL(k_nl_compact, dim3((unsigned)(((size_t)N * (cap1 + cap2) + 255) / 256)), dim3(256), 0, s,
+                           N, cap1, cap2, n1_off, n2_off, t1_j,
                            t1_shift, t1_r, t2_i, t2_k, n1_j, n1_shift, n1_r, n2_i, n2_k);
 }
 
 // Largest neighbour1 and neighbour2 counts (atomic max of block maxima into mx[0], mx[1]).
+// Largest items of two arrays: each block reduces 16 items per thread, then one
+// atomic per block and array.
 __global__ __launch_bounds__(256) void k_count_max(int N, const int *c1, const int *c2, int *mx) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    int a = i < N ? c1[i] : 0, b = i < N ? c2[i] : 0;
+    __shared__ int wa[4], wb[4];
+    int a = 0, b = 0;
+    for (int k = 0; k < 16; k++) {
+        const int i = blockIdx.x * 4096 + k * 256 + threadIdx.x;
+        if (i < N) {
+            a = max(a, c1[i]);
+            b = max(b, c2[i]);
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) {
         a = max(a, __shfl_xor(a, off, 64));
         b = max(b, __shfl_xor(b, off, 64));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&mx[0], a);
-        atomicMax(&mx[1], b);
+        wa[threadIdx.x >> 6] = a;
+        wb[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMax(&mx[0], max(max(wa[0], wa[1]), max(wa[2], wa[3])));
+        atomicMax(&mx[1], max(max(wb[0], wb[1]), max(wb[2], wb[3])));
     }
 }
 void launch_count_max(int N, const int *c1, const int *c2, int *mx, hipStream_t s) {
-    if (N > 0) hipLaunchKernelGGL(k_count_max, dim3(cdiv(N, 256)), dim3(256), 0, s, N, c1, c2, mx);
+    if (N > 0) hipLaunchKernelGGL(k_count_max, dim3(cdiv(N, 4096)), dim3(256), 0, s, N, c1, c2, mx);
 }
 
 // neighbour2(j) in the reference's append order: the global loop visits atom1
 // by (cell linear index, position in its descending cell list), then by its
-// neighbour1 entry index.  One wavefront per atom ranks its entries by that key
-// (all keys distinct; up to 64 entries are staged in LDS first) and writes them
+// neighbour1 entry index.  A group of 32 lanes per atom ranks its entries by that
+// key (all keys distinct; up to 32 entries are staged in LDS first) and writes them
 // in order; it also records, for every neighbour1 entry, the slot of its mirror
 // in neighbour2.
 __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k,
                                                     const int *cell_of, const int *n1_off, int *n2_is, int *n2_ks,
                                                     int *n1_rev) {
-    __shared__ int s_i[4][64], s_k[4][64], s_c[4][64];   // the wave's entries when they fit one row
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = blockIdx.x * 4 + wv;
-    if (j >= N) return;   // whole wavefront leaves together
+    __shared__ int s_i[8][32], s_k[8][32], s_c[8][32];   // the group's entries when they fit one row
+    const int lane = threadIdx.x & 31, wv = threadIdx.x >> 5;
+    const int j = blockIdx.x * 8 + wv;
+    if (j >= N) return;   // the whole group leaves together
     const int b = n2_off[j], n = n2_off[j + 1] - b;
-    if (n <= 64) {
+    if (n <= 32) {
         int vi = 0, vk = 0, vc = 0;
         if (lane < n) {
             vi = n2_i[b + lane];
@@ -583,7 +622,7 @@ __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, co
         }
         return;
     }
-    for (int x = lane; x < n; x += 64) {
+    for (int x = lane; x < n; x += 32) {
         const int vi = n2_i[b + x], vk = n2_k[b + x], vc = cell_of[vi];
         int rank = 0;
         for (int y = 0; y < n; y++) {
@@ -599,14 +638,14 @@ __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, co
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
                        const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s) {
     if (N > 0)
-        hipLaunchKernelGGL(k_nl_n2_sort, dim3(cdiv(N, 4)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of, n1_off,
+        hipLaunchKernelGGL(k_nl_n2_sort, dim3(cdiv(N, 8)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of, n1_off,
                            n2_is, n2_ks, n1_rev);
 }
 
 // Full per-atom neighbour view in connection_neighbour order (neighbour2 first,
 // Connection.f95:2038-2051), with diff = pos_j - pos_i + lattice.shift summed
-// m = 1..3 (:2120-2125) and the index of the mirror pair.  One wavefront per
-// atom, lanes over its entries.
+// m = 1..3 (:2120-2125) and the index of the mirror pair.  A group of 32 lanes
+// per atom, lanes over its entries (a covalent solid has about 30).
 __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict__ pos,
                                                  const NLGeom *__restrict__ geo, const int *__restrict__ sid,
                                                  const int *n1_off, const int *n1_j, const int *n1_shift,
@@ -614,8 +653,8 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
                                                  const int *n2_k, const int *n1_rev, int *nb_off, int *nb_j,
                                                  int *nb_rev, double *nb_r, double *nb_d) {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 31;
+    const int i = blockIdx.x * 8 + (threadIdx.x >> 5);
     if (i > N) return;
     const int base = n1_off[i] + n2_off[i];
     if (lane == 0) nb_off[i] = base;
@@ -623,7 +662,7 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
     const NLGeom &g = geo[sid[i]];
     const int n2c = n2_off[i + 1] - n2_off[i];
     const int n1c = n1_off[i + 1] - n1_off[i];
-    for (int q = lane; q < n2c + n1c; q += 64) {
+    for (int q = lane; q < n2c + n1c; q += 32) {
         int j, e1, sgn, rev;
         if (q < n2c) {
             j = n2_i[n2_off[i] + q];
@@ -659,7 +698,7 @@ void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 4)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j, n1_shift, n1_r,
+    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 8)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j, n1_shift, n1_r,
                        n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d);
 }
 

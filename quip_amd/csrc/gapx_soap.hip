@@ -26,6 +26,7 @@ namespace gapx {
 static __constant__ double c_ynorm[(kMaxL + 1) * (kMaxL + 2) / 2];
 static __constant__ double c_dfact[kMaxL + 2];
 static __constant__ double c_rinv[2 * kMaxL + 4];   // 1/k (k = 0 unused)
+static __constant__ double c_isq[kMaxL + 1];        // 1/sqrt(2l+1)
 
 static bool g_soap_const_ready[64] = {};   // per device
 static void ensure_soap_constants() {
@@ -46,6 +47,9 @@ static void ensure_soap_constants() {
     ri[0] = 0.0;
     for (int k = 1; k < 2 * kMaxL + 4; k++) ri[k] = 1.0 / (double)k;
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_rinv), ri, sizeof(ri));
+    double isq[kMaxL + 1];
+    for (int l = 0; l <= kMaxL; l++) isq[l] = 1.0 / sqrt(2.0 * l + 1.0);
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_isq), isq, sizeof(isq));
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn));
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df));
     g_soap_const_ready[dev] = true;
@@ -400,7 +404,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
     double *gA = rbs + N, *iA = gA + N;
     double *dcs_all = rbs + 3 * N; // cpb x K*LM
     double *csc = dcs_all + cpb * KLM;   // cpb x (dot, 1/|p|)
-    double *stg = csc + 2 * cpb;   // max(K*K + K*(2L+1), 64*PR), or max(cpb*d, 64*PR) when whole
+    double *stg = csc + 2 * cpb;   // max(K*K + K*(2L+1), 64*PR), or with whole max(cpb*(L+1)*K*K | cpb*d, 64*PR)
     const int lane = threadIdx.x;
     const int d0 = cpb * blockIdx.x;
     const int nh = min(cpb, nc - d0);
@@ -421,15 +425,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
             csc[2 * h] = dot;
             csc[2 * h + 1] = nrm > 0.0 ? 1.0 / nrm : 0.0;
         }
-        if (whole) {   // dE/dp of every centre staged together; c is read from cbuf
-            const double nrm = norm[desc];
-            const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        const double nrm = norm[desc];
+        const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        if (whole == 1) {   // dE/dp of every centre staged together; c is read from cbuf
             double *dpL = stg + h * d;
             for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
+        } else if (whole == 2) {   // the level blocks P_l (below) of every centre, all levels
+            double *Ph = stg + h * (L + 1) * K * K;
+            for (int e = lane; e < (L + 1) * K * K; e += 64) {
+                const int l = e / (K * K), r = e - l * K * K, ia = r / K, jb = r - ia * K;
+                const int hi = ia > jb ? ia : jb, lo = ia > jb ? jb : ia;
+                const int q = (hi * (hi + 1) / 2 + lo) * (L + 1) + l;
+                Ph[e] = (gg[q] - x[q] * dot) * inv * (ia == jb ? 2.0 : 1.4142135623730950488) * c_isq[l];
+            }
         }
     }
     __syncthreads();
-    if (whole) {   // dE/dc in one pass over (centre, ia, lm)
+    if (whole == 2) {   // dE/dc in one pass over (centre, ia, lm): row ia of P_l times column lm of c
+        for (int it = lane; it < nh * KLM; it += 64) {
+            const int h = it / KLM, e = it - h * KLM;
+            const int ia = e / LM, lm = e - ia * LM;
+            const int l = (int)sqrtf((float)lm);   // exact: lm < (l+1)^2 is far from a rounding step
+            const double *Pr = stg + h * (L + 1) * K * K + (l * K + ia) * K;
+            const double *cL = cbuf + (size_t)(d0 + h) * KLM + lm;
+            double acc = 0.0;
+            if (S == 1) {
+#pragma unroll
+                for (int jb = 0; jb < N; jb++) acc += Pr[jb] * cL[jb * LM];
+            } else {
+                for (int jb = 0; jb < K; jb++) acc += Pr[jb] * cL[jb * LM];
+            }
+            dcs_all[h * KLM + e] = acc;
+        }
+        __syncthreads();
+    }
+    if (whole == 1) {   // dE/dc in one pass over (centre, ia, lm)
         for (int it = lane; it < nh * KLM; it += 64) {
             const int h = it / KLM, e = it - h * KLM;
             const double *dpL = stg + h * d, *cL = cbuf + (size_t)(d0 + h) * KLM;
@@ -646,12 +676,16 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     const int LM = (sp.L + 1) * (sp.L + 1);
     const int KLM = sp.K * LM, PR = (2 * (sp.L + 2)) | 1;
     const size_t stg0 = std::max((size_t)sp.K * sp.K + (size_t)sp.K * (2 * sp.L + 1), (size_t)64 * PR);
-    // whole: dE/dp of all centres of the block staged at once (small descriptors)
-    auto stg = [&](int cpb, int whole) { return whole ? std::max(stg0, (size_t)cpb * sp.dim) : stg0; };
+    // whole: all centres of the block staged at once (small descriptors): 2, the level blocks
+    // P_l (K x K each); 1, dE/dp; 0, one level at a time
+    const size_t pl = (size_t)(sp.L + 1) * sp.K * sp.K;
+    auto stg = [&](int cpb, int whole) {
+        return whole == 2 ? std::max(stg0, cpb * pl) : whole == 1 ? std::max(stg0, (size_t)cpb * sp.dim) : stg0;
+    };
     auto need = [&](int cpb, int whole) {
         return ((size_t)sp.n * sp.n + 3 * sp.n + (size_t)cpb * (KLM + 2) + stg(cpb, whole)) * 8;
     };
-    const int whole = need(2, 1) <= 24 * 1024 ? 1 : 0;
+    const int whole = need(2, 2) <= 24 * 1024 ? 2 : need(2, 1) <= 24 * 1024 ? 1 : 0;
     const int cpb = need(2, whole) <= 24 * 1024 ? 2 : 1;   // two centres per wave when LDS stays small
     const size_t sh = need(cpb, whole);
     const int nblk = (nc + cpb - 1) / cpb;
