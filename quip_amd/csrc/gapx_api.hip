@@ -797,7 +797,8 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
     // per-call counters start at zero
     CHK(w.nl_flags.ensure(64));
-    int *flags = w.nl_flags.as<int>();   // [0] slot overflow, [1] max neighbour1 count, [2] max neighbour2 count
+    // [0] slot overflow, [1] max neighbour1 count, [2] max neighbour2 count, [3] neighbour1 total
+    int *flags = w.nl_flags.as<int>();
     {
         int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags};
         const int zn[5] = {NC, NC, N, N, 4};
@@ -832,18 +833,17 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
                         nullptr, s);
     }
     launch_scan2_max(w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), N, w.n1_off.as<int>(), w.n2_off.as<int>(), flags + 1,
-                     w.scan_part.as<int>(), w.scan_part2.as<int>(), s);
+                     flags + 3, w.scan_part.as<int>(), w.scan_part2.as<int>(), s);
     CHK(eg.ctl.ensure(64));
     int *hcount = eg.ctl.as<int>();
-    hcount[0] = -1;
-    HIPCHK(hipMemcpyAsync(hcount, w.n1_off.as<int>() + N, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(hcount + 1, flags, 12, hipMemcpyDeviceToHost, s));
+    hcount[3] = -1;
+    HIPCHK(hipMemcpyAsync(hcount, flags, 16, hipMemcpyDeviceToHost, s));   // overflow, both maxima, pair total
     HIPCHK(hipStreamSynchronize(s));
-    const int n1_total = hcount[0];
-    const bool overflow = hcount[1] != 0;
+    const int n1_total = hcount[3];
+    const bool overflow = hcount[0] != 0;
     // next build's slots: the largest counts of this one plus a quarter and 8
-    eg.nl_cap1 = (hcount[2] + hcount[2] / 4 + 8 + 3) & ~3;
-    eg.nl_cap2 = (hcount[3] + hcount[3] / 4 + 8 + 3) & ~3;
+    eg.nl_cap1 = (hcount[1] + hcount[1] / 4 + 8 + 3) & ~3;
+    eg.nl_cap2 = (hcount[2] + hcount[2] / 4 + 8 + 3) & ~3;
     if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);   // the previous call has finished
     if (n1_total < 0 || n1_total > (INT32_MAX - 2) / 2) return fail(GAPX_E_GEOMETRY, "neighbour count overflow");
     const size_t T1 = (size_t)n1_total + 1;

@@ -30,14 +30,14 @@ struct SoapDev {
 };
 
 // launchers (kernels.hip)
-int scan_partials_size(int n);
+int scan_partials_size(int n);   // ints of scan scratch per array
 void launch_scan_i32(const int *in, int n, int *out, int *partials, hipStream_t s);
 void launch_fill_i32(int *p, int n, int v, hipStream_t s);
 void launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
 constexpr int kMultiFill = 6;   // arrays one zero-fill launch covers
 void launch_zero_i32_multi(int k, int *const *p, const int *n, hipStream_t s);
 void launch_zero_f64_multi(int k, double *const *p, const size_t *n, hipStream_t s);
-void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out2, int *mx, int *part1,
+void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out2, int *mx, int *tot1, int *part1,
                       int *part2, hipStream_t s);
 // bins the atoms and counts each cell's atoms (cell_cnt zeroed by the caller)
 void launch_nl_bin(int N, const double *pos, const NLGeom *geo, const int *sid, const int *cell_base, int *cell_of,
@@ -55,7 +55,6 @@ void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, c
 void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off, const int *t1_j,
                        const int *t1_shift, const double *t1_r, const int *t2_i, const int *t2_k, int *n1_j,
                        int *n1_shift, double *n1_r, int *n2_i, int *n2_k, hipStream_t s);
-void launch_count_max(int N, const int *c1, const int *c2, int *mx, hipStream_t s);
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
                        const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s);
 void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,

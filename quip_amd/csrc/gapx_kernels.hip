@@ -122,50 +122,81 @@ __device__ __forceinline__ int block_excl_scan(int v, int *sh, int *total) {
     __syncthreads();
     return incl - v;
 }
-__global__ void k_scan_partials(const int *in, int n, int *partials) {
+// Exclusive scans of one or two int arrays (blockIdx.y or blockIdx.x selects the array);
+// out[n] gets each array's total, mx[b] (nullable) its largest item and *tot (nullable)
+// the total of array 0.  Large arrays: per-tile sums and maxima, a one-block scan of
+// the tile sums, then the tiles.  Small arrays: one block per array.
+struct ScanSet { const int *in[2]; int *out[2]; int *part[2]; };   // part: tile sums, then tile maxima
+__global__ void k_scan_partials(ScanSet ss, int n, int nb) {
     __shared__ int sh[SCAN_BS];
+    const int b = blockIdx.y;
+    const int *in = ss.in[b];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    int s = 0;
-    for (int k = 0; k < SCAN_ITEMS; k++) s += (base + k < n) ? in[base + k] : 0;
+    int s = 0, m = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        const int v = (base + k < n) ? in[base + k] : 0;
+        s += v;
+        m = max(m, v);
+    }
     int tot;
     block_excl_scan(s, sh, &tot);
-    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int bm = sh[0];
+        for (int w = 1; w < SCAN_BS / 64; w++) bm = max(bm, sh[w]);
+        ss.part[b][blockIdx.x] = tot;
+        ss.part[b][nb + blockIdx.x] = bm;
+    }
 }
-__global__ void k_scan_top(int *partials, int nb, int *total_out) {
+__global__ void k_scan_top(ScanSet ss, int n, int nb, int *mx, int *tot_out) {
     __shared__ int sh[SCAN_BS];
-    int carry = 0;
+    const int b = blockIdx.x;
+    int *partials = ss.part[b];
+    int carry = 0, m = 0;
     for (int c0 = 0; c0 < nb; c0 += SCAN_BS) {
         int i = c0 + threadIdx.x;
         int v = i < nb ? partials[i] : 0;
+        if (i < nb) m = max(m, partials[nb + i]);
         int tot;
         int ex = block_excl_scan(v, sh, &tot);
         if (i < nb) partials[i] = ex + carry;
         carry += tot;
     }
-    if (threadIdx.x == 0) *total_out = carry;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < SCAN_BS / 64; w++) m = max(m, sh[w]);
+        ss.out[b][n] = carry;
+        if (mx) mx[b] = m;
+        if (tot_out && b == 0) *tot_out = carry;
+    }
 }
-__global__ void k_scan_apply(const int *in, int n, const int *partials, int *out) {
+__global__ void k_scan_apply(ScanSet ss, int n) {
     __shared__ int sh[SCAN_BS];
+    const int b = blockIdx.y;
+    const int *in = ss.in[b];
+    int *out = ss.out[b];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
     for (int k = 0; k < SCAN_ITEMS; k++) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
     int tot;
-    int ex = block_excl_scan(s, sh, &tot) + partials[blockIdx.x];
+    int ex = block_excl_scan(s, sh, &tot) + ss.part[b][blockIdx.x];
     for (int k = 0; k < SCAN_ITEMS; k++) {
         if (base + k < n) out[base + k] = ex;
         ex += v[k];
     }
 }
-// One block of 1024 threads for n <= kScanOne.  The array passes through LDS in chunks
-// of kScanChunk items: loaded and stored coalesced, scanned as 16 contiguous items per
-// thread (item i sits at i + i/16, so a wave's 16-item runs fall on distinct banks),
-// with a wave scan and a scan of the 16 wave totals joining the runs and a carry
-// joining the chunks.  Block b of the launch scans array b; mx (nullable) gets each
-// array's largest item.
-constexpr int kScanOne = 1 << 18;
+// One block of 1024 threads per array for n <= kScanOne.  The array passes through LDS
+// in chunks of kScanChunk items: loaded and stored coalesced, scanned as 16 contiguous
+// items per thread (item i sits at i + i/16, so a wave's 16-item runs fall on distinct
+// banks), with a wave scan and a scan of the 16 wave totals joining the runs and a
+// carry joining the chunks.
+constexpr int kScanOne = 1 << 16;
 constexpr int kScanChunk = 16 * 1024;
-struct ScanSet { const int *in[2]; int *out[2]; };
-__global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx) {
+__global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx, int *tot_out) {
     __shared__ int buf[kScanChunk + kScanChunk / 16];
     __shared__ int wsum[16], wmax[16];
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -220,32 +251,31 @@ __global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx) {
     if (t == 0) {
         out[n] = carry;
         if (mx) mx[b] = big;
+        if (tot_out && b == 0) *tot_out = carry;
     }
 }
-int scan_partials_size(int n) { return cdiv(n > 0 ? n : 1, SCAN_TILE); }
+// partials: 2 * scan_partials_size(n) ints per array (tile sums, then tile maxima)
+int scan_partials_size(int n) { return 2 * cdiv(n > 0 ? n : 1, SCAN_TILE); }
+static void scan_go(const ScanSet &ss, int narr, int n, int *mx, int *tot, hipStream_t s) {
+    if (n <= kScanOne) {
+        hipLaunchKernelGGL(k_scan_one, dim3(narr), dim3(1024), 0, s, ss, n, mx, tot);
+        return;
+    }
+    const int nb = cdiv(n, SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_partials, dim3(nb, narr), dim3(SCAN_BS), 0, s, ss, n, nb);
+    hipLaunchKernelGGL(k_scan_top, dim3(narr), dim3(SCAN_BS), 0, s, ss, n, nb, mx, tot);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb, narr), dim3(SCAN_BS), 0, s, ss, n);
+}
 void launch_scan_i32(const int *in, int n, int *out, int *partials, hipStream_t s) {
-    if (n <= kScanOne) {
-        ScanSet ss = {{in, in}, {out, out}};
-        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(1024), 0, s, ss, n, nullptr);
-        return;
-    }
-    const int nb = scan_partials_size(n);
-    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BS), 0, s, in, n, partials);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BS), 0, s, partials, nb, out + n);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BS), 0, s, in, n, partials, out);
+    const ScanSet ss = {{in, in}, {out, out}, {partials, partials}};
+    scan_go(ss, 1, n, nullptr, nullptr, s);
 }
-// Both neighbour counts scanned, and their largest values written to mx[0], mx[1]
-// (mx zeroed by the caller), in one launch when n <= kScanOne.
-void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out2, int *mx, int *part1,
+// Both neighbour counts scanned, their largest values written to mx[0], mx[1] and the
+// total of in1 to *tot1.
+void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out2, int *mx, int *tot1, int *part1,
                       int *part2, hipStream_t s) {
-    if (n <= kScanOne) {
-        ScanSet ss = {{in1, in2}, {out1, out2}};
-        hipLaunchKernelGGL(k_scan_one, dim3(2), dim3(1024), 0, s, ss, n, mx);
-        return;
-    }
-    launch_scan_i32(in1, n, out1, part1, s);
-    launch_scan_i32(in2, n, out2, part2, s);
-    launch_count_max(n, in1, in2, mx, s);
+    const ScanSet ss = {{in1, in2}, {out1, out2}, {part1, part2}};
+    scan_go(ss, 2, n, mx, tot1, s);
 }
 
 // ----------------------------------------------------------------------------
@@ -516,71 +546,40 @@ void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, c
 #undef NLP
 }
 
-// Per-atom slots of the one-pass search -> the CSR tables (one thread per slot).
+// Per-atom slots of the one-pass search -> the CSR tables.
 __global__ __launch_bounds__(256) void k_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off,
                                                     const int *t1_j, const int *t1_shift, const double *t1_r,
                                                     const int *t2_i, const int *t2_k, int *n1_j, int *n1_shift,
                                                     double *n1_r, int *n2_i, int *n2_k) {
-    // one thread per slot: the N * cap1 neighbour1 slots, then the N * cap2 neighbour2 slots
-    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x, n1s = (size_t)N * cap1;
-    if (idx < n1s) {
-        const int a = (int)(idx / cap1), q = (int)(idx - (size_t)a * cap1);
-        const int b1 = n1_off[a];
-        if (q >= n1_off[a + 1] - b1) return;
-        n1_j[b1 + q] = t1_j[idx];
-        n1_shift[3 * (b1 + q)] = t1_shift[3 * idx];
-        n1_shift[3 * (b1 + q) + 1] = t1_shift[3 * idx + 1];
-        n1_shift[3 * (b1 + q) + 2] = t1_shift[3 * idx + 2];
-        n1_r[b1 + q] = t1_r[idx];
-        return;
+    // 16 lanes per atom (a covalent solid has about 15 entries in each table)
+    const int lane = threadIdx.x & 15;
+    const int a = blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (a >= N) return;
+    const int b1 = n1_off[a], c1 = n1_off[a + 1] - b1;
+    for (int q = lane; q < c1; q += 16) {
+        const size_t src = (size_t)a * cap1 + q;
+        n1_j[b1 + q] = t1_j[src];
+        n1_shift[3 * (b1 + q)] = t1_shift[3 * src];
+        n1_shift[3 * (b1 + q) + 1] = t1_shift[3 * src + 1];
+        n1_shift[3 * (b1 + q) + 2] = t1_shift[3 * src + 2];
+        n1_r[b1 + q] = t1_r[src];
     }
-    const size_t r = idx - n1s;
-    if (r >= (size_t)N * cap2) return;
-    const int a = (int)(r / cap2), q = (int)(r - (size_t)a * cap2);
-    const int b2 = n2_off[a];
-    if (q >= n2_off[a + 1] - b2) return;
-    n2_i[b2 + q] = t2_i[r];
-    n2_k[b2 + q] = t2_k[r];
+    const int b2 = n2_off[a], c2 = n2_off[a + 1] - b2;
+    for (int q = lane; q < c2; q += 16) {
+        const size_t src = (size_t)a * cap2 + q;
+        n2_i[b2 + q] = t2_i[src];
+        n2_k[b2 + q] = t2_k[src];
+    }
 }
 void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off, const int *t1_j,
                        const int *t1_shift, const double *t1_r, const int *t2_i, const int *t2_k, int *n1_j,
                        int *n1_shift, double *n1_r, int *n2_i, int *n2_k, hipStream_t s) {
     if (N > 0)
-        hipLaunchKernelGGL(k_nl_compact, dim3((unsigned)(((size_t)N * (cap1 + cap2) + 255) / 256)), dim3(256), 0, s,
-                           N, cap1, cap2, n1_off, n2_off, t1_j,
+        hipLaunchKernelGGL(k_nl_compact, dim3(cdiv(N, 16)), dim3(256), 0, s, N, cap1, cap2, n1_off, n2_off, t1_j,
                            t1_shift, t1_r, t2_i, t2_k, n1_j, n1_shift, n1_r, n2_i, n2_k);
 }
 
 // Largest neighbour1 and neighbour2 counts (atomic max of block maxima into mx[0], mx[1]).
-// Largest items of two arrays: each block reduces 16 items per thread, then one
-// atomic per block and array.
-__global__ __launch_bounds__(256) void k_count_max(int N, const int *c1, const int *c2, int *mx) {
-    __shared__ int wa[4], wb[4];
-    int a = 0, b = 0;
-    for (int k = 0; k < 16; k++) {
-        const int i = blockIdx.x * 4096 + k * 256 + threadIdx.x;
-        if (i < N) {
-            a = max(a, c1[i]);
-            b = max(b, c2[i]);
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        a = max(a, __shfl_xor(a, off, 64));
-        b = max(b, __shfl_xor(b, off, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        wa[threadIdx.x >> 6] = a;
-        wb[threadIdx.x >> 6] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicMax(&mx[0], max(max(wa[0], wa[1]), max(wa[2], wa[3])));
-        atomicMax(&mx[1], max(max(wb[0], wb[1]), max(wb[2], wb[3])));
-    }
-}
-void launch_count_max(int N, const int *c1, const int *c2, int *mx, hipStream_t s) {
-    if (N > 0) hipLaunchKernelGGL(k_count_max, dim3(cdiv(N, 4096)), dim3(256), 0, s, N, c1, c2, mx);
-}
 
 // neighbour2(j) in the reference's append order: the global loop visits atom1
 // by (cell linear index, position in its descending cell list), then by its

@@ -61,8 +61,10 @@ def _geoms():
                           np.diag([5.0, 5.0, 5.0])), 3.0
     yield "si4096", configs.si4096(), 5.0
     yield "si4096_skin", configs.si4096(), 5.5
-    # more than 16384 atoms: the count scans pass through LDS in two chunks
+    # more than 16384 atoms: the one-block count scans pass through LDS in two chunks
     yield "si17576", configs.perturb(configs.diamond(5.43, (13, 13, 13)), 0.1, configs.rng_for(0, 77)), 5.0
+    # more than 65536 atoms: the count scans take the tile-sum path
+    yield "si74088", configs.perturb(configs.diamond(5.43, (21, 21, 21)), 0.1, configs.rng_for(0, 78)), 5.0
     yield "empty", Atoms(np.zeros(0, np.int32), np.zeros((0, 3)), np.eye(3) * 5.0), 3.0
     yield "single_atom", Atoms([14], [[0.3, 0.2, 0.1]], np.eye(3) * 2.2), 5.0
 
