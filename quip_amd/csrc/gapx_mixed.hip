@@ -451,10 +451,12 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     typedef typename H::T T;
     typedef typename H::V V;
     constexpr int DP = 32 * DK;     // descriptor components handled (zero padded)
-    // LDS rows without padding; 16-byte vectors XOR-swizzled so that the 16 rows one
-    // fragment read touches fall on 16 distinct 16-byte bank groups:
+    // LDS rows without padding; 16-byte vectors XOR-swizzled so that the rows one
+    // fragment read touches fall on distinct 16-byte bank groups in every group of 8
+    // lanes (a 16-byte read moves 128 bytes per cycle):
     //   Xs rows (XSV vectors):                     vector c of row r at c ^ (r & XSM)
-    //   Xs^T and W rows (4 vectors of 8 sparse points): vector c of row q at c ^ ((q >> 2) & 3)
+    //   Xs^T and W rows (4 vectors of 8 sparse points, 64 bytes): vector c of row q at
+    //   c ^ ((q >> 1) & 3), so rows q..q+7 cover the 8 groups of 128 bytes
     // Two chunk buffers: the next chunk is written while the current one is read, so
     // each chunk needs one barrier.
     constexpr int XSV = DP / 8;                     // 8, 16 or 32
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __shared__ __attribute__((aligned(16))) T xt_l[2][2][DP * 32];
     __shared__ __attribute__((aligned(16))) T w_l[8][2][16 * 32];
     auto xs_at = [](int r, int c) { return r * DP + ((c ^ (r & XSM)) << 3); };
-    auto xt_at = [](int q, int c) { return q * 32 + ((c ^ ((q >> 2) & 3)) << 3); };
+    auto xt_at = [](int q, int c) { return q * 32 + ((c ^ ((q >> 1) & 3)) << 3); };
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
     const int row0 = blockIdx.x * 128 + wv * 16;
