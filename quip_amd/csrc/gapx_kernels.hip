@@ -1168,7 +1168,7 @@ void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z
 // ----------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int GPAD = 4;
+constexpr int GPAD = 1;   // odd: the 16 k-rows one stash write touches fall on distinct bank pairs
 
 // C = A . B (+ fused GP epilogue).  A is row-major (rows x K, k contiguous);
 // B is row-major (cols x K) when BT, else (K x cols).  4 wavefronts in a 2x2
