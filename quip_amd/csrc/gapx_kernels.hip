@@ -1176,8 +1176,11 @@ constexpr int GPAD = 1;   // odd: the 16 k-rows one stash write touches fall on 
 // The next k tile is fetched into registers while the current one feeds the
 // MFMAs from LDS.  blockIdx.z selects a k range (split-K) whose partial result
 // goes to C + z * rows * cols.
+// The occupancy hint keeps the accumulators in ordinary VGPRs for the whole kernel:
+// without it the compiler places them in AGPRs and copies all of them out and back
+// around every k tile.
 template <bool BT, bool EPI, int WM, int WN, int GBK>
-__global__ __launch_bounds__(256) void k_gemm_f64(int Mr, int Nc, int Kd, int kchunk, const double *__restrict__ A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GBK == 16 ? 3 : 2))) void k_gemm_f64(int Mr, int Nc, int Kd, int kchunk, const double *__restrict__ A,
                                                   const double *__restrict__ B, double *__restrict__ C,
                                                   const double *__restrict__ wvec, double zeta, int izeta,
                                                   double *__restrict__ e_part) {
