@@ -183,15 +183,15 @@ struct Rad {
 // c' of level l.  A macro so that both level loops (rolled and unrolled) inline it.
 #define SOAP_FWD_LEVEL(l, cur, prv)                                                                                  \
             __syncthreads();   /* the previous level's reads of Rl / Yl are done */                                  \
-            if (lane < NB) {   /* rows of neighbours outside every list stay zero for the dense loop */              \
+            if (lane < 32 * CPW) {   /* rows of neighbours outside every list stay zero for the dense loop */        \
 _Pragma("unroll")                                                                                                    \
-                for (int a = 0; a < N; a++) Rl[a * RT + lane] = s >= 0 ? fc * rs.F0[a] : 0.0;                        \
+                for (int a = 0; a < N; a++) Rh[a * RT + kl] = s >= 0 ? fc * rs.F0[a] : 0.0;                          \
             }                                                                                                        \
             if (s >= 0) {                                                                                            \
                 rs.step(l);                                                                                          \
                 legendre_level(l, uz, prv, cur);                                                                     \
                 double A = 1.0, B = 0.0;                                                                             \
-                double *Yr = Yl + lane;   /* slot m of this neighbour at Yr[m * RT] */                               \
+                double *Yr = Yh + kl;   /* slot m of this neighbour at Yr[m * RT] */                                 \
                 for (int m = 0; m <= l; m++) {                                                                       \
                     if (m > 0) {                                                                                     \
                         const double An = A * ux - B * uy;                                                           \
@@ -205,52 +205,56 @@ _Pragma("unroll")                                                               
                         Yr[(l - m) * RT] = Nn * B;                                                                   \
                     }                                                                                                \
                 }                                                                                                    \
-            } else if (lane < NB) {                                                                                  \
-                for (int m = 0; m < 2 * l + 1; m++) Yl[m * RT + lane] = 0.0;                                         \
+            } else if (lane < 32 * CPW) {                                                                            \
+                for (int m = 0; m < 2 * l + 1; m++) Yh[m * RT + kl] = 0.0;                                           \
             }                                                                                                        \
             __syncthreads();                                                                                         \
             const int W = 2 * l + 1;                                                                                 \
-            for (int it = lane; it < N * W; it += 64) {                                                              \
-                const int a = it / W, mi = it - a * W;                                                               \
+            for (int it = lane; it < CPW * N * W; it += 64) {                                                        \
+                const int hh = CPW == 1 ? 0 : it / (N * W), r_ = it - hh * (N * W);                                 \
+                const int a = r_ / W, mi = r_ - a * W;                                                               \
+                const double *Rc = Rl + hh * (N * RT), *Yc = Yl + hh * (YW * RT);                                   \
+                double *cPc = cP + hh * KLMp;                                                                        \
                 if (S == 1) {   /* one species: every row of the chunk in pairs, zero rows included */            \
-                    const double2 *R2 = (const double2 *)(Rl + a * RT), *Y2 = (const double2 *)(Yl + mi * RT);      \
+                    const double2 *R2 = (const double2 *)(Rc + a * RT), *Y2 = (const double2 *)(Yc + mi * RT);      \
                     double acc0 = 0.0, acc1 = 0.0;                                                                   \
 _Pragma("unroll 4")                                                                                                  \
-                    for (int k2 = 0; k2 < NB / 2; k2++) {                                                               \
+                    for (int k2 = 0; k2 < NB / 2; k2++) {                                                            \
                         const double2 rv = R2[k2], yv = Y2[k2];                                                      \
                         acc0 += rv.x * yv.x;                                                                         \
                         acc1 += rv.y * yv.y;                                                                         \
                     }                                                                                                \
-                    cP[a * LM + l * l + mi] += acc0 + acc1;                                                          \
+                    cPc[a * LM + l * l + mi] += acc0 + acc1;                                                         \
                     continue;                                                                                        \
                 }                                                                                                    \
                 for (int si = 0; si < S; si++) {                                                                     \
-                    const int n = ns[si];                                                                            \
-                    const int *ki = kidx + si * NB;                                                                  \
+                    const int n = ns[hh * S + si];                                                                   \
+                    const int *ki = kidx + (hh * S + si) * NB;                                                       \
                     double acc0 = 0.0, acc1 = 0.0;                                                                   \
                     int q = 0;                                                                                       \
 _Pragma("unroll 2")                                                                                                  \
                     for (; q + 1 < n; q += 2) {                                                                      \
                         const int k0 = ki[q], k1 = ki[q + 1];                                                        \
-                        acc0 += Rl[a * RT + k0] * Yl[mi * RT + k0];                                                  \
-                        acc1 += Rl[a * RT + k1] * Yl[mi * RT + k1];                                                  \
+                        acc0 += Rc[a * RT + k0] * Yc[mi * RT + k0];                                                  \
+                        acc1 += Rc[a * RT + k1] * Yc[mi * RT + k1];                                                  \
                     }                                                                                                \
                     if (q < n) {                                                                                     \
                         const int k0 = ki[q];                                                                        \
-                        acc0 += Rl[a * RT + k0] * Yl[mi * RT + k0];                                                  \
+                        acc0 += Rc[a * RT + k0] * Yc[mi * RT + k0];                                                  \
                     }                                                                                                \
-                    cP[(si * N + a) * LM + l * l + mi] += acc0 + acc1;                                               \
+                    cPc[(si * N + a) * LM + l * l + mi] += acc0 + acc1;                                              \
                 }                                                                                                    \
             }
 
 // PREG: each lane's Legendre levels in registers, with the level loop unrolled (no
 // LDS rows, more registers: two wavefronts per SIMD); otherwise two LDS rows per
-// lane and a rolled level loop (three wavefronts per SIMD).  Measured: registers are
-// faster for launches below 16K centres and for n >= 10 or l >= 8 (whose LDS
-// already limits the occupancy), the LDS rows for large single-species batches.
-template <int N, int L, bool PREG>
+// lane and a rolled level loop (three wavefronts per SIMD).
+// CPW: centres per wavefront.  A centre's chunk of 32 neighbours occupies 32 lanes,
+// so with two centres lanes 0-31 and 32-63 stage the neighbours of one centre each
+// and every staging instruction serves two centres.
+template <int N, int L, bool PREG, int CPW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREG ? 2 : WPE_FWD, 8))) void k_soap_fwd_x(
-    SoapDev sp, const int *__restrict__ centres, const int *__restrict__ Z, const int *__restrict__ nb_off,
+    SoapDev sp, int nc, const int *__restrict__ centres, const int *__restrict__ Z, const int *__restrict__ nb_off,
     const int *__restrict__ nb_j, const double *__restrict__ nb_r, const double *__restrict__ nb_d, double *X,
     double *norm_out, double *cbuf) {
     extern __shared__ double lds[];
@@ -258,42 +262,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREG ? 2 : W
     constexpr int YW = 2 * L + 1;
     constexpr int NB = kSoapChunk;
     const int K = sp.K, KLM = K * LM, S = sp.S;
+    const int KLMp = (KLM + 1) & ~1;   // per-centre c' rows, 16-byte aligned
     double *Ts = lds;              // N*N
     double *rbs = Ts + N * N;      // N grid points, then exp(-alpha r_a^2) and 1/a (N each)
     double *gA = rbs + N, *iA = gA + N;
-    double *cP = rbs + 3 * N;      // K*LM: c' during accumulation, c after the transform
+    double *cP = rbs + 3 * N;      // CPW x K*LM: c' during accumulation, c after the transform
     // The staged levels are stored one row per radial point a (Rl) and per m slot (Yl),
     // neighbours along the row: the writes of 32 lanes are consecutive, and the dense
     // accumulation reads two neighbours per 16-byte access.  The row stride RT = NB + 2
-    // keeps rows 16-byte aligned (N (N + 3) is even, and Rl's offset is rounded up to even)
-    // and puts successive rows 16 bytes apart in the bank cycle.
+    // keeps rows 16-byte aligned (N (N + 3) and KLMp are even) and puts successive rows
+    // 16 bytes apart in the bank cycle.
     constexpr int RT = NB + 2;     // row stride of Rl and Yl
     constexpr int PR = (2 * (L + 2)) | 1;   // row stride of a lane's two Legendre rows (odd: distinct banks)
-    double *Rl = cP + ((KLM + 1) & ~1);   // N x RT: fc F_l(a) of neighbour k at Rl[a * RT + k]
-    double *Yl = Rl + N * RT;      // YW x RT: Y_l,m of neighbour k at Yl[slot * RT + k]
-    double *Pq0 = Yl + YW * RT;    // NB x PR: the lanes' Legendre rows (not PREG)
-    int *kidx = (int *)(Pq0 + (PREG ? 0 : NB * PR));   // S x NB: the chunk's neighbours of each species
-    int *ns = kidx + S * NB;       // their counts
+    double *Rl = cP + CPW * KLMp;  // CPW x N x RT: fc F_l(a) of neighbour k of centre h at Rl[(h N + a) RT + k]
+    double *Yl = Rl + CPW * N * RT;   // CPW x YW x RT: Y_l,m of neighbour k at Yl[(h YW + slot) RT + k]
+    double *Pq0 = Yl + CPW * YW * RT; // 32 CPW x PR: the staging lanes' Legendre rows (not PREG)
+    int *kidx = (int *)(Pq0 + (PREG ? 0 : 32 * CPW * PR));   // CPW x S x NB: the chunk's neighbours of each species
+    int *ns = kidx + CPW * S * NB; // their counts
     const int lane = threadIdx.x;
-    const int desc = blockIdx.x;
-    const int ci = centres[desc];
+    const int h = CPW == 2 ? lane >> 5 : 0, kl = lane & 31;   // this lane's centre and neighbour slot
+    const int desc = CPW * blockIdx.x + h;
+    const bool valid = desc < nc;   // this lane's centre exists (the same for all lanes with CPW 1)
+    const int ci = valid ? centres[desc] : 0;
+    double *Rh = Rl + h * (N * RT), *Yh = Yl + h * (YW * RT);
     for (int e = lane; e < N * N; e += 64) Ts[e] = sp.T[e];
     if (lane < N) {
         rbs[lane] = sp.rb[lane];
         gA[lane] = exp(-sp.alpha * sp.rb[lane] * sp.rb[lane]);
         iA[lane] = lane > 0 ? 1.0 / (double)lane : 0.0;
     }
-    for (int e = lane; e < KLM; e += 64) cP[e] = 0.0;
+    for (int e = lane; e < CPW * KLMp; e += 64) cP[e] = 0.0;
     __syncthreads();
-    const int p0 = nb_off[ci], np = nb_off[ci + 1] - p0;
-    for (int c0 = 0; c0 < np; c0 += NB) {
-        const int nk = min(NB, np - c0);
+    const int p0 = valid ? nb_off[ci] : 0, np = valid ? nb_off[ci + 1] - p0 : 0;
+    int npmax = np;   // chunks of the wave (uniform): those of its larger centre
+    if (CPW == 2) npmax = max(np, __shfl_xor(np, 32, 64));
+    for (int c0 = 0; c0 < npmax; c0 += NB) {
+        const int nk = max(0, min(NB, np - c0));
         __syncthreads();   // the previous chunk's reads of the lists are done
         int s = -1;
         double fc = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
         Rad<N> rs;
-        if (lane < nk) {
-            const int p = p0 + c0 + lane;
+        if (kl < nk && lane < 32 * CPW) {
+            const int p = p0 + c0 + kl;
             const double r = nb_r[p];
             s = species_s(sp, Z[nb_j[p]]);
             if (r > sp.rc) s = -1;
@@ -305,10 +315,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREG ? 2 : W
             }
         }
         for (int si = 0; S > 1 && si < S; si++) {   // neighbours beyond the cutoff are in no list
-            const bool mine = lane < nk && s == si;
+            const bool mine = lane < 32 * CPW && s == si;
             const unsigned long long mk = __ballot(mine);
-            if (mine) kidx[si * NB + __popcll(mk & ((1ull << lane) - 1ull))] = lane;
-            if (lane == 0) ns[si] = __popcll(mk);
+            const unsigned long long mh = (mk >> (32 * h)) & 0xffffffffull;   // this centre's half
+            if (mine) kidx[(h * S + si) * NB + __popcll(mh & ((1ull << kl) - 1ull))] = kl;
+            if (kl == 0 && lane < 32 * CPW) ns[h * S + si] = __popcll(mh);
         }
         if constexpr (PREG) {
             double P0[L + 2], P1[L + 2];
@@ -329,51 +340,73 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PREG ? 2 : W
         }
     }
     __syncthreads();
-    // c = T^T c', one (species, lm) column per lane, in place
-    for (int it = lane; it < S * LM; it += 64) {
-        const int si = it / LM, lm = it - si * LM;
+    // c = T^T c', one (centre, species, lm) column per lane, in place
+    for (int it = lane; it < CPW * S * LM; it += 64) {
+        const int hh = CPW == 1 ? 0 : it / (S * LM), r_ = it - hh * (S * LM), si = r_ / LM, lm = r_ - si * LM;
+        double *cc = cP + hh * KLMp;
         RELOAD_BARRIER();   // T is re-read from LDS for every column
         double cp[N];
 #pragma unroll
-        for (int a = 0; a < N; a++) cp[a] = cP[(si * N + a) * LM + lm];
+        for (int a = 0; a < N; a++) cp[a] = cc[(si * N + a) * LM + lm];
 #pragma unroll
         for (int b = 0; b < N; b++) {
             double acc = 0.0;
 #pragma unroll
             for (int a = 0; a < N; a++) acc += Ts[a * N + b] * cp[a];
-            cP[(si * N + b) * LM + lm] = acc;
+            cc[(si * N + b) * LM + lm] = acc;
             RELOAD_BARRIER();   // one column of T in registers at a time
         }
     }
     __syncthreads();
-    if (lane == 0) {   // centre atom term
+    if (kl == 0 && valid && lane < 32 * CPW) {   // centre atom term
         const int sc = species_s(sp, Z[ci]);
-        if (sc >= 0) cP[(sc * N) * LM] += sp.cw * sp.u00 * c_ynorm[0];
+        if (sc >= 0) cP[h * KLMp + (sc * N) * LM] += sp.cw * sp.u00 * c_ynorm[0];
     }
     __syncthreads();
     // power spectrum, written unnormalised and rescaled in place by the same lanes
     const int d = sp.dim;
-    double *xo = X + (size_t)desc * d;
-    double ss = 0.0;
-    for (int q = lane; q < d - 1; q += 64) {
+    const int d0 = CPW * blockIdx.x;
+    double ss[CPW];
+#pragma unroll
+    for (int hh = 0; hh < CPW; hh++) ss[hh] = 0.0;
+    for (int it = lane; it < CPW * (d - 1); it += 64) {
+        const int hh = CPW == 1 ? 0 : it / (d - 1), q = it - hh * (d - 1);
+        if (d0 + hh >= nc) continue;
+        const double *cc = cP + hh * KLMp;
         const int code = sp.qmap[q];
         const int ia = code >> 16, jb = (code >> 8) & 0xff, l = code & 0xff;
         double v = 0.0;
-        for (int m = -l; m <= l; m++) v += cP[ia * LM + l * l + l + m] * cP[jb * LM + l * l + l + m];
+        for (int m = -l; m <= l; m++) v += cc[ia * LM + l * l + l + m] * cc[jb * LM + l * l + l + m];
         v *= (ia != jb ? 1.4142135623730950488 : 1.0) / sqrt(2.0 * l + 1.0);
-        xo[q] = v;
-        ss += v * v;
+        X[(size_t)(d0 + hh) * d + q] = v;
+#pragma unroll
+        for (int g2 = 0; g2 < CPW; g2++)
+            if (g2 == hh) ss[g2] += v * v;
     }
-    ss = wave_sum_s(ss);
-    const double nrm = sqrt(ss);
-    const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-    for (int q = lane; q < d - 1; q += 64) xo[q] = xo[q] * inv;
-    if (lane == 0) {
-        xo[d - 1] = 0.0;
-        norm_out[desc] = nrm;
+    double inv[CPW];
+#pragma unroll
+    for (int hh = 0; hh < CPW; hh++) {
+        const double nrm = sqrt(wave_sum_s(ss[hh]));
+        inv[hh] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        if (lane == 0 && d0 + hh < nc) {
+            X[(size_t)(d0 + hh) * d + d - 1] = 0.0;
+            norm_out[d0 + hh] = nrm;
+        }
     }
-    double *co = cbuf + (size_t)desc * KLM;
-    for (int e = lane; e < KLM; e += 64) co[e] = cP[e];
+    for (int it = lane; it < CPW * (d - 1); it += 64) {
+        const int hh = CPW == 1 ? 0 : it / (d - 1), q = it - hh * (d - 1);
+        if (d0 + hh >= nc) continue;
+        double sc = inv[0];
+#pragma unroll
+        for (int g2 = 1; g2 < CPW; g2++)
+            if (g2 == hh) sc = inv[g2];
+        X[(size_t)(d0 + hh) * d + q] *= sc;
+    }
+    for (int hh = 0; hh < CPW; hh++) {
+        if (d0 + hh >= nc) break;
+        double *co = cbuf + (size_t)(d0 + hh) * KLM;
+        for (int e = lane; e < KLM; e += 64) co[e] = cP[hh * KLMp + e];
+    }
 }
 
 // Adjoint.  dE/dc in the orthonormal basis is transformed once per centre to
@@ -621,6 +654,32 @@ int soap_fast_supported(int n, int L) {
     return 0;
 }
 
+// Two centres per wave: single-species models with n <= 8, l <= 6 whose register
+// state fits without spilling (n l >= 32: 8/4, 6/6, 8/6).
+static constexpr bool fwd_pair_ok(int n, int L) { return n <= 8 && L <= 6 && n * L >= 32; }
+// Only the variants the launcher can pick are instantiated.
+template <int N, int L>
+static void fwd_go(bool preg, int cpw, int nblk, size_t sh, hipStream_t s, const SoapDev &sp, int nc,
+                   const int *centres, const int *Z, const int *nb_off, const int *nb_j, const double *nb_r,
+                   const double *nb_d, double *X, double *norm, double *cbuf) {
+    if constexpr (fwd_pair_ok(N, L)) {
+        if (cpw == 2) {
+            hipLaunchKernelGGL((k_soap_fwd_x<N, L, true, 2>), dim3(nblk), dim3(64), sh, s, sp, nc, centres, Z,
+                               nb_off, nb_j, nb_r, nb_d, X, norm, cbuf);
+            return;
+        }
+    }
+    if constexpr (N < 10 && L < 8) {
+        if (!preg) {
+            hipLaunchKernelGGL((k_soap_fwd_x<N, L, false, 1>), dim3(nblk), dim3(64), sh, s, sp, nc, centres, Z,
+                               nb_off, nb_j, nb_r, nb_d, X, norm, cbuf);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_soap_fwd_x<N, L, true, 1>), dim3(nblk), dim3(64), sh, s, sp, nc, centres, Z, nb_off,
+                       nb_j, nb_r, nb_d, X, norm, cbuf);
+}
+
 bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                           const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
                           double *cbuf, hipStream_t s) {
@@ -632,17 +691,20 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     // neighbour chunk of 32: covers a centre's neighbours within 5 A in one pass
     // for covalent solids and keeps the block's LDS small
     const int NB = kSoapChunk;
-    const bool preg = sp.n >= 10 || sp.L >= 8 || nc < 16384;
-    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + KLM + 1 + (NB + 2) * (sp.n + 2 * sp.L + 1) +
-                               (preg ? 0 : NB * ((2 * (sp.L + 2)) | 1))) * 8 + (size_t)sp.S * (NB + 1) * 4;
+    // Two centres per wave with the Legendre levels in registers for single-species
+    // models up to n 8, l 6 (measured on the 8/6 model: batch512 1.22 -> 1.11 ms,
+    // si4096 0.049 -> 0.043 ms); one centre per wave otherwise (the two-species 12/8
+    // model runs 3.4 ms with one and 5.6 ms with two, its LDS per wave doubling).
+    const int cpw = sp.S == 1 && fwd_pair_ok(sp.n, sp.L) ? 2 : 1;
+    const bool preg = cpw == 2 || sp.n >= 10 || sp.L >= 8 || nc < 16384;
+    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + cpw * ((KLM + 1) & ~1) +
+                               cpw * (NB + 2) * (sp.n + 2 * sp.L + 1) +
+                               (preg ? 0 : 32 * cpw * ((2 * (sp.L + 2)) | 1))) * 8 +
+                      (size_t)cpw * sp.S * (NB + 1) * 4;
+    const int nblk = (nc + cpw - 1) / cpw;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
-        if (preg)                                                                                               \
-            hipLaunchKernelGGL((k_soap_fwd_x<NN, LL, true>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off,  \
-                               nb_j, nb_r, nb_d, X, norm, cbuf);                                                \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_soap_fwd_x<NN, LL, false>), dim3(nc), dim3(64), sh, s, sp, centres, Z, nb_off, \
-                               nb_j, nb_r, nb_d, X, norm, cbuf);                                                \
+        fwd_go<NN, LL>(preg, cpw, nblk, sh, s, sp, nc, centres, Z, nb_off, nb_j, nb_r, nb_d, X, norm, cbuf);     \
         return true;                                                                                            \
     }
     SOAP_FAST(LAUNCH)
