@@ -137,6 +137,24 @@ def test_fp64_matches_oracle(models, name):
     _compare(r, ro, len(at))
 
 
+@pytest.mark.parametrize("nl", [(4, 4), (5, 4), (6, 4), (8, 4), (6, 6), (10, 6), (12, 6), (8, 8), (10, 8)])
+def test_descriptor_variants_match_oracle(nl, tmp_path):
+    """Every (n_max, l_max) the specialised kernels cover, on a cell with an odd number of
+    centres (the forward's two-centres-per-wave form leaves its last wave half empty)."""
+    n, lm = nl
+    train = [configs.perturb(configs.diamond(5.43, (1, 1, 1)), 0.1, configs.rng_for(3, 700 + k)) for k in range(3)]
+    m = configs.soap_model(_desc_fn, train, 16, 3, [configs.soap_string(n_max=n, l_max=lm)], "GAP_nl%d_%d" % nl,
+                           alpha_std=1e-2)
+    path = str(tmp_path / "m.xml")
+    write_gap_xml(path, m)
+    full = configs.si64()
+    at = Atoms(full.numbers[1:], full.positions[1:], full.cell)   # 63 atoms
+    print("case", nl, len(at), flush=True)
+    r = Potential(param_filename=path).calc(at.copy(), force=True, virial=True, local_energy=True,
+                                            local_virial=True)
+    _compare(r, O.gap_calc(m, at), len(at))
+
+
 def test_skin_and_nonperiodic(models):
     """A neighbour cutoff above the model cutoff (QUIP's cutoff_skin) and a non-periodic axis."""
     m, path = models["si64"]
