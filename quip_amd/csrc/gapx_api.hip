@@ -1168,11 +1168,15 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     double *o_lv = (want & GAPX_WANT_LOCAL_V) && d_lv ? d_lv : w.lv.as<double>();
 
     // The launch sequence (stage events only when timing, never inside a graph).
+    // pair energies exist only with a distance_2b term; without one the gather skips them
+    bool has_d2b = false;
+    for (size_t k = 0; k < NCOORD; k++) has_d2b = has_d2b || m->coords[k]->kind == 2;
+    double *pairE = has_d2b ? w.pairE.as<double>() : nullptr;
     auto launch_all = [&](bool marks) {
         {
-            double *zp[3] = {w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>()};
-            const size_t zn[3] = {(size_t)P * 3, (size_t)P, (size_t)N};
-            launch_zero_f64_multi(3, zp, zn, s);
+            double *zp[3] = {w.pairF.as<double>(), w.esite.as<double>(), pairE};
+            const size_t zn[3] = {(size_t)P * 3, (size_t)N, (size_t)P};
+            launch_zero_f64_multi(has_d2b ? 3 : 2, zp, zn, s);
         }
         for (size_t k = 0; k < NCOORD; k++) {
             Coord &c = *m->coords[k];
@@ -1180,7 +1184,7 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
                 const int ta = marks ? tm.mark(s) : -1;
                 launch_d2b(N, w.Z.as<int>(), c.Z1, c.Z2, c.rc, c.ctw, c.M, c.d_Xs.as<double>(), c.d_w.as<double>(),
                            c.d_theta.as<double>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
-                           w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), cmask, s);
+                           w.nb_d.as<double>(), w.pairF.as<double>(), pairE, cmask, s);
                 if (marks) tm.stage(5, ta, tm.mark(s));
                 continue;
             }
@@ -1228,7 +1232,7 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
         }
         const int tg = marks ? tm.mark(s) : -1;
         launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
-                      w.nb_d.as<double>(), w.pairF.as<double>(), w.pairE.as<double>(), w.esite.as<double>(), o_f, o_le,
+                      w.nb_d.as<double>(), w.pairF.as<double>(), pairE, w.esite.as<double>(), o_f, o_le,
                       o_lv, cmask, s);
         launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s);
         if (marks) tm.stage(5, tg, tm.mark(s));

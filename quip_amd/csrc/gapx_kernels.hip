@@ -1472,7 +1472,7 @@ __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const
         v[3] -= dx * gx; v[4] -= dy * gx; v[5] -= dz * gx;
         v[6] -= dx * gy; v[7] -= dy * gy; v[8] -= dz * gy;
         v[9] -= dx * gz; v[10] -= dy * gz; v[11] -= dz * gz;
-        v[12] += pairE[q] + pairE[r];
+        if (pairE) v[12] += pairE[q] + pairE[r];   // null without a distance_2b term
     }
 #pragma unroll
     for (int k = 0; k < 13; k++) v[k] = half_sum_g(v[k]);

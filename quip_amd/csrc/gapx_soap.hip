@@ -67,10 +67,19 @@ __device__ __forceinline__ double fc_s(double r, double rc, double w) {
     if (r > rc - w) return 0.5 * (cos(GAPX_PI_S * (r - rc + w) / w) + 1.0);
     return 1.0;
 }
-__device__ __forceinline__ double dfc_s(double r, double rc, double w) {
-    if (r > rc) return 0.0;
-    if (r > rc - w) return -0.5 * GAPX_PI_S * sin(GAPX_PI_S * (r - rc + w) / w) / w;
-    return 0.0;
+// fc and dfc/dr together: one sincos for the transition region
+__device__ __forceinline__ void fc_dfc_s(double r, double rc, double w, double &fc, double &dfc) {
+    fc = 0.0;
+    dfc = 0.0;
+    if (r > rc) return;
+    if (r > rc - w) {
+        double sn, cs;
+        sincos(GAPX_PI_S * (r - rc + w) / w, &sn, &cs);
+        fc = 0.5 * (cs + 1.0);
+        dfc = -0.5 * GAPX_PI_S * sn / w;
+        return;
+    }
+    fc = 1.0;
 }
 __device__ __forceinline__ double wave_sum_s(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -574,7 +583,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
         const double r = nb_r[p];
         const int s = species_s(sp, Z[nb_j[p]]);
         if (r > sp.rc || s < 0) continue;
-        const double fc = fc_s(r, sp.rc, sp.ctw), dfc = dfc_s(r, sp.rc, sp.ctw);
+        double fc, dfc;
+        fc_dfc_s(r, sp.rc, sp.ctw, fc, dfc);
         const double ir = 1.0 / r;
         const double ux = nb_d[3 * p] * ir, uy = nb_d[3 * p + 1] * ir, uz = nb_d[3 * p + 2] * ir;
         const double *dcs = dcP + s * N * LM;
