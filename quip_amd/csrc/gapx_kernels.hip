@@ -378,8 +378,10 @@ __device__ __forceinline__ int wave_excl_scan_i(int v, int *total) {
 // exact CSR tables; MODE 2 fills per-atom slots of cap1 / cap2 entries
 // (neighbour1 of atom a at a*cap1, neighbour2 of atom a at a*cap2), counts both,
 // and raises *ovf when a count exceeds its capacity.
+// Latency-bound (most of its time waits on the candidates' global loads): the occupancy
+// hint gives five wavefronts per SIMD (96 VGPRs, no spills) instead of four.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_nl_pairs_w(int N, const double *__restrict__ pos,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_nl_pairs_w(int N, const double *__restrict__ pos,
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
                                                     const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
                                                     const int *__restrict__ cell_atoms,
