@@ -61,6 +61,8 @@ def _geoms():
                           np.diag([5.0, 5.0, 5.0])), 3.0
     yield "si4096", configs.si4096(), 5.0
     yield "si4096_skin", configs.si4096(), 5.5
+    # about 50 neighbour2 entries per atom: the ranking takes its path beyond 32 entries
+    yield "si64_rc8", configs.si64(), 8.0
     # more than 16384 atoms: the one-block count scans pass through LDS in two chunks
     yield "si17576", configs.perturb(configs.diamond(5.43, (13, 13, 13)), 0.1, configs.rng_for(0, 77)), 5.0
     # more than 65536 atoms: the count scans take the tile-sum path
