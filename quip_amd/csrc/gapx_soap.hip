@@ -446,7 +446,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
     double *gA = rbs + N, *iA = gA + N;
     double *dcs_all = rbs + 3 * N; // cpb x K*LM
     double *csc = dcs_all + cpb * KLM;   // cpb x (dot, 1/|p|)
-    double *stg = csc + 2 * cpb;   // max(K*K + K*(2L+1), 64*PR), or with whole max(cpb*(L+1)*K*K | cpb*d, 64*PR)
+    double *stg = csc + 2 * cpb;   // max(K*K + K*(2L+1), 64*PR), or with whole max(cpb*(L+1)*(K*K+2) | cpb*d, 64*PR)
     const int lane = threadIdx.x;
     const int d0 = cpb * blockIdx.x;
     const int nh = min(cpb, nc - d0);
@@ -473,12 +473,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
             double *dpL = stg + h * d;
             for (int q = lane; q < d; q += 64) dpL[q] = (gg[q] - x[q] * dot) * inv;
         } else if (whole == 2) {   // the level blocks P_l (below) of every centre, all levels
-            double *Ph = stg + h * (L + 1) * K * K;
+            // level blocks K*K + 2 doubles apart: rows of different levels read by one group
+            // of lanes then fall on different bank groups
+            const int KKp = K * K + 2;
+            double *Ph = stg + h * (L + 1) * KKp;
             for (int e = lane; e < (L + 1) * K * K; e += 64) {
                 const int l = e / (K * K), r = e - l * K * K, ia = r / K, jb = r - ia * K;
                 const int hi = ia > jb ? ia : jb, lo = ia > jb ? jb : ia;
                 const int q = (hi * (hi + 1) / 2 + lo) * (L + 1) + l;
-                Ph[e] = (gg[q] - x[q] * dot) * inv * (ia == jb ? 2.0 : 1.4142135623730950488) * c_isq[l];
+                Ph[l * KKp + r] = (gg[q] - x[q] * dot) * inv * (ia == jb ? 2.0 : 1.4142135623730950488) * c_isq[l];
             }
         }
     }
@@ -488,7 +491,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) vo
             const int h = it / KLM, e = it - h * KLM;
             const int ia = e / LM, lm = e - ia * LM;
             const int l = (int)sqrtf((float)lm);   // exact: lm < (l+1)^2 is far from a rounding step
-            const double *Pr = stg + h * (L + 1) * K * K + (l * K + ia) * K;
+            const double *Pr = stg + h * (L + 1) * (K * K + 2) + l * (K * K + 2) + ia * K;
             const double *cL = cbuf + (size_t)(d0 + h) * KLM + lm;
             double acc = 0.0;
             if (S == 1) {
@@ -750,7 +753,7 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     const size_t stg0 = std::max((size_t)sp.K * sp.K + (size_t)sp.K * (2 * sp.L + 1), (size_t)64 * PR);
     // whole: all centres of the block staged at once (small descriptors): 2, the level blocks
     // P_l (K x K each); 1, dE/dp; 0, one level at a time
-    const size_t pl = (size_t)(sp.L + 1) * sp.K * sp.K;
+    const size_t pl = (size_t)(sp.L + 1) * (sp.K * sp.K + 2);
     auto stg = [&](int cpb, int whole) {
         return whole == 2 ? std::max(stg0, cpb * pl) : whole == 1 ? std::max(stg0, (size_t)cpb * sp.dim) : stg0;
     };
