@@ -616,42 +616,141 @@ int orc_soap_grad_atom(const orc_coord *c, int N, const double *pos, const int *
 /* ------------------------------------------------------------------ */
 /* gp_predict                                                          */
 /* ------------------------------------------------------------------ */
+/* Covariance of x with sparse point s, without alpha and sparseCutoff:
+ * dot_product delta^2 (x.X_s)^zeta, ard_se delta^2 exp(-1/2 sum_d (x_d-X_sd)^2/theta_d^2).
+ * dk (nullable) gets dk/dx. */
+static double gp_cov(const orc_coord *c, const double *x, int s, double *dk) {
+    int d = c->dim;
+    const double *xs = c->sparseX + (size_t)s * d;
+    double d2 = c->delta * c->delta;
+    if (c->covariance == 2) {
+        double k = 0;
+        for (int q = 0; q < d; q++) k += x[q] * xs[q];
+        if (dk) {
+            double g = d2 * c->zeta * pow(k, c->zeta - 1.0);
+            for (int q = 0; q < d; q++) dk[q] = g * xs[q];
+        }
+        return d2 * pow(k, c->zeta);
+    }
+    double a = 0;
+    for (int q = 0; q < d; q++) { double t = (x[q] - xs[q]) / c->theta[q]; a += t * t; }
+    double k = d2 * exp(-0.5 * a);
+    if (dk)
+        for (int q = 0; q < d; q++) dk[q] = -k * (x[q] - xs[q]) / (c->theta[q] * c->theta[q]);
+    return k;
+}
+
+/* e = sum_s alpha_s sparseCutoff_s k_s(x); gradPredict = de/dx (dot_product:
+ * k_s = delta^2 (x.X_s)^zeta, gradient alpha delta^2 zeta (x.X_s)^(zeta-1) X_s). */
 static double gp_predict(const orc_coord *c, const double *x, double *gradPredict) {
     int d = c->dim;
-    double e = 0, d2 = c->delta * c->delta;
+    double e = 0;
+    double *dk = gradPredict ? (double *)malloc(sizeof(double) * d) : NULL;
     if (gradPredict) memset(gradPredict, 0, sizeof(double) * d);
     for (int s = 0; s < c->n_sparse; s++) {
-        const double *xs = c->sparseX + (size_t)s * d;
-        double w = c->alpha[s] * d2 * c->sparse_cutoff[s];
-        if (c->covariance == 2) {
-            double k = 0;
-            for (int q = 0; q < d; q++) k += x[q] * xs[q];
-            e += w * pow(k, c->zeta);
-            if (gradPredict) {
-                double gk = w * c->zeta * pow(k, c->zeta - 1.0);
-                for (int q = 0; q < d; q++) gradPredict[q] += gk * xs[q];
-            }
-        } else {
-            double a = 0;
-            for (int q = 0; q < d; q++) { double t = (x[q] - xs[q]) / c->theta[q]; a += t * t; }
-            double k = exp(-0.5 * a);
-            e += w * k;
-            if (gradPredict)
-                for (int q = 0; q < d; q++) gradPredict[q] -= w * k * (x[q] - xs[q]) / (c->theta[q] * c->theta[q]);
+        double w = c->alpha[s] * c->sparse_cutoff[s];
+        e += w * gp_cov(c, x, s, dk);
+        if (gradPredict)
+            for (int q = 0; q < d; q++) gradPredict[q] += w * dk[q];
+    }
+    free(dk);
+    return e;
+}
+
+double orc_gp_predict(const orc_coord *c, const double *x, double *gradPredict) {
+    return gp_predict(c, x, gradPredict);
+}
+
+/* Variance of the sparse GP prediction (projected-process form):
+ *   var(x) = k(x,x) - k_x^T (K_MM + reg I)^-1 k_x,   k_x[s] = cov(x, X_s),
+ *   K_MM[s][t] = cov(X_s, X_t), k(x,x) = cov(x, x);
+ *   d var/dx = dk(x,x)/dx - 2 (dk_x/dx)^T (K_MM + reg I)^-1 k_x.
+ * The reference's gpCoordinates_initialise_variance_estimate and the variance
+ * branch of gp_predict live in the absent GAP module; this is the textbook
+ * estimate that its gap_variance_regularisation key (IPModel_GAP.f95:332)
+ * regularises.  The factor L of K_MM + reg I (lower, row-major) is formed once
+ * per coordinate. */
+typedef struct { int M; double *L; } var_setup;
+
+static int var_prepare(const orc_coord *c, double reg, var_setup *v) {
+    int M = c->n_sparse, d = c->dim;
+    v->M = M;
+    v->L = (double *)calloc((size_t)M * M + 1, sizeof(double));
+    double *K = (double *)malloc(sizeof(double) * ((size_t)M * M + 1));
+    for (int s = 0; s < M; s++)
+        for (int t = 0; t < M; t++) K[(size_t)s * M + t] = gp_cov(c, c->sparseX + (size_t)t * d, s, NULL);
+    for (int s = 0; s < M; s++) K[(size_t)s * M + s] += reg;
+    for (int j = 0; j < M; j++) {
+        double a = K[(size_t)j * M + j];
+        for (int k = 0; k < j; k++) a -= v->L[(size_t)j * M + k] * v->L[(size_t)j * M + k];
+        if (!(a > 0)) { free(K); return 1; }
+        v->L[(size_t)j * M + j] = sqrt(a);
+        for (int i = j + 1; i < M; i++) {
+            double t = K[(size_t)i * M + j];
+            for (int k = 0; k < j; k++) t -= v->L[(size_t)i * M + k] * v->L[(size_t)j * M + k];
+            v->L[(size_t)i * M + j] = t / v->L[(size_t)j * M + j];
         }
     }
-    return e;
+    free(K);
+    return 0;
+}
+
+/* var(x); gvar (nullable) gets d var/dx */
+static double gp_variance(const orc_coord *c, const var_setup *v, const double *x, double *gvar) {
+    int M = v->M, d = c->dim;
+    double *k = (double *)malloc(sizeof(double) * (M + 1)), *y = (double *)malloc(sizeof(double) * (M + 1));
+    double *dk = (double *)malloc(sizeof(double) * (gvar ? (size_t)M * d : 1));
+    for (int s = 0; s < M; s++) k[s] = gp_cov(c, x, s, gvar ? dk + (size_t)s * d : NULL);
+    for (int i = 0; i < M; i++) {   /* y = L^-T L^-1 k */
+        double t = k[i];
+        for (int j = 0; j < i; j++) t -= v->L[(size_t)i * M + j] * y[j];
+        y[i] = t / v->L[(size_t)i * M + i];
+    }
+    for (int i = M - 1; i >= 0; i--) {
+        double t = y[i];
+        for (int j = i + 1; j < M; j++) t -= v->L[(size_t)j * M + i] * y[j];
+        y[i] = t / v->L[(size_t)i * M + i];
+    }
+    /* k(x,x) and its gradient */
+    double kxx;
+    if (c->covariance == 2) {
+        double xx = 0;
+        for (int q = 0; q < d; q++) xx += x[q] * x[q];
+        kxx = c->delta * c->delta * pow(xx, c->zeta);
+        if (gvar) {
+            double g = 2.0 * c->delta * c->delta * c->zeta * pow(xx, c->zeta - 1.0);
+            for (int q = 0; q < d; q++) gvar[q] = g * x[q];
+        }
+    } else {
+        kxx = c->delta * c->delta;
+        if (gvar) memset(gvar, 0, sizeof(double) * d);
+    }
+    double var = kxx;
+    for (int s = 0; s < M; s++) var -= k[s] * y[s];
+    if (gvar)
+        for (int s = 0; s < M; s++)
+            for (int q = 0; q < d; q++) gvar[q] -= 2.0 * dk[(size_t)s * d + q] * y[s];
+    free(k); free(y); free(dk);
+    return var;
 }
 
 /* ------------------------------------------------------------------ */
 /* IPModel_GAP_Calc                                                    */
 /* ------------------------------------------------------------------ */
-typedef struct { double *f, *vir, *le; } accum;
+typedef struct { double *f, *vir, *le, *lgv, *gvg; } accum;
 
+/* One descriptor instance (IPModel_GAP.f95:452-498): local energy e_i*cov/size(ci)
+ * to each ci; f(:,j) -= gradPredict.grad_data(:,:,n)*cov + e_i*grad_cov(:,n);
+ * virial_in(:,:,j) -= (pos_n - pos_first) (x) f_gp.  With a variance (var not NULL):
+ * local_gap_variance(ci) += var*cov^2/size(ci) and gap_variance_gradient(:,j) +=
+ * gvar.grad_data(:,:,n)*cov^2 + 2 var cov grad_cov(:,n) (:464-470, :484-488). */
 static void scatter(int N, accum *A, double e_i, double cov, const double *gcov, int nci, const int *ci,
-                    const double *gp, const double *grad, int dim, int ne, const int *ii, const double *epos) {
+                    const double *gp, const double *grad, int dim, int ne, const int *ii, const double *epos,
+                    const double *var, const double *gvar) {
     double ec = e_i * cov / nci;
     for (int q = 0; q < nci; q++) A->le[ci[q]] += ec;
+    if (var)
+        for (int q = 0; q < nci; q++) A->lgv[ci[q]] += *var * cov * cov / nci;
     if (!grad) return;
     for (int e = 0; e < ne; e++) {
         double fg[3];
@@ -666,35 +765,67 @@ static void scatter(int N, accum *A, double e_i, double cov, const double *gcov,
         for (int b = 0; b < 3; b++)
             for (int a = 0; a < 3; a++)
                 A->vir[9 * j + 3 * b + a] -= (epos[3 * e + a] - epos[a]) * fg[b];
+        if (var && gvar)
+            for (int k = 0; k < 3; k++) {
+                double v = 0;
+                const double *g = grad + (size_t)(e * 3 + k) * dim;
+                for (int q = 0; q < dim; q++) v += gvar[q] * g[q];
+                A->gvg[3 * j + k] += v * cov * cov + 2.0 * *var * cov * (gcov ? gcov[3 * e + k] : 0.0);
+            }
     }
     (void)N;
 }
 
-int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0, double E_scale, int N,
-                 const double *pos, const int *Z, const double *lat, const int *pbc, double nl_cutoff,
-                 int n_threads, double *e, double *local_e, double *f, double *virial, double *local_virial) {
+static void record_instance(const orc_calc_opts *o, int coord, double var, double cut2, int a, int b) {
+    if (!o || !o->n_inst) return;
+    int k = *o->n_inst;
+    if (k < o->inst_cap) {
+        if (o->inst_var) o->inst_var[k] = var;
+        if (o->inst_cut2) o->inst_cut2[k] = cut2;
+        if (o->inst_coord) o->inst_coord[k] = coord;
+        if (o->inst_atoms) { o->inst_atoms[2 * k] = a; o->inst_atoms[2 * k + 1] = b; }
+    }
+    *o->n_inst = k + 1;
+}
+
+int orc_gap_calc_ex(int n_coord, const orc_coord *coords, const double *e0, double E_scale, int N,
+                    const double *pos, const int *Z, const double *lat, const int *pbc, double nl_cutoff,
+                    int n_threads, double *e, double *local_e, double *f, double *virial, double *local_virial,
+                    const orc_calc_opts *opt) {
     int err;
     orc_nl *nl = orc_nl_build(N, pos, lat, pbc, nl_cutoff, &err);
     if (!nl) return 3;
-    int want_grad = (f || virial || local_virial);
+    const unsigned char *mask = opt ? opt->atom_mask : NULL;
+    const int want_var = opt && opt->want_variance;
+    const int want_grad = (f || virial || local_virial || (want_var && opt->gap_variance_gradient));
 #ifdef _OPENMP
     if (n_threads <= 0) n_threads = omp_get_max_threads();
 #else
     n_threads = 1;
 #endif
+    if (want_var || (opt && opt->n_inst)) n_threads = 1;   /* instances recorded in the reference's order */
+    if (opt && opt->n_inst) *opt->n_inst = 0;
     accum *acc = (accum *)calloc(n_threads, sizeof(accum));
     for (int t = 0; t < n_threads; t++) {
         acc[t].f = (double *)calloc(3 * (size_t)N + 1, sizeof(double));
         acc[t].vir = (double *)calloc(9 * (size_t)N + 1, sizeof(double));
         acc[t].le = (double *)calloc((size_t)N + 1, sizeof(double));
+        acc[t].lgv = (double *)calloc((size_t)N + 1, sizeof(double));
+        acc[t].gvg = (double *)calloc(3 * (size_t)N + 1, sizeof(double));
     }
     int status = 0;
     for (int ic = 0; ic < n_coord; ic++) {
         const orc_coord *c = coords + ic;
+        /* only_descriptor selects one coordinate when there are several (IPModel_GAP.f95:398-403) */
+        if (opt && opt->only_descriptor > 0 && n_coord > 1 && ic != opt->only_descriptor - 1) continue;
+        double epc = 0.0;
+        var_setup vs = {0, NULL};
+        if (want_var && var_prepare(c, opt->variance_regularisation, &vs)) { status = 6; free(vs.L); break; }
         if (c->kind == 1) {
             soap_setup s;
-            if (soap_prepare(c, &s)) { status = 2; break; }
+            if (soap_prepare(c, &s)) { status = 2; free(vs.L); break; }
             int dim = s.dim;
+            double *ei_all = (double *)calloc((size_t)N + 1, sizeof(double));
 #pragma omp parallel num_threads(n_threads)
             {
                 int tid = 0;
@@ -703,10 +834,11 @@ int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0, double 
 #endif
                 int maxe = 0;
                 double *x = (double *)malloc(sizeof(double) * dim), *gp = (double *)malloc(sizeof(double) * dim);
+                double *gv = (double *)malloc(sizeof(double) * dim);
                 double *grad = NULL, *epos = NULL; int *ii = NULL;
 #pragma omp for schedule(static)
                 for (int i = 0; i < N; i++) {
-                    if (!is_centre(c, Z[i])) continue;
+                    if (!is_centre(c, Z[i]) || (mask && !mask[i])) continue;
                     int need = nl_count(nl, i) + 1;
                     if (need > maxe) {
                         maxe = need;
@@ -716,14 +848,24 @@ int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0, double 
                     }
                     int ne = soap_instance(c, &s, nl, pos, Z, lat, i, x, grad, ii, epos, maxe + 1);
                     double ei = gp_predict(c, x, want_grad ? gp : NULL);
+                    ei_all[i] = ei;
+                    double var = 0;
+                    if (want_var) {
+                        var = gp_variance(c, &vs, x, want_grad ? gv : NULL);
+                        record_instance(opt, ic, var, 1.0, i, -1);
+                    }
                     int ci1[1] = {i};
-                    scatter(N, &acc[tid], ei, 1.0, NULL, 1, ci1, gp, grad, dim, ne, ii, epos);
+                    scatter(N, &acc[tid], ei, 1.0, NULL, 1, ci1, gp, grad, dim, ne, ii, epos,
+                            want_var ? &var : NULL, want_var ? gv : NULL);
                 }
-                free(x); free(gp); free(grad); free(epos); free(ii);
+                free(x); free(gp); free(gv); free(grad); free(epos); free(ii);
             }
+            for (int i = 0; i < N; i++) epc += ei_all[i];
+            free(ei_all);
         } else if (c->kind == 2) {
             double w = c->cutoff_transition_width;
             for (int i = 0; i < N; i++) {
+                if (mask && !mask[i]) continue;
                 int zi1 = (c->Z1 == 0) || (Z[i] == c->Z1), zi2 = (c->Z2 == 0) || (Z[i] == c->Z2);
                 int nnb = nl_count(nl, i);
                 for (int q = 0; q < nnb; q++) {
@@ -733,17 +875,26 @@ int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0, double 
                     int zj1 = (c->Z1 == 0) || (Z[j] == c->Z1), zj2 = (c->Z2 == 0) || (Z[j] == c->Z2);
                     if (!((zi1 && zj2) || (zi2 && zj1))) continue;
                     double u[3] = {d[0] / r, d[1] / r, d[2] / r};
-                    double x1 = r, gp1;
+                    double x1 = r, gp1, gv1 = 0;
                     double ei = gp_predict(c, &x1, &gp1);
                     double cov = fcut(r, c->cutoff, w), dcov = dfcut(r, c->cutoff, w);
+                    epc += ei * cov;
+                    double var = 0;
+                    if (want_var) {
+                        var = gp_variance(c, &vs, &x1, &gv1);
+                        record_instance(opt, ic, var, cov * cov, i, j);
+                    }
                     double grad[6] = {-u[0], -u[1], -u[2], u[0], u[1], u[2]};
                     double gcov[6] = {-dcov * u[0], -dcov * u[1], -dcov * u[2], dcov * u[0], dcov * u[1], dcov * u[2]};
                     int ii[2] = {i, j}, ci[2] = {i, j};
                     double epos[6] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], pos[3 * i] + d[0], pos[3 * i + 1] + d[1], pos[3 * i + 2] + d[2]};
-                    scatter(N, &acc[0], ei, cov, gcov, 2, ci, &gp1, want_grad ? grad : NULL, 1, 2, ii, epos);
+                    scatter(N, &acc[0], ei, cov, gcov, 2, ci, &gp1, want_grad ? grad : NULL, 1, 2, ii, epos,
+                            want_var ? &var : NULL, want_var ? &gv1 : NULL);
                 }
             }
-        } else { status = 5; break; }
+        } else { status = 5; free(vs.L); break; }
+        free(vs.L);
+        if (opt && opt->energy_per_coordinate) opt->energy_per_coordinate[ic] = epc;
     }
     if (!status) {
         /* fixed-order reduction over thread-private arrays */
@@ -753,14 +904,15 @@ int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0, double 
             for (size_t q = 0; q < 9 * (size_t)N; q++) Vv[q] += acc[t].vir[q];
             for (size_t q = 0; q < (size_t)N; q++) LE[q] += acc[t].le[q];
         }
+        /* e0 for every atom, or for the masked atoms only (IPModel_GAP.f95:576-590) */
         if (f) for (size_t q = 0; q < 3 * (size_t)N; q++) f[q] = E_scale * F[q];
         if (e) {
             double s = 0;
             for (int i = 0; i < N; i++) s += LE[i];
-            for (int i = 0; i < N; i++) s += e0[Z[i]];
+            for (int i = 0; i < N; i++) if (!mask || mask[i]) s += e0[Z[i]];
             *e = E_scale * s;
         }
-        if (local_e) for (int i = 0; i < N; i++) local_e[i] = E_scale * (LE[i] + e0[Z[i]]);
+        if (local_e) for (int i = 0; i < N; i++) local_e[i] = E_scale * (LE[i] + ((!mask || mask[i]) ? e0[Z[i]] : 0.0));
         if (virial) {
             for (int q = 0; q < 9; q++) {
                 double s = 0;
@@ -769,9 +921,19 @@ int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0, double 
             }
         }
         if (local_virial) for (size_t q = 0; q < 9 * (size_t)N; q++) local_virial[q] = E_scale * Vv[q];
+        /* the variance outputs are not scaled by E_scale (:592-596 scales e, f, virials only) */
+        if (want_var && opt->local_gap_variance) memcpy(opt->local_gap_variance, acc[0].lgv, sizeof(double) * N);
+        if (want_var && opt->gap_variance_gradient) memcpy(opt->gap_variance_gradient, acc[0].gvg, sizeof(double) * 3 * N);
     }
-    for (int t = 0; t < n_threads; t++) { free(acc[t].f); free(acc[t].vir); free(acc[t].le); }
+    for (int t = 0; t < n_threads; t++) { free(acc[t].f); free(acc[t].vir); free(acc[t].le); free(acc[t].lgv); free(acc[t].gvg); }
     free(acc);
     orc_nl_free(nl);
     return status;
+}
+
+int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0, double E_scale, int N,
+                 const double *pos, const int *Z, const double *lat, const int *pbc, double nl_cutoff,
+                 int n_threads, double *e, double *local_e, double *f, double *virial, double *local_virial) {
+    return orc_gap_calc_ex(n_coord, coords, e0, E_scale, N, pos, Z, lat, pbc, nl_cutoff, n_threads, e, local_e, f,
+                           virial, local_virial, NULL);
 }

@@ -77,11 +77,40 @@ int orc_soap_grad_atom(const orc_coord *c, int N, const double *pos, const int *
                        const double *lat, const int *pbc, double nl_cutoff, int atom,
                        double *x, double *grad, int *ii, int max_entries, int *n_entries);
 
+/* gp_predict of one descriptor vector x (dim entries): returns the prediction
+ * sum_s alpha_s sparseCutoff_s k(x, X_s), dot_product k = delta^2 (x.X_s)^zeta,
+ * ard_se k = delta^2 exp(-1/2 sum_d ((x_d - X_sd)/theta_d)^2); gradPredict
+ * (nullable, dim entries) gets its gradient. */
+double orc_gp_predict(const orc_coord *c, const double *x, double *gradPredict);
+
+/* Options of IPModel_GAP_Calc's args_str (IPModel_GAP.f95:324-335) and their outputs. */
+typedef struct {
+    const unsigned char *atom_mask;   /* atom_mask_name: nullable, N bytes; only masked atoms are
+                                         descriptor centres / first atoms of pairs and get e0 */
+    int only_descriptor;              /* 0: all coordinates; k >= 1: coordinate k alone when n_coord > 1 */
+    int want_variance;                /* local_gap_variance / print_gap_variance */
+    double variance_regularisation;   /* gap_variance_regularisation */
+    double *energy_per_coordinate;    /* nullable, n_coord: sum of e_i * covariance_cutoff */
+    double *local_gap_variance;       /* nullable, N */
+    double *gap_variance_gradient;    /* nullable, 3N (f(3,N) layout) */
+    /* per-instance variances in the reference's order (print_gap_variance), nullable */
+    int inst_cap;
+    double *inst_var, *inst_cut2;     /* variance, covariance_cutoff^2 */
+    int *inst_coord, *inst_atoms;     /* coordinate; first atom, second atom (-1 for soap) */
+    int *n_inst;
+} orc_calc_opts;
+
 /* Full IPModel_GAP_Calc restatement.  Any output pointer may be NULL. */
 int orc_gap_calc(int n_coord, const orc_coord *coords, const double *e0 /* [119] by Z */,
                  double E_scale, int N, const double *pos, const int *Z, const double *lat,
                  const int *pbc, double nl_cutoff, int n_threads,
                  double *e, double *local_e, double *f, double *virial, double *local_virial);
+
+/* As orc_gap_calc with the options above (opt may be NULL). */
+int orc_gap_calc_ex(int n_coord, const orc_coord *coords, const double *e0, double E_scale, int N,
+                    const double *pos, const int *Z, const double *lat, const int *pbc, double nl_cutoff,
+                    int n_threads, double *e, double *local_e, double *f, double *virial, double *local_virial,
+                    const orc_calc_opts *opt);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,14 @@ class OrcCoord(C.Structure):
                 ("sparse_cutoff", C.POINTER(C.c_double)), ("theta", C.POINTER(C.c_double))]
 
 
+class OrcCalcOpts(C.Structure):
+    _fields_ = [("atom_mask", C.c_void_p), ("only_descriptor", C.c_int), ("want_variance", C.c_int),
+                ("variance_regularisation", C.c_double), ("energy_per_coordinate", C.c_void_p),
+                ("local_gap_variance", C.c_void_p), ("gap_variance_gradient", C.c_void_p), ("inst_cap", C.c_int),
+                ("inst_var", C.c_void_p), ("inst_cut2", C.c_void_p), ("inst_coord", C.c_void_p),
+                ("inst_atoms", C.c_void_p), ("n_inst", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -52,6 +60,9 @@ def lib():
                                             C.c_int, C.POINTER(C.c_int)]
         _lib.orc_gap_calc.argtypes = [C.c_int, C.POINTER(OrcCoord), C.c_void_p, C.c_double, C.c_int, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int] + [C.c_void_p] * 5
+        _lib.orc_gap_calc_ex.argtypes = _lib.orc_gap_calc.argtypes + [C.POINTER(OrcCalcOpts)]
+        _lib.orc_gp_predict.restype = C.c_double
+        _lib.orc_gp_predict.argtypes = [C.POINTER(OrcCoord), C.c_void_p, C.c_void_p]
     return _lib
 
 
@@ -178,3 +189,59 @@ def gap_calc(model, at, nl_cutoff=None, n_threads=0, E_scale=1.0):
     if st:
         raise RuntimeError("orc_gap_calc failed (%d)" % st)
     return dict(energy=e[0], local_energy=le, forces=f, virial=v.T.copy(), local_virial=lv)
+
+
+def gp_predict(coord, x, grad=False):
+    """gp_predict of one descriptor vector: prediction (and its gradient)."""
+    arr, keep = make_coords([coord])
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.zeros(len(x)) if grad else None
+    e = lib().orc_gp_predict(arr, _p(x), _p(g))
+    return (e, g) if grad else e
+
+
+def gap_calc_ex(model, at, nl_cutoff=None, E_scale=1.0, atom_mask=None, only_descriptor=0, variance=False,
+                variance_regularisation=1e-3, n_threads=0, max_instances=0):
+    """IPModel_GAP_Calc with its args_str options: atom_mask (N booleans), only_descriptor
+    (1-based coordinate), energy_per_coordinate, and with `variance` local_gap_variance,
+    gap_variance_gradient (N,3) and the per-instance variances (print_gap_variance)."""
+    arr, keep = make_coords(model.coords)
+    pos, Z, lat, pbc = _geom(at)
+    N = len(at)
+    e = np.zeros(1)
+    le = np.zeros(N)
+    f = np.zeros((N, 3))
+    v = np.zeros((3, 3))
+    lv = np.zeros((N, 9))
+    epc = np.zeros(len(model.coords))
+    lgv = np.zeros(N)
+    gvg = np.zeros((N, 3))
+    cap = max_instances
+    ivar, icut, icoord, iatoms = np.zeros(cap + 1), np.zeros(cap + 1), np.zeros(cap + 1, np.int32), \
+        np.zeros((cap + 1, 2), np.int32)
+    ninst = np.zeros(1, np.int32)
+    mask = None if atom_mask is None else np.ascontiguousarray(np.asarray(atom_mask, dtype=bool).astype(np.uint8))
+    o = OrcCalcOpts()
+    o.atom_mask = _p(mask)
+    o.only_descriptor = int(only_descriptor)
+    o.want_variance = 1 if variance else 0
+    o.variance_regularisation = variance_regularisation
+    o.energy_per_coordinate = _p(epc)
+    o.local_gap_variance = _p(lgv)
+    o.gap_variance_gradient = _p(gvg)
+    o.inst_cap = cap
+    o.inst_var, o.inst_cut2, o.inst_coord, o.inst_atoms = _p(ivar), _p(icut), _p(icoord), _p(iatoms)
+    o.n_inst = _p(ninst) if cap > 0 else None
+    e0 = np.ascontiguousarray(model.e0, dtype=np.float64)
+    rc = nl_cutoff if nl_cutoff is not None else model.cutoff
+    st = lib().orc_gap_calc_ex(len(model.coords), arr, _p(e0), E_scale, N, _p(pos), _p(Z), _p(lat), _p(pbc), rc,
+                               n_threads, _p(e), _p(le), _p(f), _p(v), _p(lv), C.byref(o))
+    if st:
+        raise RuntimeError("orc_gap_calc_ex failed (%d)" % st)
+    out = dict(energy=e[0], local_energy=le, forces=f, virial=v.T.copy(), local_virial=lv,
+               energy_per_coordinate=epc)
+    if variance:
+        n = min(int(ninst[0]), cap)
+        out.update(local_gap_variance=lgv, gap_variance_gradient=gvg, instance_variance=ivar[:n],
+                   instance_cutoff2=icut[:n], instance_coord=icoord[:n], instance_atoms=iatoms[:n])
+    return out

@@ -71,19 +71,24 @@ static bool xml_parse(const char *p, const char *end, XNode &root, std::string &
             p = q;
             continue;
         }
+        // the buffer is length-delimited and need not be NUL-terminated: every search
+        // stays inside [p, end)
         if (end - p >= 4 && !strncmp(p, "<!--", 4)) {
-            const char *q = strstr(p, "-->");
-            if (!q) { err = "unterminated comment"; return false; }
+            static const char tail[] = "-->";
+            const char *q = std::search(p + 4, end, tail, tail + 3);
+            if (q == end) { err = "unterminated comment"; return false; }
             p = q + 3;
             continue;
         }
         if (end - p >= 9 && !strncmp(p, "<![CDATA[", 9)) {
-            const char *q = strstr(p, "]]>");
-            if (!q) { err = "unterminated CDATA"; return false; }
+            static const char tail[] = "]]>";
+            const char *q = std::search(p + 9, end, tail, tail + 3);
+            if (q == end) { err = "unterminated CDATA"; return false; }
             stack.back()->text += std::string(p + 9, q);
             p = q + 3;
             continue;
         }
+        if (end - p < 2) { err = "truncated tag"; return false; }
         if (p[1] == '?' || p[1] == '!') {
             const char *q = (const char *)memchr(p, '>', end - p);
             if (!q) { err = "unterminated declaration"; return false; }
@@ -104,12 +109,12 @@ static bool xml_parse(const char *p, const char *end, XNode &root, std::string &
         while (q < end && !isspace((unsigned char)*q) && *q != '>' && *q != '/') q++;
         auto node = std::make_unique<XNode>();
         node->tag.assign(p + 1, q);
-        bool selfclose = false;
+        bool selfclose = false, closed = false;
         while (q < end) {
             while (q < end && isspace((unsigned char)*q)) q++;
             if (q >= end) break;
             if (*q == '/') { selfclose = true; q++; continue; }
-            if (*q == '>') { q++; break; }
+            if (*q == '>') { q++; closed = true; break; }
             const char *k0 = q;
             while (q < end && *q != '=' && !isspace((unsigned char)*q) && *q != '>') q++;
             std::string key(k0, q);
@@ -117,14 +122,17 @@ static bool xml_parse(const char *p, const char *end, XNode &root, std::string &
             if (q < end && *q == '=') {
                 q++;
                 while (q < end && isspace((unsigned char)*q)) q++;
+                if (q >= end) { err = "truncated attribute " + key; return false; }
                 char quote = *q;
                 if (quote != '"' && quote != '\'') { err = "unquoted attribute " + key; return false; }
                 const char *v0 = ++q;
                 while (q < end && *q != quote) q++;
+                if (q >= end) { err = "unterminated value of attribute " + key; return false; }
                 node->attrs.emplace_back(key, xml_unescape(std::string(v0, q)));
                 q++;
             }
         }
+        if (!closed) { err = "unterminated tag <" + node->tag + ">"; return false; }
         XNode *raw = node.get();
         stack.back()->kids.push_back(std::move(node));
         if (!selfclose) stack.push_back(raw);

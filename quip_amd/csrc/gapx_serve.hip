@@ -240,8 +240,11 @@ extern "C" int gapx_server_wait(gapx_server *srv, int64_t ticket, double *e, dou
         auto it = srv->tickets.find(ticket);
         if (it == srv->tickets.end()) return fail(GAPX_E_ARG, "unknown ticket %lld", (long long)ticket);
         r = it->second;
+        // the lock is released while waiting: other threads may insert tickets (which
+        // can rehash the map) or wait on the same ticket, so it is erased by key
         srv->cv_done.wait(lk, [&] { return r->done; });
-        srv->tickets.erase(it);
+        if (srv->tickets.erase(ticket) == 0)
+            return fail(GAPX_E_ARG, "ticket %lld was waited on by another thread", (long long)ticket);
     }
     if (latency_s) *latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
     if (r->status != GAPX_OK) return fail(r->status, "%s", r->err.c_str());
