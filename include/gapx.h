@@ -43,6 +43,9 @@ extern "C" {
 #define GAPX_WANT_V 4u
 #define GAPX_WANT_LOCAL_E 8u
 #define GAPX_WANT_LOCAL_V 16u
+/* form the GAP variance (gapx_calc_ex / gapx_calc_device_ex; implied by a non-NULL
+ * local_gap_variance or gap_variance_gradient) */
+#define GAPX_WANT_VARIANCE 32u
 
 /* arithmetic of the descriptor x sparse-point GEMMs */
 #define GAPX_PREC_FP64 0    /* plain form: FP64 throughout                          */
@@ -97,6 +100,45 @@ int gapx_calc_device(gapx_model *model, int n_atoms, const double *d_pos, const 
                      const int *pbc, double nl_cutoff, unsigned want, double *d_e, double *d_local_e, double *d_f,
                      double *d_virial, double *d_local_virial);
 int gapx_sync(gapx_model *model);
+
+/* gapx_calc with the args_str keys of IPModel_GAP_Calc that change what it computes
+ * (IPModel_GAP.f95:324-335); each replaces the reference's handling of one key:
+ *   atom_mask (nullable, n_atoms bytes)   atom_mask_name (:324-326, :343-347): only atoms
+ *       with a non-zero byte are descriptor centres and first atoms of pair descriptors
+ *       (the descriptor calc's mask) and get e0 (:576-590); forces still reach every atom;
+ *   only_descriptor   (:334, :398-403): 0 evaluates every coordinate; k >= 1 evaluates
+ *       coordinate k (1-based) alone when the model has more than one (e0 is still added);
+ *   energy_per_coordinate (nullable, one value per coordinate)   (:335, :462, :568): the
+ *       sum of e_i * covariance_cutoff per coordinate, before e0 and E_scale;
+ *   GAPX_WANT_VARIANCE in want, or a non-NULL local_gap_variance / gap_variance_gradient:
+ *       local_gap_variance (:330-332, :464-470, :558-561) and print_gap_variance (:506-516;
+ *       the per-instance values from gapx_descriptor_variances), with
+ *       variance_regularisation = gap_variance_regularisation (:332, default 0.001);
+ *       local_gap_variance[N] gets var * fc^2 / size(ci) per instance and atom of ci;
+ *       gap_variance_gradient[3N] (f layout) its gradient, with the reference's plus sign
+ *       (:484-488).  The variance is var(x) = k(x,x) - k_x^T (K_MM + reg I)^-1 k_x with
+ *       k = delta^2 kernel, formed in FP64 whatever the GEMM precision; models with a
+ *       sparseCutoff other than 1 return GAPX_E_UNSUPPORTED for it.  Neither variance
+ *       output is scaled by E_scale (:592-596 scales e, f and the virials only). */
+int gapx_calc_ex(gapx_model *model, int n_atoms, const double *pos, const int *Z, const double *lattice,
+                 const int *pbc, double nl_cutoff, unsigned want, const unsigned char *atom_mask, int only_descriptor,
+                 double variance_regularisation, double *e, double *local_e, double *f, double *virial,
+                 double *local_virial, double *energy_per_coordinate, double *local_gap_variance,
+                 double *gap_variance_gradient);
+/* As gapx_calc_ex with positions and every output in device memory (atom_mask stays a
+ * host array); queued on the model's stream like gapx_calc_device. */
+int gapx_calc_device_ex(gapx_model *model, int n_atoms, const double *d_pos, const int *Z, const double *lattice,
+                        const int *pbc, double nl_cutoff, unsigned want, const unsigned char *atom_mask,
+                        int only_descriptor, double variance_regularisation, double *d_e, double *d_local_e,
+                        double *d_f, double *d_virial, double *d_local_virial, double *d_energy_per_coordinate,
+                        double *d_local_gap_variance, double *d_gap_variance_gradient);
+/* Per-instance variances of coordinate `coord` from the last calc that formed the
+ * variance, in the reference's instance order (the values print_gap_variance prints,
+ * IPModel_GAP.f95:506-516): variance, covariance_cutoff^2 and the instance's atoms
+ * (soap: the centre and -1; distance_2b: the directed pair i, j).  Up to max_entries are
+ * written; n_entries gets the count. */
+int gapx_descriptor_variances(gapx_model *model, int coord, int max_entries, double *variance, double *cutoff_sq,
+                              int *atoms, int *n_entries);
 
 /* Many structures of uneven size in one pass (struct-of-arrays with offsets):
  * structure s owns atoms [atom_off[s], atom_off[s+1]), atom_off[0] = 0;

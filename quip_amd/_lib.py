@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgapx.so")
 
-WANT_E, WANT_F, WANT_V, WANT_LOCAL_E, WANT_LOCAL_V = 1, 2, 4, 8, 16
+WANT_E, WANT_F, WANT_V, WANT_LOCAL_E, WANT_LOCAL_V, WANT_VARIANCE = 1, 2, 4, 8, 16, 32
 PREC = {"fp64": 0, "bf16x3": 1, "fp16x3": 2}
 
 
@@ -38,6 +38,12 @@ _SIGS = [
     ("gapx_model_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("gapx_calc", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                             C.c_uint] + [C.c_void_p] * 5),
+    ("gapx_calc_ex", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                               C.c_uint, C.c_void_p, C.c_int, C.c_double] + [C.c_void_p] * 8),
+    ("gapx_calc_device_ex", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_double, C.c_uint, C.c_void_p, C.c_int, C.c_double] + [C.c_void_p] * 8),
+    ("gapx_descriptor_variances", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.POINTER(C.c_int)]),
     ("gapx_calc_lammps", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
      + [C.c_void_p] * 10),
     ("gapx_calc_device", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,

@@ -2,6 +2,8 @@
 // reader, device-resident model state and the per-call pipeline
 //   neighbour list -> SOAP forward -> GEMM1 (K, fused GP epilogue) ->
 //   GEMM2 (G) -> SOAP adjoint -> distance_2b -> deterministic gather.
+#include <thread>
+
 #include "gapx_host.h"
 
 using namespace gapx;
@@ -275,6 +277,49 @@ static int read_sparse_file(const std::string &path, size_t count, std::vector<d
     return GAPX_OK;
 }
 
+// Fortran list-directed logical: T, .true., t... are true
+static bool fortran_true(const std::string &v) {
+    size_t i = 0;
+    while (i < v.size() && (isspace((unsigned char)v[i]) || v[i] == '.')) i++;
+    return i < v.size() && (v[i] == 'T' || v[i] == 't');
+}
+
+// Descriptor options that change the descriptor and are not built: each is accepted
+// only at its default (GAP's descriptor parser ignores keys it does not know, since
+// gap_fit's own keys ride in the same string, so unknown keys are ignored here too).
+struct OptDefault { const char *key; bool logical; double value; };
+static const OptDefault kSoapDefaults[] = {
+    {"normalise", true, 1}, {"normalize", true, 1}, {"average", true, 0}, {"diagonal_radial", true, 0},
+    {"central_reference_all_species", true, 0}, {"covariance_sigma0", false, 0.0}, {"cutoff_dexp", false, 0.0},
+    {"cutoff_scale", false, 1.0}, {"cutoff_rate", false, 1.0}, {"radial_enhancement", false, 0.0},
+    {"nu_R", false, 2.0}, {"nu_S", false, 2.0}, {"Z_mix", true, 0}, {"R_mix", true, 0}, {"sym_mix", true, 0},
+    {"compress_file", false, 0.0}, {"compress_mode", false, 0.0}};
+static const OptDefault kD2bDefaults[] = {
+    {"only_intra", true, 0}, {"only_inter", true, 0}, {"n_exponents", false, 1.0}, {"exponents", false, 1.0},
+    {"resid_name", false, 0.0}};
+
+static int check_defaults(const std::vector<std::pair<std::string, std::string>> &a, const OptDefault *tab, int n,
+                          const std::string &desc) {
+    for (int q = 0; q < n; q++) {
+        const std::string *v = arg_get(a, tab[q].key);
+        if (!v) continue;
+        bool ok;
+        if (tab[q].logical) ok = fortran_true(*v) == (tab[q].value != 0);
+        else if (!strcmp(tab[q].key, "compress_file") || !strcmp(tab[q].key, "compress_mode") ||
+                 !strcmp(tab[q].key, "resid_name"))
+            ok = v->empty();
+        else {
+            char *e = nullptr;
+            const double x = strtod(v->c_str(), &e);
+            ok = e != v->c_str() && x == tab[q].value;
+        }
+        if (!ok)
+            return fail(GAPX_E_UNSUPPORTED, "descriptor option %s=%s is not built (only its default): '%s'",
+                        tab[q].key, v->c_str(), desc.c_str());
+    }
+    return GAPX_OK;
+}
+
 static int parse_descriptor(Coord &c, const std::string &s) {
     c.desc = s;
     std::string kind;
@@ -283,6 +328,7 @@ static int parse_descriptor(Coord &c, const std::string &s) {
     auto a = parse_args(sp == std::string::npos ? "" : s.substr(sp));
     auto num = [&](const char *k, double dflt) { auto v = arg_get(a, k); return v ? atof(v->c_str()) : dflt; };
     if (kind == "soap") {
+        CHK(check_defaults(a, kSoapDefaults, (int)(sizeof(kSoapDefaults) / sizeof(kSoapDefaults[0])), s));
         c.kind = 1;
         if (!arg_get(a, "cutoff") || !arg_get(a, "n_max") || !arg_get(a, "l_max"))
             return fail(GAPX_E_MODEL, "soap descriptor needs cutoff, n_max and l_max: '%s'", s.c_str());
@@ -308,6 +354,7 @@ static int parse_descriptor(Coord &c, const std::string &s) {
         return soap_basis(c);
     }
     if (kind == "distance_2b") {
+        CHK(check_defaults(a, kD2bDefaults, (int)(sizeof(kD2bDefaults) / sizeof(kD2bDefaults[0])), s));
         c.kind = 2;
         c.rc = num("cutoff", 0);
         c.ctw = num("cutoff_transition_width", 0.5);
@@ -318,6 +365,8 @@ static int parse_descriptor(Coord &c, const std::string &s) {
     }
     return fail(GAPX_E_UNSUPPORTED, "descriptor type '%s' is not supported", kind.c_str());
 }
+
+static SoapDev soap_dev(const Coord &c);
 
 static int upload(DBuf &b, const void *h, size_t bytes, hipStream_t s = nullptr) {
     CHK(b.ensure(bytes));
@@ -359,12 +408,21 @@ static int model_load(const char *xml_text, int64_t xml_len, const char *label, 
     const char *ver = gp->attr("gap_version");
     if (ver && atoll(ver) > 1527075646LL)
         return fail(GAPX_E_MODEL, "gap_version %s is newer than this reader (1527075646)", ver);
-    if (const XNode *data = gp->child("GAP_data"))
+    if (const XNode *data = gp->child("GAP_data")) {
+        // a scalar e0 attribute sets every species, then <e0 Z value> entries override
+        // (IPModel_GAP_read_params_xml, IPModel_GAP.f95:670-674, 685-701)
+        if (const char *e0 = data->attr("e0"))
+            for (int z = 0; z < 119; z++) m->e0[z] = atof(e0);
+        if (const char *pca = data->attr("do_pca"))
+            if (fortran_true(pca)) return fail(GAPX_E_UNSUPPORTED, "GAP_data do_pca=T is not built");
         for (auto &e : data->kids)
-            if (e->tag == "e0" && e->attr("Z") && e->attr("value")) {
+            if (e->tag == "e0") {
+                if (!e->attr("Z") || !e->attr("value")) return fail(GAPX_E_MODEL, "GAP_data e0 needs Z and value");
                 int z = atoi(e->attr("Z"));
-                if (z >= 0 && z < 119) m->e0[z] = atof(e->attr("value"));
+                if (z < 0 || z >= 119) return fail(GAPX_E_MODEL, "GAP_data e0 Z=%d out of range", z);
+                m->e0[z] = atof(e->attr("value"));
             }
+    }
     const XNode *sparse = gp->child("gpSparse");
     if (!sparse) return fail(GAPX_E_MODEL, "no gpSparse in GAP_params");
     std::string dir = xml_dir && *xml_dir ? xml_dir : ".";
@@ -388,6 +446,13 @@ static int model_load(const char *xml_text, int64_t xml_len, const char *label, 
         c->cov = atoi(ct);
         c->signal_mean = g->attr("signal_mean") ? atof(g->attr("signal_mean")) : 0.0;
         c->zeta = g->attr("zeta") ? atof(g->attr("zeta")) : 1.0;
+        if (c->signal_mean != 0.0)
+            return fail(GAPX_E_UNSUPPORTED, "gpCoordinates %s: signal_mean=%s is not built (only 0)", c->label.c_str(),
+                        g->attr("signal_mean"));
+        if (const char *np = g->attr("n_permutations"))
+            if (atoi(np) > 1)
+                return fail(GAPX_E_UNSUPPORTED, "gpCoordinates %s: n_permutations=%s is not built (only 1)",
+                            c->label.c_str(), np);
         if (c->cov != 1 && c->cov != 2) return fail(GAPX_E_UNSUPPORTED, "covariance_type %d not supported", c->cov);
         if (c->kind == 1 && c->cov != 2) return fail(GAPX_E_UNSUPPORTED, "soap with covariance_type %d", c->cov);
         if (c->kind == 2 && c->cov != 1) return fail(GAPX_E_UNSUPPORTED, "distance_2b with covariance_type %d", c->cov);
@@ -415,6 +480,20 @@ static int model_load(const char *xml_text, int64_t xml_len, const char *label, 
         }
         if (c->kind == 2 && c->theta.size() != 1) return fail(GAPX_E_MODEL, "distance_2b needs one theta");
         CHK(read_sparse_file(dir + "/" + fn, (size_t)c->M * c->dim, c->Xs));
+        m->cutoff = std::max(m->cutoff, c->rc);
+        m->coords.push_back(std::move(c));
+    }
+    if (m->coords.empty()) return fail(GAPX_E_MODEL, "no gpCoordinates in gpSparse");
+    return GAPX_OK;
+}
+
+// Device half of the model load, on the current device: sparse points, coefficients and
+// the SOAP tables; refuses descriptor shapes whose kernels need more LDS than a block has.
+static int model_upload(gapx_model *m) {
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device));
+    for (auto &cp : m->coords) {
+        Coord *c = cp.get();
         std::vector<double> w(c->M);
         for (int s = 0; s < c->M; s++) w[s] = c->alpha[s] * c->delta * c->delta * c->scut[s];
         CHK(upload(c->d_Xs, c->Xs.data(), c->Xs.size() * 8));
@@ -423,13 +502,15 @@ static int model_load(const char *xml_text, int64_t xml_len, const char *label, 
             CHK(upload(c->d_rb, c->rb.data(), c->rb.size() * 8));
             CHK(upload(c->d_T, c->T.data(), c->T.size() * 8));
             CHK(upload(c->d_qmap, c->qmap.data(), c->qmap.size() * 4));
+            const size_t need = soap_lds_bytes(soap_dev(*c));
+            if (need > (size_t)lds)
+                return fail(GAPX_E_UNSUPPORTED,
+                            "soap n_max=%d l_max=%d with %d species needs %zu bytes of LDS per block (%d available)",
+                            c->n, c->L, c->S, need, lds);
         } else {
             CHK(upload(c->d_theta, c->theta.data(), c->theta.size() * 8));
         }
-        m->cutoff = std::max(m->cutoff, c->rc);
-        m->coords.push_back(std::move(c));
     }
-    if (m->coords.empty()) return fail(GAPX_E_MODEL, "no gpCoordinates in gpSparse");
     CHK(upload(m->d_e0, m->e0, sizeof(m->e0)));
     return GAPX_OK;
 }
@@ -439,6 +520,8 @@ int gapx::new_engine(gapx_model *m, std::unique_ptr<Engine> &out) {
     hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) return fail(GAPX_E_DEVICE, "hipStreamCreate: %s", hipGetErrorString(he));
     e->ws.centres.resize(m->coords.size());
+    e->ws.var_inst.resize(m->coords.size());
+    e->ws.cut2_inst.resize(m->coords.size());
     e->h_centres.resize(m->coords.size());
     out = std::move(e);
     return GAPX_OK;
@@ -448,16 +531,20 @@ extern "C" int gapx_model_load(const char *xml_text, int64_t xml_len, const char
                                int device, gapx_model **out) {
     if (!out) return fail(GAPX_E_ARG, "out is NULL");
     *out = nullptr;
+    // the XML and sparseX files are read and checked before any device is touched
+    std::unique_ptr<gapx_model> parsed(new gapx_model());
+    parsed->device = device;
+    CHK(model_load(xml_text, xml_len, label, xml_dir, device, parsed.get()));
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(GAPX_E_ARG, "device %d out of range (%d visible)", device, ndev);
     HIPCHK(hipSetDevice(device));
-    auto m = new gapx_model();
-    m->device = device;
-    int st = model_load(xml_text, xml_len, label, xml_dir, device, m);
+    auto m = parsed.release();
+    int st = model_upload(m);
     if (st == GAPX_OK) {
-        soap_constants_init();
-        st = new_engine(m, m->eng);
+        const hipError_t ce = soap_constants_init();
+        st = ce != hipSuccess ? fail(GAPX_E_DEVICE, "uploading the SOAP constant tables: %s", hipGetErrorString(ce))
+                              : new_engine(m, m->eng);
     }
     if (st != GAPX_OK) {
         gapx_model_free(m);
@@ -1050,6 +1137,97 @@ static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out)
     return GAPX_OK;
 }
 
+// GAP variance set-up (gpCoordinates_initialise_variance_estimate's role, IPModel_GAP.f95:412-414):
+// Cinv = (K_MM + reg I)^-1 with K_MM[s][t] = delta^2 kernel(X_s, X_t), formed once per
+// coordinate and regularisation on the host (Cholesky, triangular inverse, Linv^T Linv;
+// rows split over host threads) and kept on the device.
+template <class F> static void parallel_rows(int n, F fn) {
+    const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), n / 64 + 1));
+    if (nt <= 1) {
+        for (int i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&] {
+            for (int i = next++; i < n; i = next++) fn(i);
+        });
+    for (auto &x : th) x.join();
+}
+
+static int var_prepare(Coord &c, double reg, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(c.var_mu);
+    if (c.var_reg == reg) return GAPX_OK;
+    if (!(reg >= 0)) return fail(GAPX_E_ARG, "gap_variance_regularisation must be >= 0 (got %g)", reg);
+    for (double sc : c.scut)
+        if (sc != 1.0)
+            return fail(GAPX_E_UNSUPPORTED, "gpCoordinates %s: the GAP variance with sparseCutoff != 1 is not built",
+                        c.label.c_str());
+    if (c.kind == 2 && c.M > kMaxVarSparse)
+        return fail(GAPX_E_UNSUPPORTED, "distance_2b variance with %d sparse points (at most %d)", c.M, kMaxVarSparse);
+    const int M = c.M, d = c.dim;
+    const double d2 = c.delta * c.delta;
+    std::vector<double> L((size_t)M * M, 0.0), LT((size_t)M * M, 0.0), C((size_t)M * M, 0.0);
+    parallel_rows(M, [&](int a) {   // K_MM + reg I, lower triangle
+        const double *xa = c.Xs.data() + (size_t)a * d;
+        for (int b = 0; b <= a; b++) {
+            const double *xb = c.Xs.data() + (size_t)b * d;
+            double k;
+            if (c.kind == 1) {
+                double dot = 0.0;
+                for (int q = 0; q < d; q++) dot += xa[q] * xb[q];
+                k = d2 * std::pow(dot, c.zeta);
+            } else {
+                const double t = (xa[0] - xb[0]) / c.theta[0];
+                k = d2 * std::exp(-0.5 * t * t);
+            }
+            L[(size_t)a * M + b] = k + (a == b ? reg : 0.0);
+        }
+    });
+    for (int i = 0; i < M; i++) {   // Cholesky-Crout in place: row i from rows 0..i-1
+        double *li = L.data() + (size_t)i * M;
+        for (int j = 0; j <= i; j++) {
+            const double *lj = L.data() + (size_t)j * M;
+            double t = li[j];
+            for (int k = 0; k < j; k++) t -= li[k] * lj[k];
+            if (j < i) li[j] = t / lj[j];
+            else {
+                if (!(t > 0))
+                    return fail(GAPX_E_MODEL, "gpCoordinates %s: K_MM + %g I is not positive definite", c.label.c_str(),
+                                reg);
+                li[i] = std::sqrt(t);
+            }
+        }
+    }
+    parallel_rows(M, [&](int col) {   // LT[col][k] = (L^-1)[k][col], k >= col
+        double *t = LT.data() + (size_t)col * M;
+        t[col] = 1.0 / L[(size_t)col * M + col];
+        for (int i = col + 1; i < M; i++) {
+            const double *li = L.data() + (size_t)i * M;
+            double a = 0.0;
+            for (int k = col; k < i; k++) a += li[k] * t[k];
+            t[i] = -a / li[i];
+        }
+    });
+    parallel_rows(M, [&](int a) {   // C = L^-T L^-1: C[a][b] = sum_k LT[a][k] LT[b][k], k >= max(a, b)
+        const double *ta = LT.data() + (size_t)a * M;
+        for (int b = 0; b <= a; b++) {
+            const double *tb = LT.data() + (size_t)b * M;
+            double v = 0.0;
+            for (int k = a; k < M; k++) v += ta[k] * tb[k];
+            C[(size_t)a * M + b] = v;
+        }
+    });
+    for (int a = 0; a < M; a++)
+        for (int b = a + 1; b < M; b++) C[(size_t)a * M + b] = C[(size_t)b * M + a];
+    CHK(c.d_Cinv.ensure((size_t)M * M * 8 + 8));
+    HIPCHK(hipMemcpyAsync(c.d_Cinv.p, C.data(), (size_t)M * M * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c.var_reg = reg;
+    return GAPX_OK;
+}
+
 // Device-resident evaluation of S structures in one pass (IPModel_GAP_Calc per
 // structure).  Outputs: d_e[S], d_v[9S] and the per-atom arrays; any may be NULL,
 // in which case the engine's scratch takes it.
@@ -1106,7 +1284,8 @@ static int connect_external(Engine &eg, const ExtNL &x, int N, const double *d_p
 
 static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
                       const int *Z, const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
-                      double *d_le, double *d_f, double *d_v, double *d_lv, const ExtNL *ext = nullptr) {
+                      double *d_le, double *d_f, double *d_v, double *d_lv, const ExtNL *ext = nullptr,
+                      const CalcOpts *opt = nullptr) {
     CHK(check_batch(S, atom_off, Z, lats, pbcs));
     const int N = atom_off[S];
     if (N > 0 && !d_pos) return fail(GAPX_E_ARG, "positions are NULL");
@@ -1117,10 +1296,18 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     const double rc = nl_cutoff > 0 ? nl_cutoff : m->cutoff;
     if (rc < m->cutoff) return fail(GAPX_E_ARG, "nl_cutoff %g is below the model cutoff %g", rc, m->cutoff);
     const bool grad = (want & (GAPX_WANT_F | GAPX_WANT_V | GAPX_WANT_LOCAL_V)) != 0;
-    if (tm.on) tm.begin(s);
     if (ext && S != 1) return fail(GAPX_E_ARG, "a supplied connection covers one structure");
-    CHK(stage_inputs(m, eg, S, atom_off, Z, ext ? ext->local.data() : nullptr));
-    const unsigned char *cmask = ext ? w.cmask.as<unsigned char>() : nullptr;
+    if (ext && opt) return fail(GAPX_E_ARG, "calc options with a supplied connection");
+    if (opt && opt->only_coord < 0) return fail(GAPX_E_ARG, "only_descriptor must be >= 0");
+    const bool want_var = opt && opt->variance;
+    const bool want_vgrad = want_var && opt->d_gvg;
+    if (tm.on) tm.begin(s);
+    const unsigned char *hmask = ext ? ext->local.data() : (opt ? opt->mask : nullptr);
+    CHK(stage_inputs(m, eg, S, atom_off, Z, hmask));
+    const unsigned char *cmask = hmask ? w.cmask.as<unsigned char>() : nullptr;
+    // only_descriptor: with several coordinates, coordinate k alone (IPModel_GAP.f95:398-403)
+    const size_t NC0 = m->coords.size();
+    auto skipped = [&](size_t k) { return opt && opt->only_coord > 0 && NC0 > 1 && (int)k != opt->only_coord - 1; };
     const int t_nl0 = tm.on ? tm.mark(s) : -1;
     int P = 0;
     if (ext) CHK(connect_external(eg, *ext, N, d_pos, lats, &P));
@@ -1133,10 +1320,23 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     const size_t NCOORD = m->coords.size();
     std::vector<GemmTune> tunes(NCOORD);
     std::vector<char> fused(NCOORD, 0);
-    size_t mX = 8, mN = 8, mC = 8, mW = 8, mE = 8, mGp = 8, mWf = 8, mSw = 8;
+    size_t mX = 8, mN = 8, mC = 8, mW = 8, mE = 8, mGp = 8, mWf = 8, mSw = 8, mV = 8;
+    bool has_d2b_var = false;
     for (size_t k = 0; k < NCOORD; k++) {
-        const Coord &c = *m->coords[k];
+        Coord &c = *m->coords[k];
         const int nc = (int)eg.h_centres[k].size();
+        if (skipped(k)) continue;
+        if (want_var) {
+            CHK(var_prepare(c, opt->reg, s));
+            if (c.kind == 2) {
+                has_d2b_var = true;
+                CHK(w.var_inst[k].ensure((size_t)(P + 1) * 8));
+                CHK(w.cut2_inst[k].ensure((size_t)(P + 1) * 8));
+            } else {
+                CHK(w.var_inst[k].ensure((size_t)(nc + 1) * 8));
+                mV = std::max(mV, (size_t)nc * c.M * 8);
+            }
+        }
         if (c.kind != 1 || nc == 0) continue;
         const size_t KLM = (size_t)c.S * c.n * (c.L + 1) * (c.L + 1);
         const int dpad = (c.dim + 31) / 32 * 32, Mpad = (c.M + 31) / 32 * 32;
@@ -1161,11 +1361,19 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     CHK(w.pairF.ensure((size_t)(P + 1) * 24));
     CHK(w.pairE.ensure((size_t)(P + 1) * 8));
     CHK(w.esite.ensure((size_t)(N + 1) * 8));
+    if (opt && opt->d_epc) CHK(w.eatom.ensure((size_t)(N + 1) * 8));
+    if (want_var) {
+        CHK(w.vK.ensure(mV)); CHK(w.vk.ensure(mV)); CHK(w.vV.ensure(mV));
+        CHK(w.Gv.ensure(mX));
+        CHK(w.pairFv.ensure((size_t)(P + 1) * 24));
+        CHK(w.pairVE.ensure((size_t)(P + 1) * 8));
+        CHK(w.varsite.ensure((size_t)(N + 1) * 8));
+    }
     CHK(ensure_outputs(w, S, N));
     for (size_t k = 0; k < NCOORD; k++) {
         Coord &c = *m->coords[k];
         const int nc = (int)eg.h_centres[k].size();
-        if (c.kind != 1 || nc == 0) continue;
+        if (c.kind != 1 || nc == 0 || skipped(k)) continue;
         if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tunes[k]));
         else CHK(ensure_split(m, c, m->precision, s));
     }
@@ -1180,19 +1388,40 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     bool has_d2b = false;
     for (size_t k = 0; k < NCOORD; k++) has_d2b = has_d2b || m->coords[k]->kind == 2;
     double *pairE = has_d2b ? w.pairE.as<double>() : nullptr;
+    double *d_epc = opt ? opt->d_epc : nullptr;
     auto launch_all = [&](bool marks) {
         {
             double *zp[3] = {w.pairF.as<double>(), w.esite.as<double>(), pairE};
             const size_t zn[3] = {(size_t)P * 3, (size_t)N, (size_t)P};
             launch_zero_f64_multi(has_d2b ? 3 : 2, zp, zn, s);
         }
+        if (d_epc) launch_fill_f64(d_epc, NCOORD, 0.0, s);
+        if (want_var) {
+            double *zp[3] = {w.pairFv.as<double>(), w.varsite.as<double>(), w.pairVE.as<double>()};
+            const size_t zn[3] = {(size_t)P * 3, (size_t)N, (size_t)P};
+            launch_zero_f64_multi(has_d2b_var ? 3 : 2, zp, zn, s);
+        }
         for (size_t k = 0; k < NCOORD; k++) {
             Coord &c = *m->coords[k];
+            if (skipped(k)) continue;
+            const double d2 = c.delta * c.delta;
             if (c.kind == 2) {
                 const int ta = marks ? tm.mark(s) : -1;
                 launch_d2b(N, w.Z.as<int>(), c.Z1, c.Z2, c.rc, c.ctw, c.M, c.d_Xs.as<double>(), c.d_w.as<double>(),
                            c.d_theta.as<double>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_r.as<double>(),
-                           w.nb_d.as<double>(), w.pairF.as<double>(), pairE, cmask, s);
+                           w.nb_d.as<double>(), w.pairF.as<double>(), pairE, cmask,
+                           d_epc ? w.eatom.as<double>() : nullptr, s);
+                if (d_epc) launch_sum_fixed(N, w.eatom.as<double>(), d_epc + k, s);
+                if (want_var) {
+                    const double nan = std::nan("");
+                    launch_fill_f64(w.var_inst[k].as<double>(), P, nan, s);
+                    launch_fill_f64(w.cut2_inst[k].as<double>(), P, nan, s);
+                    launch_d2b_var(N, w.Z.as<int>(), c.Z1, c.Z2, c.rc, c.ctw, c.M, c.d_Xs.as<double>(), d2,
+                                   c.d_theta.as<double>(), c.d_Cinv.as<double>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
+                                   w.nb_r.as<double>(), w.nb_d.as<double>(), w.pairFv.as<double>(),
+                                   w.pairVE.as<double>(), w.var_inst[k].as<double>(), w.cut2_inst[k].as<double>(),
+                                   cmask, s);
+                }
                 if (marks) tm.stage(5, ta, tm.mark(s));
                 continue;
             }
@@ -1220,7 +1449,25 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
             }
             const int t2 = marks ? tm.mark(s) : -1;
             launch_energy_rows(nc, ntile, centres, w.e_part.as<double>(), w.esite.as<double>(), s);
+            if (d_epc) launch_sum_fixed((size_t)nc * ntile, w.e_part.as<double>(), d_epc + k, s);
             if (marks) { tm.stage(1, t0, t1); tm.stage(2, t1, t2); }
+            if (want_var) {   // FP64 whatever the GEMM precision
+                launch_gemm_nt_f64(nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(), w.vK.as<double>(), s);
+                launch_var_cov(nc, c.M, w.vK.as<double>(), d2, c.zeta, w.vk.as<double>(), s);
+                launch_gemm2_f64(1, 1, nc, c.M, c.M, w.vk.as<double>(), c.d_Cinv.as<double>(), w.vV.as<double>(),
+                                 nullptr, s);
+                launch_var_rows(nc, c.M, c.dim, w.X.as<double>(), w.vK.as<double>(), w.vk.as<double>(),
+                                w.vV.as<double>(), d2, c.zeta, centres, w.var_inst[k].as<double>(),
+                                w.varsite.as<double>(), s);
+                if (want_vgrad) {
+                    launch_gemm2_f64(1, 1, nc, c.M, c.dim, w.vV.as<double>(), c.d_Xs.as<double>(), w.Gv.as<double>(),
+                                     nullptr, s);
+                    launch_var_self_grad(nc, c.dim, w.X.as<double>(), d2, c.zeta, w.Gv.as<double>(), s);
+                    launch_soap_bwd(sp, nc, centres, w.Z.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
+                                    w.nb_r.as<double>(), w.nb_d.as<double>(), w.X.as<double>(), w.Gv.as<double>(),
+                                    w.norm.as<double>(), w.cbuf.as<double>(), w.pairFv.as<double>(), s);
+                }
+            }
             if (grad) {
                 if (m->precision == GAPX_PREC_FP64) {
                     launch_gemm2_f64(tu.g2, tu.parts, nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(),
@@ -1243,10 +1490,16 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
                       w.nb_d.as<double>(), w.pairF.as<double>(), pairE, w.esite.as<double>(), o_f, o_le,
                       o_lv, cmask, s);
         launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s);
+        if (want_var)
+            launch_gather_var(N, w.nb_off.as<int>(), w.nb_rev.as<int>(), w.pairFv.as<double>(),
+                              has_d2b_var ? w.pairVE.as<double>() : nullptr, w.varsite.as<double>(), opt->d_gvg,
+                              opt->d_lgv, s);
         if (marks) tm.stage(5, tg, tm.mark(s));
     };
+    eg.var_valid = want_var;
+    eg.var_P = P;
 
-    if (m->graphs && !tm.on && !ext) {
+    if (m->graphs && !tm.on && !ext && !opt) {
         // replay the captured sequence when nothing it depends on has changed
         std::vector<long long> key = {S, N, P, m->precision, (long long)want, (long long)(uintptr_t)o_e,
                                       (long long)(uintptr_t)o_v, (long long)(uintptr_t)o_f, (long long)(uintptr_t)o_le,
@@ -1313,7 +1566,8 @@ extern "C" int gapx_sync(gapx_model *m) {
 // pinned staging, results come back into the caller's arrays.
 int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *pos, const int *Z,
                      const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *e, double *le,
-                     double *f, double *v, double *lv, const ExtNL *ext) {
+                     double *f, double *v, double *lv, const ExtNL *ext, const CalcOpts *opt, double *h_epc,
+                     double *h_lgv, double *h_gvg) {
     if ((want & GAPX_WANT_E) && !e) return fail(GAPX_E_ARG, "energy requested but e is NULL");
     if ((want & GAPX_WANT_F) && !f) return fail(GAPX_E_ARG, "forces requested but f is NULL");
     if ((want & GAPX_WANT_V) && !v) return fail(GAPX_E_ARG, "virial requested but virial is NULL");
@@ -1330,7 +1584,18 @@ int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     // one pinned block: positions in, then results out
     const size_t bpos = (size_t)N * 24, be = (size_t)S * 8, bv = (size_t)S * 72, bf = (size_t)N * 24,
                  ble = (size_t)N * 8, blv = (size_t)N * 72;
-    CHK(eg.stage_io.ensure(std::max(bpos, be + bv + bf + ble + blv) + 64));
+    // option outputs: device scratch read back with the rest (sized before any copy is queued)
+    CalcOpts o2;
+    const size_t bepc = m->coords.size() * 8, blgv = (size_t)N * 8, bgvg = (size_t)N * 24;
+    size_t bopt = 0;
+    if (opt) {
+        o2 = *opt;
+        o2.d_epc = o2.d_lgv = o2.d_gvg = nullptr;
+        if (h_epc) { CHK(w.o_epc.ensure(bepc + 8)); o2.d_epc = w.o_epc.as<double>(); bopt += bepc; }
+        if (opt->variance && h_lgv) { CHK(w.o_lgv.ensure(blgv + 8)); o2.d_lgv = w.o_lgv.as<double>(); bopt += blgv; }
+        if (opt->variance && h_gvg) { CHK(w.o_gvg.ensure(bgvg + 8)); o2.d_gvg = w.o_gvg.as<double>(); bopt += bgvg; }
+    }
+    CHK(eg.stage_io.ensure(std::max(bpos, be + bv + bf + ble + blv + bopt) + 64));
     char *h = eg.stage_io.as<char>();
     if (N) {
         memcpy(h, pos, bpos);
@@ -1339,7 +1604,7 @@ int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     CHK(eval_batch(m, eg, S, atom_off, w.pos.as<double>(), pos, Z, lats, pbcs, nl_cutoff,
                    want | GAPX_WANT_E | GAPX_WANT_V | GAPX_WANT_F | GAPX_WANT_LOCAL_E | GAPX_WANT_LOCAL_V,
                    w.e.as<double>(), w.le.as<double>(), w.f.as<double>(), w.vir.as<double>(), w.lv.as<double>(),
-                   ext));
+                   ext, opt ? &o2 : nullptr));
     char *o = h;
     auto get = [&](bool on, const DBuf &src, size_t bytes, char **dst) -> int {
         *dst = nullptr;
@@ -1355,7 +1620,16 @@ int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     CHK(get(want & GAPX_WANT_F, w.f, bf, &pf));
     CHK(get(want & GAPX_WANT_LOCAL_E, w.le, ble, &ple));
     CHK(get(want & GAPX_WANT_LOCAL_V, w.lv, blv, &plv));
+    char *pepc = nullptr, *plgv = nullptr, *pgvg = nullptr;
+    if (opt) {
+        CHK(get(o2.d_epc != nullptr, w.o_epc, bepc, &pepc));
+        CHK(get(o2.d_lgv != nullptr, w.o_lgv, blgv, &plgv));
+        CHK(get(o2.d_gvg != nullptr, w.o_gvg, bgvg, &pgvg));
+    }
     HIPCHK(hipStreamSynchronize(s));
+    if (pepc) memcpy(h_epc, pepc, bepc);
+    if (plgv) memcpy(h_lgv, plgv, blgv);
+    if (pgvg) memcpy(h_gvg, pgvg, bgvg);
     if (pe) memcpy(e, pe, be);
     if (pv) memcpy(v, pv, bv);
     if (pf) memcpy(f, pf, bf);
@@ -1370,6 +1644,101 @@ extern "C" int gapx_calc(gapx_model *m, int N, const double *pos, const int *Z, 
     if (N < 0) return fail(GAPX_E_ARG, "n_atoms < 0");
     const int off[2] = {0, N};
     return calc_host(m, *m->eng, 1, off, pos, Z, lat, pbc, nl_cutoff, want, e, le, f, v, lv);
+}
+
+static int make_opts(gapx_model *m, unsigned want, const unsigned char *atom_mask, int only_descriptor,
+                     double reg, double *epc, double *lgv, double *gvg, CalcOpts &o) {
+    if (only_descriptor < 0) return fail(GAPX_E_ARG, "only_descriptor must be >= 0 (got %d)", only_descriptor);
+    o.mask = atom_mask;
+    o.only_coord = only_descriptor;
+    o.d_epc = epc;
+    o.variance = (want & GAPX_WANT_VARIANCE) || lgv || gvg;
+    o.reg = reg;
+    o.d_lgv = lgv;
+    o.d_gvg = gvg;
+    (void)m;
+    return GAPX_OK;
+}
+
+extern "C" int gapx_calc_ex(gapx_model *m, int N, const double *pos, const int *Z, const double *lat, const int *pbc,
+                            double nl_cutoff, unsigned want, const unsigned char *atom_mask, int only_descriptor,
+                            double variance_regularisation, double *e, double *le, double *f, double *v, double *lv,
+                            double *energy_per_coordinate, double *local_gap_variance,
+                            double *gap_variance_gradient) {
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    if (N < 0) return fail(GAPX_E_ARG, "n_atoms < 0");
+    CalcOpts o;
+    CHK(make_opts(m, want, atom_mask, only_descriptor, variance_regularisation, energy_per_coordinate,
+                  local_gap_variance, gap_variance_gradient, o));
+    const int off[2] = {0, N};
+    return calc_host(m, *m->eng, 1, off, pos, Z, lat, pbc, nl_cutoff, want, e, le, f, v, lv, nullptr, &o,
+                     energy_per_coordinate, local_gap_variance, gap_variance_gradient);
+}
+
+extern "C" int gapx_calc_device_ex(gapx_model *m, int N, const double *d_pos, const int *Z, const double *lat,
+                                   const int *pbc, double nl_cutoff, unsigned want, const unsigned char *atom_mask,
+                                   int only_descriptor, double variance_regularisation, double *d_e, double *d_le,
+                                   double *d_f, double *d_v, double *d_lv, double *d_energy_per_coordinate,
+                                   double *d_local_gap_variance, double *d_gap_variance_gradient) {
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    if (N < 0) return fail(GAPX_E_ARG, "n_atoms < 0");
+    CalcOpts o;
+    CHK(make_opts(m, want, atom_mask, only_descriptor, variance_regularisation, d_energy_per_coordinate,
+                  d_local_gap_variance, d_gap_variance_gradient, o));
+    const int off[2] = {0, N};
+    return eval_batch(m, *m->eng, 1, off, d_pos, nullptr, Z, lat, pbc, nl_cutoff, want, d_e, d_le, d_f, d_v, d_lv,
+                      nullptr, &o);
+}
+
+// The per-instance variances of coordinate `coord` from the last calc that formed them,
+// in the reference's instance order (print_gap_variance, IPModel_GAP.f95:506-516):
+// soap, one per centre atom (atoms[2k] = centre, atoms[2k+1] = -1, cutoff_sq = 1);
+// distance_2b, one per directed pair (i, j) with i ascending and j in i's neighbour order.
+extern "C" int gapx_descriptor_variances(gapx_model *m, int coord, int max_entries, double *variance,
+                                         double *cutoff_sq, int *atoms, int *n_entries) {
+    if (!m || !n_entries) return fail(GAPX_E_ARG, "NULL argument");
+    if (coord < 0 || coord >= (int)m->coords.size()) return fail(GAPX_E_ARG, "coordinate %d out of range", coord);
+    Engine &eg = *m->eng;
+    if (!eg.var_valid) return fail(GAPX_E_ARG, "no variance formed by the last calc (ask for GAPX_WANT_VARIANCE)");
+    HIPCHK(hipSetDevice(m->device));
+    hipStream_t s = eg.stream;
+    HIPCHK(hipStreamSynchronize(s));
+    const Coord &c = *m->coords[coord];
+    Workspace &w = eg.ws;
+    int k = 0;
+    auto put = [&](double var, double c2, int a, int b) {
+        if (k < max_entries) {
+            if (variance) variance[k] = var;
+            if (cutoff_sq) cutoff_sq[k] = c2;
+            if (atoms) { atoms[2 * k] = a; atoms[2 * k + 1] = b; }
+        }
+        k++;
+    };
+    if (!w.var_inst[coord].p) {   // a coordinate the last calc skipped (only_descriptor)
+        *n_entries = 0;
+        return GAPX_OK;
+    }
+    if (c.kind == 1) {
+        const auto &cen = eg.h_centres[coord];
+        std::vector<double> v(cen.size());
+        if (!cen.empty()) HIPCHK(hipMemcpy(v.data(), w.var_inst[coord].p, cen.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < cen.size(); q++) put(v[q], 1.0, cen[q], -1);
+    } else {
+        const int N = (int)eg.h_Z.size(), P = eg.var_P;
+        std::vector<int> off(N + 1), nj(P + 1);
+        std::vector<double> v(P + 1), c2(P + 1);
+        HIPCHK(hipMemcpy(off.data(), w.nb_off.p, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost));
+        if (P) {
+            HIPCHK(hipMemcpy(nj.data(), w.nb_j.p, (size_t)P * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(v.data(), w.var_inst[coord].p, (size_t)P * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(c2.data(), w.cut2_inst[coord].p, (size_t)P * 8, hipMemcpyDeviceToHost));
+        }
+        for (int i = 0; i < N; i++)
+            for (int p = off[i]; p < off[i + 1]; p++)
+                if (!std::isnan(v[p])) put(v[p], c2[p], i, nj[p]);
+    }
+    *n_entries = k;
+    return GAPX_OK;
 }
 
 // LAMMPS pair style entry (quip_lammps_wrapper.f95:30-156).  The connection is

@@ -105,6 +105,10 @@ struct Coord {
     // FP64 GEMM tile choices per centre-count bucket (ceil log2 nc), from the autotuner
     std::mutex tune_mu;
     std::map<int, GemmTune> tune;
+    // GAP variance: (K_MM + reg I)^-1 (M x M, FP64) for the regularisation last asked for
+    std::mutex var_mu;
+    double var_reg = -1.0;
+    DBuf d_Cinv;
 };
 
 struct Workspace {
@@ -118,6 +122,22 @@ struct Workspace {
     DBuf cmask;                  // descriptor-centre mask of a caller-supplied connection
     DBuf t1_j, t1_shift, t1_r, t2_i, t2_k, nl_flags;   // per-atom slots of the one-pass neighbour search
     std::vector<DBuf> centres;   // per coordinate
+    DBuf eatom;                  // per-atom distance_2b energies of one coordinate (energy_per_coordinate)
+    // GAP variance: K, k = delta^2 K^zeta, V = k Cinv (each nc x M), per-centre G of the
+    // variance, pair sums, per-atom variances; per coordinate the instance variances
+    DBuf vK, vk, vV, Gv, pairFv, pairVE, varsite;
+    std::vector<DBuf> var_inst, cut2_inst;
+    DBuf o_epc, o_lgv, o_gvg;    // device outputs of the host-buffer entry points' options
+};
+
+// IPModel_GAP_Calc's args_str options that change what is computed (IPModel_GAP.f95:324-335)
+struct CalcOpts {
+    const unsigned char *mask = nullptr;   // atom_mask_name: host, N bytes
+    int only_coord = 0;                    // only_descriptor: 1-based coordinate, 0 = all
+    double *d_epc = nullptr;               // energy_per_coordinate: device, one per coordinate
+    bool variance = false;                 // local_gap_variance / print_gap_variance
+    double reg = 1e-3;                     // gap_variance_regularisation
+    double *d_lgv = nullptr, *d_gvg = nullptr;   // device: local_gap_variance[N], gap_variance_gradient[3N]
 };
 
 // A connection handed in by the caller instead of calc_connect (a LAMMPS pair
@@ -240,6 +260,9 @@ struct Engine {
     int n1_total_last = 0;
     int nl_cap1 = 0, nl_cap2 = 0;   // slot sizes of the one-pass neighbour search (0: count first)
     int ncell_last = 0;
+    // the last calc that formed GAP variances: its pair count (instance buffers are per coordinate)
+    bool var_valid = false;
+    int var_P = 0;
     ~Engine() {
         if (gexec) hipGraphExecDestroy(gexec);
         if (stream) {
@@ -270,9 +293,12 @@ int ensure_split(gapx_model *m, Coord &c, int prec, hipStream_t s);
 int check_batch(int S, const int *atom_off, const int *Z, const double *lats, const int *pbcs);
 // Host-buffer evaluation of S structures on engine eg (pinned staging both ways).
 // ext: a caller-supplied connection (one structure), else calc_connect.
+// opt (nullable): per-call options; their host outputs h_epc[n_coord], h_lgv[N], h_gvg[3N]
+// (nullable) are filled from device scratch.
 int calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *pos, const int *Z,
               const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *e, double *le, double *f,
-              double *v, double *lv, const ExtNL *ext = nullptr);
+              double *v, double *lv, const ExtNL *ext = nullptr, const CalcOpts *opt = nullptr,
+              double *h_epc = nullptr, double *h_lgv = nullptr, double *h_gvg = nullptr);
 
 }  // namespace gapx
 

@@ -70,10 +70,13 @@ void launch_minmax_frac(int S, const int *atom_off, const double *pos, const dou
 
 // upload the constant tables of the SOAP kernels to the current device (once per
 // device; done at model load so that no launch sequence contains an upload)
-void soap_constants_init();
-void soap_fast_constants_init();
+hipError_t soap_constants_init();
+hipError_t soap_fast_constants_init();
 int soap_supported(int n, int L);
 int soap_fast_supported(int n, int L);
+// LDS bytes per block the descriptor kernels of this shape use (largest launch variant)
+size_t soap_lds_bytes(const SoapDev &sp);
+size_t soap_fast_lds_bytes(const SoapDev &sp);
 bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                           const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
                           double *cbuf, hipStream_t s);
@@ -121,10 +124,41 @@ void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double 
                          void *hi, void *lo, hipStream_t s);
 
 void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_part, double *esite, hipStream_t s);
-// mask (nullable): only atoms with mask[i] != 0 are the first atom of a pair descriptor
+// mask (nullable): only atoms with mask[i] != 0 are the first atom of a pair descriptor.
+// e_atom (nullable): e_atom[i] = sum of e * fc over the instances whose first atom is i
+// (written for every atom; zero where there are none).
 void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
                 const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
-                const double *nb_d, double *pairF, double *pairE, const unsigned char *mask, hipStream_t s);
+                const double *nb_d, double *pairF, double *pairE, const unsigned char *mask, double *e_atom,
+                hipStream_t s);
+// out[0] = v[0] + ... + v[n-1], summed in a fixed order by one block
+void launch_sum_fixed(size_t n, const double *v, double *out, hipStream_t s);
+
+// GAP variance (projected-process estimate), FP64.
+// C = A (rows x kd) . B^T (cols x kd), no epilogue
+void launch_gemm_nt_f64(int rows, int cols, int kd, const double *A, const double *B, double *C, hipStream_t s);
+// kv[i][s] = delta2 K[i][s]^zeta
+void launch_var_cov(int nc, int M, const double *K, double delta2, double zeta, double *kv, hipStream_t s);
+// Per centre row i (V = kv . Cinv): var_i = delta2 (x_i.x_i)^zeta - sum_s kv_is V_is, then
+// V_is <- -2 zeta delta2 K_is^(zeta-1) V_is (the variance's weight on X_s);
+// var_inst[i] = var_i and varsite[centres[i]] += var_i.
+void launch_var_rows(int nc, int M, int d, const double *X, const double *K, const double *kv, double *V,
+                     double delta2, double zeta, const int *centres, double *var_inst, double *varsite,
+                     hipStream_t s);
+// Gv_i += 2 zeta delta2 (x_i.x_i)^(zeta-1) x_i  (the gradient of k(x, x))
+void launch_var_self_grad(int nc, int d, const double *X, double delta2, double zeta, double *Gv, hipStream_t s);
+// distance_2b variance with the ard_se kernel (M <= kMaxVarSparse): per directed pair p of a
+// descriptor instance, var = delta2 - k^T Cinv k; pairVE[p] += var fc^2, pairFv[p] += d(var fc^2)/dr u;
+// var_inst[p] = var and cut2_inst[p] = fc^2 (NaN-filled by the caller for non-instances).
+constexpr int kMaxVarSparse = 128;
+void launch_d2b_var(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
+                    double delta2, const double *theta, const double *Cinv, const int *nb_off, const int *nb_j,
+                    const double *nb_r, const double *nb_d, double *pairFv, double *pairVE, double *var_inst,
+                    double *cut2_inst, const unsigned char *mask, hipStream_t s);
+// gap_variance_gradient(:, j) = -sum_q (pairFv[q] - pairFv[rev q]) (the plus-signed gradient);
+// local_gap_variance(j) = varsite[j] + (pairVE[q] + pairVE[rev q]) / 2 summed over q
+void launch_gather_var(int N, const int *nb_off, const int *nb_rev, const double *pairFv, const double *pairVE,
+                       const double *varsite, double *gvg, double *lgv, hipStream_t s);
 // mask (nullable): e0 is added to the local energy of masked atoms only
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,

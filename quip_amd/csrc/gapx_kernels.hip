@@ -28,9 +28,12 @@ __constant__ double c_dfact[kMaxL + 2];                       // (2m-1)!!
 
 // constant tables are uploaded once per device (each device has its own copy)
 static bool g_const_ready[64] = {};
-static void ensure_constants() {
+static hipError_t ensure_constants() {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_const_ready[dev]) return;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (g_const_ready[dev]) return hipSuccess;
     double yn[(kMaxL + 1) * (kMaxL + 2) / 2], df[kMaxL + 2];
     for (int l = 0; l <= kMaxL; l++)
         for (int m = 0; m <= l; m++) {
@@ -42,14 +45,15 @@ static void ensure_constants() {
         }
     df[0] = 1.0;
     for (int m = 1; m < kMaxL + 2; m++) df[m] = df[m - 1] * (2.0 * m - 1.0);
-    hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn));
-    hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df));
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df))) != hipSuccess) return e;
     g_const_ready[dev] = true;
+    return hipSuccess;
 }
 
-void soap_constants_init() {
-    ensure_constants();
-    soap_fast_constants_init();
+hipError_t soap_constants_init() {
+    hipError_t e = ensure_constants();
+    return e != hipSuccess ? e : soap_fast_constants_init();
 }
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -1116,6 +1120,28 @@ int soap_supported(int n, int L) {
 
 static int pick_nmax(int n) { return n <= 8 ? 8 : 16; }
 
+// generic forward: c' accumulators plus the staged neighbour rows of NB neighbours
+static size_t fwd_generic_lds(const SoapDev &sp, int NB) {
+    const int LM = (sp.L + 1) * (sp.L + 1);
+    const int RL = sp.n * (sp.L + 1) + LM;
+    const int KLM = sp.K * LM;
+    return (size_t)(KLM + std::max((size_t)NB * RL, (size_t)sp.dim)) * 8 + NB * 4;
+}
+static int fwd_generic_nb(const SoapDev &sp) {
+    int NB = 64;
+    while (NB > 8 && fwd_generic_lds(sp, NB) > 60 * 1024) NB /= 2;
+    return NB;
+}
+static size_t bwd_generic_lds(const SoapDev &sp) {
+    const int LM = (sp.L + 1) * (sp.L + 1);
+    return (size_t)(2 * sp.K * LM + sp.dim) * 8;
+}
+
+size_t soap_lds_bytes(const SoapDev &sp) {
+    if (soap_fast_supported(sp.n, sp.L)) return soap_fast_lds_bytes(sp);
+    return std::max(fwd_generic_lds(sp, fwd_generic_nb(sp)), bwd_generic_lds(sp));
+}
+
 void launch_soap_fwd(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                      const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
                      double *cbuf, hipStream_t s) {
@@ -1124,12 +1150,9 @@ void launch_soap_fwd(const SoapDev &sp, int nc, const int *centres, const int *Z
     if (nc <= 0) return;
     const int LM = (sp.L + 1) * (sp.L + 1);
     const int RL = sp.n * (sp.L + 1) + LM;
-    const int KLM = sp.K * LM;
-    int NB = 64;
-    auto need = [&](int nb) { return (size_t)(KLM + std::max((size_t)nb * RL, (size_t)sp.dim)) * 8 + nb * 4; };
-    while (NB > 8 && need(NB) > 60 * 1024) NB /= 2;
+    const int NB = fwd_generic_nb(sp);
     const int rows_cap = (int)std::max((size_t)NB * RL, (size_t)sp.dim);
-    size_t sh = need(NB);
+    size_t sh = fwd_generic_lds(sp, NB);
     const int nm = pick_nmax(sp.n);
 #define LAUNCH(LL, NN)                                                                                          \
     if (sp.L == LL && nm == NN) {                                                                               \
@@ -1147,8 +1170,7 @@ void launch_soap_bwd(const SoapDev &sp, int nc, const int *centres, const int *Z
     if (launch_soap_bwd_fast(sp, nc, centres, Z, nb_off, nb_j, nb_r, nb_d, X, G, norm, cbuf, pairF, s)) return;
     ensure_constants();
     if (nc <= 0) return;
-    const int LM = (sp.L + 1) * (sp.L + 1);
-    size_t sh = (size_t)(2 * sp.K * LM + sp.dim) * 8;
+    size_t sh = bwd_generic_lds(sp);
     const int nm = pick_nmax(sp.n);
 #define LAUNCH(LL, NN)                                                                                          \
     if (sp.L == LL && nm == NN) {                                                                               \
@@ -1406,9 +1428,15 @@ void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_p
 // ----------------------------------------------------------------------------
 __global__ void k_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
                       const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
-                      const double *nb_d, double *pairF, double *pairE, const unsigned char *mask) {
+                      const double *nb_d, double *pairF, double *pairE, const unsigned char *mask,
+                      double *e_atom) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || (mask && !mask[i])) return;
+    if (i >= N) return;
+    if (mask && !mask[i]) {
+        if (e_atom) e_atom[i] = 0.0;
+        return;
+    }
+    double esum = 0.0;
     const bool zi1 = (Z1 == 0) || (Z[i] == Z1), zi2 = (Z2 == 0) || (Z[i] == Z2);
     const double it2 = 1.0 / (theta[0] * theta[0]);
     for (int p = nb_off[i]; p < nb_off[i + 1]; p++) {
@@ -1428,17 +1456,206 @@ __global__ void k_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw
         const double dE = de * fc + e * dfc;
         const double ir = 1.0 / r;
         pairE[p] += e * fc;
+        esum += e * fc;
         pairF[3 * p] += dE * nb_d[3 * p] * ir;
         pairF[3 * p + 1] += dE * nb_d[3 * p + 1] * ir;
         pairF[3 * p + 2] += dE * nb_d[3 * p + 2] * ir;
     }
+    if (e_atom) e_atom[i] = esum;
 }
 void launch_d2b(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
                 const double *wvec, const double *theta, const int *nb_off, const int *nb_j, const double *nb_r,
-                const double *nb_d, double *pairF, double *pairE, const unsigned char *mask, hipStream_t s) {
+                const double *nb_d, double *pairF, double *pairE, const unsigned char *mask, double *e_atom,
+                hipStream_t s) {
     if (N > 0)
         hipLaunchKernelGGL(k_d2b, dim3(cdiv(N, 128)), dim3(128), 0, s, N, Z, Z1, Z2, rc, ctw, M, xs, wvec, theta,
-                           nb_off, nb_j, nb_r, nb_d, pairF, pairE, mask);
+                           nb_off, nb_j, nb_r, nb_d, pairF, pairE, mask, e_atom);
+}
+
+// ----------------------------------------------------------------------------
+// fixed-order sum of n doubles by one block (energy per coordinate)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sum_fixed(size_t n, const double *v, double *out) {
+    __shared__ double red[256];
+    double a = 0.0;
+    for (size_t q = threadIdx.x; q < n; q += 256) a += v[q];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+void launch_sum_fixed(size_t n, const double *v, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(256), 0, s, n, v, out);
+}
+
+// ----------------------------------------------------------------------------
+// GAP variance of the dot_product (SOAP) and ard_se (distance_2b) predictions:
+//   var(x) = k(x,x) - k_x^T (K_MM + reg I)^-1 k_x,  k_x[s] = delta^2 kernel(x, X_s)
+// Cinv = (K_MM + reg I)^-1 is formed on the host when the model is asked for it.
+// ----------------------------------------------------------------------------
+template <int WM, int WN, int BK>
+static void gemm_nt_go(int rows, int cols, int kd, const double *A, const double *B, double *C, hipStream_t s) {
+    dim3 grid(cdiv(rows, 32 * WM), cdiv(cols, 32 * WN), 1);
+    hipLaunchKernelGGL((k_gemm_f64<true, false, WM, WN, BK>), grid, dim3(256), 0, s, rows, cols, kd, kd, A, B, C,
+                       (const double *)nullptr, 0.0, 0, (double *)nullptr);
+}
+void launch_gemm_nt_f64(int rows, int cols, int kd, const double *A, const double *B, double *C, hipStream_t s) {
+    if (rows > 0 && cols > 0) gemm_nt_go<2, 2, 16>(rows, cols, kd, A, B, C, s);
+}
+
+__device__ __forceinline__ double powz(double x, double zeta, int iz) {
+    if (iz > 0) {
+        double r = 1.0;
+        for (int q = 0; q < iz; q++) r *= x;
+        return r;
+    }
+    if (iz == 0 && zeta == 0.0) return 1.0;
+    return pow(x, zeta);
+}
+
+__global__ void k_var_cov(size_t n, const double *K, double delta2, double zeta, int iz, double *kv) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) kv[q] = delta2 * powz(K[q], zeta, iz);
+}
+void launch_var_cov(int nc, int M, const double *K, double delta2, double zeta, double *kv, hipStream_t s) {
+    const size_t n = (size_t)nc * M;
+    if (n) hipLaunchKernelGGL(k_var_cov, dim3(cdiv((long)n, 256)), dim3(256), 0, s, n, K, delta2, zeta,
+                              int_zeta(zeta), kv);
+}
+
+__device__ __forceinline__ double wave_sum_v(double v) {   // fixed butterfly over the 64 lanes
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// one wavefront per centre row
+__global__ __launch_bounds__(256) void k_var_rows(int nc, int M, int d, const double *X, const double *K,
+                                                  const double *kv, double *V, double delta2, double zeta, int iz,
+                                                  int iz1, const int *centres, double *var_inst, double *varsite) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nc) return;
+    const double *x = X + (size_t)i * d;
+    double xx = 0.0;
+    for (int q = lane; q < d; q += 64) xx += x[q] * x[q];
+    xx = wave_sum_v(xx);
+    const double *Ki = K + (size_t)i * M, *ki = kv + (size_t)i * M;
+    double *Vi = V + (size_t)i * M;
+    double kck = 0.0;
+    const double g = -2.0 * zeta * delta2;
+    for (int q = lane; q < M; q += 64) {
+        const double v = Vi[q];
+        kck += ki[q] * v;
+        Vi[q] = g * powz(Ki[q], zeta - 1.0, iz1) * v;
+    }
+    kck = wave_sum_v(kck);
+    if (lane == 0) {
+        const double var = delta2 * powz(xx, zeta, iz) - kck;
+        var_inst[i] = var;
+        varsite[centres[i]] += var;
+    }
+}
+void launch_var_rows(int nc, int M, int d, const double *X, const double *K, const double *kv, double *V,
+                     double delta2, double zeta, const int *centres, double *var_inst, double *varsite,
+                     hipStream_t s) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_var_rows, dim3(cdiv(nc, 4)), dim3(256), 0, s, nc, M, d, X, K, kv, V, delta2, zeta,
+                           int_zeta(zeta), int_zeta(zeta - 1.0), centres, var_inst, varsite);
+}
+
+__global__ __launch_bounds__(256) void k_var_self_grad(int nc, int d, const double *X, double delta2, double zeta,
+                                                       int iz1, double *Gv) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nc) return;
+    const double *x = X + (size_t)i * d;
+    double xx = 0.0;
+    for (int q = lane; q < d; q += 64) xx += x[q] * x[q];
+    xx = wave_sum_v(xx);
+    const double g = 2.0 * zeta * delta2 * powz(xx, zeta - 1.0, iz1);
+    for (int q = lane; q < d; q += 64) Gv[(size_t)i * d + q] += g * x[q];
+}
+void launch_var_self_grad(int nc, int d, const double *X, double delta2, double zeta, double *Gv, hipStream_t s) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_var_self_grad, dim3(cdiv(nc, 4)), dim3(256), 0, s, nc, d, X, delta2, zeta,
+                           int_zeta(zeta - 1.0), Gv);
+}
+
+// one thread per first atom over its pairs; the covariance vector sits in a per-lane array
+__global__ void k_d2b_var(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
+                          double delta2, const double *theta, const double *Cinv, const int *nb_off, const int *nb_j,
+                          const double *nb_r, const double *nb_d, double *pairFv, double *pairVE, double *var_inst,
+                          double *cut2_inst, const unsigned char *mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || (mask && !mask[i])) return;
+    const bool zi1 = (Z1 == 0) || (Z[i] == Z1), zi2 = (Z2 == 0) || (Z[i] == Z2);
+    const double it2 = 1.0 / (theta[0] * theta[0]);
+    double kk[kMaxVarSparse], dk[kMaxVarSparse];
+    for (int p = nb_off[i]; p < nb_off[i + 1]; p++) {
+        const double r = nb_r[p];
+        if (r > rc) continue;
+        const int j = nb_j[p];
+        const bool zj1 = (Z1 == 0) || (Z[j] == Z1), zj2 = (Z2 == 0) || (Z[j] == Z2);
+        if (!((zi1 && zj2) || (zi2 && zj1))) continue;
+        for (int a = 0; a < M; a++) {
+            const double t = r - xs[a];
+            kk[a] = delta2 * exp(-0.5 * t * t * it2);
+            dk[a] = -kk[a] * t * it2;
+        }
+        double var = delta2, dvar = 0.0;
+        for (int a = 0; a < M; a++) {
+            double y = 0.0;   // (Cinv k)_a
+            for (int b = 0; b < M; b++) y += Cinv[(size_t)a * M + b] * kk[b];
+            var -= kk[a] * y;
+            dvar -= 2.0 * dk[a] * y;
+        }
+        const double fc = fcut_d(r, rc, ctw), dfc = dfcut_d(r, rc, ctw);
+        const double gr = dvar * fc * fc + 2.0 * var * fc * dfc;
+        const double ir = 1.0 / r;
+        pairVE[p] += var * fc * fc;
+        pairFv[3 * p] += gr * nb_d[3 * p] * ir;
+        pairFv[3 * p + 1] += gr * nb_d[3 * p + 1] * ir;
+        pairFv[3 * p + 2] += gr * nb_d[3 * p + 2] * ir;
+        var_inst[p] = var;
+        cut2_inst[p] = fc * fc;
+    }
+}
+void launch_d2b_var(int N, const int *Z, int Z1, int Z2, double rc, double ctw, int M, const double *xs,
+                    double delta2, const double *theta, const double *Cinv, const int *nb_off, const int *nb_j,
+                    const double *nb_r, const double *nb_d, double *pairFv, double *pairVE, double *var_inst,
+                    double *cut2_inst, const unsigned char *mask, hipStream_t s) {
+    if (N > 0 && M > 0 && M <= kMaxVarSparse)
+        hipLaunchKernelGGL(k_d2b_var, dim3(cdiv(N, 64)), dim3(64), 0, s, N, Z, Z1, Z2, rc, ctw, M, xs, delta2,
+                           theta, Cinv, nb_off, nb_j, nb_r, nb_d, pairFv, pairVE, var_inst, cut2_inst, mask);
+}
+
+__global__ void k_gather_var(int N, const int *nb_off, const int *nb_rev, const double *pairFv,
+                             const double *pairVE, const double *varsite, double *gvg, double *lgv) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, ve = 0.0;
+    for (int q = nb_off[j]; q < nb_off[j + 1]; q++) {
+        const int r = nb_rev[q];
+        g0 += pairFv[3 * q] - pairFv[3 * r];
+        g1 += pairFv[3 * q + 1] - pairFv[3 * r + 1];
+        g2 += pairFv[3 * q + 2] - pairFv[3 * r + 2];
+        if (pairVE) ve += pairVE[q] + pairVE[r];
+    }
+    if (gvg) {
+        gvg[3 * (size_t)j] = -g0;
+        gvg[3 * (size_t)j + 1] = -g1;
+        gvg[3 * (size_t)j + 2] = -g2;
+    }
+    if (lgv) lgv[j] = varsite[j] + 0.5 * ve;
+}
+void launch_gather_var(int N, const int *nb_off, const int *nb_rev, const double *pairFv, const double *pairVE,
+                       const double *varsite, double *gvg, double *lgv, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_gather_var, dim3(cdiv(N, 128)), dim3(128), 0, s, N, nb_off, nb_rev, pairFv, pairVE,
+                           varsite, gvg, lgv);
 }
 
 // ----------------------------------------------------------------------------

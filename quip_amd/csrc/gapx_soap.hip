@@ -29,9 +29,12 @@ static __constant__ double c_rinv[2 * kMaxL + 4];   // 1/k (k = 0 unused)
 static __constant__ double c_isq[kMaxL + 1];        // 1/sqrt(2l+1)
 
 static bool g_soap_const_ready[64] = {};   // per device
-static void ensure_soap_constants() {
+static hipError_t ensure_soap_constants() {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64 || g_soap_const_ready[dev]) return;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (g_soap_const_ready[dev]) return hipSuccess;
     double yn[(kMaxL + 1) * (kMaxL + 2) / 2], df[kMaxL + 2];
     for (int l = 0; l <= kMaxL; l++)
         for (int m = 0; m <= l; m++) {
@@ -46,16 +49,17 @@ static void ensure_soap_constants() {
     double ri[2 * kMaxL + 4];
     ri[0] = 0.0;
     for (int k = 1; k < 2 * kMaxL + 4; k++) ri[k] = 1.0 / (double)k;
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_rinv), ri, sizeof(ri));
     double isq[kMaxL + 1];
     for (int l = 0; l <= kMaxL; l++) isq[l] = 1.0 / sqrt(2.0 * l + 1.0);
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_isq), isq, sizeof(isq));
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn));
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df));
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_rinv), ri, sizeof(ri))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_isq), isq, sizeof(isq))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_ynorm), yn, sizeof(yn))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_dfact), df, sizeof(df))) != hipSuccess) return e;
     g_soap_const_ready[dev] = true;
+    return hipSuccess;
 }
 
-void soap_fast_constants_init() { ensure_soap_constants(); }
+hipError_t soap_fast_constants_init() { return ensure_soap_constants(); }
 
 #define RELOAD_BARRIER() __asm__ volatile("" ::: "memory")
 
@@ -693,6 +697,43 @@ static void fwd_go(bool preg, int cpw, int nblk, size_t sh, hipStream_t s, const
                        nb_j, nb_r, nb_d, X, norm, cbuf);
 }
 
+// LDS bytes of the forward block: transform, radial grid, c' accumulators, the staged
+// neighbour chunk and (without registers for the Legendre levels) their LDS rows
+static size_t fwd_fast_lds(const SoapDev &sp, int cpw, bool preg) {
+    const int LM = (sp.L + 1) * (sp.L + 1), KLM = sp.K * LM, NB = kSoapChunk;
+    return (size_t)(sp.n * sp.n + 3 * sp.n + cpw * ((KLM + 1) & ~1) + cpw * (NB + 2) * (sp.n + 2 * sp.L + 1) +
+                    (preg ? 0 : 32 * cpw * ((2 * (sp.L + 2)) | 1))) * 8 +
+           (size_t)cpw * sp.S * (NB + 1) * 4;
+}
+
+// LDS bytes of the adjoint block, with its staging choice: whole = 2 stages the level
+// blocks P_l of all its centres, 1 dE/dp, 0 one level at a time; cpb centres per wave
+static size_t bwd_fast_lds(const SoapDev &sp, int *cpb_out, int *whole_out) {
+    const int LM = (sp.L + 1) * (sp.L + 1);
+    const int KLM = sp.K * LM, PR = (2 * (sp.L + 2)) | 1;
+    const size_t stg0 = std::max((size_t)sp.K * sp.K + (size_t)sp.K * (2 * sp.L + 1), (size_t)64 * PR);
+    const size_t pl = (size_t)(sp.L + 1) * (sp.K * sp.K + 2);
+    auto stg = [&](int cpb, int whole) {
+        return whole == 2 ? std::max(stg0, cpb * pl) : whole == 1 ? std::max(stg0, (size_t)cpb * sp.dim) : stg0;
+    };
+    auto need = [&](int cpb, int whole) {
+        return ((size_t)sp.n * sp.n + 3 * sp.n + (size_t)cpb * (KLM + 2) + stg(cpb, whole)) * 8;
+    };
+    const int whole = need(2, 2) <= 24 * 1024 ? 2 : need(2, 1) <= 24 * 1024 ? 1 : 0;
+    const int cpb = need(2, whole) <= 24 * 1024 ? 2 : 1;   // two centres per wave when LDS stays small
+    if (cpb_out) *cpb_out = cpb;
+    if (whole_out) *whole_out = whole;
+    return need(cpb, whole);
+}
+
+size_t soap_fast_lds_bytes(const SoapDev &sp) {
+    if (!soap_fast_supported(sp.n, sp.L)) return 0;
+    const int cpw = sp.S == 1 && fwd_pair_ok(sp.n, sp.L) ? 2 : 1;
+    size_t f = fwd_fast_lds(sp, cpw, true);
+    if (cpw == 1 && sp.n < 10 && sp.L < 8) f = std::max(f, fwd_fast_lds(sp, 1, false));
+    return std::max(f, bwd_fast_lds(sp, nullptr, nullptr));
+}
+
 bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const int *Z, const int *nb_off,
                           const int *nb_j, const double *nb_r, const double *nb_d, double *X, double *norm,
                           double *cbuf, hipStream_t s) {
@@ -710,10 +751,8 @@ bool launch_soap_fwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     // model runs 3.4 ms with one and 5.6 ms with two, its LDS per wave doubling).
     const int cpw = sp.S == 1 && fwd_pair_ok(sp.n, sp.L) ? 2 : 1;
     const bool preg = cpw == 2 || sp.n >= 10 || sp.L >= 8 || nc < 16384;
-    const size_t sh = (size_t)(sp.n * sp.n + 3 * sp.n + cpw * ((KLM + 1) & ~1) +
-                               cpw * (NB + 2) * (sp.n + 2 * sp.L + 1) +
-                               (preg ? 0 : 32 * cpw * ((2 * (sp.L + 2)) | 1))) * 8 +
-                      (size_t)cpw * sp.S * (NB + 1) * 4;
+    const size_t sh = fwd_fast_lds(sp, cpw, preg);
+    (void)KLM; (void)NB;
     const int nblk = (nc + cpw - 1) / cpw;
 #define LAUNCH(NN, LL)                                                                                          \
     if (sp.n == NN && sp.L == LL) {                                                                             \
@@ -748,21 +787,10 @@ bool launch_soap_bwd_fast(const SoapDev &sp, int nc, const int *centres, const i
     if (!soap_fast_supported(sp.n, sp.L)) return false;
     ensure_soap_constants();
     if (nc <= 0) return true;
-    const int LM = (sp.L + 1) * (sp.L + 1);
-    const int KLM = sp.K * LM, PR = (2 * (sp.L + 2)) | 1;
-    const size_t stg0 = std::max((size_t)sp.K * sp.K + (size_t)sp.K * (2 * sp.L + 1), (size_t)64 * PR);
     // whole: all centres of the block staged at once (small descriptors): 2, the level blocks
     // P_l (K x K each); 1, dE/dp; 0, one level at a time
-    const size_t pl = (size_t)(sp.L + 1) * (sp.K * sp.K + 2);
-    auto stg = [&](int cpb, int whole) {
-        return whole == 2 ? std::max(stg0, cpb * pl) : whole == 1 ? std::max(stg0, (size_t)cpb * sp.dim) : stg0;
-    };
-    auto need = [&](int cpb, int whole) {
-        return ((size_t)sp.n * sp.n + 3 * sp.n + (size_t)cpb * (KLM + 2) + stg(cpb, whole)) * 8;
-    };
-    const int whole = need(2, 2) <= 24 * 1024 ? 2 : need(2, 1) <= 24 * 1024 ? 1 : 0;
-    const int cpb = need(2, whole) <= 24 * 1024 ? 2 : 1;   // two centres per wave when LDS stays small
-    const size_t sh = need(cpb, whole);
+    int cpb = 1, whole = 0;
+    const size_t sh = bwd_fast_lds(sp, &cpb, &whole);
     const int nblk = (nc + cpb - 1) / cpb;
     // three wavefronts per SIMD for n <= 8 (161 VGPRs without spills) once the launch has
     // at least four per SIMD to schedule

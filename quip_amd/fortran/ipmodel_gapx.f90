@@ -10,7 +10,7 @@ module gapx_c_binding
   implicit none
 
   integer(c_int), parameter :: GAPX_WANT_E = 1, GAPX_WANT_F = 2, GAPX_WANT_V = 4, &
-       GAPX_WANT_LOCAL_E = 8, GAPX_WANT_LOCAL_V = 16
+       GAPX_WANT_LOCAL_E = 8, GAPX_WANT_LOCAL_V = 16, GAPX_WANT_VARIANCE = 32
 
   interface
      integer(c_int) function gapx_api_version() bind(c, name="gapx_api_version")
@@ -70,6 +70,40 @@ module gapx_c_binding
        integer(c_int), value :: want
        type(c_ptr), value :: e, local_e, f, virial, local_virial
      end function gapx_calc
+
+     integer(c_int) function gapx_calc_ex(model, n, pos, z, lattice, pbc, nl_cutoff, want, atom_mask, &
+          only_descriptor, reg, e, local_e, f, virial, local_virial, epc, lgv, gvg) bind(c, name="gapx_calc_ex")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: model
+       integer(c_int), value :: n
+       real(c_double), dimension(*) :: pos
+       integer(c_int), dimension(*) :: z
+       real(c_double), dimension(*) :: lattice
+       integer(c_int), dimension(*) :: pbc
+       real(c_double), value :: nl_cutoff
+       integer(c_int), value :: want
+       type(c_ptr), value :: atom_mask
+       integer(c_int), value :: only_descriptor
+       real(c_double), value :: reg
+       type(c_ptr), value :: e, local_e, f, virial, local_virial, epc, lgv, gvg
+     end function gapx_calc_ex
+
+     integer(c_int) function gapx_descriptor_variances(model, coord, max_entries, variance, cutoff_sq, atoms, &
+          n_entries) bind(c, name="gapx_descriptor_variances")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: model
+       integer(c_int), value :: coord, max_entries
+       type(c_ptr), value :: variance, cutoff_sq, atoms
+       integer(c_int) :: n_entries
+     end function gapx_descriptor_variances
+
+     integer(c_int) function gapx_model_info(model, n_coord, coord, dim, n_sparse) bind(c, name="gapx_model_info")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: model
+       integer(c_int) :: n_coord
+       integer(c_int), value :: coord
+       type(c_ptr), value :: dim, n_sparse
+     end function gapx_model_info
   end interface
 end module gapx_c_binding
 
@@ -83,7 +117,7 @@ module IPModel_GAPX_module
   integer, parameter :: dp = kind(1.0d0)
 
   public :: IPModel_GAPX, IPModel_GAPX_Initialise_str, IPModel_GAPX_Finalise, IPModel_GAPX_Calc, &
-       IPModel_GAPX_Print, IPModel_GAPX_cutoff, IPModel_GAPX_last_error
+       IPModel_GAPX_Print, IPModel_GAPX_cutoff, IPModel_GAPX_last_error, IPModel_GAPX_n_coordinates
 
   type IPModel_GAPX
      type(c_ptr) :: model = c_null_ptr
@@ -190,6 +224,15 @@ contains
     c = this%cutoff
   end function IPModel_GAPX_cutoff
 
+  function IPModel_GAPX_n_coordinates(this) result(nc)
+    type(IPModel_GAPX), intent(in) :: this
+    integer :: nc
+    integer(c_int) :: n, st
+    n = 0
+    st = gapx_model_info(this%model, n, -1_c_int, c_null_ptr, c_null_ptr)
+    nc = int(n)
+  end function IPModel_GAPX_n_coordinates
+
   subroutine IPModel_GAPX_Print(this)
     type(IPModel_GAPX), intent(in) :: this
     write(*, '(a,a,a,f10.5,a,a)') 'IPModel_GAPX : label=', trim(this%label), ' cutoff=', this%cutoff, &
@@ -212,8 +255,17 @@ contains
   ! IPModel_GAP_Calc (IPModel_GAP.f95:233-605) with the Atoms object unpacked:
   ! pos(3,n), Z(n), lattice(3,3), pbc(3).  Outputs are optional and
   ! caller-allocated; wrong sizes are reported like check_size (:283-298).
+  ! The args_str keys of the reference that change what is computed (:324-335) take
+  ! their data through optional arguments in place of Atoms properties:
+  !   atom_mask_name=NAME        needs atom_mask(n) (the logical property NAME)
+  !   only_descriptor=k          coordinate k alone when there are several
+  !   energy_per_coordinate=NAME needs energy_per_coordinate(n_coordinates)
+  !   local_gap_variance=NAME    needs local_gap_variance(n); gap_variance_gradient(3,n)
+  !                              is filled when present and forces or virials are asked for
+  !   print_gap_variance=T       prints the per-instance variances (:506-516)
+  !   gap_variance_regularisation=x (default 0.001)
   subroutine IPModel_GAPX_Calc(this, n, pos, Z, lattice, pbc, e, local_e, f, virial, local_virial, &
-       args_str, error)
+       args_str, error, atom_mask, energy_per_coordinate, local_gap_variance, gap_variance_gradient)
     type(IPModel_GAPX), intent(inout) :: this
     integer, intent(in) :: n
     real(dp), intent(in), target :: pos(3, n), lattice(3, 3)
@@ -222,12 +274,17 @@ contains
     real(dp), intent(out), optional, target :: e, local_e(:), f(:, :), virial(3, 3), local_virial(:, :)
     character(len=*), intent(in), optional :: args_str
     integer, intent(out), optional :: error
+    logical, intent(in), optional :: atom_mask(:)
+    real(dp), intent(out), optional, target :: energy_per_coordinate(:), local_gap_variance(:), &
+         gap_variance_gradient(:, :)
     integer(c_int), allocatable :: zc(:)
-    integer(c_int) :: pbcc(3), want, st
-    real(c_double) :: rc
+    integer(c_int) :: pbcc(3), want, st, only, ncoord
+    real(c_double) :: rc, reg
     real(dp), target :: e_dummy
-    type(c_ptr) :: pe, ple, pf, pv, plv
+    character(kind=c_char), allocatable, target :: maskc(:)
+    type(c_ptr) :: pe, ple, pf, pv, plv, pmask, pepc, plgv, pgvg
     character(len=256) :: v
+    logical :: print_var, grad
     if (present(error)) error = 0
     if (.not. c_associated(this%model)) then
        call raise(1_c_int, 'IPModel_GAPX_Calc: model not initialised', error)
@@ -236,6 +293,7 @@ contains
     want = GAPX_WANT_E
     pe = c_loc(e_dummy)
     ple = c_null_ptr; pf = c_null_ptr; pv = c_null_ptr; plv = c_null_ptr
+    pmask = c_null_ptr; pepc = c_null_ptr; plgv = c_null_ptr; pgvg = c_null_ptr
     if (present(e)) pe = c_loc(e)
     if (present(local_e)) then
        if (size(local_e) /= n) then
@@ -261,7 +319,11 @@ contains
        end if
        want = ior(want, GAPX_WANT_LOCAL_V); plv = c_loc(local_virial)
     end if
+    grad = present(f) .or. present(virial) .or. present(local_virial)
     rc = 0.0_dp
+    only = 0
+    reg = 1.0e-3_dp
+    print_var = .false.
     if (present(args_str)) then
        v = arg_value(args_str, 'cutoff')
        if (len_trim(v) > 0) read(v, *) rc
@@ -269,12 +331,103 @@ contains
           call raise(1_c_int, 'IPModel_GAPX_Calc: r_scale / E_scale cannot be set at calc time', error)
           return
        end if
+       v = arg_value(args_str, 'atom_mask_name')
+       if (len_trim(v) > 0) then
+          if (.not. present(atom_mask)) then
+             call raise(1_c_int, 'IPModel_GAPX_Calc did not find '//trim(v)//' property (pass atom_mask)', error)
+             return
+          end if
+          if (size(atom_mask) /= n) then
+             call raise(1_c_int, 'IPModel_GAPX_Calc: atom_mask has the wrong size', error)
+             return
+          end if
+          allocate(maskc(n))
+          maskc = merge(achar(1, c_char), achar(0, c_char), atom_mask)
+          pmask = c_loc(maskc)
+       end if
+       v = arg_value(args_str, 'only_descriptor')
+       if (len_trim(v) > 0) read(v, *) only
+       v = arg_value(args_str, 'gap_variance_regularisation')
+       if (len_trim(v) > 0) read(v, *) reg
+       v = arg_value(args_str, 'print_gap_variance')
+       print_var = len_trim(v) > 0 .and. (v(1:1) == 'T' .or. v(1:1) == 't' .or. v(1:2) == '.T' .or. v(1:2) == '.t')
+       if (print_var) want = ior(want, GAPX_WANT_VARIANCE)
+       if (len_trim(arg_value(args_str, 'energy_per_coordinate')) > 0) then
+          st = gapx_model_info(this%model, ncoord, -1_c_int, c_null_ptr, c_null_ptr)
+          if (.not. present(energy_per_coordinate)) then
+             call raise(1_c_int, 'IPModel_GAPX_Calc: energy_per_coordinate needs its output array', error)
+             return
+          end if
+          if (size(energy_per_coordinate) /= ncoord) then
+             call raise(1_c_int, 'IPModel_GAPX_Calc: energy_per_coordinate has the wrong size', error)
+             return
+          end if
+          pepc = c_loc(energy_per_coordinate)
+       end if
+       if (len_trim(arg_value(args_str, 'local_gap_variance')) > 0) then
+          if (.not. present(local_gap_variance)) then
+             call raise(1_c_int, 'IPModel_GAPX_Calc: local_gap_variance needs its output array', error)
+             return
+          end if
+          if (size(local_gap_variance) /= n) then
+             call raise(1_c_int, 'IPModel_GAPX_Calc: local_gap_variance has the wrong size', error)
+             return
+          end if
+          plgv = c_loc(local_gap_variance)
+          if (present(gap_variance_gradient) .and. grad) then
+             if (size(gap_variance_gradient, 1) /= 3 .or. size(gap_variance_gradient, 2) /= n) then
+                call raise(1_c_int, 'IPModel_GAPX_Calc: gap_variance_gradient has the wrong shape', error)
+                return
+             end if
+             pgvg = c_loc(gap_variance_gradient)
+          end if
+       end if
     end if
     allocate(zc(n))
     zc = Z
     pbcc = merge(1, 0, pbc)
-    st = gapx_calc(this%model, int(n, c_int), pos, zc, lattice, pbcc, rc, want, pe, ple, pf, pv, plv)
-    if (st /= 0) call raise(st, 'IPModel_GAPX_Calc: '//trim(IPModel_GAPX_last_error()), error)
+    st = gapx_calc_ex(this%model, int(n, c_int), pos, zc, lattice, pbcc, rc, want, pmask, only, reg, pe, ple, pf, &
+         pv, plv, pepc, plgv, pgvg)
+    if (st /= 0) then
+       call raise(st, 'IPModel_GAPX_Calc: '//trim(IPModel_GAPX_last_error()), error)
+       return
+    end if
+    if (print_var) call print_variances(this, only)
   end subroutine IPModel_GAPX_Calc
+
+  ! print_gap_variance (IPModel_GAP.f95:401-404, 506-516)
+  subroutine print_variances(this, only)
+    type(IPModel_GAPX), intent(in) :: this
+    integer(c_int), intent(in) :: only
+    integer(c_int) :: ncoord, k, ne, st
+    real(c_double), allocatable, target :: var(:), c2(:)
+    integer(c_int), allocatable, target :: at2(:, :)
+    integer :: i
+    st = gapx_model_info(this%model, ncoord, -1_c_int, c_null_ptr, c_null_ptr)
+    write(*, '(a,a,a,i0,a)') 'GAP_VARIANCE potential ', trim(this%label), ' calculating for ', ncoord, ' descriptors'
+    do k = 0, ncoord - 1
+       if (only > 0 .and. ncoord > 1 .and. k /= only - 1) cycle
+       st = gapx_descriptor_variances(this%model, k, 0_c_int, c_null_ptr, c_null_ptr, c_null_ptr, ne)
+       if (st /= 0) return
+       if (ne == 0) then
+          write(*, '(a,a,a,i0,a)') 'GAP_VARIANCE potential ', trim(this%label), ' descriptor ', k + 1, ' not found'
+          cycle
+       end if
+       allocate(var(ne), c2(ne), at2(2, ne))
+       st = gapx_descriptor_variances(this%model, k, ne, c_loc(var), c_loc(c2), c_loc(at2), ne)
+       do i = 1, ne
+          write(*, '(a,a,a,i0,a,i0,a,es25.16,a,es25.16,a)') 'GAP_VARIANCE potential ', trim(this%label), &
+               ' descriptor ', k + 1, ' var( ', i, ' ) = ', var(i), ' * ', c2(i), ' cutoff'
+          if (at2(2, i) >= 0) then
+             write(*, '(a,a,a,i0,a,i0,a,2i8)') 'GAP_VARIANCE potential ', trim(this%label), ' descriptor ', k + 1, &
+                  ' ii( ', i, ' ) = ', at2(1, i) + 1, at2(2, i) + 1
+          else
+             write(*, '(a,a,a,i0,a,i0,a,i8)') 'GAP_VARIANCE potential ', trim(this%label), ' descriptor ', k + 1, &
+                  ' ii( ', i, ' ) = ', at2(1, i) + 1
+          end if
+       end do
+       deallocate(var, c2, at2)
+    end do
+  end subroutine print_variances
 
 end module IPModel_GAPX_module

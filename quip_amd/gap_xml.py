@@ -52,6 +52,41 @@ def parse_args_str(s):
     return out
 
 
+def _fortran_true(v):
+    """Fortran list-directed logical: T, .true., t... are true."""
+    v = (v or "").strip().lstrip(".")
+    return v[:1] in ("T", "t")
+
+
+# Descriptor options that change the descriptor and are not built: accepted only at their
+# default.  Keys GAP does not read are ignored, as its parser does (gap_fit's keys ride in
+# the same string).
+SOAP_ONLY_DEFAULT = {"normalise": True, "normalize": True, "average": False, "diagonal_radial": False,
+                     "central_reference_all_species": False, "covariance_sigma0": 0.0, "cutoff_dexp": 0.0,
+                     "cutoff_scale": 1.0, "cutoff_rate": 1.0, "radial_enhancement": 0.0, "nu_R": 2.0, "nu_S": 2.0,
+                     "Z_mix": False, "R_mix": False, "sym_mix": False, "compress_file": "", "compress_mode": ""}
+D2B_ONLY_DEFAULT = {"only_intra": False, "only_inter": False, "n_exponents": 1.0, "exponents": 1.0,
+                    "resid_name": ""}
+
+
+def _check_defaults(kv, table, s):
+    for k, dflt in table.items():
+        if k not in kv:
+            continue
+        v = kv[k]
+        if isinstance(dflt, bool):
+            ok = _fortran_true(v) == dflt
+        elif isinstance(dflt, str):
+            ok = v == dflt
+        else:
+            try:
+                ok = float(v) == dflt
+            except ValueError:
+                ok = False
+        if not ok:
+            raise NotImplementedError("descriptor option %s=%s is not built (only its default): %r" % (k, v, s))
+
+
 @dataclass
 class Descriptor:
     """Parsed descriptor init string (soap or distance_2b)."""
@@ -84,6 +119,7 @@ class Descriptor:
         kind = toks[0]
         kv = parse_args_str(" ".join(toks[1:]))
         if kind == "soap":
+            _check_defaults(kv, SOAP_ONLY_DEFAULT, s)
             n_species = int(kv.get("n_species", 1))
             species = [int(z) for z in kv["species_Z"].split()] if "species_Z" in kv else [0] * n_species
             n_Z = int(kv.get("n_Z", 1))
@@ -97,6 +133,7 @@ class Descriptor:
                        float(kv.get("basis_error_exponent", 10.0)),
                        int(kv["n_max"]), int(kv["l_max"]), n_species, species, n_Z, Z, string=s)
         if kind == "distance_2b":
+            _check_defaults(kv, D2B_ONLY_DEFAULT, s)
             return cls("distance_2b", float(kv["cutoff"]),
                        float(kv.get("cutoff_transition_width", 0.5)),
                        Z1=int(kv.get("Z1", 0)), Z2=int(kv.get("Z2", 0)), string=s)
@@ -156,6 +193,10 @@ def read_gap_xml(path=None, xml_text=None, label=None, xml_dir=None):
     e0 = np.zeros(119)
     data = params.find("GAP_data")
     if data is not None:
+        if data.get("e0") is not None:          # every species, then per-Z entries (IPModel_GAP.f95:670-674)
+            e0[:] = float(data.get("e0").split()[0])
+        if _fortran_true(data.get("do_pca", "F")):
+            raise NotImplementedError("GAP_data do_pca=T is not built")
         for e in data.findall("e0"):
             e0[int(e.get("Z"))] = float(e.get("value"))
     coords = []
@@ -178,6 +219,12 @@ def read_gap_xml(path=None, xml_text=None, label=None, xml_dir=None):
         if d != desc.dim:
             raise ValueError("gpCoordinates %s: dimensions %d != descriptor dimension %d"
                              % (gc.get("label"), d, desc.dim))
+        if float(gc.get("signal_mean", "0")) != 0.0:
+            raise NotImplementedError("gpCoordinates %s: signal_mean=%s is not built (only 0)"
+                                      % (gc.get("label"), gc.get("signal_mean")))
+        if int(gc.get("n_permutations", "1")) > 1:
+            raise NotImplementedError("gpCoordinates %s: n_permutations=%s is not built (only 1)"
+                                      % (gc.get("label"), gc.get("n_permutations")))
         coords.append(GPCoordinate(gc.get("label"), desc, cov, float(gc.get("signal_variance")),
                                    float(gc.get("signal_mean", "0")), zeta, X, alpha, scut, theta))
     cl = params.find("command_line")

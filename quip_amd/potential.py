@@ -44,6 +44,7 @@ class Potential:
         L.check(lib.gapx_model_load(raw, len(raw), label.encode() if label else None, xml_dir.encode(), device,
                                     C.byref(h)), "gapx_model_load")
         self._h = h
+        self.label = label
         self.calc_args = calc_args
         self.set_precision(precision)
         if "E_scale" in kv:
@@ -106,66 +107,141 @@ class Potential:
         L.check(L.lib().gapx_timing_totals(self._h, L.ptr(t), C.byref(n)), "timing_totals")
         return dict(zip(self.STAGES, t)), n.value
 
+    # args_str keys this host reads (IPModel_GAP.f95:324-335 plus the output names); others
+    # are ignored as ParamReader's ignore_unknown does
     def calc(self, at, energy=None, force=None, virial=None, local_energy=None, local_virial=None, args_str="",
              cutoff=None):
-        """IPModel_GAP_Calc.  `energy`, `force`... may be True (allocate) or caller arrays to fill."""
+        """IPModel_GAP_Calc (IPModel_GAP.f95:233-605).
+
+        `energy`, `force`, `virial`, `local_energy`, `local_virial` may be True (the
+        result is allocated) or caller-owned float64 arrays that are filled: energy of
+        one element, force (N, 3) or QUIP's (3, N), virial (3, 3), local_energy (N,),
+        local_virial (N, 9) or (9, N).  The results are computed into fresh buffers and
+        copied into the caller's arrays with the layout their shape names.
+
+        args_str keys: the output names above; `atom_mask_name` (a boolean array of
+        at.arrays: only those atoms are descriptor centres / first atoms of pair
+        descriptors and get e0); `only_descriptor` (1-based coordinate); `energy_per_coordinate`
+        (name of the at.info entry that receives one energy per coordinate);
+        `local_gap_variance` (name of the at.arrays entry for the per-atom variance; with
+        forces or virials also at.arrays["gap_variance_gradient"]); `print_gap_variance`;
+        `gap_variance_regularisation` (default 0.001); `cutoff`.  `r_scale` / `E_scale` are
+        errors at calc time, as in the reference (:348-350)."""
         kv = parse_args_str((self.calc_args or "") + " " + (args_str or ""))
         if "r_scale" in kv or "E_scale" in kv:
             raise ValueError("IPModel_GAP_Calc: r_scale / E_scale cannot be given at calc time "
                              "(IPModel_GAP.f95:348-350)")
-        want = 0
         N = len(at)
-        out = {}
-
-        def req(flag, name, val, shape):
-            nonlocal want
-            on = (val is not None and val is not False) or name in kv
-            if not on:
-                return None
-            want |= flag
-            if isinstance(val, np.ndarray):
-                if val.size != int(np.prod(shape)):
-                    raise ValueError("%s has %d elements, expected %s" % (name, val.size, shape))
-                return val
-            return np.zeros(shape)
-
-        e = req(L.WANT_E, "energy", energy, (1,))
-        if e is None:
-            e = np.zeros(1)
-            want |= L.WANT_E
-        f = req(L.WANT_F, "force", force, (N, 3))
-        v = req(L.WANT_V, "virial", virial, (3, 3))
-        le = req(L.WANT_LOCAL_E, "local_energy", local_energy, (N,))
-        lv = req(L.WANT_LOCAL_V, "local_virial", local_virial, (N, 9))
+        want = L.WANT_E
+        shapes = {"energy": (1,), "force": (N, 3), "virial": (3, 3), "local_energy": (N,), "local_virial": (N, 9)}
+        flags = {"energy": L.WANT_E, "force": L.WANT_F, "virial": L.WANT_V, "local_energy": L.WANT_LOCAL_E,
+                 "local_virial": L.WANT_LOCAL_V}
+        given = {"energy": energy, "force": force, "virial": virial, "local_energy": local_energy,
+                 "local_virial": local_virial}
+        bufs = {}
+        for name, val in given.items():
+            if (val is not None and val is not False) or name in kv:
+                want |= flags[name]
+                if isinstance(val, np.ndarray) and val.dtype != np.float64:
+                    raise TypeError("%s must be a float64 array (got %s)" % (name, val.dtype))
+                bufs[name] = np.zeros(shapes[name])
+        mask = None
+        if "atom_mask_name" in kv:
+            name = kv["atom_mask_name"]
+            if name not in at.arrays:
+                raise ValueError("IPModel_GAP_Calc did not find %s property in the atoms object" % name)
+            mask = np.ascontiguousarray(np.asarray(at.arrays[name]).astype(bool).astype(np.uint8))
+            if mask.shape != (N,):
+                raise ValueError("atom mask %s has shape %s, expected (%d,)" % (name, mask.shape, N))
+        only = int(kv.get("only_descriptor", 0))
+        epc_name = kv.get("energy_per_coordinate", "")
+        lgv_name = kv.get("local_gap_variance", "")
+        print_var = kv.get("print_gap_variance", "F").upper().startswith(("T", ".T"))
+        reg = float(kv.get("gap_variance_regularisation", 1e-3))
+        grad = bool(want & (L.WANT_F | L.WANT_V | L.WANT_LOCAL_V))
+        ncoord = self.n_coordinates()
+        epc = np.zeros(ncoord) if epc_name else None
+        lgv = np.zeros(N) if lgv_name else None
+        gvg = np.zeros((N, 3)) if lgv_name and grad else None
+        if print_var:
+            want |= L.WANT_VARIANCE
         pos, Z = L.f64(at.positions), L.i32(at.numbers)
         lat, pbc = L.f64(at.cell), L.i32(at.pbc)
-        buf = {k: (np.zeros(x.shape) if x is not None and not x.flags["C_CONTIGUOUS"] else x)
-               for k, x in (("f", f), ("v", v), ("le", le), ("lv", lv))}
-        vbuf = np.zeros(9) if v is not None else None
+        vflat = np.zeros(9) if "virial" in bufs else None
+        e = bufs.get("energy", np.zeros(1))
         rc = float(cutoff) if cutoff is not None else float(kv.get("cutoff", 0.0))
-        L.check(L.lib().gapx_calc(self._h, N, L.ptr(pos), L.ptr(Z), L.ptr(lat), L.ptr(pbc), rc, want, L.ptr(e),
-                                  L.ptr(buf["le"]), L.ptr(buf["f"]), L.ptr(vbuf), L.ptr(buf["lv"])), "gapx_calc")
-        out["energy"] = float(e[0])
+        L.check(L.lib().gapx_calc_ex(self._h, N, L.ptr(pos), L.ptr(Z), L.ptr(lat), L.ptr(pbc), rc, want,
+                                     L.ptr(mask), only, reg, L.ptr(e), L.ptr(bufs.get("local_energy")),
+                                     L.ptr(bufs.get("force")), L.ptr(vflat), L.ptr(bufs.get("local_virial")),
+                                     L.ptr(epc), L.ptr(lgv), L.ptr(gvg)), "gapx_calc_ex")
+        out = {"energy": float(e[0])}
         at.info["energy"] = out["energy"]
-        if f is not None:
-            f[...] = buf["f"].reshape(f.shape)
-            out["force"] = f.reshape(N, 3)
+        if "force" in bufs:
+            out["force"] = bufs["force"]
             at.arrays["force"] = out["force"]
-        if v is not None:
-            vv = vbuf.reshape(3, 3).T      # stored column-major: vbuf[3*b + a] = virial(a, b)
-            v[...] = vv.reshape(v.shape)
-            out["virial"] = vv
-            at.info["virial"] = vv
-        if le is not None:
-            le[...] = buf["le"].reshape(le.shape)
-            out["local_energy"] = le
-            at.arrays["local_energy"] = le
-        if lv is not None:
-            lv[...] = buf["lv"].reshape(lv.shape)
-            out["local_virial"] = lv.reshape(N, 9)
+        if vflat is not None:
+            out["virial"] = vflat.reshape(3, 3).T.copy()   # stored column-major: vflat[3*b + a] = virial(a, b)
+            at.info["virial"] = out["virial"]
+        if "local_energy" in bufs:
+            out["local_energy"] = bufs["local_energy"]
+            at.arrays["local_energy"] = out["local_energy"]
+        if "local_virial" in bufs:
+            out["local_virial"] = bufs["local_virial"]
             at.arrays["local_virial"] = out["local_virial"]
+        if epc is not None:
+            out["energy_per_coordinate"] = epc
+            at.info[epc_name] = epc
+        if lgv is not None:
+            out["local_gap_variance"] = lgv
+            at.arrays[lgv_name] = lgv
+            if gvg is not None:
+                out["gap_variance_gradient"] = gvg
+                at.arrays["gap_variance_gradient"] = gvg
+        if print_var:
+            out["gap_variance"] = self._print_variances(ncoord, only)
+        # caller-owned arrays receive the results in the layout their shape names
+        for name, val in given.items():
+            if isinstance(val, np.ndarray):
+                res = out[name] if name != "energy" else np.array([out["energy"]])
+                if name in ("force", "local_virial") and val.shape == res.T.shape and val.shape != res.shape:
+                    np.copyto(val, res.T)
+                elif val.size == res.size:
+                    np.copyto(val, res.reshape(val.shape))
+                else:
+                    raise ValueError("%s has shape %s, expected %s" % (name, val.shape, res.shape))
         self.results = out
         return out
+
+    def descriptor_variances(self, coord):
+        """Per-instance GAP variances of coordinate `coord` (0-based) from the last calc
+        that formed them: (variance, covariance_cutoff^2, atoms (n, 2); -1 for soap)."""
+        n = C.c_int()
+        L.check(L.lib().gapx_descriptor_variances(self._h, coord, 0, None, None, None, C.byref(n)),
+                "gapx_descriptor_variances")
+        v, c2, a = np.zeros(n.value), np.zeros(n.value), np.zeros((n.value, 2), np.int32)
+        L.check(L.lib().gapx_descriptor_variances(self._h, coord, n.value, L.ptr(v), L.ptr(c2), L.ptr(a),
+                                                  C.byref(n)), "gapx_descriptor_variances")
+        return v, c2, a
+
+    def _print_variances(self, ncoord, only):
+        """print_gap_variance (IPModel_GAP.f95:401-404, 506-516): one line per instance."""
+        label = self.label or ""
+        print("GAP_VARIANCE potential %s calculating for %d descriptors" % (label, ncoord))
+        res = []
+        for k in range(ncoord):
+            if only > 0 and ncoord > 1 and k != only - 1:
+                continue
+            v, c2, a = self.descriptor_variances(k)
+            res.append((v, c2, a))
+            if len(v) == 0:
+                print("GAP_VARIANCE potential %s descriptor %d not found" % (label, k + 1))
+            for i in range(len(v)):
+                print("GAP_VARIANCE potential %s descriptor %d var( %d ) = %.16g * %.16g cutoff"
+                      % (label, k + 1, i + 1, v[i], c2[i]))
+                ii = [x + 1 for x in a[i] if x >= 0]
+                print("GAP_VARIANCE potential %s descriptor %d ii( %d ) = %s"
+                      % (label, k + 1, i + 1, " ".join(str(x) for x in ii)))
+        return res
 
     # --- ASE-calculator style accessors (quippy/quippy/potential.py:177-300) ---
     def get_property(self, name, at):

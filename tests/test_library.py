@@ -97,3 +97,97 @@ def test_quip_cli_arguments_and_report():
     assert np.allclose(back.arrays["force"], at.arrays["force"])
     assert back.info["Energy"] == -42.5
     assert np.allclose(np.array(back.info["virial"].split(), float).reshape(3, 3, order="F"), V)
+
+
+# ---------------------------------------------------------------- model-file refusals
+
+def _water_xml(golden):
+    return open(os.path.join(golden, "gap_water", "GAP.xml")).read()
+
+
+def _load_status(xml, golden):
+    """gapx_model_load's status: the XML and sparseX files are read and checked before
+    any device is touched, so a refusal shows here without a GPU; a file that passes
+    them stops at the device (GAPX_E_DEVICE or GAPX_E_ARG for a missing device)."""
+    import ctypes as C
+    raw = xml.encode() if isinstance(xml, str) else xml
+    h = C.c_void_p()
+    st = L.lib().gapx_model_load(raw, len(raw), None, os.path.join(golden, "gap_water").encode(), 0, C.byref(h))
+    if st == 0:
+        L.lib().gapx_model_free(h)
+    return st, L.last_error()
+
+
+GAPX_E_ARG, GAPX_E_MODEL, GAPX_E_DEVICE, GAPX_E_UNSUPPORTED = 1, 2, 4, 5
+
+
+@pytest.mark.parametrize("edit,word", [
+    (('do_core="F"', 'do_core="F" do_pca="T"'), "do_pca"),
+    (('signal_mean=".00000000000000000E+000"', 'signal_mean=".25000000000000000E+000"'), "signal_mean"),
+    (('n_permutations="1"', 'n_permutations="2"'), "n_permutations"),
+    (("Z1=8 Z2=8</descriptor>", "Z1=8 Z2=8 only_intra=T</descriptor>"), "only_intra"),
+    (("Z1=8 Z2=8</descriptor>", "Z1=8 Z2=8 exponents=2</descriptor>"), "exponents"),
+])
+def test_model_options_not_built_are_refused(golden, edit, word):
+    xml = _water_xml(golden)
+    assert edit[0] in xml
+    st, msg = _load_status(xml.replace(edit[0], edit[1], 1), golden)
+    assert st == GAPX_E_UNSUPPORTED and word in msg, (st, msg)
+    with pytest.raises(NotImplementedError):
+        read_gap_xml(xml_text=xml.replace(edit[0], edit[1], 1), xml_dir=os.path.join(golden, "gap_water"))
+
+
+def test_soap_options_not_built_are_refused(golden, tmp_path):
+    """normalise=F, average=T and friends change the descriptor: refused unless at their
+    default; keys GAP's parser does not read (gap_fit's) are ignored, as it ignores them."""
+    d = configs.soap_string(cutoff=4.0, n_max=4, l_max=4)
+    X = np.zeros((2, 4 * 5 // 2 * 5 + 1))
+    from quip_amd.gap_xml import Descriptor, GAPModel, GPCoordinate
+    for extra, refused in (("normalise=F", True), ("average=T", True), ("covariance_sigma0=0.5", True),
+                           ("normalise=T average=F", False), ("n_sparse=100 sparse_method=cur_points delta=3", False)):
+        s = d + " " + extra
+        if refused:
+            with pytest.raises(NotImplementedError):
+                Descriptor.from_string(s)
+        c = GPCoordinate("c", Descriptor.from_string(d), 2, 1.0, 0.0, 4.0, X, np.ones(2), np.ones(2))
+        c.descriptor.string = s
+        path = str(tmp_path / "m.xml")
+        write_gap_xml(path, GAPModel("M", 1527075646, np.zeros(119), [c]))
+        import ctypes as C
+        raw = open(path).read().encode()
+        h = C.c_void_p()
+        st = L.lib().gapx_model_load(raw, len(raw), None, str(tmp_path).encode(), 0, C.byref(h))
+        if st == 0:
+            L.lib().gapx_model_free(h)
+        if refused:
+            assert st == GAPX_E_UNSUPPORTED and extra.split("=")[0] in L.last_error()
+        else:
+            assert st in (0, GAPX_E_ARG, GAPX_E_DEVICE), (st, L.last_error())
+
+
+def test_scalar_e0_attribute_sets_every_species(golden):
+    """<GAP_data e0=...> sets e0 of every species before the per-Z entries override it
+    (IPModel_GAP.f95:670-674); the Python reader mirrors the C reader."""
+    xml = _water_xml(golden).replace('do_core="F"', 'do_core="F" e0="-3.5"', 1)
+    m = read_gap_xml(xml_text=xml, xml_dir=os.path.join(golden, "gap_water"))
+    base = read_gap_xml(os.path.join(golden, "gap_water", "GAP.xml"))
+    listed = set()
+    import re as _re
+    for z in _re.findall(r'<e0 Z="(\d+)"', xml):
+        listed.add(int(z))
+    for z in range(1, 119):
+        assert m.e0[z] == (base.e0[z] if z in listed else -3.5)
+    assert _load_status(xml, golden)[0] in (0, GAPX_E_ARG, GAPX_E_DEVICE)
+
+
+def test_truncated_xml_buffers_are_errors(golden):
+    """The reader takes a length-delimited buffer: a comment, CDATA section, tag or
+    attribute cut off by the length is an error, and nothing past it is read."""
+    for cut in ("<GAP><!-- open comment", "<GAP><c><![CDATA[abc", "<GAP><e0 Z=\"1\" value=\"1.0", "<GAP><e0 Z=",
+                "<GAP><", "<GAP><e0 Z=\"1\""):
+        raw = cut.encode() + b"-->]]>\"/></GAP>"     # the closing text lies beyond the length passed
+        import ctypes as C
+        buf = C.create_string_buffer(raw, len(raw))
+        h = C.c_void_p()
+        st = L.lib().gapx_model_load(buf, len(cut), None, b".", 0, C.byref(h))
+        assert st == GAPX_E_MODEL, (cut, st, L.last_error())
