@@ -101,6 +101,23 @@ int gapx_calc_device(gapx_model *model, int n_atoms, const double *d_pos, const 
                      double *d_virial, double *d_local_virial);
 int gapx_sync(gapx_model *model);
 
+/* gapx_calc_device without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving loops, code
+ * under a capture or harness that forbids hipMalloc).  Every work buffer is carved
+ * from one file-scope device array of 160 MiB per GPU, held until the process ends;
+ * nothing is allocated or freed, on the device or in pinned host memory (staging is
+ * ordinary host memory).  Requests whose buffers do not fit return GAPX_E_CAPACITY
+ * (about 4000 atoms with a 2000-point FP64 SOAP model, more with the split forms).
+ * All calls on one GPU share the array and calls that overlap on one GPU race: this
+ * entry point serialises its own calls within a process and returns only when its
+ * work is done.  The neighbour list is rebuilt on every call (no skin reuse), the GEMM
+ * tiles are the autotuner's choices where it has made them and the defaults otherwise,
+ * and a split precision must be set (gapx_model_set_precision) before the call.
+ * Results equal gapx_calc_device's. */
+int gapx_calc_device_noalloc(gapx_model *model, int n_atoms, const double *d_pos, const int *Z, const double *lattice,
+                             const int *pbc, double nl_cutoff, unsigned want, double *d_e, double *d_local_e,
+                             double *d_f, double *d_virial, double *d_local_virial);
+
 /* gapx_calc with the args_str keys of IPModel_GAP_Calc that change what it computes
  * (IPModel_GAP.f95:324-335); each replaces the reference's handling of one key:
  *   atom_mask (nullable, n_atoms bytes)   atom_mask_name (:324-326, :343-347): only atoms

@@ -49,6 +49,8 @@ _SIGS = [
     ("gapx_calc_device", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                    C.c_uint] + [C.c_void_p] * 5),
     ("gapx_sync", C.c_int, [C.c_void_p]),
+    ("gapx_calc_device_noalloc", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_double, C.c_uint] + [C.c_void_p] * 5),
     ("gapx_calc_batch", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_double, C.c_uint] + [C.c_void_p] * 5),
     ("gapx_calc_batch_device", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -546,6 +546,11 @@ extern "C" int gapx_model_load(const char *xml_text, int64_t xml_len, const char
         st = ce != hipSuccess ? fail(GAPX_E_DEVICE, "uploading the SOAP constant tables: %s", hipGetErrorString(ce))
                               : new_engine(m, m->eng);
     }
+    if (st == GAPX_OK) st = new_engine(m, m->eng_na);
+    if (st == GAPX_OK) {
+        Engine &na = *m->eng_na;
+        for (HBuf *h : {&na.stage_in, &na.stage_geo, &na.stage_io, &na.ctl}) h->pageable = true;
+    }
     if (st != GAPX_OK) {
         gapx_model_free(m);
         return st;
@@ -558,8 +563,27 @@ extern "C" int gapx_model_free(gapx_model *m) {
     if (!m) return GAPX_OK;
     hipSetDevice(m->device);
     m->eng.reset();
+    m->eng_na.reset();
     delete m;
     return GAPX_OK;
+}
+
+void gapx::reset_arena_views(Engine &eg) {
+    Workspace &w = eg.ws;
+    for (DBuf *b : {&w.pos, &w.Z, &w.sid, &w.atom_off, &w.geo, &w.cell_base, &w.ginv, &w.cell_of, &w.map_shift,
+                    &w.cell_cnt, &w.cell_start, &w.cell_fill, &w.cell_atoms, &w.cell_unsorted, &w.scan_part,
+                    &w.scan_part2, &w.n1_cnt, &w.n1_off, &w.n1_j, &w.n1_shift, &w.n1_r, &w.n1_rev, &w.n2_cnt,
+                    &w.n2_off, &w.n2_fill, &w.n2_i, &w.n2_k, &w.n2_is, &w.n2_ks, &w.nb_off, &w.nb_j, &w.nb_rev,
+                    &w.nb_r, &w.nb_d, &w.pairF, &w.pairE, &w.esite, &w.X, &w.norm, &w.cbuf, &w.W, &w.Wf, &w.e_part,
+                    &w.G, &w.Gparts, &w.split_ws, &w.f, &w.le, &w.lv, &w.e, &w.vir, &w.minmax, &w.last_pos,
+                    &w.skin_flags, &w.cmask, &w.t1_j, &w.t1_shift, &w.t1_r, &w.t2_i, &w.t2_k, &w.nl_flags, &w.eatom,
+                    &w.vK, &w.vk, &w.vV, &w.Gv, &w.pairFv, &w.pairVE, &w.varsite, &w.o_epc, &w.o_lgv, &w.o_gvg})
+        b->drop_view();
+    for (auto *v : {&w.centres, &w.var_inst, &w.cut2_inst})
+        for (DBuf &b : *v) b.drop_view();
+    // uploads and neighbour lists refer to the old views
+    eg.have_inputs = eg.have_geo = eg.nl_valid = false;
+    eg.var_valid = false;
 }
 
 extern "C" int gapx_model_cutoff(const gapx_model *m, double *cutoff) {
@@ -576,6 +600,12 @@ extern "C" int gapx_model_set_precision(gapx_model *m, int p) {
     if (!m) return fail(GAPX_E_ARG, "NULL model");
     if (p != GAPX_PREC_FP64 && p != GAPX_PREC_BF16X3 && p != GAPX_PREC_FP16X3)
         return fail(GAPX_E_ARG, "unknown precision %d", p);
+    if (p != GAPX_PREC_FP64) {   // the split copies of the sparse points are made here, at set-up
+        HIPCHK(hipSetDevice(m->device));
+        for (auto &cp : m->coords)
+            if (cp->kind == 1) CHK(ensure_split(m, *cp, p, m->eng->stream));
+        HIPCHK(hipStreamSynchronize(m->eng->stream));
+    }
     m->precision = p;
     return GAPX_OK;
 }
@@ -1071,7 +1101,7 @@ int gapx::check_batch(int S, const int *atom_off, const int *Z, const double *la
 // call's own operands (X is already computed), and keeps the fastest.  Later
 // calls in the bucket reuse the choice.  Without autotuning: GEMM 1 (4,2),
 // GEMM 2 (2,2) with the default split-K factor.
-static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out) {
+static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out, bool cached_only = false) {
     out = GemmTune();
     out.parts = gemm2_parts(nc, c.dim);
     if (!m->autotune || nc <= 0) return GAPX_OK;
@@ -1085,6 +1115,7 @@ static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out)
             return GAPX_OK;
         }
     }
+    if (cached_only) return GAPX_OK;   // the defaults: timing needs events
     Workspace &w = eg.ws;
     hipStream_t s = eg.stream;
     hipEvent_t ea, eb;
@@ -1282,10 +1313,10 @@ static int connect_external(Engine &eg, const ExtNL &x, int N, const double *d_p
     return GAPX_OK;
 }
 
-static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
-                      const int *Z, const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
-                      double *d_le, double *d_f, double *d_v, double *d_lv, const ExtNL *ext = nullptr,
-                      const CalcOpts *opt = nullptr) {
+int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
+                     const int *Z, const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
+                     double *d_le, double *d_f, double *d_v, double *d_lv, const ExtNL *ext, const CalcOpts *opt,
+                     bool noalloc) {
     CHK(check_batch(S, atom_off, Z, lats, pbcs));
     const int N = atom_off[S];
     if (N > 0 && !d_pos) return fail(GAPX_E_ARG, "positions are NULL");
@@ -1374,8 +1405,11 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
         Coord &c = *m->coords[k];
         const int nc = (int)eg.h_centres[k].size();
         if (c.kind != 1 || nc == 0 || skipped(k)) continue;
-        if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tunes[k]));
-        else CHK(ensure_split(m, c, m->precision, s));
+        if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tunes[k], noalloc));
+        else if (c.split_prec != m->precision) {
+            if (noalloc) return fail(GAPX_E_ARG, "split copies of the sparse points are missing (set the precision first)");
+            CHK(ensure_split(m, c, m->precision, s));
+        }
     }
     double *o_e = (want & GAPX_WANT_E) && d_e ? d_e : w.e.as<double>();
     double *o_v = (want & GAPX_WANT_V) && d_v ? d_v : w.vir.as<double>();
@@ -1499,7 +1533,7 @@ static int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     eg.var_valid = want_var;
     eg.var_P = P;
 
-    if (m->graphs && !tm.on && !ext && !opt) {
+    if (m->graphs && !tm.on && !ext && !opt && !noalloc) {
         // replay the captured sequence when nothing it depends on has changed
         std::vector<long long> key = {S, N, P, m->precision, (long long)want, (long long)(uintptr_t)o_e,
                                       (long long)(uintptr_t)o_v, (long long)(uintptr_t)o_f, (long long)(uintptr_t)o_le,

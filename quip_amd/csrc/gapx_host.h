@@ -51,14 +51,44 @@ inline long long alloc_generation(bool bump = false) {
     return bump ? ++gen : gen.load();
 }
 
+// A fixed block of device memory that buffers are carved from instead of hipMalloc
+// (the allocation-free entry point).  Active for the calling thread between
+// ArenaScope's construction and destruction.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+};
+inline Arena *&active_arena() {
+    static thread_local Arena *a = nullptr;
+    return a;
+}
+struct ArenaScope {
+    explicit ArenaScope(Arena *a) { active_arena() = a; }
+    ~ArenaScope() { active_arena() = nullptr; }
+};
+
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool owned = true;   // false: a view into an arena, never freed here
     int ensure(size_t bytes) {
         if (bytes <= cap) return GAPX_OK;
-        if (p) hipFree(p);
+        if (Arena *a = active_arena()) {   // carve, 256-byte aligned; nothing is allocated
+            const size_t want = (std::max<size_t>(bytes, 256) + 255) & ~(size_t)255;
+            if (a->used + want > a->size)
+                return fail(GAPX_E_CAPACITY, "the allocation-free entry point's device block (%zu bytes) is too small: "
+                            "%zu more bytes needed after %zu", a->size, want, a->used);
+            if (p && owned) hipFree(p);
+            p = a->base + a->used;
+            a->used += want;
+            cap = want;
+            owned = false;
+            return GAPX_OK;
+        }
+        if (p && owned) hipFree(p);
         p = nullptr;
         cap = 0;
+        owned = true;
         size_t want = std::max<size_t>(bytes + bytes / 4, 256);
         hipError_t e = hipMalloc(&p, want);
         if (e != hipSuccess) return fail(GAPX_E_DEVICE, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
@@ -66,8 +96,12 @@ struct DBuf {
         alloc_generation(true);
         return GAPX_OK;
     }
+    // forget an arena view (the arena is re-carved by the next call)
+    void drop_view() {
+        if (!owned) { p = nullptr; cap = 0; owned = true; }
+    }
     template <class T> T *as() const { return (T *)p; }
-    ~DBuf() { if (p) hipFree(p); }
+    ~DBuf() { if (p && owned) hipFree(p); }
 };
 
 // ---------------------------------------------------------------------------
@@ -149,12 +183,21 @@ struct ExtNL {
     std::vector<unsigned char> local;
 };
 
-// Pinned host staging (hipHostMalloc), grow-only.
+// Pinned host staging (hipHostMalloc), grow-only; `pageable` staging (ordinary host
+// memory, for the allocation-free entry point) when set before the first use.
 struct HBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool pageable = false;
+    std::vector<char> host;
     int ensure(size_t bytes) {
         if (bytes <= cap) return GAPX_OK;
+        if (pageable) {
+            host.resize(std::max<size_t>(bytes + bytes / 4, 4096));
+            p = host.data();
+            cap = host.size();
+            return GAPX_OK;
+        }
         if (p) hipHostFree(p);
         p = nullptr;
         cap = 0;
@@ -165,7 +208,7 @@ struct HBuf {
         return GAPX_OK;
     }
     template <class T> T *as() const { return (T *)p; }
-    ~HBuf() { if (p) hipHostFree(p); }
+    ~HBuf() { if (p && !pageable) hipHostFree(p); }
 };
 
 // Stage times from HIP events on an engine's stream.  Two event sets alternate
@@ -285,12 +328,22 @@ struct ModelState {
     bool graphs = false;
     bool nl_reuse = true;   // keep a neighbour list within its skin (calc_connect with cutoff_skin)
     std::unique_ptr<Engine> eng;   // the engine of the plain entry points
+    std::unique_ptr<Engine> eng_na;   // the allocation-free entry point's engine (pageable staging)
 };
 
 // engine entry points (gapx_api.hip)
 int new_engine(gapx_model *m, std::unique_ptr<Engine> &out);
 int ensure_split(gapx_model *m, Coord &c, int prec, hipStream_t s);
 int check_batch(int S, const int *atom_off, const int *Z, const double *lats, const int *pbcs);
+// Device-resident evaluation of S structures on engine eg (IPModel_GAP_Calc per structure).
+// noalloc: no autotuning, graphs or timing (nothing that allocates); device buffers come
+// from the caller's active arena.
+int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
+               const int *Z, const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
+               double *d_le, double *d_f, double *d_v, double *d_lv, const ExtNL *ext = nullptr,
+               const CalcOpts *opt = nullptr, bool noalloc = false);
+// Forget every arena view of a workspace and the state that refers to its buffers.
+void reset_arena_views(Engine &eg);
 // Host-buffer evaluation of S structures on engine eg (pinned staging both ways).
 // ext: a caller-supplied connection (one structure), else calc_connect.
 // opt (nullable): per-call options; their host outputs h_epc[n_coord], h_lgv[N], h_gvg[3N]
