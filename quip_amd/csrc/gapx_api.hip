@@ -766,6 +766,16 @@ static void host_minmax(const double *pos, int a0, int a1, const double *gi, dou
 }
 #pragma clang fp contract(on)
 
+// Pinned staging is reused call after call; in a speculative call nothing waits for the
+// stream, so staging written while an earlier upload from it may be queued waits first.
+static int staging_ready(Engine &eg) {
+    if (eg.staging_dirty) {
+        HIPCHK(hipStreamSynchronize(eg.stream));
+        eg.staging_dirty = false;
+    }
+    return GAPX_OK;
+}
+
 static bool geo_equal(const NLGeom &x, const NLGeom &y) { return memcmp(&x, &y, sizeof(NLGeom)) == 0; }
 
 static bool z_in(const std::vector<int> &v, int z) {
@@ -807,9 +817,10 @@ static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, c
     CHK(w.Z.ensure((size_t)N * 4 + 4));
     CHK(w.sid.ensure((size_t)N * 4 + 4));
     CHK(w.atom_off.ensure((size_t)(S + 1) * 4));
-    // the previous call's uploads finished before its neighbour-count synchronisation
+    CHK(staging_ready(eg));   // an earlier upload from the staging may still be queued
     CHK(eg.stage_in.ensure(tot + 64));
     char *h = eg.stage_in.as<char>();
+    eg.staging_dirty = true;
     size_t o = 0;
     auto put = [&](DBuf &dst, const void *src, size_t bytes) -> int {
         if (!bytes) return GAPX_OK;
@@ -836,13 +847,48 @@ static int stage_inputs(gapx_model *m, Engine &eg, int S, const int *atom_off, c
 
 // calc_connect for S structures at once (Connection.f95:1035-1310): per-structure
 // geometry, one set of kernels over all atoms; pairs never cross structures.
-// h_pos (nullable) lets the host compute the lat_eff extents of non-periodic axes.
-static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
-                    const double *lats, const int *pbcs, double cutoff, int *P_out) {
+//
+// Two ways to size the tables:
+//  - exact: the candidate search counts (or fills per-atom slots sized from the last
+//    build's largest counts), the neighbour1 total is read back with one host
+//    synchronisation, and the tables are sized to it;
+//  - speculative (spec): every table is sized from capacities kept from earlier builds
+//    (per-atom slots, the neighbour1 total), nothing is read back, and the whole build can
+//    sit in a captured graph.  A device guard empties every list when a slot or the
+//    total overflows, so the kernels after it stay inside their buffers; the flags are
+//    read back when the caller synchronises and an overflowed call is redone exactly.
+//    For structures with a non-periodic axis, lat_eff comes from a device kernel on this
+//    call's extents; the cell counts are the host's (from the previous call's extents),
+//    and a structure whose counts would change flags the call for redoing as well.
+static int ensure_nl_tables(Workspace &w, int T1cap);
+
+struct NLPlan {
+    int N = 0, S = 0, NC = 0;
+    bool spec = false, one_pass = false, any_np = false, dev_geo = false;
+    int cap1 = 0, cap2 = 0, capT1 = 0;
+};
+
+// the lattice-derived fields of two geometries (not lat_eff, which the device may refresh)
+static bool geo_equal_static(const NLGeom &x, const NLGeom &y) {
+    return !memcmp(x.lat, y.lat, sizeof(x.lat)) && !memcmp(x.ginv, y.ginv, sizeof(x.ginv)) &&
+           !memcmp(x.cells, y.cells, sizeof(x.cells)) && !memcmp(x.img, y.img, sizeof(x.img)) &&
+           !memcmp(x.pbc, y.pbc, sizeof(x.pbc)) && x.cutoff == y.cutoff;
+}
+
+// Host geometry, uploads and buffers of a build.  Extents of non-periodic axes: from
+// host positions when the caller has them, else from the device (exact: read back now;
+// speculative: the last read extents for the cell counts, and this call's own on the
+// device for lat_eff).
+static int nl_prepare(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
+                      const double *lats, const int *pbcs, double cutoff, bool spec, NLPlan &pl) {
+    (void)m;
     Workspace &w = eg.ws;
     hipStream_t s = eg.stream;
     const int N = atom_off[S];
     if (!(cutoff > 0)) return fail(GAPX_E_GEOMETRY, "calc_connect: cutoff must be positive (got %g)", cutoff);
+    pl = NLPlan();
+    pl.N = N;
+    pl.S = S;
     std::vector<double> mm;
     bool any_np = false;
     for (int st = 0; st < S; st++) {
@@ -852,6 +898,12 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
         const int *pbc = pbcs + 3 * st;
         if ((!pbc[0] || !pbc[1] || !pbc[2]) && atom_off[st + 1] > atom_off[st]) any_np = true;
     }
+    pl.any_np = any_np;
+    // a speculative build needs the slot and table capacities of an earlier build, and for
+    // non-periodic axes read from the device the last read extents of as many structures
+    if (spec && (eg.nl_cap1 <= 0 || eg.nl_cap2 <= 0 || eg.nl_capT1 <= 0)) spec = false;
+    if (spec && any_np && !h_pos && eg.h_mm.size() != 6 * (size_t)S) spec = false;
+    const bool dev_geo = spec && any_np && !h_pos;
     if (any_np) {
         mm.assign(6 * (size_t)S, 0.0);
         std::vector<double> gi(9 * (size_t)S);
@@ -862,14 +914,25 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
         } else {
             CHK(w.ginv.ensure(gi.size() * 8));
             CHK(w.minmax.ensure(mm.size() * 8));
-            CHK(eg.stage_geo.ensure(gi.size() * 8 + mm.size() * 8));
-            double *hg = eg.stage_geo.as<double>();
-            memcpy(hg, gi.data(), gi.size() * 8);
-            HIPCHK(hipMemcpyAsync(w.ginv.p, hg, gi.size() * 8, hipMemcpyHostToDevice, s));
-            launch_minmax_frac(S, w.atom_off.as<int>(), d_pos, w.ginv.as<double>(), w.minmax.as<double>(), s);
-            HIPCHK(hipMemcpyAsync(hg + gi.size(), w.minmax.p, mm.size() * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            memcpy(mm.data(), hg + gi.size(), mm.size() * 8);
+            if (eg.h_ginv != gi) {
+                CHK(staging_ready(eg));
+                CHK(eg.stage_geo.ensure(gi.size() * 8 + mm.size() * 8));
+                memcpy(eg.stage_geo.p, gi.data(), gi.size() * 8);
+                HIPCHK(hipMemcpyAsync(w.ginv.p, eg.stage_geo.p, gi.size() * 8, hipMemcpyHostToDevice, s));
+                eg.staging_dirty = true;
+                eg.h_ginv = gi;
+            }
+            if (dev_geo) {
+                mm = eg.h_mm;   // cell counts from the last read extents; lat_eff from this call's (device)
+            } else {
+                CHK(eg.ctl.ensure(mm.size() * 8 + 64));
+                launch_minmax_frac(S, w.atom_off.as<int>(), d_pos, w.ginv.as<double>(), w.minmax.as<double>(), s);
+                HIPCHK(hipMemcpyAsync(eg.ctl.p, w.minmax.p, mm.size() * 8, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                eg.staging_dirty = false;
+                memcpy(mm.data(), eg.ctl.p, mm.size() * 8);
+                eg.h_mm = mm;
+            }
         }
     }
     std::vector<NLGeom> geo(S);
@@ -886,18 +949,25 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     }
     cbase[S] = (int)ncell;
     const int NC = (int)ncell;
-    if (!(eg.have_geo && eg.h_geo.size() == geo.size() && eg.h_cell_base == cbase &&
-          std::equal(geo.begin(), geo.end(), eg.h_geo.begin(), geo_equal))) {
+    pl.NC = NC;
+    // With lat_eff refreshed on the device (dev_geo), only the lattice-derived fields decide
+    // whether the table is uploaded again; the last upload then came from an exact build or
+    // an earlier speculative one of the same structures.
+    const bool same = eg.have_geo && eg.h_geo.size() == geo.size() && eg.h_cell_base == cbase &&
+                      std::equal(geo.begin(), geo.end(), eg.h_geo.begin(), dev_geo ? geo_equal_static : geo_equal);
+    if (!same) {
         eg.have_geo = false;
         CHK(w.geo.ensure((size_t)S * sizeof(NLGeom) + 8));
         CHK(w.cell_base.ensure((size_t)(S + 1) * 4));
         const size_t gb = (size_t)S * sizeof(NLGeom), cb = (size_t)(S + 1) * 4;
-        CHK(eg.stage_geo.ensure(gb + cb + 64));   // the extents read-back above was synchronised
+        CHK(staging_ready(eg));
+        CHK(eg.stage_geo.ensure(gb + cb + 64));
         char *h = eg.stage_geo.as<char>();
         memcpy(h, geo.data(), gb);
         memcpy(h + gb, cbase.data(), cb);
         HIPCHK(hipMemcpyAsync(w.geo.p, h, gb, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(w.cell_base.p, h + gb, cb, hipMemcpyHostToDevice, s));
+        eg.staging_dirty = true;
         eg.h_geo = geo;
         eg.h_cell_base = cbase;
         eg.have_geo = true;
@@ -918,60 +988,34 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     CHK(w.n2_fill.ensure((size_t)N * 4 + 4));
     CHK(w.nb_off.ensure((size_t)(N + 1) * 4));
     CHK(w.cell_unsorted.ensure((size_t)N * 4 + 4));
-    const NLGeom *dgeo = w.geo.as<NLGeom>();
-    const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
-    // per-call counters start at zero
     CHK(w.nl_flags.ensure(64));
-    // [0] slot overflow, [1] max neighbour1 count, [2] max neighbour2 count, [3] neighbour1 total
-    int *flags = w.nl_flags.as<int>();
-    {
-        int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags};
-        const int zn[5] = {NC, NC, N, N, 4};
-        launch_zero_i32_multi(5, zp, zn, s);
-    }
-    launch_nl_bin(N, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(), w.cell_cnt.as<int>(), s);
-    launch_scan_i32(w.cell_cnt.as<int>(), NC, w.cell_start.as<int>(), w.scan_part.as<int>(), s);
-    launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
-                        w.cell_unsorted.as<int>(), s);
-    launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
-                        w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
     // The candidate search runs once, into per-atom slots sized from the largest counts
     // of the previous build; its counts are exact either way.  The first build counts
     // first, and a build whose slots overflowed fills the exact tables in a second
     // pass, as the first does.
-    const int cap1 = eg.nl_cap1, cap2 = eg.nl_cap2;
-    const bool one_pass = cap1 > 0 && cap2 > 0 && (size_t)N * std::max(cap1, cap2) < (size_t)INT32_MAX / 4;
-    if (one_pass) {
-        CHK(w.t1_j.ensure((size_t)N * cap1 * 4 + 4));
-        CHK(w.t1_shift.ensure((size_t)N * cap1 * 12 + 12));
-        CHK(w.t1_r.ensure((size_t)N * cap1 * 8 + 8));
-        CHK(w.t2_i.ensure((size_t)N * cap2 * 4 + 4));
-        CHK(w.t2_k.ensure((size_t)N * cap2 * 4 + 4));
-        launch_nl_pairs(2, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
-                        w.t1_j.as<int>(), w.t1_shift.as<int>(), w.t1_r.as<double>(), w.n2_cnt.as<int>(), nullptr,
-                        nullptr, w.t2_i.as<int>(), w.t2_k.as<int>(), cap1, cap2, flags, s);
-    } else {
-        launch_nl_pairs(0, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
-                        nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, 0, 0,
-                        nullptr, s);
+    pl.cap1 = eg.nl_cap1;
+    pl.cap2 = eg.nl_cap2;
+    pl.one_pass = pl.cap1 > 0 && pl.cap2 > 0 && (size_t)N * std::max(pl.cap1, pl.cap2) < (size_t)INT32_MAX / 4;
+    if (!pl.one_pass) spec = false;
+    pl.spec = spec;
+    pl.dev_geo = spec && dev_geo;
+    if (pl.one_pass) {
+        CHK(w.t1_j.ensure((size_t)N * pl.cap1 * 4 + 4));
+        CHK(w.t1_shift.ensure((size_t)N * pl.cap1 * 12 + 12));
+        CHK(w.t1_r.ensure((size_t)N * pl.cap1 * 8 + 8));
+        CHK(w.t2_i.ensure((size_t)N * pl.cap2 * 4 + 4));
+        CHK(w.t2_k.ensure((size_t)N * pl.cap2 * 4 + 4));
     }
-    launch_scan2_max(w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), N, w.n1_off.as<int>(), w.n2_off.as<int>(), flags + 1,
-                     flags + 3, w.scan_part.as<int>(), w.scan_part2.as<int>(), s);
-    CHK(eg.ctl.ensure(64));
-    int *hcount = eg.ctl.as<int>();
-    hcount[3] = -1;
-    HIPCHK(hipMemcpyAsync(hcount, flags, 16, hipMemcpyDeviceToHost, s));   // overflow, both maxima, pair total
-    HIPCHK(hipStreamSynchronize(s));
-    const int n1_total = hcount[3];
-    const bool overflow = hcount[0] != 0;
-    // next build's slots: the largest counts of this one plus a quarter and 8
-    eg.nl_cap1 = (hcount[1] + hcount[1] / 4 + 8 + 3) & ~3;
-    eg.nl_cap2 = (hcount[2] + hcount[2] / 4 + 8 + 3) & ~3;
-    if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);   // the previous call has finished
-    if (n1_total < 0 || n1_total > (INT32_MAX - 2) / 2) return fail(GAPX_E_GEOMETRY, "neighbour count overflow");
-    const size_t T1 = (size_t)n1_total + 1;
+    if (spec) {
+        pl.capT1 = eg.nl_capT1;
+        CHK(ensure_nl_tables(w, pl.capT1));
+    }
+    return GAPX_OK;
+}
+
+// The neighbour1/neighbour2 tables and the full view for up to T1 neighbour1 entries.
+static int ensure_nl_tables(Workspace &w, int T1cap) {
+    const size_t T1 = (size_t)T1cap + 1, P = 2 * (size_t)T1cap + 1;
     CHK(w.n1_j.ensure(T1 * 4));
     CHK(w.n1_shift.ensure(T1 * 12));
     CHK(w.n1_r.ensure(T1 * 8));
@@ -980,13 +1024,63 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
     CHK(w.n2_k.ensure(T1 * 4));
     CHK(w.n2_is.ensure(T1 * 4));
     CHK(w.n2_ks.ensure(T1 * 4));
-    const size_t P = 2 * (size_t)n1_total + 1;
     CHK(w.nb_j.ensure(P * 4));
     CHK(w.nb_rev.ensure(P * 4));
     CHK(w.nb_r.ensure(P * 8));
     CHK(w.nb_d.ensure(P * 24));
-    if (one_pass && !overflow) {
-        launch_nl_compact(N, cap1, cap2, w.n1_off.as<int>(), w.n2_off.as<int>(), w.t1_j.as<int>(),
+    return GAPX_OK;
+}
+
+// Binning, the candidate search and the count scans (flags: [0] overflow, [1] largest
+// neighbour1 count, [2] largest neighbour2 count, [3] neighbour1 total).
+static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    const int N = pl.N, NC = pl.NC;
+    const NLGeom *dgeo = w.geo.as<NLGeom>();
+    const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
+    int *flags = w.nl_flags.as<int>();
+    {
+        int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags};
+        const int zn[5] = {NC, NC, N, N, 4};
+        launch_zero_i32_multi(5, zp, zn, s);
+    }
+    if (pl.dev_geo) {
+        launch_minmax_frac(pl.S, w.atom_off.as<int>(), d_pos, w.ginv.as<double>(), w.minmax.as<double>(), s);
+        launch_geom_np(pl.S, w.atom_off.as<int>(), w.minmax.as<double>(), w.geo.as<NLGeom>(), flags, s);
+    }
+    launch_nl_bin(N, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(), w.cell_cnt.as<int>(), s);
+    launch_scan_i32(w.cell_cnt.as<int>(), NC, w.cell_start.as<int>(), w.scan_part.as<int>(), s);
+    launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
+                        w.cell_unsorted.as<int>(), s);
+    launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
+                        w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
+    if (pl.one_pass) {
+        launch_nl_pairs(2, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        w.t1_j.as<int>(), w.t1_shift.as<int>(), w.t1_r.as<double>(), w.n2_cnt.as<int>(), nullptr,
+                        nullptr, w.t2_i.as<int>(), w.t2_k.as<int>(), pl.cap1, pl.cap2, flags, s);
+    } else {
+        launch_nl_pairs(0, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, 0, 0,
+                        nullptr, s);
+    }
+    launch_scan2_max(w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), N, w.n1_off.as<int>(), w.n2_off.as<int>(), flags + 1,
+                     flags + 3, w.scan_part.as<int>(), w.scan_part2.as<int>(), s);
+    if (pl.spec) launch_nl_guard(N, flags, pl.capT1, w.n1_off.as<int>(), w.n2_off.as<int>(), s);
+}
+
+// The tables from the slots (or a second, filling pass), neighbour2 in the reference's
+// order and the full per-atom view.
+static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bool compact) {
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    const int N = pl.N;
+    const NLGeom *dgeo = w.geo.as<NLGeom>();
+    const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
+    if (compact) {
+        launch_nl_compact(N, pl.cap1, pl.cap2, w.n1_off.as<int>(), w.n2_off.as<int>(), w.t1_j.as<int>(),
                           w.t1_shift.as<int>(), w.t1_r.as<double>(), w.t2_i.as<int>(), w.t2_k.as<int>(),
                           w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), w.n2_i.as<int>(),
                           w.n2_k.as<int>(), s);
@@ -1002,8 +1096,40 @@ static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const
                    w.n1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(),
                    w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_rev.as<int>(), w.nb_r.as<double>(),
                    w.nb_d.as<double>(), s);
+}
+
+// next builds' capacities from a build's flags: per-atom slots of the largest counts plus a
+// quarter and 8, the neighbour1 tables of the total plus an eighth and 1024 (grow-only)
+static void update_nl_caps(Engine &eg, const int *flags) {
+    eg.nl_cap1 = (flags[1] + flags[1] / 4 + 8 + 3) & ~3;
+    eg.nl_cap2 = (flags[2] + flags[2] / 4 + 8 + 3) & ~3;
+    const long t = (long)flags[3] + flags[3] / 8 + 1024;
+    if (t < INT32_MAX / 4) eg.nl_capT1 = std::max(eg.nl_capT1, (int)t);
+}
+
+// The exact build: one host synchronisation for the neighbour1 total.
+static int build_nl(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
+                    const double *lats, const int *pbcs, double cutoff, int *P_out) {
+    NLPlan pl;
+    CHK(nl_prepare(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, cutoff, false, pl));
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    nl_launch_front(eg, pl, d_pos);
+    CHK(eg.ctl.ensure(64));
+    int *hcount = eg.ctl.as<int>();
+    hcount[3] = -1;
+    HIPCHK(hipMemcpyAsync(hcount, w.nl_flags.p, 16, hipMemcpyDeviceToHost, s));   // overflow, both maxima, total
+    HIPCHK(hipStreamSynchronize(s));
+    eg.staging_dirty = false;
+    const int n1_total = hcount[3];
+    const bool overflow = hcount[0] != 0;
+    update_nl_caps(eg, hcount);
+    if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);   // the previous call has finished
+    if (n1_total < 0 || n1_total > (INT32_MAX - 2) / 2) return fail(GAPX_E_GEOMETRY, "neighbour count overflow");
+    CHK(ensure_nl_tables(w, n1_total));
+    nl_launch_back(eg, pl, d_pos, pl.one_pass && !overflow);
     eg.n1_total_last = n1_total;
-    *P_out = (int)(P - 1);
+    *P_out = 2 * n1_total;
     return GAPX_OK;
 }
 
@@ -1032,6 +1158,7 @@ static int connect(gapx_model *m, Engine &eg, int S, const int *atom_off, const 
         CHK(eg.ctl.ensure(64));
         HIPCHK(hipMemcpyAsync(eg.ctl.p, w.skin_flags.p, 16, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        eg.staging_dirty = false;
         if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);   // the previous call has finished
         double maxd;
         memcpy(&maxd, eg.ctl.p, 8);
@@ -1313,6 +1440,8 @@ static int connect_external(Engine &eg, const ExtNL &x, int N, const double *d_p
     return GAPX_OK;
 }
 
+static int resolve_pending_list(gapx_model *m, Engine &eg, std::vector<Engine::Pending> *);
+
 int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const double *d_pos, const double *h_pos,
                      const int *Z, const double *lats, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
                      double *d_le, double *d_f, double *d_v, double *d_lv, const ExtNL *ext, const CalcOpts *opt,
@@ -1332,6 +1461,20 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     if (opt && opt->only_coord < 0) return fail(GAPX_E_ARG, "only_descriptor must be >= 0");
     const bool want_var = opt && opt->variance;
     const bool want_vgrad = want_var && opt->d_gvg;
+    // The speculative neighbour list: no read-back inside the call, so the whole call is
+    // one launch sequence (and graph).  Not for the first build, a skin (whose reuse test
+    // reads back), a supplied connection, calc options or the allocation-free entry.
+    const bool spec_ok = !ext && !opt && !noalloc && !eg.force_exact && !(rc > m->cutoff && m->nl_reuse);
+    // Calls queued speculatively are read (and redone if they overflowed) before anything
+    // of this call is staged or queued: before a call that is not speculative, and when
+    // the read-back area is full.
+    if (!eg.force_exact) {
+        const size_t need_max = (16 + 48 * (size_t)S + 15) & ~(size_t)15;
+        if (!spec_ok || eg.pending.size() >= 64 || eg.checks_used + need_max > eg.checks.cap) {
+            CHK(resolve_pending_list(m, eg, nullptr));
+            if (spec_ok) CHK(eg.checks.ensure(std::max<size_t>(need_max * 64, 4096)));
+        }
+    }
     if (tm.on) tm.begin(s);
     const unsigned char *hmask = ext ? ext->local.data() : (opt ? opt->mask : nullptr);
     CHK(stage_inputs(m, eg, S, atom_off, Z, hmask));
@@ -1339,11 +1482,35 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     // only_descriptor: with several coordinates, coordinate k alone (IPModel_GAP.f95:398-403)
     const size_t NC0 = m->coords.size();
     auto skipped = [&](size_t k) { return opt && opt->only_coord > 0 && NC0 > 1 && (int)k != opt->only_coord - 1; };
+    NLPlan pl;
+    if (spec_ok) CHK(nl_prepare(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, true, pl));
+    const bool spec = spec_ok && pl.spec;
     const int t_nl0 = tm.on ? tm.mark(s) : -1;
     int P = 0;
-    if (ext) CHK(connect_external(eg, *ext, N, d_pos, lats, &P));
+    if (spec) {
+        P = 2 * pl.capT1;   // a capacity: kernels that loop over pairs read their counts on the device
+        eg.nl_valid = false;
+        eg.last_rebuilt = true;
+    } else if (spec_ok) {   // prepared, but without capacities yet: the exact build from here
+        nl_launch_front(eg, pl, d_pos);
+        CHK(eg.ctl.ensure(64));
+        int *hc = eg.ctl.as<int>();
+        HIPCHK(hipMemcpyAsync(hc, w.nl_flags.p, 16, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        eg.staging_dirty = false;
+        update_nl_caps(eg, hc);
+        if (eg.timer.on) eg.timer.resolve(eg.timer.cur ^ 1);
+        const int n1_total = hc[3];
+        if (n1_total < 0 || n1_total > (INT32_MAX - 2) / 2) return fail(GAPX_E_GEOMETRY, "neighbour count overflow");
+        CHK(ensure_nl_tables(w, n1_total));
+        nl_launch_back(eg, pl, d_pos, pl.one_pass && hc[0] == 0);
+        eg.n1_total_last = n1_total;
+        eg.nl_valid = false;
+        eg.last_rebuilt = true;
+        P = 2 * n1_total;
+    } else if (ext) CHK(connect_external(eg, *ext, N, d_pos, lats, &P));
     else CHK(connect(m, eg, S, atom_off, d_pos, h_pos, lats, pbcs, rc, &P));
-    if (tm.on) tm.stage(0, t_nl0, tm.mark(s));
+    if (tm.on && !spec) tm.stage(0, t_nl0, tm.mark(s));
 
     // Prepare: every buffer the launches use is sized here (to the largest
     // coordinate), tiles are chosen and split copies made, so that the launch
@@ -1424,7 +1591,17 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     double *pairE = has_d2b ? w.pairE.as<double>() : nullptr;
     double *d_epc = opt ? opt->d_epc : nullptr;
     auto launch_all = [&](bool marks) {
-        {
+        if (spec) {   // the neighbour list joins the sequence; pair arrays are cleared to its count
+            nl_launch_front(eg, pl, d_pos);
+            nl_launch_back(eg, pl, d_pos, true);
+            if (marks) tm.stage(0, t_nl0, tm.mark(s));
+            const int *cnt = w.nb_off.as<int>() + N;
+            launch_zero_counted(w.pairF.as<double>(), cnt, 3, (size_t)P * 3, s);
+            if (pairE) launch_zero_counted(pairE, cnt, 1, (size_t)P, s);
+            double *zp[1] = {w.esite.as<double>()};
+            const size_t zn[1] = {(size_t)N};
+            launch_zero_f64_multi(1, zp, zn, s);
+        } else {
             double *zp[3] = {w.pairF.as<double>(), w.esite.as<double>(), pairE};
             const size_t zn[3] = {(size_t)P * 3, (size_t)N, (size_t)P};
             launch_zero_f64_multi(has_d2b ? 3 : 2, zp, zn, s);
@@ -1537,7 +1714,9 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         // replay the captured sequence when nothing it depends on has changed
         std::vector<long long> key = {S, N, P, m->precision, (long long)want, (long long)(uintptr_t)o_e,
                                       (long long)(uintptr_t)o_v, (long long)(uintptr_t)o_f, (long long)(uintptr_t)o_le,
-                                      (long long)(uintptr_t)o_lv, alloc_generation()};
+                                      (long long)(uintptr_t)o_lv, alloc_generation(), spec ? 1 : 0};
+        if (spec)
+            key.insert(key.end(), {(long long)(uintptr_t)d_pos, pl.NC, pl.cap1, pl.cap2, pl.capT1, pl.dev_geo ? 1 : 0});
         long long es;
         memcpy(&es, &m->E_scale, 8);
         key.push_back(es);
@@ -1567,9 +1746,66 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         launch_all(tm.on);
     }
     HIPCHK(hipGetLastError());
+    if (spec) {   // the flags (and this call's extents) are read when the caller synchronises
+        const size_t need = 16 + (pl.dev_geo ? 48 * (size_t)S : 0);
+        Engine::Pending pc;
+        pc.S = S;
+        pc.off.assign(atom_off, atom_off + S + 1);
+        pc.Z.assign(Z, Z + N);
+        pc.lats.assign(lats, lats + 9 * (size_t)S);
+        pc.pbcs.assign(pbcs, pbcs + 3 * (size_t)S);
+        pc.d_pos = d_pos;
+        pc.nl_cutoff = nl_cutoff;
+        pc.want = want;
+        pc.d_e = d_e; pc.d_le = d_le; pc.d_f = d_f; pc.d_v = d_v; pc.d_lv = d_lv;
+        pc.slot = eg.checks_used;
+        pc.extents = pl.dev_geo;
+        char *slot = eg.checks.as<char>() + pc.slot;
+        HIPCHK(hipMemcpyAsync(slot, w.nl_flags.p, 16, hipMemcpyDeviceToHost, s));
+        if (pl.dev_geo) HIPCHK(hipMemcpyAsync(slot + 16, w.minmax.p, 48 * (size_t)S, hipMemcpyDeviceToHost, s));
+        eg.checks_used += (need + 15) & ~(size_t)15;
+        eg.pending.push_back(std::move(pc));
+    }
     if (tm.on) tm.end();
     return GAPX_OK;
 }
+
+// Read the pending calls' flags (after one synchronisation), refresh the capacities and
+// extents from them, and redo in order every call from the first that overflowed.
+static int resolve_pending_list(gapx_model *m, Engine &eg, std::vector<Engine::Pending> *) {
+    if (eg.pending.empty()) return GAPX_OK;
+    HIPCHK(hipStreamSynchronize(eg.stream));
+    eg.staging_dirty = false;
+    std::vector<Engine::Pending> todo;
+    todo.swap(eg.pending);
+    int first_bad = -1;
+    for (size_t k = 0; k < todo.size(); k++) {
+        const char *slot = eg.checks.as<char>() + todo[k].slot;
+        int fl[4];
+        memcpy(fl, slot, 16);
+        update_nl_caps(eg, fl);
+        if (todo[k].extents) {
+            eg.h_mm.resize(6 * (size_t)todo[k].S);
+            memcpy(eg.h_mm.data(), slot + 16, 48 * (size_t)todo[k].S);
+        }
+        if (fl[0] != 0 && first_bad < 0) first_bad = (int)k;
+    }
+    eg.checks_used = 0;
+    if (first_bad < 0) return GAPX_OK;
+    eg.force_exact = true;
+    int st = GAPX_OK;
+    for (size_t k = first_bad; k < todo.size() && st == GAPX_OK; k++) {
+        const Engine::Pending &p = todo[k];
+        st = eval_batch(m, eg, p.S, p.off.data(), p.d_pos, nullptr, p.Z.data(), p.lats.data(), p.pbcs.data(),
+                        p.nl_cutoff, p.want, p.d_e, p.d_le, p.d_f, p.d_v, p.d_lv);
+    }
+    eg.force_exact = false;
+    CHK(st);
+    HIPCHK(hipStreamSynchronize(eg.stream));
+    return GAPX_OK;
+}
+
+int gapx::resolve_pending(gapx_model *m, Engine &eg) { return resolve_pending_list(m, eg, nullptr); }
 
 extern "C" int gapx_calc_device(gapx_model *m, int N, const double *d_pos, const int *Z, const double *lat,
                                 const int *pbc, double nl_cutoff, unsigned want, double *d_e, double *d_le,
@@ -1592,6 +1828,7 @@ extern "C" int gapx_calc_batch_device(gapx_model *m, int n_struct, const int *at
 extern "C" int gapx_sync(gapx_model *m) {
     if (!m) return fail(GAPX_E_ARG, "NULL model");
     HIPCHK(hipSetDevice(m->device));
+    CHK(resolve_pending(m, *m->eng));
     HIPCHK(hipStreamSynchronize(m->eng->stream));
     return GAPX_OK;
 }
@@ -1639,6 +1876,7 @@ int gapx::calc_host(gapx_model *m, Engine &eg, int S, const int *atom_off, const
                    want | GAPX_WANT_E | GAPX_WANT_V | GAPX_WANT_F | GAPX_WANT_LOCAL_E | GAPX_WANT_LOCAL_V,
                    w.e.as<double>(), w.le.as<double>(), w.f.as<double>(), w.vir.as<double>(), w.lv.as<double>(),
                    ext, opt ? &o2 : nullptr));
+    CHK(resolve_pending(m, eg));   // a speculative call that overflowed is redone before its results are read
     char *o = h;
     auto get = [&](bool on, const DBuf &src, size_t bytes, char **dst) -> int {
         *dst = nullptr;
@@ -1955,6 +2193,7 @@ extern "C" int gapx_descriptors(gapx_model *m, int coord, int N, const double *p
     if (c.kind != 1) return fail(GAPX_E_UNSUPPORTED, "gapx_descriptors: coordinate %d is not soap", coord);
     HIPCHK(hipSetDevice(m->device));
     Engine &eg = *m->eng;
+    CHK(resolve_pending(m, eg));
     Workspace &w = eg.ws;
     hipStream_t s = eg.stream;
     const int off[2] = {0, N};

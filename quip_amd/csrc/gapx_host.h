@@ -306,6 +306,27 @@ struct Engine {
     // the last calc that formed GAP variances: its pair count (instance buffers are per coordinate)
     bool var_valid = false;
     int var_P = 0;
+    // Speculative neighbour lists (no read-back inside a call): the capacity of the
+    // neighbour1 tables, the inverse lattices and the last read extents of non-periodic
+    // axes, and the calls whose overflow flags are not read yet, with what redoes them.
+    int nl_capT1 = 0;
+    bool force_exact = false;        // redoing calls: count first
+    bool staging_dirty = false;      // an upload from pinned staging may still be queued
+    std::vector<double> h_mm, h_ginv;
+    struct Pending {
+        int S = 0;
+        std::vector<int> off, Z, pbcs;
+        std::vector<double> lats;
+        const double *d_pos = nullptr;
+        double nl_cutoff = 0;
+        unsigned want = 0;
+        double *d_e = nullptr, *d_le = nullptr, *d_f = nullptr, *d_v = nullptr, *d_lv = nullptr;
+        size_t slot = 0;             // byte offset of its flags (+ extents) in `checks`
+        bool extents = false;
+    };
+    std::vector<Pending> pending;
+    HBuf checks;                     // pinned read-back slots of the pending calls
+    size_t checks_used = 0;
     ~Engine() {
         if (gexec) hipGraphExecDestroy(gexec);
         if (stream) {
@@ -344,6 +365,9 @@ int eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, const doub
                const CalcOpts *opt = nullptr, bool noalloc = false);
 // Forget every arena view of a workspace and the state that refers to its buffers.
 void reset_arena_views(Engine &eg);
+// Read the overflow flags of the engine's speculative calls (one synchronisation) and
+// redo, exactly and in order, every call from the first that overflowed.
+int resolve_pending(gapx_model *m, Engine &eg);
 // Host-buffer evaluation of S structures on engine eg (pinned staging both ways).
 // ext: a caller-supplied connection (one structure), else calc_connect.
 // opt (nullable): per-call options; their host outputs h_epc[n_coord], h_lgv[N], h_gvg[3N]

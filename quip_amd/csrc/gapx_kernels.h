@@ -67,6 +67,13 @@ void launch_nl_skin_check(int N, const double *pos, const double *last, const NL
                           const int *n1_off, const int *n1_j, const int *n1_shift, double *n1_r,
                           unsigned long long *max_disp, int *overlap, hipStream_t s);
 void launch_minmax_frac(int S, const int *atom_off, const double *pos, const double *g, double *out, hipStream_t s);
+// lat_eff of structures with a non-periodic axis from this call's extents mm (6 per
+// structure) into geo; flags[0] |= 4 when a structure's cell counts differ from geo's
+void launch_geom_np(int S, const int *atom_off, const double *mm, NLGeom *geo, int *flags, hipStream_t s);
+// empty every list (offsets zeroed) after a slot or capacity overflow; flags[0] |= 2 for capacity
+void launch_nl_guard(int N, int *flags, int cap, int *n1_off, int *n2_off, hipStream_t s);
+// p[0 .. mult * (*cnt)) = 0 with *cnt read on the device; cap bounds the grid
+void launch_zero_counted(double *p, const int *cnt, int mult, size_t cap, hipStream_t s);
 
 // upload the constant tables of the SOAP kernels to the current device (once per
 // device; done at model load so that no launch sequence contains an upload)

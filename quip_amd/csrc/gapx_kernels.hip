@@ -812,6 +812,94 @@ void launch_minmax_frac(int S, const int *atom_off, const double *pos, const dou
     if (S > 0) hipLaunchKernelGGL(k_minmax_frac, dim3(S), dim3(256), 0, s, atom_off, pos, g, out);
 }
 
+// calc_lat_eff (Connection.f95:740-763) and divide_cell (:1674-1691) on the device for
+// structures with a non-periodic axis, in the host's operation order (no contraction):
+// the fractional extents of this call give lat_eff, its inverse and the offsets in
+// geo[st]; the cell counts were fixed on the host from an earlier call's extents, and a
+// structure whose counts would differ sets flags[0] |= 4 (the call is then redone).
+__device__ __forceinline__ void cross3_d(const double *x, const double *y, double *o) {
+#pragma clang fp contract(off)
+    o[0] = +(x[1] * y[2] - x[2] * y[1]);
+    o[1] = -(x[0] * y[2] - x[2] * y[0]);
+    o[2] = +(x[0] * y[1] - x[1] * y[0]);
+}
+__device__ __forceinline__ double triple3_d(const double *x, const double *y, const double *z) {
+#pragma clang fp contract(off)
+    return +x[0] * (y[1] * z[2] - y[2] * z[1]) - x[1] * (y[0] * z[2] - y[2] * z[0]) + x[2] * (y[0] * z[1] - y[1] * z[0]);
+}
+__device__ __forceinline__ double norm3_d(const double *v) {
+#pragma clang fp contract(off)
+    return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+__global__ void k_geom_np(int S, const int *atom_off, const double *mm, NLGeom *geo, int *flags) {
+#pragma clang fp contract(off)
+    const int st = blockIdx.x * blockDim.x + threadIdx.x;
+    if (st >= S) return;
+    NLGeom &g = geo[st];
+    if (g.pbc[0] && g.pbc[1] && g.pbc[2]) return;
+    if (atom_off[st + 1] <= atom_off[st]) return;
+    double eff[9], off[3];
+    for (int k = 0; k < 9; k++) eff[k] = g.lat[k];
+    for (int a = 0; a < 3; a++) off[a] = 0.0;
+    for (int i = 0; i < 3; i++) {
+        if (g.pbc[i]) continue;
+        const double mn = mm[6 * st + i], mx = mm[6 * st + 3 + i];
+        if (mn <= 0.5 || mx >= 0.5) {
+            const double ext = (mx - mn) * 1.1, nv = norm3_d(eff + 3 * i);
+            for (int k = 0; k < 3; k++) eff[3 * i + k] = eff[3 * i + k] / nv * ext;
+            off[i] = mn - 0.05;
+        }
+    }
+    const double stp = triple3_d(eff, eff + 3, eff + 6);
+    double inv[9], c[3];
+    for (int r = 0; r < 3; r++) {
+        cross3_d(eff + 3 * ((r + 1) % 3), eff + 3 * ((r + 2) % 3), c);
+        for (int k = 0; k < 3; k++) inv[3 * k + r] = c[k] / stp;
+    }
+    const double vol = fabs(stp);
+    bool same = true;
+    for (int a = 0; a < 3; a++) {
+        cross3_d(eff + 3 * ((a + 1) % 3), eff + 3 * ((a + 2) % 3), c);
+        const int n = max(1, (int)(vol / (g.cutoff * norm3_d(c))));
+        same = same && n == g.cells[a];
+    }
+    for (int k = 0; k < 9; k++) g.inv_eff[k] = inv[k];
+    for (int a = 0; a < 3; a++) g.offset[a] = off[a];
+    if (!same) atomicOr(flags, 4);
+}
+void launch_geom_np(int S, const int *atom_off, const double *mm, NLGeom *geo, int *flags, hipStream_t s) {
+    if (S > 0) hipLaunchKernelGGL(k_geom_np, dim3(cdiv(S, 64)), dim3(64), 0, s, S, atom_off, mm, geo, flags);
+}
+
+// Overflow guard of a neighbour list built into capacity-sized tables without a host
+// read-back: when a per-atom slot overflowed (flags[0] != 0) or the neighbour1 total
+// flags[3] exceeds the tables' capacity cap, every offset is set to zero, so the
+// kernels after it see empty lists and stay inside their buffers; flags[0] gets bit 2
+// for a capacity overflow.  The host redoes such a call.
+__global__ void k_nl_guard(int N, int *flags, int cap, int *n1_off, int *n2_off) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tot = flags[3];
+    const bool over = tot > cap;
+    if (!(over || flags[0] != 0) || i > N) return;
+    n1_off[i] = 0;
+    n2_off[i] = 0;
+    if (i == 0 && over) atomicOr(flags, 2);
+}
+void launch_nl_guard(int N, int *flags, int cap, int *n1_off, int *n2_off, hipStream_t s) {
+    hipLaunchKernelGGL(k_nl_guard, dim3(cdiv(N + 1, 256)), dim3(256), 0, s, N, flags, cap, n1_off, n2_off);
+}
+
+// zero the first mult * count doubles of p, count = *cnt read on the device
+__global__ void k_zero_counted(double *p, const int *cnt, int mult) {
+    const size_t n = (size_t)(*cnt) * mult;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        p[q] = 0.0;
+}
+void launch_zero_counted(double *p, const int *cnt, int mult, size_t cap, hipStream_t s) {
+    const int blocks = std::max(1, std::min(cdiv((long)cap, 256), 2048));
+    hipLaunchKernelGGL(k_zero_counted, dim3(blocks), dim3(256), 0, s, p, cnt, mult);
+}
+
 // ----------------------------------------------------------------------------
 // SOAP
 // ----------------------------------------------------------------------------
