@@ -254,6 +254,18 @@ int gapx_descriptors(gapx_model *model, int coord, int n_atoms, const double *po
  * count) with the measured milliseconds of one launch. */
 int gapx_model_set_autotune(gapx_model *model, int on);
 
+/* Split-form autotuner (on with gapx_model_set_autotune).  No counterpart in the reference.
+ * For each gpCoordinate, split precision and centre-count bucket the first evaluation
+ * times the fused GP kernel (descriptors up to 256 components) with 1, 2, 4 and 8
+ * sparse-point parts, or the split GEMMs (larger descriptors) with 64 x 64 and 128 x 128
+ * LDS tiles, and keeps the fastest.  gapx_model_tuning_split lists the choices:
+ * fused_parts (0 where the GEMMs are used), gemm1_lds / gemm2_lds (-1 where the fused
+ * kernel is used) and the measured milliseconds of one launch (fused kernel with its
+ * partial sums, or GEMM 1; GEMM 2). */
+int gapx_model_tuning_split(gapx_model *model, int coord, int max_entries, int *precision, int *bucket,
+                            int *fused_parts, int *gemm1_lds, int *gemm2_lds, double *fused_or_gemm1_ms,
+                            double *gemm2_ms, int *n_entries);
+
 /* Replay of the launch sequence after the neighbour count as a HIP graph (off by
  * default).  The sequence is captured on the first call and re-captured when a
  * size, a buffer, an output pointer or E_scale changes; calls with stage timing

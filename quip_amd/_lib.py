@@ -74,6 +74,7 @@ _SIGS = [
     ("gapx_model_set_nl_reuse", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_last_nl_rebuilt", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("gapx_model_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.POINTER(C.c_int)]),
+    ("gapx_model_tuning_split", C.c_int, [C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 7 + [C.POINTER(C.c_int)]),
     ("gapx_model_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_last_timings", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gapx_timing_totals", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -1295,6 +1295,131 @@ static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out,
     return GAPX_OK;
 }
 
+// Split-form autotuner, as gemm_tune for the FP64 tiles: per coordinate, precision and
+// centre-count bucket the first evaluation times the fused kernel (with its partial-G sum)
+// for 1, 2, 4 and 8 sparse-point parts, or the split GEMMs with 64 x 64 and 128 x 128
+// tiles, on the call's own operands, and keeps the fastest (one warm-up, best of three).
+static int fused_part_candidates(const Coord &c, int nc, std::vector<int> &out) {
+    out.clear();
+    const int Mpad = (c.M + 31) / 32 * 32;
+    for (int p = 1; p <= 8; p *= 2) {
+        const int v = gp_fused_parts_valid(p, Mpad);
+        if ((v == 1 || (size_t)v * nc * c.dim * 4 <= kTuneScratchCap) &&
+            std::find(out.begin(), out.end(), v) == out.end())
+            out.push_back(v);
+    }
+    return GAPX_OK;
+}
+
+static int mixed_tune(gapx_model *m, Engine &eg, Coord &c, int nc, bool fused, GemmTune &out, bool cached_only) {
+    out = GemmTune();
+    if (!m->autotune || nc <= 0) return GAPX_OK;
+    int bucket = 0;
+    while ((1 << bucket) < nc) bucket++;
+    const int key = (m->precision << 8) | bucket;
+    {
+        std::lock_guard<std::mutex> lk(c.tune_mu);
+        auto it = c.mtune.find(key);
+        if (it != c.mtune.end()) {
+            out = it->second;
+            return GAPX_OK;
+        }
+    }
+    if (cached_only) return GAPX_OK;
+    Workspace &w = eg.ws;
+    hipStream_t s = eg.stream;
+    hipEvent_t ea, eb;
+    HIPCHK(hipEventCreate(&ea));
+    if (hipEventCreate(&eb) != hipSuccess) {
+        hipEventDestroy(ea);
+        return fail(GAPX_E_DEVICE, "hipEventCreate failed");
+    }
+    auto timed = [&](auto launch, float *ms) -> int {
+        launch();
+        *ms = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            hipEventRecord(ea, s);
+            launch();
+            hipEventRecord(eb, s);
+            HIPCHK(hipEventSynchronize(eb));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ea, eb));
+            *ms = std::min(*ms, t);
+        }
+        return GAPX_OK;
+    };
+    GemmTune best;
+    best.g1_ms = best.g2_ms = 1e30f;
+    int st = GAPX_OK;
+    if (fused) {
+        std::vector<int> cand;
+        fused_part_candidates(c, nc, cand);
+        for (int parts : cand) {
+            float t = 0;
+            int nt = 0;
+            st = timed([&] {
+                launch_gp_fused_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, parts, w.X.as<double>(),
+                                      c.d_Xs_hi.p, c.d_Xs_lo.p, c.d_XsT_hi.p, c.d_XsT_lo.p, c.d_w.as<double>(),
+                                      c.zeta, c.wscale, w.G.as<double>(), w.Gparts.as<float>(),
+                                      w.e_part.as<double>(), &nt, s);
+                launch_gp_fused_reduce(nc, c.dim, nt, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
+            }, &t);
+            if (st != GAPX_OK) break;
+            if (t < best.g1_ms) { best.g1_ms = t; best.fparts = parts; }
+        }
+        best.g2_ms = 0;
+    } else {
+        for (int lds = 0; lds <= 1 && st == GAPX_OK; lds++) {
+            float t = 0;
+            int nt = 0;
+            st = timed([&] { launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, lds, w.X.as<double>(),
+                                                c.d_Xs_hi.p, c.d_Xs_lo.p, c.d_w.as<double>(), c.zeta, c.wscale,
+                                                w.Wf.as<float>(), w.e_part.as<double>(), &nt, w.split_ws.p, s); }, &t);
+            if (st == GAPX_OK && t < best.g1_ms) { best.g1_ms = t; best.g1lds = lds; }
+        }
+        for (int lds = 0; lds <= 1 && st == GAPX_OK; lds++) {
+            float t = 0;
+            st = timed([&] { launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, lds, w.Wf.as<float>(),
+                                                c.d_XsT_hi.p, c.d_XsT_lo.p, c.wscale, w.G.as<double>(), s); }, &t);
+            if (st == GAPX_OK && t < best.g2_ms) { best.g2_ms = t; best.g2lds = lds; }
+        }
+    }
+    hipEventDestroy(ea);
+    hipEventDestroy(eb);
+    CHK(st);
+    HIPCHK(hipGetLastError());
+    {
+        std::lock_guard<std::mutex> lk(c.tune_mu);
+        c.mtune[key] = best;
+    }
+    out = best;
+    return GAPX_OK;
+}
+
+extern "C" int gapx_model_tuning_split(gapx_model *m, int coord, int max_entries, int *precision, int *bucket,
+                                       int *fused_parts, int *gemm1_lds, int *gemm2_lds, double *fused_or_gemm1_ms,
+                                       double *gemm2_ms, int *n_entries) {
+    if (!m || !n_entries) return fail(GAPX_E_ARG, "NULL argument");
+    if (coord < 0 || coord >= (int)m->coords.size()) return fail(GAPX_E_ARG, "coordinate %d out of range", coord);
+    Coord &c = *m->coords[coord];
+    std::lock_guard<std::mutex> lk(c.tune_mu);
+    int k = 0;
+    for (auto &e : c.mtune) {
+        if (k < max_entries) {
+            if (precision) precision[k] = e.first >> 8;
+            if (bucket) bucket[k] = e.first & 255;
+            if (fused_parts) fused_parts[k] = e.second.fparts;
+            if (gemm1_lds) gemm1_lds[k] = e.second.g1lds;
+            if (gemm2_lds) gemm2_lds[k] = e.second.g2lds;
+            if (fused_or_gemm1_ms) fused_or_gemm1_ms[k] = e.second.g1_ms;
+            if (gemm2_ms) gemm2_ms[k] = e.second.g2_ms;
+        }
+        k++;
+    }
+    *n_entries = k;
+    return GAPX_OK;
+}
+
 // GAP variance set-up (gpCoordinates_initialise_variance_estimate's role, IPModel_GAP.f95:412-414):
 // Cinv = (K_MM + reg I)^-1 with K_MM[s][t] = delta^2 kernel(X_s, X_t), formed once per
 // coordinate and regularisation on the host (Cholesky, triangular inverse, Linv^T Linv;
@@ -1547,7 +1672,13 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
             mW = std::max(mW, (size_t)nc * c.M * 8);
             mGp = std::max(mGp, (size_t)8 * nc * c.dim * 8);
         } else if (fused[k]) {
-            mGp = std::max(mGp, (size_t)gp_fused_parts(nc, Mpad) * nc * c.dim * 4);
+            size_t parts = gp_fused_parts(nc, Mpad);
+            if (m->autotune && !noalloc) {   // the tuner's candidates
+                std::vector<int> cand;
+                fused_part_candidates(c, nc, cand);
+                for (int p : cand) parts = std::max(parts, (size_t)p);
+            }
+            mGp = std::max(mGp, parts * nc * c.dim * 4);
         } else {
             mWf = std::max(mWf, (size_t)nc * Mpad * 4);
             mSw = std::max(mSw, (size_t)(nc + 63) / 64 * 64 * dpad * 4);
@@ -1573,9 +1704,13 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         const int nc = (int)eg.h_centres[k].size();
         if (c.kind != 1 || nc == 0 || skipped(k)) continue;
         if (m->precision == GAPX_PREC_FP64) CHK(gemm_tune(m, eg, c, nc, tunes[k], noalloc));
-        else if (c.split_prec != m->precision) {
-            if (noalloc) return fail(GAPX_E_ARG, "split copies of the sparse points are missing (set the precision first)");
-            CHK(ensure_split(m, c, m->precision, s));
+        else {
+            if (c.split_prec != m->precision) {
+                if (noalloc)
+                    return fail(GAPX_E_ARG, "split copies of the sparse points are missing (set the precision first)");
+                CHK(ensure_split(m, c, m->precision, s));
+            }
+            CHK(mixed_tune(m, eg, c, nc, fused[k], tunes[k], noalloc));
         }
     }
     double *o_e = (want & GAPX_WANT_E) && d_e ? d_e : w.e.as<double>();
@@ -1650,11 +1785,13 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                 launch_gemm1_f64(tu.g1, nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(), c.d_w.as<double>(),
                                  c.zeta, w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
             } else if (fused[k]) {   // split forms: K -> W -> G in one kernel
-                launch_gp_fused_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.X.as<double>(), c.d_Xs_hi.p,
+                launch_gp_fused_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, tu.fparts, w.X.as<double>(),
+                                      c.d_Xs_hi.p,
                                       c.d_Xs_lo.p, c.d_XsT_hi.p, c.d_XsT_lo.p, c.d_w.as<double>(), c.zeta, c.wscale,
                                       w.G.as<double>(), w.Gparts.as<float>(), w.e_part.as<double>(), &ntile, s);
             } else {
-                launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, w.X.as<double>(), c.d_Xs_hi.p, c.d_Xs_lo.p,
+                launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, tu.g1lds, w.X.as<double>(), c.d_Xs_hi.p,
+                                   c.d_Xs_lo.p,
                                    c.d_w.as<double>(), c.zeta, c.wscale, w.Wf.as<float>(), w.e_part.as<double>(),
                                    &ntile, w.split_ws.p, s);
             }
@@ -1686,7 +1823,8 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                 } else if (fused[k]) {
                     launch_gp_fused_reduce(nc, c.dim, ntile, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
                 } else {
-                    launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, w.Wf.as<float>(), c.d_XsT_hi.p,
+                    launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, tu.g2lds, w.Wf.as<float>(),
+                                       c.d_XsT_hi.p,
                                        c.d_XsT_lo.p, c.wscale, w.G.as<double>(), s);
                 }
                 const int t3 = marks ? tm.mark(s) : -1;
@@ -1721,7 +1859,8 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         memcpy(&es, &m->E_scale, 8);
         key.push_back(es);
         for (size_t k = 0; k < NCOORD; k++)
-            key.insert(key.end(), {(long long)eg.h_centres[k].size(), tunes[k].g1, tunes[k].g2, tunes[k].parts});
+            key.insert(key.end(), {(long long)eg.h_centres[k].size(), tunes[k].g1, tunes[k].g2, tunes[k].parts,
+                                   tunes[k].fparts, tunes[k].g1lds, tunes[k].g2lds});
         if (!eg.gexec || key != eg.gkey) {
             if (eg.gexec) {
                 hipGraphExecDestroy(eg.gexec);

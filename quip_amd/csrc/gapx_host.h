@@ -110,7 +110,11 @@ struct DBuf {
 struct GemmTune {
     int g1 = 0, g2 = 1, parts = 1;   // GEMM 1 tile, GEMM 2 tile, GEMM 2 split-K factor
     float g1_ms = 0, g2_ms = 0;      // measured times of the chosen configurations
+    // split forms: sparse-point parts of the fused kernel (0 = its default rule), and the
+    // split GEMMs' tiles (-1 = the default rule, 0 = 64 x 64, 1 = 128 x 128 LDS tiles)
+    int fparts = 0, g1lds = -1, g2lds = -1;
 };
+constexpr size_t kTuneScratchCap = (size_t)256 << 20;   // largest partial-G buffer a tuning candidate may use
 
 struct Coord {
     int kind = 0;          // 1 soap, 2 distance_2b
@@ -139,6 +143,7 @@ struct Coord {
     // FP64 GEMM tile choices per centre-count bucket (ceil log2 nc), from the autotuner
     std::mutex tune_mu;
     std::map<int, GemmTune> tune;
+    std::map<int, GemmTune> mtune;   // split forms: key (precision << 8) | bucket
     // GAP variance: (K_MM + reg I)^-1 (M x M, FP64) for the regularisation last asked for
     std::mutex var_mu;
     double var_reg = -1.0;

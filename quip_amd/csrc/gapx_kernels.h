@@ -112,20 +112,25 @@ void launch_gemm2_f64(int cfg, int parts, int nc, int M, int d, const double *W,
                       double *Gparts, hipStream_t s);
 // mixed precision forms: Xs pre-split into hi/lo (bf16 or fp16) copies, row- and column-major
 // W is scaled by wscale (a power of two) before its hi/lo split; GEMM2 divides it out.
-void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *X, const void *Xs_hi,
+// lds_mode: -1 the 128 x 128 LDS tiles once they fill the GPU, else 64 x 64; 0 / 1 force
+// the 64 x 64 / 128 x 128 tiles (an element's MFMA passes run in the same order in both)
+void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, int lds_mode, const double *X, const void *Xs_hi,
                         const void *Xs_lo, const double *wvec, double zeta, double wscale, float *Wf,
                         double *e_part, int *ntile_out, void *xsplit_ws, hipStream_t s);
-void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, const float *Wf, const void *XsT_hi,
-                        const void *XsT_lo, double wscale, double *G, hipStream_t s);
+void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, int lds_mode, const float *Wf,
+                        const void *XsT_hi, const void *XsT_lo, double wscale, double *G, hipStream_t s);
 // fused K -> epilogue -> G for descriptors of up to 256 components (split forms):
 // one launch computes the energy partials e_part[i*parts + p] and, with parts > 1,
 // FP32 partial G tiles (parts x nc x d in Gpart) that launch_gp_fused_reduce sums;
 // with parts == 1 it writes G directly.  *ntile_out = parts.
 bool gp_fused_supported(int dpad);
-int gp_fused_parts(int nc, int Mpad);
-void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, const double *X, const void *Xs_hi,
-                           const void *Xs_lo, const void *XsT_hi, const void *XsT_lo, const double *wvec, double zeta,
-                           double wscale, double *G, float *Gpart, double *e_part, int *ntile_out, hipStream_t s);
+int gp_fused_parts(int nc, int Mpad);              // the default number of sparse-point parts
+int gp_fused_parts_valid(int parts, int Mpad);     // a requested number, as the launch rounds it
+// parts_req > 0 asks for that many parts (rounded so that every part has chunks), 0 the default
+void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, int parts_req, const double *X,
+                           const void *Xs_hi, const void *Xs_lo, const void *XsT_hi, const void *XsT_lo,
+                           const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
+                           int *ntile_out, hipStream_t s);
 void launch_gp_fused_reduce(int nc, int d, int parts, const float *Gpart, double wscale, double *G, hipStream_t s);
 void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
                          void *hi, void *lo, hipStream_t s);

@@ -417,7 +417,9 @@ __global__ __launch_bounds__(256) void k_gemm2_split_lds(int nc, int d, int Mpad
 }
 
 // 128 x 128 tiles when they alone fill the GPU (two per CU); 64 x 64 tiles otherwise
-static bool use_lds_tiles(int rows, int cols) {
+// (lds_mode: -1 this rule, 0 the 64 x 64 tiles, 1 the 128 x 128 tiles)
+static bool use_lds_tiles(int rows, int cols, int lds_mode = -1) {
+    if (lds_mode >= 0) return lds_mode == 1;
     return (long)cdivm(rows, kLT) * cdivm(cols, kLT) >= 512;
 }
 
@@ -659,12 +661,12 @@ static int int_zeta_m(double z) {
 }
 
 
-void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, const double *X, const void *Xs_hi,
+void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, int lds_mode, const double *X, const void *Xs_hi,
                         const void *Xs_lo, const double *wvec, double zeta, double wscale, float *Wf,
                         double *e_part, int *ntile_out, void *xsplit_ws, hipStream_t s) {
     const int Mpad = (M + 31) / 32 * 32;
     dim3 grid(cdivm(nc, 64), cdivm(M, 64));
-    const bool lds = use_lds_tiles(nc, M);
+    const bool lds = use_lds_tiles(nc, M, lds_mode);
     const int ntn = cdivm(M, kLT), nblk = cdivm(nc, kLT) * ntn;
     *ntile_out = lds ? ntn : (int)grid.y;
     if (nc <= 0) return;
@@ -721,10 +723,18 @@ static void gp_fused_go(int nc, int M, int d, int dpad, int Mpad, int parts, con
                            1.0 / wscale, G, Gpart, e_part);
 }
 
-void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, const double *X, const void *Xs_hi,
-                           const void *Xs_lo, const void *XsT_hi, const void *XsT_lo, const double *wvec, double zeta,
-                           double wscale, double *G, float *Gpart, double *e_part, int *ntile_out, hipStream_t s) {
-    const int parts = gp_fused_parts(nc, Mpad);
+int gp_fused_parts_valid(int parts, int Mpad) {
+    const int chunks = Mpad / 32;
+    parts = std::max(1, std::min(parts, chunks));
+    const int per = cdivm(chunks, parts);
+    return cdivm(chunks, per);
+}
+
+void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, int parts_req, const double *X,
+                           const void *Xs_hi, const void *Xs_lo, const void *XsT_hi, const void *XsT_lo,
+                           const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
+                           int *ntile_out, hipStream_t s) {
+    const int parts = parts_req > 0 ? gp_fused_parts_valid(parts_req, Mpad) : gp_fused_parts(nc, Mpad);
     *ntile_out = parts;
     if (nc <= 0) return;
     const int dk = dpad <= 64 ? 2 : (dpad <= 128 ? 4 : 8);
@@ -743,15 +753,15 @@ void launch_gp_fused_reduce(int nc, int d, int parts, const float *Gpart, double
     hipLaunchKernelGGL(k_sum_parts_f32, dim3(cdivm((long)n, 256)), dim3(256), 0, s, n, parts, Gpart, 1.0 / wscale, G);
 }
 
-void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, const float *Wf, const void *XsT_hi,
-                        const void *XsT_lo, double wscale, double *G, hipStream_t s) {
+void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, int lds_mode, const float *Wf,
+                        const void *XsT_hi, const void *XsT_lo, double wscale, double *G, hipStream_t s) {
     (void)M; (void)dpad;
     dim3 grid(cdivm(nc, 64), cdivm(d, 64));
     if (nc <= 0) return;
     const char *wf = (const char *)Wf;
     const void *Whi = wf, *Wlo = wf + (size_t)nc * Mpad * 2;
     const double inv = 1.0 / wscale;
-    if (use_lds_tiles(nc, d)) {
+    if (use_lds_tiles(nc, d, lds_mode)) {
         const int ntn = cdivm(d, kLT), nblk = cdivm(nc, kLT) * ntn;
         if (prec == 1)
             hipLaunchKernelGGL(k_gemm2_split_lds<1>, dim3(nblk), dim3(256), 0, s, nc, d, Mpad, ntn,

@@ -161,3 +161,23 @@ def test_slab_batch_device_geometry(path, graphs):
             assert np.array_equal(out["f"].reshape(-1, 3), np.vstack([r["force"] for r in ref])), step
     finally:
         buf.free()
+
+
+def test_split_tuning_is_listed(path):
+    """The split-form autotuner times the fused kernel's sparse-point parts for the
+    centre-count bucket of a call and lists its choice; the tuned call's results stay
+    within the split form's budget of the FP64 ones."""
+    at = configs.perturb(configs.diamond(5.43, (4, 4, 4)), 0.1, np.random.default_rng(9))
+    pot = Potential(param_filename=path, precision="bf16x3")
+    r = pot.calc(at.copy(), force=True)
+    n = C.c_int()
+    prec, bucket, fparts, g1, g2 = (np.zeros(8, np.int32) for _ in range(5))
+    ms1, ms2 = np.zeros(8), np.zeros(8)
+    L.check(L.lib().gapx_model_tuning_split(pot.handle, 0, 8, L.ptr(prec), L.ptr(bucket), L.ptr(fparts), L.ptr(g1),
+                                            L.ptr(g2), L.ptr(ms1), L.ptr(ms2), C.byref(n)), "tuning_split")
+    assert n.value == 1
+    assert prec[0] == 1 and bucket[0] == 9                      # 512 centres: 2^8 < 512 <= 2^9
+    assert fparts[0] in (1, 2, 4, 8) and g1[0] == -1 and g2[0] == -1 and ms1[0] > 0
+    r64 = Potential(param_filename=path).calc(at.copy(), force=True)
+    assert abs(r["energy"] - r64["energy"]) / len(at) < 1e-3
+    assert np.sqrt(np.mean((r["force"] - r64["force"]) ** 2)) < 1e-2

@@ -474,6 +474,8 @@ def test_graph_replay_matches_eager(models, prec):
     eager = Potential(param_filename=path, precision=prec)
     graph = Potential(param_filename=path, precision=prec)
     graph.set_graphs(True)
+    for p in (eager, graph):   # two models: fixed tile rules, so that neither tunes differently
+        L.check(L.lib().gapx_model_set_autotune(p.handle, 0), "set_autotune")
     a = configs.si64(1)
     b = configs.si64(2)
     rng = np.random.default_rng(8)
