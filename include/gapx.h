@@ -203,7 +203,8 @@ int gapx_calc_lammps(gapx_model *model, int nlocal, int nghost, const int *atomi
  * evaluates the batch in one pass; slots overlap on the device.  The model's
  * precision must not change while a server runs.
  * gapx_server_submit copies the structure and returns a ticket at once
- * (GAPX_E_CAPACITY if it can never fit a slot).  gapx_server_wait blocks until
+ * (GAPX_E_CAPACITY if it can never fit a slot, or if its footprint exceeds the
+ * device memory free now (hipMemGetInfo) plus what a slot's engine already holds).  gapx_server_wait blocks until
  * that request is done, copies the requested outputs (as gapx_calc) and gives
  * its latency from submission to completion; a ticket is waited on once.
  * gapx_server_free finishes the queued requests and stops the threads. */
@@ -216,6 +217,27 @@ int gapx_server_wait(gapx_server *server, int64_t ticket, double *e, double *loc
                      double *local_virial, double *latency_s);
 int gapx_server_stats(gapx_server *server, int64_t *n_done, int64_t *n_batches, int64_t *atoms_done,
                       double *busy_s);
+/* Sessions: a client that resubmits one structure as it moves (MD) keeps it resident.
+ * No counterpart in the reference beyond calc_connect's skin (Connection.f95:1085-1127).
+ * gapx_server_session_open gives the session its own engine (stream + workspace) and a
+ * skin in Angstrom; gapx_server_session_submit queues a request of the session (waited
+ * on with gapx_server_wait like any ticket).  A session's requests are evaluated in
+ * order with the neighbour list built at the model cutoff plus the skin and kept while
+ * no atom has moved by half the skin (the distances recomputed as calc_dists does);
+ * its descriptor and GP buffers stay resident; a resubmission with bitwise identical
+ * positions, species, cell and pbc returns the resident results without device work.
+ * Sessions run on the slot threads, overlapping on the device with each other and with
+ * the batches.  Requests are admitted against the budget's slot share and against
+ * hipMemGetInfo (free device memory plus what the session's engine already holds).
+ * gapx_server_session_stats counts the requests done, the neighbour lists reused and
+ * the results reused; gapx_server_session_close finishes the queued requests and frees
+ * the session's device memory. */
+int gapx_server_session_open(gapx_server *server, double skin, int64_t *session);
+int gapx_server_session_submit(gapx_server *server, int64_t session, int n_atoms, const double *pos, const int *Z,
+                               const double *lattice, const int *pbc, unsigned want, int64_t *ticket);
+int gapx_server_session_stats(gapx_server *server, int64_t session, int64_t *n_done, int64_t *nl_reused,
+                              int64_t *results_reused);
+int gapx_server_session_close(gapx_server *server, int64_t session);
 int gapx_server_free(gapx_server *server);
 
 /* Skin reuse of the neighbour list (calc_connect with cutoff_skin,

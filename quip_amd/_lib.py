@@ -62,6 +62,11 @@ _SIGS = [
     ("gapx_server_wait", C.c_int, [C.c_void_p, C.c_int64] + [C.c_void_p] * 5 + [C.POINTER(C.c_double)]),
     ("gapx_server_stats", C.c_int, [C.c_void_p] + [C.POINTER(C.c_int64)] * 3 + [C.POINTER(C.c_double)]),
     ("gapx_server_free", C.c_int, [C.c_void_p]),
+    ("gapx_server_session_open", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int64)]),
+    ("gapx_server_session_submit", C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_uint, C.POINTER(C.c_int64)]),
+    ("gapx_server_session_stats", C.c_int, [C.c_void_p, C.c_int64] + [C.POINTER(C.c_int64)] * 3),
+    ("gapx_server_session_close", C.c_int, [C.c_void_p, C.c_int64]),
     ("gapx_connect", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                C.POINTER(C.c_void_p)]),
     ("gapx_nl_counts", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p]),

@@ -10,9 +10,24 @@
 // so the work of different slots overlaps on the device.  Workspaces grow to
 // the largest batch a slot has run and stay resident; after warm-up no request
 // allocates device memory.
+//
+// Sessions serve clients that resubmit one structure as it moves (an MD run): a
+// session owns an engine whose neighbour list, built with the model cutoff plus the
+// session's skin, and whose descriptor buffers stay resident on the device between
+// its requests.  A resubmission reuses the list while no atom has moved by half the
+// skin (calc_connect with cutoff_skin, Connection.f95:1085-1127; the distances are
+// recomputed as calc_dists does), and a resubmission with bitwise identical
+// positions, species and cell takes the resident results without device work.  A
+// session's requests are evaluated in order by whichever slot thread is free, so
+// sessions overlap on the device with each other and with the batches.
+//
+// Admission: a request is refused (GAPX_E_CAPACITY) when its estimated device
+// footprint exceeds the slot's share of the budget, or exceeds the device memory that
+// is free now (hipMemGetInfo) plus what the engine that would run it already holds.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -40,6 +55,25 @@ struct Request {
 
 }  // namespace
 
+namespace {
+
+// A client's resident structure: its engine (stream, workspace, neighbour list kept
+// within the skin) and the results of its last evaluated request.
+struct Session {
+    std::unique_ptr<Engine> eng;
+    double skin = 0;
+    std::deque<std::shared_ptr<Request>> queue;
+    bool busy = false, closing = false;
+    bool have_last = false;
+    std::vector<double> last_pos, last_lat, last_f, last_le, last_lv;
+    std::vector<int> last_Z, last_pbc;
+    double last_e = 0, last_vir[9] = {0};
+    int64_t n_done = 0, nl_reused = 0, results_reused = 0;
+    size_t held = 0;                 // device bytes its engine held after its last request
+};
+
+}  // namespace
+
 struct gapx_server {
     gapx_model *m = nullptr;
     int n_slots = 1;
@@ -51,11 +85,42 @@ struct gapx_server {
     std::condition_variable cv_queue, cv_done;
     std::deque<std::shared_ptr<Request>> queue;
     std::unordered_map<int64_t, std::shared_ptr<Request>> tickets;
-    int64_t next_ticket = 1;
+    std::map<int64_t, std::unique_ptr<Session>> sessions;
+    std::vector<size_t> slot_held;   // device bytes each slot's engine held after its last batch
+    int64_t next_ticket = 1, next_session = 1;
     bool stop = false;
     int64_t n_done = 0, n_batches = 0, atoms_done = 0;
     double busy_s = 0;
 };
+
+// device bytes an engine's workspace holds
+static size_t engine_bytes(const Engine &eg) {
+    const Workspace &w = eg.ws;
+    size_t b = 0;
+    for (const DBuf *x : {&w.pos, &w.Z, &w.sid, &w.atom_off, &w.geo, &w.cell_base, &w.ginv, &w.cell_of, &w.map_shift,
+                          &w.cell_cnt, &w.cell_start, &w.cell_fill, &w.cell_atoms, &w.cell_unsorted, &w.scan_part,
+                          &w.scan_part2, &w.n1_cnt, &w.n1_off, &w.n1_j, &w.n1_shift, &w.n1_r, &w.n1_rev, &w.n2_cnt,
+                          &w.n2_off, &w.n2_fill, &w.n2_i, &w.n2_k, &w.n2_is, &w.n2_ks, &w.nb_off, &w.nb_j, &w.nb_rev,
+                          &w.nb_r, &w.nb_d, &w.pairF, &w.pairE, &w.esite, &w.X, &w.norm, &w.cbuf, &w.W, &w.Wf,
+                          &w.e_part, &w.G, &w.Gparts, &w.split_ws, &w.f, &w.le, &w.lv, &w.e, &w.vir, &w.minmax,
+                          &w.last_pos, &w.skin_flags, &w.cmask, &w.t1_j, &w.t1_shift, &w.t1_r, &w.t2_i, &w.t2_k,
+                          &w.nl_flags})
+        b += x->cap;
+    for (const DBuf &x : w.centres) b += x.cap;
+    return b;
+}
+
+// admission against the device: the footprint must fit the memory free now plus what
+// the engine that would run the request already holds (its workspace is reused)
+static int admit_device(gapx_server *srv, size_t footprint, size_t held) {
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipSetDevice(srv->m->device));
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    if (footprint > free_b + held)
+        return fail(GAPX_E_CAPACITY, "request needs ~%zu device bytes; %zu are free and the engine holds %zu",
+                    footprint, free_b, held);
+    return GAPX_OK;
+}
 
 // Estimated device bytes of one structure of N atoms inside a batch: per-atom
 // neighbour-list, pair, descriptor and GEMM buffers of every coordinate.  Pairs
@@ -77,6 +142,80 @@ static size_t footprint_bytes(const gapx_model *m, int N) {
     return (size_t)(per_atom * N) + (1u << 20);
 }
 
+// The session whose oldest queued request is the oldest of all idle sessions (null if none).
+static Session *ready_session(gapx_server *srv, Clock::time_point *t) {
+    Session *best = nullptr;
+    for (auto &kv : srv->sessions) {
+        Session &ss = *kv.second;
+        if (ss.busy || ss.queue.empty()) continue;
+        if (!best || ss.queue.front()->t_submit < best->queue.front()->t_submit) best = &ss;
+    }
+    if (best && t) *t = best->queue.front()->t_submit;
+    return best;
+}
+
+// One request of a session on the session's engine: the neighbour list with the
+// session's skin is kept while no atom has moved by half of it, and bitwise identical
+// inputs take the results of the session's last request.
+static void run_session_request(gapx_server *srv, Session &ss, Request &r) {
+    gapx_model *m = srv->m;
+    const auto t0 = Clock::now();
+    r.t_start = t0;
+    const size_t n = r.N;
+    const bool same = ss.have_last && ss.last_pos == r.pos && ss.last_Z == r.Z &&
+                      std::equal(r.lat, r.lat + 9, ss.last_lat.begin()) &&
+                      std::equal(r.pbc, r.pbc + 3, ss.last_pbc.begin());
+    int st = GAPX_OK;
+    std::string msg;
+    bool reused_nl = false;
+    if (!same) {
+        const unsigned all = GAPX_WANT_E | GAPX_WANT_F | GAPX_WANT_V | GAPX_WANT_LOCAL_E | GAPX_WANT_LOCAL_V;
+        ss.last_f.resize(3 * n);
+        ss.last_le.resize(n);
+        ss.last_lv.resize(9 * n);
+        const int off[2] = {0, r.N};
+        st = calc_host(m, *ss.eng, 1, off, r.pos.data(), r.Z.data(), r.lat, r.pbc, m->cutoff + ss.skin, all,
+                       &ss.last_e, ss.last_le.data(), ss.last_f.data(), ss.last_vir, ss.last_lv.data());
+        if (st == GAPX_OK) {
+            reused_nl = !ss.eng->last_rebuilt;
+            ss.last_pos = r.pos;
+            ss.last_Z = r.Z;
+            ss.last_lat.assign(r.lat, r.lat + 9);
+            ss.last_pbc.assign(r.pbc, r.pbc + 3);
+            ss.have_last = true;
+        } else {
+            msg = last_error_text();
+            ss.have_last = false;
+        }
+    }
+    const size_t held = engine_bytes(*ss.eng);
+    const auto t1 = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        r.status = st;
+        r.err = msg;
+        if (st == GAPX_OK) {
+            r.e = ss.last_e;
+            std::copy(ss.last_vir, ss.last_vir + 9, r.vir);
+            if (r.want & GAPX_WANT_F) r.f = ss.last_f;
+            if (r.want & GAPX_WANT_LOCAL_E) r.le = ss.last_le;
+            if (r.want & GAPX_WANT_LOCAL_V) r.lv = ss.last_lv;
+        }
+        r.t_done = t1;
+        r.done = true;
+        ss.busy = false;
+        ss.n_done++;
+        ss.held = held;
+        if (same) ss.results_reused++;
+        if (reused_nl) ss.nl_reused++;
+        srv->n_done += 1;
+        srv->atoms_done += r.N;
+        srv->busy_s += std::chrono::duration<double>(t1 - t0).count();
+    }
+    srv->cv_done.notify_all();
+    srv->cv_queue.notify_one();   // the session may have more requests queued
+}
+
 static void worker(gapx_server *srv, int slot) {
     Engine &eg = *srv->engines[slot];
     gapx_model *m = srv->m;
@@ -86,13 +225,23 @@ static void worker(gapx_server *srv, int slot) {
     hipSetDevice(m->device);
     for (;;) {
         batch.clear();
+        Session *sess = nullptr;
+        std::shared_ptr<Request> sreq;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
-            srv->cv_queue.wait(lk, [&] { return srv->stop || !srv->queue.empty(); });
-            if (srv->queue.empty()) return;   // stop requested and nothing left
+            srv->cv_queue.wait(lk, [&] { return srv->stop || !srv->queue.empty() || ready_session(srv, nullptr); });
+            Clock::time_point ts;
+            Session *rs = ready_session(srv, &ts);
+            if (srv->queue.empty() && !rs) return;   // stop requested and nothing left
+            if (rs && (srv->queue.empty() || ts < srv->queue.front()->t_submit)) {   // oldest first
+                sess = rs;
+                sreq = rs->queue.front();
+                rs->queue.pop_front();
+                rs->busy = true;
+            }
             size_t fp = 0;
             long atoms = 0;
-            while (!srv->queue.empty()) {
+            while (!sess && !srv->queue.empty()) {
                 auto &r = srv->queue.front();
                 if (!batch.empty() &&
                     (fp + r->footprint > srv->slot_budget || atoms + r->N > (long)srv->max_batch_atoms))
@@ -102,6 +251,10 @@ static void worker(gapx_server *srv, int slot) {
                 batch.push_back(r);
                 srv->queue.pop_front();
             }
+        }
+        if (sess) {
+            run_session_request(srv, *sess, *sreq);
+            continue;
         }
         const auto t0 = Clock::now();
         const int S = (int)batch.size();
@@ -159,6 +312,7 @@ static void worker(gapx_server *srv, int slot) {
             srv->n_done += S;
             srv->n_batches += 1;
             srv->busy_s += std::chrono::duration<double>(t1 - t0).count();
+            srv->slot_held[slot] = engine_bytes(eg);
         }
         srv->cv_done.notify_all();
     }
@@ -186,6 +340,7 @@ extern "C" int gapx_server_create(gapx_model *m, int n_slots, int64_t hbm_budget
         CHK(new_engine(m, e));
         srv->engines.push_back(std::move(e));
     }
+    srv->slot_held.assign(n_slots, 0);
     for (int k = 0; k < n_slots; k++) srv->threads.emplace_back(worker, srv.get(), k);
     *out = srv.release();
     return GAPX_OK;
@@ -211,6 +366,12 @@ extern "C" int gapx_server_submit(gapx_server *srv, int N, const double *pos, co
     if (r->footprint > srv->slot_budget || N > srv->max_batch_atoms)
         return fail(GAPX_E_CAPACITY, "structure of %d atoms needs ~%zu device bytes; a slot admits %zu bytes and %d atoms",
                     N, r->footprint, srv->slot_budget, srv->max_batch_atoms);
+    size_t held = 0;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        for (size_t h : srv->slot_held) held = std::max(held, h);
+    }
+    CHK(admit_device(srv, r->footprint, held));
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         r->t_submit = Clock::now();
@@ -219,6 +380,94 @@ extern "C" int gapx_server_submit(gapx_server *srv, int N, const double *pos, co
         srv->queue.push_back(r);
     }
     srv->cv_queue.notify_one();
+    return GAPX_OK;
+}
+
+extern "C" int gapx_server_session_open(gapx_server *srv, double skin, int64_t *session) {
+    if (!srv || !session) return fail(GAPX_E_ARG, "NULL argument");
+    if (!(skin >= 0) || skin > 10.0) return fail(GAPX_E_ARG, "skin must be in [0, 10] A (got %g)", skin);
+    HIPCHK(hipSetDevice(srv->m->device));
+    auto ss = std::make_unique<Session>();
+    CHK(new_engine(srv->m, ss->eng));
+    ss->skin = skin;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->sessions.size() >= 256) return fail(GAPX_E_CAPACITY, "at most 256 open sessions");
+    *session = srv->next_session++;
+    srv->sessions[*session] = std::move(ss);
+    return GAPX_OK;
+}
+
+extern "C" int gapx_server_session_submit(gapx_server *srv, int64_t session, int N, const double *pos, const int *Z,
+                                          const double *lat, const int *pbc, unsigned want, int64_t *ticket) {
+    if (!srv || !ticket) return fail(GAPX_E_ARG, "NULL argument");
+    if (N < 0 || (N > 0 && (!pos || !Z)) || !lat || !pbc) return fail(GAPX_E_ARG, "bad structure arguments");
+    const int off[2] = {0, N};
+    CHK(check_batch(1, off, Z, lat, pbc));
+    const double vol = lat[0] * (lat[4] * lat[8] - lat[5] * lat[7]) - lat[1] * (lat[3] * lat[8] - lat[5] * lat[6]) +
+                       lat[2] * (lat[3] * lat[7] - lat[4] * lat[6]);
+    if (!(std::fabs(vol) > 0)) return fail(GAPX_E_GEOMETRY, "lattice is singular");
+    auto r = std::make_shared<Request>();
+    r->N = N;
+    r->want = want;
+    r->pos.assign(pos, pos + 3 * (size_t)N);
+    r->Z.assign(Z, Z + N);
+    std::copy(lat, lat + 9, r->lat);
+    std::copy(pbc, pbc + 3, r->pbc);
+    r->footprint = footprint_bytes(srv->m, N);
+    if (r->footprint > srv->slot_budget)
+        return fail(GAPX_E_CAPACITY, "structure of %d atoms needs ~%zu device bytes; a session admits %zu", N,
+                    r->footprint, srv->slot_budget);
+    size_t held = 0;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->sessions.find(session);
+        if (it == srv->sessions.end() || it->second->closing)
+            return fail(GAPX_E_ARG, "unknown session %lld", (long long)session);
+        held = it->second->held;
+    }
+    CHK(admit_device(srv, r->footprint, held));
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->sessions.find(session);
+        if (it == srv->sessions.end() || it->second->closing)
+            return fail(GAPX_E_ARG, "unknown session %lld", (long long)session);
+        r->t_submit = Clock::now();
+        *ticket = srv->next_ticket++;
+        srv->tickets[*ticket] = r;
+        it->second->queue.push_back(r);
+    }
+    srv->cv_queue.notify_one();
+    return GAPX_OK;
+}
+
+extern "C" int gapx_server_session_stats(gapx_server *srv, int64_t session, int64_t *n_done, int64_t *nl_reused,
+                                         int64_t *results_reused) {
+    if (!srv) return fail(GAPX_E_ARG, "NULL server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->sessions.find(session);
+    if (it == srv->sessions.end()) return fail(GAPX_E_ARG, "unknown session %lld", (long long)session);
+    if (n_done) *n_done = it->second->n_done;
+    if (nl_reused) *nl_reused = it->second->nl_reused;
+    if (results_reused) *results_reused = it->second->results_reused;
+    return GAPX_OK;
+}
+
+extern "C" int gapx_server_session_close(gapx_server *srv, int64_t session) {
+    if (!srv) return fail(GAPX_E_ARG, "NULL server");
+    std::unique_ptr<Session> gone;
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        auto it = srv->sessions.find(session);
+        if (it == srv->sessions.end() || it->second->closing)
+            return fail(GAPX_E_ARG, "unknown session %lld", (long long)session);
+        Session *ss = it->second.get();
+        ss->closing = true;   // no new requests; the queued ones are finished first
+        srv->cv_done.wait(lk, [&] { return ss->queue.empty() && !ss->busy; });
+        gone = std::move(srv->sessions[session]);
+        srv->sessions.erase(session);
+    }
+    HIPCHK(hipSetDevice(srv->m->device));
+    gone.reset();   // the engine: its stream and resident buffers
     return GAPX_OK;
 }
 

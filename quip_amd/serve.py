@@ -6,6 +6,14 @@
 
 Requests are evaluated with the physics of Potential.calc; a free slot batches
 whatever is queued (first in, first out) within its share of the HBM budget.
+
+A client that resubmits one structure as it moves opens a session, which keeps its
+neighbour list (with a skin) and descriptor buffers resident on the device:
+
+    with srv.session(skin=0.5) as s:
+        t = s.submit(at)                # evaluated in order, list reused within the skin
+        r = srv.wait(t)
+        s.stats()                       # {"requests", "nl_reused", "results_reused"}
 """
 import ctypes as C
 
@@ -23,18 +31,27 @@ class Server:
         self._h = h
         self._n = {}
 
-    def submit(self, at, want=("energy", "force", "virial")):
+    @staticmethod
+    def _want(want):
         w = L.WANT_E
         for key, bit in (("force", L.WANT_F), ("virial", L.WANT_V), ("local_energy", L.WANT_LOCAL_E),
                          ("local_virial", L.WANT_LOCAL_V)):
             if key in want:
                 w |= bit
+        return w
+
+    def submit(self, at, want=("energy", "force", "virial")):
+        w = self._want(want)
         t = C.c_int64()
         pos, Z, lat, pbc = L.f64(at.positions), L.i32(at.numbers), L.f64(at.cell), L.i32(at.pbc)
         L.check(L.lib().gapx_server_submit(self._h, len(at), L.ptr(pos), L.ptr(Z), L.ptr(lat), L.ptr(pbc), w,
                                            C.byref(t)), "gapx_server_submit")
         self._n[t.value] = (len(at), w)
         return t.value
+
+    def session(self, skin=0.5):
+        """A resident client structure (gapx_server_session_open)."""
+        return Session(self, skin)
 
     def done(self, ticket):
         d = C.c_int()
@@ -75,6 +92,44 @@ class Server:
             self.close()
         except Exception:
             pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Session:
+    """A client's resident structure on a Server: requests are evaluated in order with
+    the neighbour list (model cutoff + skin) kept within the skin."""
+
+    def __init__(self, server, skin=0.5):
+        self.server = server
+        h = C.c_int64()
+        L.check(L.lib().gapx_server_session_open(server._h, float(skin), C.byref(h)), "gapx_server_session_open")
+        self._id = h.value
+
+    def submit(self, at, want=("energy", "force", "virial")):
+        w = Server._want(want)
+        t = C.c_int64()
+        pos, Z, lat, pbc = L.f64(at.positions), L.i32(at.numbers), L.f64(at.cell), L.i32(at.pbc)
+        L.check(L.lib().gapx_server_session_submit(self.server._h, self._id, len(at), L.ptr(pos), L.ptr(Z),
+                                                   L.ptr(lat), L.ptr(pbc), w, C.byref(t)),
+                "gapx_server_session_submit")
+        self.server._n[t.value] = (len(at), w)
+        return t.value
+
+    def stats(self):
+        n, nl, rr = C.c_int64(), C.c_int64(), C.c_int64()
+        L.check(L.lib().gapx_server_session_stats(self.server._h, self._id, C.byref(n), C.byref(nl), C.byref(rr)),
+                "gapx_server_session_stats")
+        return dict(requests=n.value, nl_reused=nl.value, results_reused=rr.value)
+
+    def close(self):
+        if self._id is not None and self.server._h:
+            L.check(L.lib().gapx_server_session_close(self.server._h, self._id), "gapx_server_session_close")
+        self._id = None
 
     def __enter__(self):
         return self

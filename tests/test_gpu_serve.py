@@ -120,3 +120,68 @@ def test_server_reports_errors_per_request(sic_model):
         for t, at in zip([tickets[0], tickets[2], tickets[3]], good):
             r = srv.wait(t)
             assert abs(r["energy"] - alone.calc(at.copy())["energy"]) < 1e-12 * len(at)
+
+
+def test_session_keeps_neighbour_list_and_results(sic_model):
+    """A session resubmitting a moving structure keeps its neighbour list within the skin
+    (calc_connect with cutoff_skin, Connection.f95:1085-1127): its results equal, bit for
+    bit, those of one model evaluating the same trajectory with the same list cutoff
+    (the plain entry point's skin reuse), and fresh builds to rounding -- a reused list's
+    distances come from calc_dists (Connection.f95:1393-1395), whose operation order
+    differs from the build's, in the reference too.  A bitwise identical resubmission
+    takes the resident results; a move beyond half the skin rebuilds the list."""
+    m, path = sic_model
+    skin = 0.5
+    at0 = configs.perturb(configs.diamond(4.36, (2, 2, 2), species=(14, 6)), 0.05, configs.rng_for(4, 31))
+    rng = np.random.default_rng(12)
+    traj = [at0]
+    for _ in range(5):
+        a = traj[-1].copy()
+        a.positions = a.positions + rng.normal(0.0, 2e-3, a.positions.shape)
+        traj.append(a)
+    far = traj[-1].copy()
+    far.positions = far.positions + rng.normal(0.0, 0.3, far.positions.shape)
+    pot = Potential(param_filename=path)
+    rc = pot.cutoff() + skin
+    with Server(pot, n_slots=2, hbm_budget=8 << 30, max_batch_atoms=6000) as srv:
+        with srv.session(skin=skin) as s:
+            res = [srv.wait(s.submit(a)) for a in traj]
+            again = srv.wait(s.submit(traj[-1].copy()))
+            st = s.stats()
+            moved = srv.wait(s.submit(far))
+            st2 = s.stats()
+    n = len(traj)
+    assert st["requests"] == n + 1
+    assert st["nl_reused"] == n - 1 and st["results_reused"] == 1
+    assert st2["nl_reused"] == st["nl_reused"]                     # a move beyond half the skin rebuilt it
+    ref = Potential(param_filename=path)
+    for k, (a, r) in enumerate(zip(traj + [far], res + [moved])):
+        rr = ref.calc(a.copy(), force=True, virial=True, cutoff=rc)
+        assert ref.nl_rebuilt() == (k == 0 or k == n)
+        assert r["energy"] == rr["energy"], k
+        assert np.array_equal(r["force"], rr["force"]), k
+        assert np.array_equal(r["virial"], rr["virial"]), k
+        fresh = Potential(param_filename=path).calc(a.copy(), force=True, cutoff=rc)
+        assert abs(r["energy"] - fresh["energy"]) < 1e-12 * len(a), k
+        assert np.abs(r["force"] - fresh["force"]).max() < 1e-11, k
+    assert again["energy"] == res[-1]["energy"] and np.array_equal(again["force"], res[-1]["force"])
+
+
+def test_submit_while_another_thread_waits(sic_model):
+    """Tickets are erased by key after a wait: submissions from one thread while another
+    waits (which can rehash the ticket table) leave every wait valid."""
+    m, path = sic_model
+    pot = Potential(param_filename=path)
+    frames = [configs.perturb(configs.diamond(4.36, (1, 1, 1), species=(14, 6)), 0.05, configs.rng_for(4, 50 + k))
+              for k in range(40)]
+    with Server(pot, n_slots=2, hbm_budget=4 << 30, max_batch_atoms=4000) as srv:
+        first = srv.submit(frames[0])
+        out = {}
+        th = threading.Thread(target=lambda: out.setdefault("r", srv.wait(first)))
+        th.start()
+        tickets = [srv.submit(a) for a in frames[1:]]       # many inserts while the waiter blocks
+        th.join()
+        rest = [srv.wait(t) for t in tickets]
+    ref = Potential(param_filename=path)
+    for a, r in zip(frames, [out["r"]] + rest):
+        assert abs(r["energy"] - ref.calc(a.copy())["energy"]) < 1e-12 * len(a)
