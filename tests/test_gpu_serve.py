@@ -185,3 +185,55 @@ def test_submit_while_another_thread_waits(sic_model):
     ref = Potential(param_filename=path)
     for a, r in zip(frames, [out["r"]] + rest):
         assert abs(r["energy"] - ref.calc(a.copy())["energy"]) < 1e-12 * len(a)
+
+
+@pytest.fixture(scope="module")
+def sic_full(tmp_path_factory):
+    """The served model at full size: Si and C centres, n_max 12, l_max 8 (d = 2701),
+    2 x 2000 sparse points from GPU descriptors of 64 seeded training cells."""
+    tmp = tmp_path_factory.mktemp("sicfull")
+    cache = {}
+
+    def fn(d, at):
+        if d.string not in cache:
+            from quip_amd.gap_xml import GAPModel
+            mm = GAPModel("D", 1527075646, np.zeros(119),
+                          [GPCoordinate("D1", d, 2, 1.0, 0.0, 1.0, np.zeros((1, d.dim)), np.zeros(1), np.ones(1))])
+            p = str(tmp / ("d%d.xml" % len(cache)))
+            write_gap_xml(p, mm)
+            cache[d.string] = Potential(param_filename=p)
+        return cache[d.string].descriptors(at)[0]
+
+    m = configs.model_sic(fn)
+    path = str(tmp / "sic.xml")
+    write_gap_xml(path, m)
+    return m, path
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_cfg4_scale_served_equals_alone(sic_full, prec):
+    """Requests of the serving workload (Si/C, N log-uniform on [16, 4096], bulk / slab /
+    vacancies) arrive as a Poisson stream on the full model; each served result equals
+    the structure evaluated alone (FP64 to 1e-12 eV/atom and 1e-11 eV/A; the split form's
+    large descriptors run the split GEMMs, whose element results do not depend on the
+    batch, to the same bounds)."""
+    m, path = sic_full
+    frames = configs.sic_requests(20)
+    assert max(len(a) for a in frames) > 1000
+    pot = Potential(param_filename=path, precision=prec)
+    rng = np.random.default_rng(17)
+    gaps = rng.exponential(2e-3, len(frames))
+    with Server(pot, n_slots=2, hbm_budget=64 << 30, max_batch_atoms=16384) as srv:
+        tickets = []
+        for a, g in zip(frames, gaps):
+            time.sleep(float(g))
+            tickets.append(srv.submit(a))
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    print("cfg4", prec, "requests", st["requests"], "batches", st["batches"], flush=True)
+    alone = Potential(param_filename=path, precision=prec)
+    for a, r in zip(frames, res):
+        ra = alone.calc(a.copy(), force=True, virial=True)
+        assert abs(r["energy"] - ra["energy"]) / len(a) < 1e-12
+        assert np.abs(r["force"] - ra["force"]).max() < 1e-11
+        assert np.abs(r["virial"] - ra["virial"]).max() < 1e-9
