@@ -4,10 +4,15 @@
     python -m quip_amd.prof_summary counters <run_counter_collection.csv> [--match S]
     python -m quip_amd.prof_summary traffic  <fetch_counter.csv> <write_counter.csv> --kernel S [--grid G]
                                             [--probe-bytes B]
+    python -m quip_amd.prof_summary stages   <kernel_trace.csv> <fetch.csv> <write.csv> <sq.csv>
+                                            [--match A|B|...] [--probe-bytes B]
 
 kernels:  calls, total and average duration per (kernel, grid size), the
           --stats view split by launch shape.
 counters: average counter value per dispatch per (kernel, grid size, counter).
+stages:   per kernel shape the achieved HBM GB/s and FP64 VALU TFLOP/s against the
+          MI355X peaks, from a kernel trace, FETCH_SIZE and WRITE_SIZE runs and a run
+          of SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, SQ_INSTS_VALU, SQ_INSTS_LDS, SQ_WAVES.
 traffic:  HBM bytes per launch of one kernel shape from a FETCH_SIZE run and a
           WRITE_SIZE run (KiB per dispatch), scaled by what the same counters
           read for the device-to-device copy kernel (__amd_rocclr_copyBuffer)
@@ -97,9 +102,58 @@ def traffic(fetch_csv, write_csv, kernel, grid, probe_bytes):
     return "\n".join(lines)
 
 
+FP64_PEAK = 78.6e12     # MI355X FP64 vector and matrix peak, FLOP/s
+HBM_PEAK = 8.0e12       # MI355X HBM3E, bytes/s
+
+
+def stages(trace_csv, fetch_csv, write_csv, sq_csv, match, probe_bytes):
+    """Per kernel shape: average duration (kernel trace), HBM bytes per launch
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy probe), FP64 VALU FLOPs per launch
+    (64 lanes x (ADD + MUL + TRANS + 2 FMA) wave instructions, full wavefronts assumed)
+    and what they give against the 8 TB/s and 78.6 TFLOP/s peaks; VALU and LDS
+    instructions per wavefront."""
+    dur = defaultdict(list)
+    for r in csv.DictReader(open(trace_csv)):   # keyed by total grid threads, as the counter files are
+        g = str(int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]))
+        dur[(short(r["Kernel_Name"]), g)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    f, w, q = _counter_avgs(fetch_csv), _counter_avgs(write_csv), _counter_avgs(sq_csv)
+    pf = max((v for k, v in f.items() if k[0] == "__amd_rocclr_copyBuffer" and k[2] == "FETCH_SIZE"),
+             key=lambda v: v[1])
+    pw = max((v for k, v in w.items() if k[0] == "__amd_rocclr_copyBuffer" and k[2] == "WRITE_SIZE"),
+             key=lambda v: v[1])
+    ff, fw = probe_bytes / pf[1], probe_bytes / pw[1]
+    out = ["copy probe: %d bytes read + written; %.2f bytes per FETCH_SIZE unit, %.2f per WRITE_SIZE unit"
+           % (probe_bytes, ff, fw),
+           "%-34s %-12s %6s %9s %10s %8s %6s %10s %8s %6s %9s %8s" % (
+               "kernel", "grid", "calls", "avg_us", "hbm_MB", "GB/s", "hbm%", "fp64_GF", "TF/s", "valu%",
+               "valu/wave", "lds/wave")]
+    shapes = sorted({(k[0], k[1]) for k in f} | {(k[0], k[1]) for k in q})
+    for name, grid in shapes:
+        if match and not any(m_ in name for m_ in match.split("|")):
+            continue
+        d = dur.get((name, grid))
+        if not d:
+            continue
+        avg = sum(d) / len(d)
+
+        def cv(src, cn):
+            v = src.get((name, grid, cn))
+            return v[1] if v else 0.0
+        hbm = cv(f, "FETCH_SIZE") * ff + cv(w, "WRITE_SIZE") * fw
+        flops = 64.0 * (cv(q, "SQ_INSTS_VALU_ADD_F64") + cv(q, "SQ_INSTS_VALU_MUL_F64") +
+                        cv(q, "SQ_INSTS_VALU_TRANS_F64") + 2.0 * cv(q, "SQ_INSTS_VALU_FMA_F64"))
+        waves = max(cv(q, "SQ_WAVES"), 1.0)
+        t = avg * 1e-6
+        out.append("%-34s %-12s %6d %9.1f %10.2f %8.0f %6.1f %10.2f %8.2f %6.1f %9.0f %8.0f" % (
+            name[:34], grid, len(d), avg, hbm / 1e6, hbm / t / 1e9, 100.0 * hbm / t / HBM_PEAK, flops / 1e9,
+            flops / t / 1e12, 100.0 * flops / t / FP64_PEAK, cv(q, "SQ_INSTS_VALU") / waves,
+            cv(q, "SQ_INSTS_LDS") / waves))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["kernels", "counters", "traffic"])
+    ap.add_argument("what", choices=["kernels", "counters", "traffic", "stages"])
     ap.add_argument("files", nargs="+")
     ap.add_argument("--match")
     ap.add_argument("--top", type=int, default=40)
@@ -111,8 +165,10 @@ def main():
         print(kernels(a.files[0], a.match, a.top))
     elif a.what == "counters":
         print(counters(a.files[0], a.match))
-    else:
+    elif a.what == "traffic":
         print(traffic(a.files[0], a.files[1], a.kernel, a.grid, a.probe_bytes))
+    else:
+        print(stages(a.files[0], a.files[1], a.files[2], a.files[3], a.match, a.probe_bytes))
 
 
 if __name__ == "__main__":
