@@ -9,8 +9,12 @@ positions already resident in HBM, through gapx_calc_device.
 
 Reported beside the FP64 plain form: the bf16x3 and fp16x3 split-precision
 forms (throughput and error against FP64), a per-stage breakdown from HIP
-events on the library's stream, the roofline of the dominant kernel and the
-oracle's FP64 CPU throughput on this host (a reported baseline).
+events on the library's stream, the roofline of the dominant kernel, the HBM
+GB/s and FP64-VALU fraction of each descriptor stage (per-launch bytes and flops
+from the committed counter profile over this run's stage times), the mixed/FP64
+ratio on each workload measured in both forms, and the oracle's FP64 CPU
+throughput on this host's cores (si4096, with bounded samples of si64, sic32768
+and batch512, and the host's lscpu figures) -- a reported baseline.
 """
 import argparse
 import ctypes as C
@@ -32,6 +36,8 @@ from quip_amd.potential import Potential  # noqa: E402
 
 MFMA_FP64_PEAK = 78.6e12
 MFMA_BF16_PEAK = 2.5e15
+VALU_FP64_PEAK = 78.6e12
+HBM_PEAK = 8.0e12
 
 
 def log(*a):
@@ -151,22 +157,75 @@ def run_batch(pot, frames, dev, steps, warmup, graphs=False):
     return dt, N, dict(energy=E, force=F, natoms=np.diff(off)), {k: tot[k] / max(n, 1) for k in tot}
 
 
-def cpu_baseline(model, at, budget_s=12.0):
-    from oracle import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+def host_cpu():
+    """lscpu model / sockets / cores / threads and the cores this process may use."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["nproc"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def _time_oracle(fn, budget_s, max_reps=20):
     t0 = time.perf_counter()
-    O.gap_calc(model, at, n_threads=threads)          # warm-up
+    r = fn()                                           # warm-up
     first = time.perf_counter() - t0
-    reps = max(1, min(20, int(budget_s / max(first, 1e-3))))
+    reps = max(1, min(max_reps, int(budget_s / max(first, 1e-3))))
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        r = O.gap_calc(model, at, n_threads=threads)
+        r = fn()
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
+    return float(np.median(times)), reps, r
+
+
+def cpu_baseline(model, at, budget_s=12.0):
+    """The oracle's FP64 E+F+virial (OpenMP over centres) on the si4096 workload."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    med, reps, r = _time_oracle(lambda: O.gap_calc(model, at, n_threads=threads), budget_s)
     return dict(value=len(at) / med, unit="atoms/s", cores=threads, kind="port",
                 sample="si4096 (4055 atoms) full E+F+virial, median of %d runs after 1 warm-up, "
                        "oracle/liboracle.so FP64 with OpenMP" % reps), r
+
+
+def cpu_baseline_configs(si64_model, si4096_model, sic_model, frames, threads, sic_centres=128,
+                         batch_structs=64):
+    """Bounded samples of the other workloads on the same host cores: si64 whole (model with
+    500 sparse points and a distance_2b term), sic32768 with the first `sic_centres` atoms as
+    the only centres (atom_mask: the per-centre work of the full structure), and the first
+    `batch_structs` structures of batch512 (the si4096 model)."""
+    from oracle import oracle as O
+    out = {}
+    at0 = configs.si64()
+    med, reps, _ = _time_oracle(lambda: O.gap_calc(si64_model, at0, n_threads=threads), 1.5, 40)
+    out["si64"] = dict(atoms_per_s=len(at0) / med, sample="si64 (64 atoms) whole, median of %d runs" % reps)
+    if sic_model is not None:
+        sat = configs.sic32768()
+        mask = np.zeros(len(sat), bool)
+        mask[:sic_centres] = True
+        med, reps, _ = _time_oracle(lambda: O.gap_calc_ex(sic_model, sat, atom_mask=mask, n_threads=threads),
+                                    4.0, 3)
+        out["sic32768"] = dict(atoms_per_s=sic_centres / med,
+                               sample="sic32768 (32768 atoms), the first %d atoms as centres (atom_mask), median "
+                                      "of %d runs" % (sic_centres, reps))
+    sub = frames[:batch_structs]
+    n = sum(len(a) for a in sub)
+
+    def run():
+        for a in sub:
+            O.gap_calc(si4096_model, a, n_threads=threads)
+    med, reps, _ = _time_oracle(run, 3.0, 3)
+    out["batch512"] = dict(atoms_per_s=n / med, sample="the first %d structures of batch512 (%d atoms) one after "
+                                                        "another, median of %d runs" % (len(sub), n, reps))
+    return out
 
 
 def run_serving(path, prec, n_pool, duration, fracs, n_slots, max_batch_atoms):
@@ -243,6 +302,57 @@ def gemm1_traffic():
     return None
 
 
+def stage_counters():
+    """Per-launch HBM bytes (FETCH_SIZE + WRITE_SIZE scaled by a device copy), FP64 VALU
+    flops and kernel-trace time of each stage's dominant kernel, from the committed
+    counter summary: {(workload, stage): (kernel, bytes, flops, avg_us)}."""
+    path = os.path.join(ROOT, "profiles", "descriptor_stages_si4096_batch512.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if f and f[0] == "stage_counter":
+                out[(f[1], f[2])] = (f[3], float(f[5]), float(f[6]), float(f[7]))
+    return out
+
+
+SINGLE_KERNEL_STAGES = ("soap_forward", "soap_adjoint", "gemm1")
+
+
+def stage_roofline(forms, model, nc_batch):
+    """Achieved HBM GB/s and FP64-VALU fraction per descriptor stage.  Stages that are one
+    kernel (forward, adjoint, the GP kernel) divide the committed per-launch bytes and
+    flops by the stage time measured in this run with HIP events on the library's stream;
+    the neighbour-list and gather stages span several kernels, so their dominant kernel
+    is priced at its kernel-trace time and the live stage time is given beside it."""
+    cnt = stage_counters()
+    if not cnt:
+        return None
+    live = {"si4096_fp64": forms.get("fp64", {}).get("stage_ms", {}),
+            "batch512_bf16x3": forms.get("batch512_bf16x3", {}).get("stage_ms", {})}
+    out = {}
+    for (wl, stage), (kernel, nbytes, flops, avg_us) in sorted(cnt.items()):
+        st_ms = live.get(wl, {}).get(stage)
+        if stage in SINGLE_KERNEL_STAGES and st_ms:
+            t, src = st_ms * 1e-3, "HIP events, this run"
+        else:
+            t, src = avg_us * 1e-6, "kernel trace of the committed profile"
+        e = dict(kernel=kernel, hbm_bytes_per_launch=nbytes, fp64_valu_flops_per_launch=flops,
+                 kernel_ms=round(t * 1e3, 4), time_source=src, stage_ms_live=st_ms,
+                 hbm_GBps=round(nbytes / t / 1e9, 1), hbm_frac=round(nbytes / t / HBM_PEAK, 4),
+                 fp64_valu_TFps=round(flops / t / 1e12, 3), fp64_valu_frac=round(flops / t / VALU_FP64_PEAK, 4))
+        if stage == "gemm1" and wl == "batch512_bf16x3":   # split GP kernel: MFMA is its roof
+            c = model.coords[0]
+            alg = 4.0 * nc_batch * c.n_sparse * c.descriptor.dim
+            e.update(bound="mfma", algorithmic_flops=alg, mfma_flops_executed=3 * alg,
+                     mfma_frac_algorithmic=round(alg / t / MFMA_BF16_PEAK, 4),
+                     mfma_frac_executed=round(3 * alg / t / MFMA_BF16_PEAK, 4))
+        out.setdefault(wl, {})[stage] = e
+    out["source"] = ("bytes and flops per launch: profiles/descriptor_stages_si4096_batch512.txt (rocprofv3 "
+                     "FETCH_SIZE, WRITE_SIZE, SQ_INSTS_VALU_*_F64); peaks HBM 8 TB/s, FP64 VALU 78.6 TFLOP/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,10 +421,10 @@ def main():
         forms[prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - ref["force"]) ** 2)))
         forms[prec]["speedup_vs_fp64"] = forms[prec]["atoms_per_s"] / forms["fp64"]["atoms_per_s"]
     batch_list = [f for f in args.batch.split(",") if f]
+    frames = configs.batch512()
     if batch_list:
         if "fp64" not in batch_list:
             batch_list.insert(0, "fp64")
-        frames = configs.batch512()
         bres = {}
         for prec in batch_list:
             pot = Potential(param_filename=path, precision=prec)
@@ -344,7 +454,8 @@ def main():
     serving = None
     serve_list = [f for f in args.serve.split(",") if f]
     sic_list = [f for f in args.sic.split(",") if f]
-    if serve_list or sic_list:
+    smodel = None
+    if serve_list or sic_list or not args.no_cpu:
         log("building the two-species SiC model from GPU descriptors")
         smodel = configs.model_sic(gpu_descriptor_fn(tmp))
         spath = os.path.join(tmp, "sic.xml")
@@ -397,6 +508,18 @@ def main():
         cpu, ro = cpu_baseline(model, at)
         forms["fp64"]["dE_per_atom_vs_oracle"] = abs(ref["energy"] - ro["energy"]) / N
         forms["fp64"]["max_dF_vs_oracle"] = float(np.abs(ref["force"] - ro["forces"]).max())
+        log("cpu baseline, other workloads")
+        m64 = configs.model_si64(gpu_descriptor_fn(tmp))
+        cpu["other_workloads"] = cpu_baseline_configs(m64, model, smodel, frames, cpu["cores"])
+        cpu["host"] = host_cpu()
+    same = {}
+    for key, a, b in (("si4096_eager", "bf16x3", "fp64"), ("si4096_graph", "bf16x3_graph", "fp64_graph"),
+                      ("batch512_eager", "batch512_bf16x3", "batch512_fp64"),
+                      ("sic32768_eager", "sic32768_bf16x3", "sic32768_fp64")):
+        if a in forms and b in forms:
+            same[key] = forms[a]["atoms_per_s"] / forms[b]["atoms_per_s"]
+    nc_batch = sum(len(a) for a in frames)
+    stages_roof = stage_roofline(forms, model, nc_batch)
     dev.free()
     fp = forms["fp64"]
     mixed = [k for k in forms if k.startswith("batch512_") and "fp64" not in k]
@@ -410,7 +533,9 @@ def main():
                    "atoms": N, "sparse_points": M, "descriptor_dim": d},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "stage_roofline": stages_roof,
         "mixed_batched_vs_fp64_plain": max([forms[k]["speedup_vs_fp64_plain"] for k in mixed], default=None),
+        "mixed_vs_fp64_same_workload": same,
         "forms": forms,
         "serving": serving,
     }

@@ -144,7 +144,7 @@ def stages(trace_csv, fetch_csv, write_csv, sq_csv, match, probe_bytes):
                         cv(q, "SQ_INSTS_VALU_TRANS_F64") + 2.0 * cv(q, "SQ_INSTS_VALU_FMA_F64"))
         waves = max(cv(q, "SQ_WAVES"), 1.0)
         t = avg * 1e-6
-        out.append("%-34s %-12s %6d %9.1f %10.2f %8.0f %6.1f %10.2f %8.2f %6.1f %9.0f %8.0f" % (
+        out.append("%-34s %-12s %6d %9.1f %10.3f %8.0f %6.1f %10.4f %8.2f %6.1f %9.0f %8.0f" % (
             name[:34], grid, len(d), avg, hbm / 1e6, hbm / t / 1e9, 100.0 * hbm / t / HBM_PEAK, flops / 1e9,
             flops / t / 1e12, 100.0 * flops / t / FP64_PEAK, cv(q, "SQ_INSTS_VALU") / waves,
             cv(q, "SQ_INSTS_LDS") / waves))
