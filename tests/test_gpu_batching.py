@@ -181,3 +181,30 @@ def test_split_tuning_is_listed(path):
     r64 = Potential(param_filename=path).calc(at.copy(), force=True)
     assert abs(r["energy"] - r64["energy"]) / len(at) < 1e-3
     assert np.sqrt(np.mean((r["force"] - r64["force"]) ** 2)) < 1e-2
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+def test_split_form_device_calls(path, prec):
+    """Repeated device calls of a split form (the speculative launch sequence) give the
+    split form's own results -- those of its first (exact) host call bit for bit -- and
+    they differ from the FP64 form's, within the split form's budget."""
+    at = configs.perturb(configs.diamond(5.43, (4, 4, 4)), 0.1, np.random.default_rng(10))
+    r64 = Potential(param_filename=path).calc(at.copy(), force=True)
+    rs = Potential(param_filename=path, precision=prec).calc(at.copy(), force=True)
+    pot = Potential(param_filename=path, precision=prec)
+    buf = _Buf(1, len(at))
+    try:
+        buf.put(at.positions)
+        for _ in range(5):
+            _calc_dev(pot, at, buf)
+        L.check(L.lib().gapx_sync(pot.handle), "sync")
+        out = buf.get()
+    finally:
+        buf.free()
+    f = out["f"].reshape(-1, 3)
+    print(prec, "dE/atom vs fp64", abs(out["e"][0] - r64["energy"]) / len(at),
+          "F rmse", np.sqrt(np.mean((f - r64["force"]) ** 2)))
+    assert out["e"][0] == rs["energy"] and np.array_equal(f, rs["force"])
+    assert out["e"][0] != r64["energy"] and not np.array_equal(f, r64["force"])
+    assert abs(out["e"][0] - r64["energy"]) / len(at) < 1e-3
+    assert np.sqrt(np.mean((f - r64["force"]) ** 2)) < 1e-2
