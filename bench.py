@@ -406,6 +406,7 @@ def main():
         forms[prec] = dict(atoms_per_s=N * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
                            stage_ms={k: round(v, 5) for k, v in stages.items()})
         results[prec] = res
+        log("  %s energy %.12f, |F| sum %.12f" % (prec, res["energy"], float(np.abs(res["force"]).sum())))
         del pot
     for prec in [f for f in form_list if f in ("fp64", "bf16x3")]:
         pot = Potential(param_filename=path, precision=prec)

@@ -499,6 +499,14 @@ static int model_upload(gapx_model *m) {
         CHK(upload(c->d_Xs, c->Xs.data(), c->Xs.size() * 8));
         CHK(upload(c->d_w, w.data(), w.size() * 8));
         if (c->kind == 1) {
+            if (gp_f64_fused_supported(c->dim)) {
+                const int dp = (c->dim + 31) / 32 * 32, mp = (c->M + 31) / 32 * 32;
+                std::vector<double> xp((size_t)mp * dp, 0.0);
+                for (int r = 0; r < c->M; r++)
+                    std::copy(c->Xs.begin() + (size_t)r * c->dim, c->Xs.begin() + (size_t)(r + 1) * c->dim,
+                              xp.begin() + (size_t)r * dp);
+                CHK(upload(c->d_XsP, xp.data(), xp.size() * 8));
+            }
             CHK(upload(c->d_rb, c->rb.data(), c->rb.size() * 8));
             CHK(upload(c->d_T, c->T.data(), c->T.size() * 8));
             CHK(upload(c->d_qmap, c->qmap.data(), c->qmap.size() * 4));
@@ -577,7 +585,8 @@ void gapx::reset_arena_views(Engine &eg) {
                     &w.nb_r, &w.nb_d, &w.pairF, &w.pairE, &w.esite, &w.X, &w.norm, &w.cbuf, &w.W, &w.Wf, &w.e_part,
                     &w.G, &w.Gparts, &w.split_ws, &w.f, &w.le, &w.lv, &w.e, &w.vir, &w.minmax, &w.last_pos,
                     &w.skin_flags, &w.cmask, &w.t1_j, &w.t1_shift, &w.t1_r, &w.t2_i, &w.t2_k, &w.nl_flags, &w.eatom,
-                    &w.vK, &w.vk, &w.vV, &w.Gv, &w.pairFv, &w.pairVE, &w.varsite, &w.o_epc, &w.o_lgv, &w.o_gvg})
+                    &w.vK, &w.vk, &w.vV, &w.Gv, &w.pairFv, &w.pairVE, &w.varsite, &w.o_epc, &w.o_lgv, &w.o_gvg,
+                    &w.spart})
         b->drop_view();
     for (auto *v : {&w.centres, &w.var_inst, &w.cut2_inst})
         for (DBuf &b : *v) b.drop_view();
@@ -649,11 +658,12 @@ extern "C" int gapx_model_tuning(gapx_model *m, int coord, int max_entries, int 
     for (auto &e : c.tune) {
         if (k < max_entries) {
             if (bucket) bucket[k] = e.first;
-            if (gemm1_tile) gemm1_tile[k] = e.second.g1;
-            if (gemm2_tile) gemm2_tile[k] = e.second.g2;
-            if (gemm2_split) gemm2_split[k] = e.second.parts;
-            if (gemm1_ms) gemm1_ms[k] = e.second.g1_ms;
-            if (gemm2_ms) gemm2_ms[k] = e.second.g2_ms;
+            const bool fz = e.second.f64parts > 0;   // the fused kernel: tile -1, its parts and time
+            if (gemm1_tile) gemm1_tile[k] = fz ? -1 : e.second.g1;
+            if (gemm2_tile) gemm2_tile[k] = fz ? -1 : e.second.g2;
+            if (gemm2_split) gemm2_split[k] = fz ? e.second.f64parts : e.second.parts;
+            if (gemm1_ms) gemm1_ms[k] = fz ? e.second.f64_ms : e.second.g1_ms;
+            if (gemm2_ms) gemm2_ms[k] = fz ? 0.0 : e.second.g2_ms;
         }
         k++;
     }
@@ -1040,21 +1050,32 @@ static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
     const NLGeom *dgeo = w.geo.as<NLGeom>();
     const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
     int *flags = w.nl_flags.as<int>();
-    {
-        int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags};
-        const int zn[5] = {NC, NC, N, N, 4};
-        launch_zero_i32_multi(5, zp, zn, s);
+    // the one-block front: small problems whose geometry is on the host (the device
+    // geometry kernels write flags, which the front would clear)
+    const bool small = !pl.dev_geo && N <= kNlFrontAtoms && NC <= kNlFrontCells;
+    if (small) {
+        launch_nl_front_small(N, NC, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(),
+                              w.cell_cnt.as<int>(), w.cell_start.as<int>(), w.cell_atoms.as<int>(),
+                              w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags, s);
+    } else {
+        {
+            int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(),
+                          flags};
+            const int zn[5] = {NC, NC, N, N, 4};
+            launch_zero_i32_multi(5, zp, zn, s);
+        }
+        if (pl.dev_geo) {
+            launch_minmax_frac(pl.S, w.atom_off.as<int>(), d_pos, w.ginv.as<double>(), w.minmax.as<double>(), s);
+            launch_geom_np(pl.S, w.atom_off.as<int>(), w.minmax.as<double>(), w.geo.as<NLGeom>(), flags, s);
+        }
+        launch_nl_bin(N, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(), w.cell_cnt.as<int>(),
+                      s);
+        launch_scan_i32(w.cell_cnt.as<int>(), NC, w.cell_start.as<int>(), w.scan_part.as<int>(), s);
+        launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
+                            w.cell_unsorted.as<int>(), s);
+        launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
+                            w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
     }
-    if (pl.dev_geo) {
-        launch_minmax_frac(pl.S, w.atom_off.as<int>(), d_pos, w.ginv.as<double>(), w.minmax.as<double>(), s);
-        launch_geom_np(pl.S, w.atom_off.as<int>(), w.minmax.as<double>(), w.geo.as<NLGeom>(), flags, s);
-    }
-    launch_nl_bin(N, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(), w.cell_cnt.as<int>(), s);
-    launch_scan_i32(w.cell_cnt.as<int>(), NC, w.cell_start.as<int>(), w.scan_part.as<int>(), s);
-    launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
-                        w.cell_unsorted.as<int>(), s);
-    launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
-                        w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
     if (pl.one_pass) {
         launch_nl_pairs(2, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
                         w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
@@ -1066,19 +1087,35 @@ static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
                         nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, 0, 0,
                         nullptr, s);
     }
+    if (pl.spec && scan2_guard_fits(N)) {
+        launch_scan2_guard(w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), N, w.n1_off.as<int>(), w.n2_off.as<int>(), flags,
+                           pl.capT1, s);
+        return;
+    }
     launch_scan2_max(w.n1_cnt.as<int>(), w.n2_cnt.as<int>(), N, w.n1_off.as<int>(), w.n2_off.as<int>(), flags + 1,
                      flags + 3, w.scan_part.as<int>(), w.scan_part2.as<int>(), s);
     if (pl.spec) launch_nl_guard(N, flags, pl.capT1, w.n1_off.as<int>(), w.n2_off.as<int>(), s);
 }
 
 // The tables from the slots (or a second, filling pass), neighbour2 in the reference's
-// order and the full per-atom view.
-static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bool compact) {
+// order and the full per-atom view.  slots_only (a speculative build, whose tables no
+// later call reuses): the neighbour2 sort and the full view read the one-pass slots
+// directly and the compacted neighbour1/neighbour2 copies are not made.
+static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bool compact, bool slots_only = false) {
     Workspace &w = eg.ws;
     hipStream_t s = eg.stream;
     const int N = pl.N;
     const NLGeom *dgeo = w.geo.as<NLGeom>();
     const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
+    if (compact && slots_only) {
+        launch_nl_n2_sort(N, w.n2_off.as<int>(), w.t2_i.as<int>(), w.t2_k.as<int>(), w.cell_of.as<int>(),
+                          w.n1_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), pl.cap2, s);
+        launch_nl_full(N, d_pos, dgeo, dsid, w.n1_off.as<int>(), w.t1_j.as<int>(), w.t1_shift.as<int>(),
+                       w.t1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(),
+                       w.n1_rev.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_rev.as<int>(),
+                       w.nb_r.as<double>(), w.nb_d.as<double>(), pl.cap1, s);
+        return;
+    }
     if (compact) {
         launch_nl_compact(N, pl.cap1, pl.cap2, w.n1_off.as<int>(), w.n2_off.as<int>(), w.t1_j.as<int>(),
                           w.t1_shift.as<int>(), w.t1_r.as<double>(), w.t2_i.as<int>(), w.t2_k.as<int>(),
@@ -1091,11 +1128,11 @@ static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bo
                         w.n2_fill.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), 0, 0, nullptr, s);
     }
     launch_nl_n2_sort(N, w.n2_off.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), w.cell_of.as<int>(),
-                      w.n1_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), s);
+                      w.n1_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(), 0, s);
     launch_nl_full(N, d_pos, dgeo, dsid, w.n1_off.as<int>(), w.n1_j.as<int>(), w.n1_shift.as<int>(),
                    w.n1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(), w.n1_rev.as<int>(),
                    w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_rev.as<int>(), w.nb_r.as<double>(),
-                   w.nb_d.as<double>(), s);
+                   w.nb_d.as<double>(), 0, s);
 }
 
 // next builds' capacities from a build's flags: per-atom slots of the largest counts plus a
@@ -1168,7 +1205,7 @@ static int connect(gapx_model *m, Engine &eg, int S, const int *atom_off, const 
             launch_nl_full(N, d_pos, w.geo.as<NLGeom>(), w.sid.as<int>(), w.n1_off.as<int>(), w.n1_j.as<int>(),
                            w.n1_shift.as<int>(), w.n1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(),
                            w.n2_ks.as<int>(), w.n1_rev.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(),
-                           w.nb_rev.as<int>(), w.nb_r.as<double>(), w.nb_d.as<double>(), s);
+                           w.nb_rev.as<int>(), w.nb_r.as<double>(), w.nb_d.as<double>(), 0, s);
             eg.last_rebuilt = false;
             *P_out = 2 * eg.n1_total_last;
             return GAPX_OK;
@@ -1231,6 +1268,8 @@ int gapx::check_batch(int S, const int *atom_off, const int *Z, const double *la
 static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out, bool cached_only = false) {
     out = GemmTune();
     out.parts = gemm2_parts(nc, c.dim);
+    const bool fusable = gp_f64_fused_supported(c.dim) && c.d_XsP.p;
+    if (fusable) out.f64parts = gp_f64_parts_default(nc, c.M);   // the default: the fused kernel
     if (!m->autotune || nc <= 0) return GAPX_OK;
     int bucket = 0;
     while ((1 << bucket) < nc) bucket++;
@@ -1283,6 +1322,26 @@ static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out,
                                               w.G.as<double>(), w.Gparts.as<double>(), s); }, &t);
             if (st == GAPX_OK && t < best.g2_ms) { best.g2_ms = t; best.g2 = cfg; best.parts = parts; }
         }
+    if (fusable) {   // the fused kernel (with its partial-G sum) against the two GEMMs
+        float fbest = 1e30f;
+        int fparts = 0;
+        for (int p = 1, prev = 0; p <= 8 && st == GAPX_OK; p *= 2) {
+            const int v = gp_f64_parts_valid(p, c.M);
+            if (v == prev) continue;
+            prev = v;
+            float t = 0;
+            int nt = 0;
+            st = timed([&] {
+                launch_gp_fused_f64(nc, c.M, c.dim, v, w.X.as<double>(), c.d_XsP.as<double>(), c.d_w.as<double>(),
+                                    c.zeta, w.G.as<double>(), w.Gparts.as<double>(), w.e_part.as<double>(), &nt, s);
+                launch_gp_finish_f64(nc, c.dim, nt, w.Gparts.as<double>(), w.G.as<double>(), nt, nullptr, nullptr,
+                                     nullptr, s);
+            }, &t);
+            if (st == GAPX_OK && t < fbest) { fbest = t; fparts = v; }
+        }
+        best.f64_ms = fbest;
+        best.f64parts = (st == GAPX_OK && fbest < best.g1_ms + best.g2_ms) ? fparts : 0;
+    }
     hipEventDestroy(ea);
     hipEventDestroy(eb);
     CHK(st);
@@ -1559,7 +1618,7 @@ static int connect_external(Engine &eg, const ExtNL &x, int N, const double *d_p
     launch_nl_full(N, d_pos, w.geo.as<NLGeom>(), w.sid.as<int>(), w.n1_off.as<int>(), w.n1_j.as<int>(),
                    w.n1_shift.as<int>(), w.n1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(),
                    w.n2_ks.as<int>(), w.n1_rev.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_rev.as<int>(),
-                   w.nb_r.as<double>(), w.nb_d.as<double>(), s);
+                   w.nb_r.as<double>(), w.nb_d.as<double>(), 0, s);
     eg.n1_total_last = n1_total;
     *P_out = (int)(P - 1);
     return GAPX_OK;
@@ -1725,21 +1784,16 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     for (size_t k = 0; k < NCOORD; k++) has_d2b = has_d2b || m->coords[k]->kind == 2;
     double *pairE = has_d2b ? w.pairE.as<double>() : nullptr;
     double *d_epc = opt ? opt->d_epc : nullptr;
+    if (S == 1) CHK(w.spart.ensure((size_t)(N / 8 + 2) * 80));   // the gather's block partials
     auto launch_all = [&](bool marks) {
         if (spec) {   // the neighbour list joins the sequence; pair arrays are cleared to its count
             nl_launch_front(eg, pl, d_pos);
-            nl_launch_back(eg, pl, d_pos, true);
+            nl_launch_back(eg, pl, d_pos, true, true);
             if (marks) tm.stage(0, t_nl0, tm.mark(s));
-            const int *cnt = w.nb_off.as<int>() + N;
-            launch_zero_counted(w.pairF.as<double>(), cnt, 3, (size_t)P * 3, s);
-            if (pairE) launch_zero_counted(pairE, cnt, 1, (size_t)P, s);
-            double *zp[1] = {w.esite.as<double>()};
-            const size_t zn[1] = {(size_t)N};
-            launch_zero_f64_multi(1, zp, zn, s);
+            launch_zero_step(w.pairF.as<double>(), pairE, w.esite.as<double>(), N, w.nb_off.as<int>() + N, 0,
+                             (size_t)P, s);
         } else {
-            double *zp[3] = {w.pairF.as<double>(), w.esite.as<double>(), pairE};
-            const size_t zn[3] = {(size_t)P * 3, (size_t)N, (size_t)P};
-            launch_zero_f64_multi(has_d2b ? 3 : 2, zp, zn, s);
+            launch_zero_step(w.pairF.as<double>(), pairE, w.esite.as<double>(), N, nullptr, P, (size_t)P, s);
         }
         if (d_epc) launch_fill_f64(d_epc, NCOORD, 0.0, s);
         if (want_var) {
@@ -1781,7 +1835,12 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                             w.nb_d.as<double>(), w.X.as<double>(), w.norm.as<double>(), w.cbuf.as<double>(), s);
             const int t1 = marks ? tm.mark(s) : -1;
             int ntile = 0;
-            if (m->precision == GAPX_PREC_FP64) {
+            const bool f64fused = m->precision == GAPX_PREC_FP64 && grad && tu.f64parts > 0;
+            if (f64fused) {   // K -> W -> G in one kernel
+                launch_gp_fused_f64(nc, c.M, c.dim, tu.f64parts, w.X.as<double>(), c.d_XsP.as<double>(),
+                                    c.d_w.as<double>(), c.zeta, w.G.as<double>(), w.Gparts.as<double>(),
+                                    w.e_part.as<double>(), &ntile, s);
+            } else if (m->precision == GAPX_PREC_FP64) {
                 launch_gemm1_f64(tu.g1, nc, c.M, c.dim, w.X.as<double>(), c.d_Xs.as<double>(), c.d_w.as<double>(),
                                  c.zeta, w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
             } else if (fused[k]) {   // split forms: K -> W -> G in one kernel
@@ -1796,7 +1855,16 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                                    &ntile, w.split_ws.p, s);
             }
             const int t2 = marks ? tm.mark(s) : -1;
-            launch_energy_rows(nc, ntile, centres, w.e_part.as<double>(), w.esite.as<double>(), s);
+            // the energy rows, with the partial-G sum of a fused kernel in the same launch
+            const bool split_fused = m->precision != GAPX_PREC_FP64 && fused[k];
+            if (f64fused)
+                launch_gp_finish_f64(nc, c.dim, ntile, w.Gparts.as<double>(), w.G.as<double>(), ntile, centres,
+                                     w.e_part.as<double>(), w.esite.as<double>(), s);
+            else if (split_fused && grad)
+                launch_gp_finish_f32(nc, c.dim, ntile, w.Gparts.as<float>(), 1.0 / c.wscale, w.G.as<double>(), ntile,
+                                     centres, w.e_part.as<double>(), w.esite.as<double>(), s);
+            else
+                launch_energy_rows(nc, ntile, centres, w.e_part.as<double>(), w.esite.as<double>(), s);
             if (d_epc) launch_sum_fixed((size_t)nc * ntile, w.e_part.as<double>(), d_epc + k, s);
             if (marks) { tm.stage(1, t0, t1); tm.stage(2, t1, t2); }
             if (want_var) {   // FP64 whatever the GEMM precision
@@ -1817,11 +1885,13 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                 }
             }
             if (grad) {
-                if (m->precision == GAPX_PREC_FP64) {
+                if (f64fused) {
+                    // G is complete
+                } else if (m->precision == GAPX_PREC_FP64) {
                     launch_gemm2_f64(tu.g2, tu.parts, nc, c.M, c.dim, w.W.as<double>(), c.d_Xs.as<double>(),
                                      w.G.as<double>(), w.Gparts.as<double>(), s);
                 } else if (fused[k]) {
-                    launch_gp_fused_reduce(nc, c.dim, ntile, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
+                    // G is complete (summed with the energy rows)
                 } else {
                     launch_gemm2_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, tu.g2lds, w.Wf.as<float>(),
                                        c.d_XsT_hi.p,
@@ -1837,8 +1907,9 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         const int tg = marks ? tm.mark(s) : -1;
         launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
                       w.nb_d.as<double>(), w.pairF.as<double>(), pairE, w.esite.as<double>(), o_f, o_le,
-                      o_lv, cmask, s);
-        launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s);
+                      o_lv, cmask, S == 1 ? w.spart.as<double>() : nullptr, s);
+        if (S == 1) launch_struct_finish(N, w.spart.as<double>(), o_e, o_v, s);
+        else launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s);
         if (want_var)
             launch_gather_var(N, w.nb_off.as<int>(), w.nb_rev.as<int>(), w.pairFv.as<double>(),
                               has_d2b_var ? w.pairVE.as<double>() : nullptr, w.varsite.as<double>(), opt->d_gvg,
@@ -1860,7 +1931,7 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         key.push_back(es);
         for (size_t k = 0; k < NCOORD; k++)
             key.insert(key.end(), {(long long)eg.h_centres[k].size(), tunes[k].g1, tunes[k].g2, tunes[k].parts,
-                                   tunes[k].fparts, tunes[k].g1lds, tunes[k].g2lds});
+                                   tunes[k].fparts, tunes[k].g1lds, tunes[k].g2lds, tunes[k].f64parts});
         if (!eg.gexec || key != eg.gkey) {
             if (eg.gexec) {
                 hipGraphExecDestroy(eg.gexec);

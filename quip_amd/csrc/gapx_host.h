@@ -113,6 +113,9 @@ struct GemmTune {
     // split forms: sparse-point parts of the fused kernel (0 = its default rule), and the
     // split GEMMs' tiles (-1 = the default rule, 0 = 64 x 64, 1 = 128 x 128 LDS tiles)
     int fparts = 0, g1lds = -1, g2lds = -1;
+    // FP64: sparse-point parts of the fused GP kernel (0 = the two GEMMs) and its time
+    int f64parts = 0;
+    float f64_ms = 0;
 };
 constexpr size_t kTuneScratchCap = (size_t)256 << 20;   // largest partial-G buffer a tuning candidate may use
 
@@ -135,6 +138,7 @@ struct Coord {
     int M = 0;
     std::vector<double> Xs, alpha, scut, theta;
     DBuf d_Xs, d_w, d_theta;
+    DBuf d_XsP;   // Xs zero-padded to the fused FP64 kernel's chunks (descriptors up to 256 components)
     // mixed-precision copies of Xs (split hi/lo), built on demand
     int split_prec = -1;
     int dpad = 0, Mpad = 0;
@@ -156,7 +160,7 @@ struct Workspace {
     DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k, n2_is, n2_ks;
     DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite;
     DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws;
-    DBuf f, le, lv, e, vir, minmax;
+    DBuf f, le, lv, e, vir, minmax, spart;
     DBuf last_pos, skin_flags;   // positions of the last build (skin reuse), reuse test results
     DBuf cmask;                  // descriptor-centre mask of a caller-supplied connection
     DBuf t1_j, t1_shift, t1_r, t2_i, t2_k, nl_flags;   // per-atom slots of the one-pass neighbour search

@@ -39,12 +39,23 @@ void launch_zero_i32_multi(int k, int *const *p, const int *n, hipStream_t s);
 void launch_zero_f64_multi(int k, double *const *p, const size_t *n, hipStream_t s);
 void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out2, int *mx, int *tot1, int *part1,
                       int *part2, hipStream_t s);
+// n <= 65536: both scans (maxima to flags[1], flags[2], the in1 total to flags[3]) and the
+// overflow guard of launch_nl_guard in one launch
+bool scan2_guard_fits(int n);
+void launch_scan2_guard(const int *in1, const int *in2, int n, int *out1, int *out2, int *flags, int cap,
+                        hipStream_t s);
 // bins the atoms and counts each cell's atoms (cell_cnt zeroed by the caller)
 void launch_nl_bin(int N, const double *pos, const NLGeom *geo, const int *sid, const int *cell_base, int *cell_of,
                    int *map_shift, int *cell_cnt, hipStream_t s);
 void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s);
 void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
                          int *atoms, hipStream_t s);
+// bin, scan, fill and rank in one block for N <= kNlFrontAtoms atoms in NC <= kNlFrontCells
+// cells (also zeroes n2_cnt, n2_fill and the four flags)
+constexpr int kNlFrontAtoms = 8192, kNlFrontCells = 8192;
+void launch_nl_front_small(int N, int NC, const double *pos, const NLGeom *geo, const int *sid,
+                           const int *cell_base, int *cell_of, int *map_shift, int *cell_cnt, int *cell_start,
+                           int *cell_atoms, int *n2_cnt, int *n2_fill, int *flags, hipStream_t s);
 // mode 0: count; 1: fill the CSR tables; 2: one pass into per-atom slots of cap1 / cap2
 // (counts in n1_cnt / n2_cnt, *ovf = 1 when a slot range overflows)
 void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, const int *cell_of,
@@ -55,11 +66,13 @@ void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, c
 void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off, const int *t1_j,
                        const int *t1_shift, const double *t1_r, const int *t2_i, const int *t2_k, int *n1_j,
                        int *n1_shift, double *n1_r, int *n2_i, int *n2_k, hipStream_t s);
+// cap2 > 0: the unsorted entries are read from the one-pass slots (atom j's at j * cap2)
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
-                       const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s);
+                       const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, int cap2, hipStream_t s);
+// cap1 > 0: neighbour1 data from the one-pass slots (atom a's entry k at a * cap1 + k)
 void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
-                    const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
+                    const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d, int cap1,
                     hipStream_t s);
 // skin reuse: max displacement since the last build (bits of a double, atomicMax into a
 // zeroed word) and the stored neighbour1 distances recomputed as calc_dists does
@@ -74,6 +87,10 @@ void launch_geom_np(int S, const int *atom_off, const double *mm, NLGeom *geo, i
 void launch_nl_guard(int N, int *flags, int cap, int *n1_off, int *n2_off, hipStream_t s);
 // p[0 .. mult * (*cnt)) = 0 with *cnt read on the device; cap bounds the grid
 void launch_zero_counted(double *p, const int *cnt, int mult, size_t cap, hipStream_t s);
+// the per-step clears in one launch: pairF (3 per pair) and pairE (nullable) of *cnt pairs
+// (cnt on the device) or n_pairs (cnt null), and esite (N)
+void launch_zero_step(double *pairF, double *pairE, double *esite, int N, const int *cnt, int n_pairs,
+                      size_t cap_pairs, hipStream_t s);
 
 // upload the constant tables of the SOAP kernels to the current device (once per
 // device; done at model load so that no launch sequence contains an upload)
@@ -110,6 +127,24 @@ void launch_gemm1_f64(int cfg, int nc, int M, int d, const double *X, const doub
 int gemm2_parts(int nc, int d);
 void launch_gemm2_f64(int cfg, int parts, int nc, int M, int d, const double *W, const double *Xs, double *G,
                       double *Gparts, hipStream_t s);
+// fused FP64 GP kernel for descriptors of up to 256 components: K = X Xs^T, the energy
+// partials e_part[i*parts + p] and G = W Xs in one launch (W stays in registers);
+// XsP is the sparse matrix zero-padded to ceil(M/32)*32 rows of 32*ceil(d/32) columns.
+// With parts > 1 the partial G rows go to Gparts (parts x nc x d); launch_gp_finish_f64
+// sums them (with the energy rows).  *ntile_out = parts as the launch rounds them.
+bool gp_f64_fused_supported(int d);
+int gp_f64_parts_default(int nc, int M);
+int gp_f64_parts_valid(int parts, int M);
+void launch_gp_fused_f64(int nc, int M, int d, int parts_req, const double *X, const double *XsP,
+                         const double *wvec, double zeta, double *G, double *Gparts, double *e_part, int *ntile_out,
+                         hipStream_t s);
+void launch_sum_parts_f64(int nc, int d, int parts, const double *Gparts, double *G, hipStream_t s);
+// G = scale * (fixed-order sum of the parts partial G rows) when parts > 1, and the energy
+// rows of launch_energy_rows when centres is not null, in one launch
+void launch_gp_finish_f64(int nc, int d, int parts, const double *Gparts, double *G, int ntile, const int *centres,
+                          const double *e_part, double *esite, hipStream_t s);
+void launch_gp_finish_f32(int nc, int d, int parts, const float *Gparts, double scale, double *G, int ntile,
+                          const int *centres, const double *e_part, double *esite, hipStream_t s);
 // mixed precision forms: Xs pre-split into hi/lo (bf16 or fp16) copies, row- and column-major
 // W is scaled by wscale (a power of two) before its hi/lo split; GEMM2 divides it out.
 // lds_mode: -1 the 128 x 128 LDS tiles once they fill the GPU, else 64 x 64; 0 / 1 force
@@ -172,9 +207,14 @@ void launch_d2b_var(int N, const int *Z, int Z1, int Z2, double rc, double ctw, 
 void launch_gather_var(int N, const int *nb_off, const int *nb_rev, const double *pairFv, const double *pairVE,
                        const double *varsite, double *gvg, double *lgv, hipStream_t s);
 // mask (nullable): e0 is added to the local energy of masked atoms only
+// block_part (nullable): 10 doubles per block of 8 atoms (cdiv(N, 8) blocks), the block's
+// local energy and virial sums for launch_struct_finish (the launch covers one structure)
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
-                   double *local_e, double *local_virial, const unsigned char *mask, hipStream_t s);
+                   double *local_e, double *local_virial, const unsigned char *mask, double *block_part,
+                   hipStream_t s);
+// the energy and virial of one structure of N atoms from the gather's block partials
+void launch_struct_finish(int N, const double *part, double *e_out, double *virial_out, hipStream_t s);
 void launch_struct_reduce(int S, const int *atom_off, const double *local_e, const double *local_virial,
                           double *e_out, double *virial_out, hipStream_t s);
 

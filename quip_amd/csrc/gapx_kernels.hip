@@ -200,10 +200,9 @@ __global__ void k_scan_apply(ScanSet ss, int n) {
 // carry joining the chunks.
 constexpr int kScanOne = 1 << 16;
 constexpr int kScanChunk = 16 * 1024;
-__global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx, int *tot_out) {
-    __shared__ int buf[kScanChunk + kScanChunk / 16];
-    __shared__ int wsum[16], wmax[16];
-    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+__device__ __forceinline__ void scan_one_array(const ScanSet &ss, int b, int n, int *mx, int *tot_out, int *buf,
+                                               int *wsum, int *wmax) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int *in = ss.in[b];
     int *out = ss.out[b];
     int carry = 0, big = 0;
@@ -258,6 +257,29 @@ __global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx, i
         if (tot_out && b == 0) *tot_out = carry;
     }
 }
+__global__ __launch_bounds__(1024) void k_scan_one(ScanSet ss, int n, int *mx, int *tot_out) {
+    __shared__ int buf[kScanChunk + kScanChunk / 16];
+    __shared__ int wsum[16], wmax[16];
+    scan_one_array(ss, blockIdx.x, n, mx, tot_out, buf, wsum, wmax);
+}
+// Both neighbour counts scanned by one block, then the overflow guard of a capacity-sized
+// build (k_nl_guard's rule) on the offsets just written: one launch instead of two.
+__global__ __launch_bounds__(1024) void k_scan2_guard(ScanSet ss, int n, int *flags, int cap) {
+    __shared__ int buf[kScanChunk + kScanChunk / 16];
+    __shared__ int wsum[16], wmax[16];
+    scan_one_array(ss, 0, n, flags + 1, flags + 3, buf, wsum, wmax);
+    __syncthreads();
+    scan_one_array(ss, 1, n, flags + 1, nullptr, buf, wsum, wmax);
+    __threadfence_block();
+    __syncthreads();
+    const bool over = flags[3] > cap;
+    if (!(over || flags[0] != 0)) return;
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) {
+        ss.out[0][i] = 0;
+        ss.out[1][i] = 0;
+    }
+    if (threadIdx.x == 0 && over) atomicOr(flags, 2);
+}
 // partials: 2 * scan_partials_size(n) ints per array (tile sums, then tile maxima)
 int scan_partials_size(int n) { return 2 * cdiv(n > 0 ? n : 1, SCAN_TILE); }
 static void scan_go(const ScanSet &ss, int narr, int n, int *mx, int *tot, hipStream_t s) {
@@ -281,6 +303,78 @@ void launch_scan2_max(const int *in1, const int *in2, int n, int *out1, int *out
     const ScanSet ss = {{in1, in2}, {out1, out2}, {part1, part2}};
     scan_go(ss, 2, n, mx, tot1, s);
 }
+// The same for n <= 8192: the two arrays scanned at once, threads 0-511 the first and
+// 512-1023 the second, 16 contiguous items per thread through LDS.
+constexpr int kScan2Small = 8192;
+__global__ __launch_bounds__(1024) void k_scan2_guard_small(ScanSet ss, int n, int *flags, int cap) {
+    __shared__ int buf[2][kScan2Small + kScan2Small / 16];
+    __shared__ int wsum[2][8], wmax[2][8];
+    const int t = threadIdx.x, b = t >> 9, tl = t & 511, lane = t & 63, wv = tl >> 6;
+    const int *in = ss.in[b];
+    int *out = ss.out[b];
+    int *bb = buf[b];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int i = tl + 512 * k;
+        bb[i + (i >> 4)] = i < n ? in[i] : 0;
+    }
+    __syncthreads();
+    int v[16], sum = 0, m = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        v[k] = bb[17 * tl + k];
+        sum += v[k];
+        m = max(m, v[k]);
+    }
+    int x = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if (lane == 63) wsum[b][wv] = x;
+    if (lane == 0) wmax[b][wv] = m;
+    __syncthreads();
+    int before = 0, total = 0, big = 0;
+    for (int q = 0; q < 8; q++) {
+        before += q < wv ? wsum[b][q] : 0;
+        total += wsum[b][q];
+        big = max(big, wmax[b][q]);
+    }
+    int ex = before + x - sum;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        bb[17 * tl + k] = ex;
+        ex += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int i = tl + 512 * k;
+        if (i < n) out[i] = bb[i + (i >> 4)];
+    }
+    if (tl == 0) {
+        out[n] = total;
+        flags[1 + b] = big;
+        if (b == 0) flags[3] = total;
+    }
+    __threadfence_block();
+    __syncthreads();
+    const bool over = flags[3] > cap;
+    if (!(over || flags[0] != 0)) return;
+    for (int i = t; i <= n; i += blockDim.x) {
+        ss.out[0][i] = 0;
+        ss.out[1][i] = 0;
+    }
+    if (t == 0 && over) atomicOr(flags, 2);
+}
+bool scan2_guard_fits(int n) { return n <= kScanOne; }
+void launch_scan2_guard(const int *in1, const int *in2, int n, int *out1, int *out2, int *flags, int cap,
+                        hipStream_t s) {
+    const ScanSet ss = {{in1, in2}, {out1, out2}, {nullptr, nullptr}};
+    if (n <= kScan2Small) hipLaunchKernelGGL(k_scan2_guard_small, dim3(1), dim3(1024), 0, s, ss, n, flags, cap);
+    else hipLaunchKernelGGL(k_scan2_guard, dim3(1), dim3(1024), 0, s, ss, n, flags, cap);
+}
 
 // ----------------------------------------------------------------------------
 // neighbour list
@@ -299,12 +393,12 @@ __device__ __forceinline__ void mult3_exact(const double *g, double p0, double p
     }
 }
 
-__global__ void k_nl_bin(int N, const double *__restrict__ pos, const NLGeom *__restrict__ geo,
-                         const int *__restrict__ sid, const int *__restrict__ cell_base, int *cell_of,
-                         int *map_shift, int *cell_cnt) {
+// Cell of atom n and its map shift (Connection.f95 cell_of_pos with the lattice shift
+// that folds the position into the cell): writes map_shift[3n..3n+2], returns the cell.
+__device__ __forceinline__ int nl_bin_atom(int n, const double *__restrict__ pos, const NLGeom *__restrict__ geo,
+                                           const int *__restrict__ sid, const int *__restrict__ cell_base,
+                                           int *map_shift) {
 #pragma clang fp contract(off)
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
     const int st = sid[n];
     const NLGeom &g = geo[st];
     double lp[3];
@@ -319,7 +413,14 @@ __global__ void k_nl_bin(int N, const double *__restrict__ pos, const NLGeom *__
         c = c < 1 ? 1 : (c > g.cells[a] ? g.cells[a] : c);
         ci[a] = c - 1;
     }
-    const int cell = cell_base[st] + (ci[2] * g.cells[1] + ci[1]) * g.cells[0] + ci[0];
+    return cell_base[st] + (ci[2] * g.cells[1] + ci[1]) * g.cells[0] + ci[0];
+}
+__global__ void k_nl_bin(int N, const double *__restrict__ pos, const NLGeom *__restrict__ geo,
+                         const int *__restrict__ sid, const int *__restrict__ cell_base, int *cell_of,
+                         int *map_shift, int *cell_cnt) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int cell = nl_bin_atom(n, pos, geo, sid, cell_base, map_shift);
     cell_of[n] = cell;
     atomicAdd(&cell_cnt[cell], 1);
 }
@@ -355,6 +456,89 @@ void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int 
                          int *atoms, hipStream_t s) {
     if (N > 0)
         hipLaunchKernelGGL(k_nl_cell_rank, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, start, cnt, unsorted, atoms);
+}
+
+// The front of the neighbour list for small problems in one block of 1024 threads (each
+// phase a separate launch otherwise): zero the neighbour2 counters and the flags, bin
+// the atoms (cell counts in LDS), scan the counts into cell_start, fill the cells in
+// arrival order and rank each cell's atoms by descending index, with the cells and
+// their unsorted lists in LDS.  The same tables as the separate kernels: the order
+// within a cell is fixed by the ranking, not by the atomics.
+constexpr int kFrontThreads = 1024;
+__global__ __launch_bounds__(kFrontThreads) void k_nl_front_small(int N, int NC, const double *__restrict__ pos,
+                                                                  const NLGeom *__restrict__ geo,
+                                                                  const int *__restrict__ sid,
+                                                                  const int *__restrict__ cell_base, int *cell_of,
+                                                                  int *map_shift, int *cell_cnt, int *cell_start,
+                                                                  int *cell_atoms, int *n2_cnt, int *n2_fill,
+                                                                  int *flags) {
+    __shared__ int scnt[kNlFrontCells], sfill[kNlFrontCells];
+    __shared__ int scell[kNlFrontAtoms], suns[kNlFrontAtoms];
+    __shared__ int wsum[kFrontThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int c = t; c < NC; c += kFrontThreads) scnt[c] = 0;
+    for (int i = t; i < N; i += kFrontThreads) { n2_cnt[i] = 0; n2_fill[i] = 0; }
+    if (t < 4) flags[t] = 0;
+    __syncthreads();
+    for (int n0 = t; n0 < N; n0 += 4 * kFrontThreads) {   // four atoms' loads in flight per thread
+        int cl[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int n = n0 + u * kFrontThreads;
+            cl[u] = n < N ? nl_bin_atom(n, pos, geo, sid, cell_base, map_shift) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int n = n0 + u * kFrontThreads;
+            if (n < N) {
+                scell[n] = cl[u];
+                cell_of[n] = cl[u];
+                atomicAdd(&scnt[cl[u]], 1);
+            }
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the NC counts: thread t takes a contiguous run of per items
+    const int per = (NC + kFrontThreads - 1) / kFrontThreads;
+    const int b0 = t * per, b1 = min(NC, b0 + per);
+    int run = 0;
+    for (int c = b0; c < b1; c++) run += scnt[c];
+    int x = run;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int q = 0; q < kFrontThreads / 64; q++) {
+        before += q < wv ? wsum[q] : 0;
+        total += wsum[q];
+    }
+    int ex = before + x - run;
+    for (int c = b0; c < b1; c++) {
+        const int k = scnt[c];
+        cell_cnt[c] = k;
+        cell_start[c] = ex;
+        sfill[c] = ex;   // the fill cursor starts at the cell's first slot
+        ex += k;
+    }
+    if (t == 0) cell_start[NC] = total;
+    __syncthreads();
+    for (int n = t; n < N; n += kFrontThreads) suns[atomicAdd(&sfill[scell[n]], 1)] = n;
+    __syncthreads();
+    for (int n = t; n < N; n += kFrontThreads) {
+        const int c = scell[n], m = scnt[c], st = sfill[c] - m;   // the cursor ended at the cell's end
+        int rank = 0;
+        for (int q = 0; q < m; q++) rank += suns[st + q] > n ? 1 : 0;
+        cell_atoms[st + rank] = n;
+    }
+}
+void launch_nl_front_small(int N, int NC, const double *pos, const NLGeom *geo, const int *sid,
+                           const int *cell_base, int *cell_of, int *map_shift, int *cell_cnt, int *cell_start,
+                           int *cell_atoms, int *n2_cnt, int *n2_fill, int *flags, hipStream_t s) {
+    hipLaunchKernelGGL(k_nl_front_small, dim3(1), dim3(kFrontThreads), 0, s, N, NC, pos, geo, sid, cell_base, cell_of,
+                       map_shift, cell_cnt, cell_start, cell_atoms, n2_cnt, n2_fill, flags);
 }
 
 // Wave-parallel form of the same search: one wavefront per atom1, lanes over
@@ -593,19 +777,23 @@ void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *
 // key (all keys distinct; up to 32 entries are staged in LDS first) and writes them
 // in order; it also records, for every neighbour1 entry, the slot of its mirror
 // in neighbour2.
+// SLOT: the unsorted entries are read from the per-atom slots (atom j's at j * cap2)
+// of a one-pass search, with no compacted copy.
+template <bool SLOT>
 __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k,
                                                     const int *cell_of, const int *n1_off, int *n2_is, int *n2_ks,
-                                                    int *n1_rev) {
+                                                    int *n1_rev, int cap2) {
     __shared__ int s_i[8][32], s_k[8][32], s_c[8][32];   // the group's entries when they fit one row
     const int lane = threadIdx.x & 31, wv = threadIdx.x >> 5;
     const int j = blockIdx.x * 8 + wv;
     if (j >= N) return;   // the whole group leaves together
     const int b = n2_off[j], n = n2_off[j + 1] - b;
+    const size_t bs = SLOT ? (size_t)j * cap2 : (size_t)b;   // where the unsorted entries sit
     if (n <= 32) {
         int vi = 0, vk = 0, vc = 0;
         if (lane < n) {
-            vi = n2_i[b + lane];
-            vk = n2_k[b + lane];
+            vi = n2_i[bs + lane];
+            vk = n2_k[bs + lane];
             vc = cell_of[vi];
             s_i[wv][lane] = vi;
             s_k[wv][lane] = vk;
@@ -628,10 +816,10 @@ __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, co
         return;
     }
     for (int x = lane; x < n; x += 32) {
-        const int vi = n2_i[b + x], vk = n2_k[b + x], vc = cell_of[vi];
+        const int vi = n2_i[bs + x], vk = n2_k[bs + x], vc = cell_of[vi];
         int rank = 0;
         for (int y = 0; y < n; y++) {
-            const int yi = n2_i[b + y], yk = n2_k[b + y], yc = cell_of[yi];
+            const int yi = n2_i[bs + y], yk = n2_k[bs + y], yc = cell_of[yi];
             const bool before = (yc < vc) || (yc == vc && (yi > vi || (yi == vi && yk < vk)));
             rank += before ? 1 : 0;
         }
@@ -641,23 +829,32 @@ __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, co
     }
 }
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
-                       const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, hipStream_t s) {
-    if (N > 0)
-        hipLaunchKernelGGL(k_nl_n2_sort, dim3(cdiv(N, 8)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of, n1_off,
-                           n2_is, n2_ks, n1_rev);
+                       const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, int cap2, hipStream_t s) {
+    if (N <= 0) return;
+    if (cap2 > 0)
+        hipLaunchKernelGGL(k_nl_n2_sort<true>, dim3(cdiv(N, 8)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of,
+                           n1_off, n2_is, n2_ks, n1_rev, cap2);
+    else
+        hipLaunchKernelGGL(k_nl_n2_sort<false>, dim3(cdiv(N, 8)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of,
+                           n1_off, n2_is, n2_ks, n1_rev, 0);
 }
 
 // Full per-atom neighbour view in connection_neighbour order (neighbour2 first,
 // Connection.f95:2038-2051), with diff = pos_j - pos_i + lattice.shift summed
 // m = 1..3 (:2120-2125) and the index of the mirror pair.  A group of 32 lanes
 // per atom, lanes over its entries (a covalent solid has about 30).
+// SLOT: the neighbour1 data (j, shift, r) are read from the per-atom slots of a one-pass
+// search (atom a's entry k at a * cap1 + k) instead of the compacted tables; offsets,
+// neighbour2 and the mirror slots n1_rev keep the compacted numbering.
+template <bool SLOT>
 __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict__ pos,
                                                  const NLGeom *__restrict__ geo, const int *__restrict__ sid,
                                                  const int *n1_off, const int *n1_j, const int *n1_shift,
                                                  const double *n1_r, const int *n2_off, const int *n2_i,
                                                  const int *n2_k, const int *n1_rev, int *nb_off, int *nb_j,
-                                                 int *nb_rev, double *nb_r, double *nb_d) {
+                                                 int *nb_rev, double *nb_r, double *nb_d, int cap1) {
 #pragma clang fp contract(off)
+    auto at1 = [&](int a, int k) -> size_t { return SLOT ? (size_t)a * cap1 + k : (size_t)n1_off[a] + k; };
     const int lane = threadIdx.x & 31;
     const int i = blockIdx.x * 8 + (threadIdx.x >> 5);
     if (i > N) return;
@@ -668,22 +865,25 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
     const int n2c = n2_off[i + 1] - n2_off[i];
     const int n1c = n1_off[i + 1] - n1_off[i];
     for (int q = lane; q < n2c + n1c; q += 32) {
-        int j, e1, sgn, rev;
+        int j, sgn, rev;
+        size_t e1;   // the neighbour1 entry's data
         if (q < n2c) {
             j = n2_i[n2_off[i] + q];
-            e1 = n1_off[j] + n2_k[n2_off[i] + q];
+            e1 = at1(j, n2_k[n2_off[i] + q]);
             sgn = -1;
             rev = n1_off[j] + n2_off[j] + (n2_off[j + 1] - n2_off[j]) + n2_k[n2_off[i] + q];
         } else {
-            e1 = n1_off[i] + (q - n2c);
+            e1 = at1(i, q - n2c);
             j = n1_j[e1];
             sgn = 1;
-            if (j != i) rev = n1_off[j] + n2_off[j] + n1_rev[e1];
+            if (j != i) rev = n1_off[j] + n2_off[j] + n1_rev[n1_off[i] + (q - n2c)];
             else {
                 rev = -1;
-                for (int t = n1_off[i]; t < n1_off[i + 1]; t++)
+                for (int k = 0; k < n1c; k++) {
+                    const size_t t = at1(i, k);
                     if (n1_j[t] == i && n1_shift[3 * t] == -n1_shift[3 * e1] && n1_shift[3 * t + 1] == -n1_shift[3 * e1 + 1] &&
-                        n1_shift[3 * t + 2] == -n1_shift[3 * e1 + 2]) { rev = base + n2c + (t - n1_off[i]); break; }
+                        n1_shift[3 * t + 2] == -n1_shift[3 * e1 + 2]) { rev = base + n2c + k; break; }
+                }
                 if (rev < 0) rev = base + q;   // mirror image missing (cannot happen for an exact cutoff test)
             }
         }
@@ -701,10 +901,14 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
 }
 void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
-                    const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d,
+                    const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d, int cap1,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_nl_full, dim3(cdiv(N + 1, 8)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j, n1_shift, n1_r,
-                       n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d);
+    if (cap1 > 0)
+        hipLaunchKernelGGL(k_nl_full<true>, dim3(cdiv(N + 1, 8)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j,
+                           n1_shift, n1_r, n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d, cap1);
+    else
+        hipLaunchKernelGGL(k_nl_full<false>, dim3(cdiv(N + 1, 8)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j,
+                           n1_shift, n1_r, n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d, 0);
 }
 
 // Skin reuse (calc_connect with cutoff_skin, Connection.f95:1085-1127).
@@ -898,6 +1102,25 @@ __global__ void k_zero_counted(double *p, const int *cnt, int mult) {
 void launch_zero_counted(double *p, const int *cnt, int mult, size_t cap, hipStream_t s) {
     const int blocks = std::max(1, std::min(cdiv((long)cap, 256), 2048));
     hipLaunchKernelGGL(k_zero_counted, dim3(blocks), dim3(256), 0, s, p, cnt, mult);
+}
+
+// The per-step clears in one launch: pair forces (3 per pair) and pair energies
+// (nullable) of the first *cnt pairs (cnt on the device) or of n_pairs when cnt is
+// null, and the N site energies.
+__global__ void k_zero_step(double *pairF, double *pairE, double *esite, int N, const int *cnt, int n_pairs) {
+    const size_t np = cnt ? (size_t)(*cnt) : (size_t)n_pairs;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t q = i0; q < 3 * np; q += stride) pairF[q] = 0.0;
+    if (pairE)
+        for (size_t q = i0; q < np; q += stride) pairE[q] = 0.0;
+    for (size_t q = i0; q < (size_t)N; q += stride) esite[q] = 0.0;
+}
+void launch_zero_step(double *pairF, double *pairE, double *esite, int N, const int *cnt, int n_pairs,
+                      size_t cap_pairs, hipStream_t s) {
+    const size_t mx = std::max<size_t>(3 * cap_pairs, (size_t)N);
+    const int blocks = std::max(1, std::min(cdiv((long)mx, 256), 2048));
+    hipLaunchKernelGGL(k_zero_step, dim3(blocks), dim3(256), 0, s, pairF, pairE, esite, N, cnt, n_pairs);
 }
 
 // ----------------------------------------------------------------------------
@@ -1493,10 +1716,51 @@ void launch_gemm2_f64(int cfg, int parts, int nc, int M, int d, const double *W,
         case 4: gemm2_go<2, 2, 32>(nc, M, d, parts, W, Xs, out, s); break;
         default: gemm2_go<2, 4, 32>(nc, M, d, parts, W, Xs, out, s); break;
     }
-    if (parts > 1) {
-        const size_t n = (size_t)nc * d;
-        hipLaunchKernelGGL(k_sum_parts, dim3(cdiv((long)n, 256)), dim3(256), 0, s, n, parts, Gparts, G);
+    if (parts > 1) launch_sum_parts_f64(nc, d, parts, Gparts, G, s);
+}
+
+// The end of a fused GP launch in one kernel: G = scale * (sum over parts, fixed order)
+// of the partial rows (FP64 or FP32; blocks [0, gblocks)), and the centres' energy rows
+// esite[centres[i]] += sum of e_part[i][0..ntile) (the blocks after them).
+template <typename T>
+__global__ void k_gp_finish(size_t n, int parts, const T *P, double scale, double *G, int gblocks, int nc, int ntile,
+                            const int *centres, const double *e_part, double *esite) {
+    const int b = blockIdx.x;
+    if (b < gblocks) {
+        const size_t i = (size_t)b * blockDim.x + threadIdx.x;
+        if (i >= n) return;
+        double s = 0.0;
+        for (int p = 0; p < parts; p++) s += (double)P[(size_t)p * n + i];
+        G[i] = s * scale;
+        return;
     }
+    const int i = (b - gblocks) * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    double s = 0.0;
+    for (int t = 0; t < ntile; t++) s += e_part[(size_t)i * ntile + t];
+    esite[centres[i]] += s;
+}
+template <typename T>
+static void gp_finish_go(int nc, int d, int parts, const T *Gparts, double scale, double *G, int ntile,
+                         const int *centres, const double *e_part, double *esite, hipStream_t s) {
+    const size_t n = (size_t)nc * d;
+    const int gb = parts > 1 ? cdiv((long)n, 256) : 0, eb = centres ? cdiv(nc, 256) : 0;
+    if (nc > 0 && gb + eb > 0)
+        hipLaunchKernelGGL(k_gp_finish<T>, dim3(gb + eb), dim3(256), 0, s, n, parts, Gparts, scale, G, gb, nc, ntile,
+                           centres, e_part, esite);
+}
+void launch_gp_finish_f64(int nc, int d, int parts, const double *Gparts, double *G, int ntile, const int *centres,
+                          const double *e_part, double *esite, hipStream_t s) {
+    gp_finish_go<double>(nc, d, parts, Gparts, 1.0, G, ntile, centres, e_part, esite, s);
+}
+void launch_gp_finish_f32(int nc, int d, int parts, const float *Gparts, double scale, double *G, int ntile,
+                          const int *centres, const double *e_part, double *esite, hipStream_t s) {
+    gp_finish_go<float>(nc, d, parts, Gparts, scale, G, ntile, centres, e_part, esite, s);
+}
+
+void launch_sum_parts_f64(int nc, int d, int parts, const double *Gparts, double *G, hipStream_t s) {
+    const size_t n = (size_t)nc * d;
+    if (n > 0) hipLaunchKernelGGL(k_sum_parts, dim3(cdiv((long)n, 256)), dim3(256), 0, s, n, parts, Gparts, G);
 }
 
 // e_i = sum over column tiles, fixed order
@@ -1760,7 +2024,8 @@ __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const
                                                       const int *nb_off, const int *nb_rev, const double *nb_d,
                                                       const double *pairF, const double *pairE, const double *esite,
                                                       double *f, double *local_e, double *local_virial,
-                                                      const unsigned char *mask) {
+                                                      const unsigned char *mask, double *block_part) {
+    __shared__ double sv[8][10];   // the block's atoms' local energy and virial (block partials)
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, hl = lane & 31;
     const int j = (blockIdx.x * 4 + w) * 2 + (lane >> 5);
     const bool live = j < N;
@@ -1783,62 +2048,130 @@ __global__ __launch_bounds__(GATHER_BS) void k_gather(int N, const int *Z, const
     }
 #pragma unroll
     for (int k = 0; k < 13; k++) v[k] = half_sum_g(v[k]);
-    if (!live) return;
-    if (hl < 3 && f) f[3 * j + hl] = E_scale * (hl == 0 ? v[0] : (hl == 1 ? v[1] : v[2]));
-    if (hl == 0) local_e[j] = E_scale * (esite[j] + 0.5 * v[12] + ((mask && !mask[j]) ? 0.0 : e0[Z[j]]));
-    if (hl < 9) {
-        double x = v[3];
+    const int a = (w << 1) | (lane >> 5);   // the atom's place in the block
+    if (live) {
+        if (hl < 3 && f) f[3 * j + hl] = E_scale * (hl == 0 ? v[0] : (hl == 1 ? v[1] : v[2]));
+        if (hl == 0) {
+            const double le = E_scale * (esite[j] + 0.5 * v[12] + ((mask && !mask[j]) ? 0.0 : e0[Z[j]]));
+            local_e[j] = le;
+            if (block_part) sv[a][0] = le;
+        }
+        if (hl < 9) {
+            double x = v[3];
 #pragma unroll
-        for (int k = 1; k < 9; k++) x = (hl == k) ? v[3 + k] : x;
-        local_virial[9 * (size_t)j + hl] = E_scale * x;
+            for (int k = 1; k < 9; k++) x = (hl == k) ? v[3 + k] : x;
+            local_virial[9 * (size_t)j + hl] = E_scale * x;
+            if (block_part) sv[a][1 + hl] = E_scale * x;
+        }
+    } else if (block_part && hl < 10) {
+        sv[a][hl] = 0.0;
+    }
+    if (!block_part) return;
+    __syncthreads();
+    if (threadIdx.x < 10) {   // the block's 8 atoms in order
+        double x = sv[0][threadIdx.x];
+        for (int q = 1; q < 8; q++) x += sv[q][threadIdx.x];
+        block_part[10 * (size_t)blockIdx.x + threadIdx.x] = x;
     }
 }
 void launch_gather(int N, const int *Z, const double *e0, double E_scale, const int *nb_off, const int *nb_rev,
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
-                   double *local_e, double *local_virial, const unsigned char *mask, hipStream_t s) {
+                   double *local_e, double *local_virial, const unsigned char *mask, double *block_part,
+                   hipStream_t s) {
     if (N > 0)
         hipLaunchKernelGGL(k_gather, dim3(cdiv(N, 8)), dim3(GATHER_BS), 0, s, N, Z, e0, E_scale, nb_off, nb_rev,
-                           nb_d, pairF, pairE, esite, f, local_e, local_virial, mask);
+                           nb_d, pairF, pairE, esite, f, local_e, local_virial, mask, block_part);
 }
 
 // Energy and virial of each structure: E = sum of local energies, V = sum of
-// local virials (IPModel_GAP.f95:524-526).  One block per structure; thread t
-// takes the structure's atoms t, t+256, ... and a fixed tree finishes the sum,
+// local virials (IPModel_GAP.f95:524-526).  One block of 512 threads per structure;
+// thread t sums the structure's atoms t, t+512, ... in that order (four atoms' loads
+// issued together), a fixed shuffle tree joins the lanes and threads 0..9 the 8 waves,
 // so a structure's totals do not depend on the other structures of a batch.
-__global__ __launch_bounds__(256) void k_struct_reduce(const int *atom_off, const double *local_e,
-                                                       const double *local_virial, double *e_out,
-                                                       double *virial_out) {
-    __shared__ double sred[10][256];
-    const int st = blockIdx.x;
+constexpr int kReduceThreads = 512;
+__global__ __launch_bounds__(kReduceThreads) void k_struct_reduce(const int *atom_off, const double *local_e,
+                                                                  const double *local_virial, double *e_out,
+                                                                  double *virial_out) {
+    __shared__ double sred[10][kReduceThreads / 64];
+    const int st = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int a0 = atom_off[st], a1 = atom_off[st + 1];
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
-    for (int j = a0 + threadIdx.x; j < a1; j += 256) {
-        acc[0] += local_e[j];
+    for (int j0 = a0 + t; j0 < a1; j0 += 4 * kReduceThreads) {
+        double v[4][10];
 #pragma unroll
-        for (int k = 0; k < 9; k++) acc[1 + k] += local_virial[9 * (size_t)j + k];
+        for (int u = 0; u < 4; u++) {
+            const int j = j0 + u * kReduceThreads;
+            const bool ok = j < a1;
+            v[u][0] = ok ? local_e[j] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) v[u][1 + k] = ok ? local_virial[9 * (size_t)j + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int k = 0; k < 10; k++) acc[k] += v[u][k];
     }
 #pragma unroll
-    for (int k = 0; k < 10; k++) sred[k][threadIdx.x] = acc[k];
+    for (int k = 0; k < 10; k++) {
+        double x = acc[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) sred[k][wv] = x;
+    }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
+    if (t < 10) {
+        double x = sred[t][0];
+        for (int q = 1; q < kReduceThreads / 64; q++) x += sred[t][q];
+        if (t == 0) e_out[st] = x;
+        else virial_out[9 * (size_t)st + t - 1] = x;
+    }
+}
+// One structure: the totals from the gather's block partials (10 per block, blocks of 8
+// atoms in order): thread t sums partials t, t+512, ... in turn, then the fixed tree.
+__global__ __launch_bounds__(kReduceThreads) void k_struct_finish(int nb, const double *part, double *e_out,
+                                                                  double *virial_out) {
+    __shared__ double sred[10][kReduceThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double acc[10];
 #pragma unroll
-            for (int k = 0; k < 10; k++) sred[k][threadIdx.x] += sred[k][threadIdx.x + s];
-        __syncthreads();
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    for (int b0 = t; b0 < nb; b0 += 2 * kReduceThreads) {
+        double v[2][10];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int b = b0 + u * kReduceThreads;
+#pragma unroll
+            for (int k = 0; k < 10; k++) v[u][k] = b < nb ? part[10 * (size_t)b + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int k = 0; k < 10; k++) acc[k] += v[u][k];
     }
-    if (threadIdx.x < 10) {
-        const double x = sred[threadIdx.x][0];
-        if (threadIdx.x == 0) e_out[st] = x;
-        else virial_out[9 * (size_t)st + threadIdx.x - 1] = x;
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        double x = acc[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) sred[k][wv] = x;
     }
+    __syncthreads();
+    if (t < 10) {
+        double x = sred[t][0];
+        for (int q = 1; q < kReduceThreads / 64; q++) x += sred[t][q];
+        if (t == 0) e_out[0] = x;
+        else virial_out[t - 1] = x;
+    }
+}
+void launch_struct_finish(int N, const double *part, double *e_out, double *virial_out, hipStream_t s) {
+    if (N > 0)
+        hipLaunchKernelGGL(k_struct_finish, dim3(1), dim3(kReduceThreads), 0, s, cdiv(N, 8), part, e_out, virial_out);
 }
 void launch_struct_reduce(int S, const int *atom_off, const double *local_e, const double *local_virial,
                           double *e_out, double *virial_out, hipStream_t s) {
     if (S > 0)
-        hipLaunchKernelGGL(k_struct_reduce, dim3(S), dim3(256), 0, s, atom_off, local_e, local_virial, e_out,
-                           virial_out);
+        hipLaunchKernelGGL(k_struct_reduce, dim3(S), dim3(kReduceThreads), 0, s, atom_off, local_e, local_virial,
+                           e_out, virial_out);
 }
 
 }  // namespace gapx

@@ -208,3 +208,35 @@ def test_split_form_device_calls(path, prec):
     assert out["e"][0] != r64["energy"] and not np.array_equal(f, r64["force"])
     assert abs(out["e"][0] - r64["energy"]) / len(at) < 1e-3
     assert np.sqrt(np.mean((f - r64["force"]) ** 2)) < 1e-2
+
+
+def test_forms_in_turn_on_shared_buffers(path):
+    """The bench's pattern: an FP64 model and then a split-form model write the same
+    device output buffers, each over many timed device calls; each form's results are
+    its own (the split form's differ from FP64's and equal its first exact call's)."""
+    at = configs.perturb(configs.diamond(5.43, (4, 4, 4)), 0.1, np.random.default_rng(11))
+    buf = _Buf(1, len(at))
+    outs = {}
+    try:
+        buf.put(at.positions)
+        for prec in ("fp64", "bf16x3"):
+            pot = Potential(param_filename=path, precision=prec)
+            for _ in range(3):
+                _calc_dev(pot, at, buf)
+            L.check(L.lib().gapx_sync(pot.handle), "sync")
+            pot.set_timing(True)
+            for _ in range(20):
+                _calc_dev(pot, at, buf)
+            L.check(L.lib().gapx_sync(pot.handle), "sync")
+            pot.timing_totals()
+            pot.set_timing(False)
+            outs[prec] = buf.get()
+            del pot
+    finally:
+        buf.free()
+    refs = {p: Potential(param_filename=path, precision=p).calc(at.copy(), force=True) for p in ("fp64", "bf16x3")}
+    for p in ("fp64", "bf16x3"):
+        print(p, outs[p]["e"][0], refs[p]["energy"])
+        assert outs[p]["e"][0] == refs[p]["energy"], p
+        assert np.array_equal(outs[p]["f"].reshape(-1, 3), refs[p]["force"]), p
+    assert outs["fp64"]["e"][0] != outs["bf16x3"]["e"][0]
