@@ -466,6 +466,7 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __shared__ __attribute__((aligned(16))) T xs_l[2][2][32 * DP];   // [buffer][hi, lo]
     __shared__ __attribute__((aligned(16))) T xt_l[2][2][DP * 32];
     __shared__ __attribute__((aligned(16))) T w_l[8][2][16 * 32];
+    __shared__ double wv_l[2][32];   // the chunk's w_s (0 past M), staged with it
     auto xs_at = [](int r, int c) { return r * DP + ((c ^ (r & XSM)) << 3); };
     auto xt_at = [](int q, int c) { return q * 32 + ((c ^ ((q >> 1) & 3)) << 3); };
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -500,7 +501,9 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     constexpr int XS_N = 32 * (DP / 8), XT_N = DP * 4;
     constexpr int XS_V = (XS_N + 511) / 512, XT_V = (XT_N + 511) / 512;
     V ps_h[XS_V], ps_l[XS_V], pt_h[XT_V], pt_l[XT_V];
+    double pw = 0.0;
     auto fetch = [&](int s0) {
+        if (t < 32) pw = s0 + t < M ? wvec[s0 + t] : 0.0;
 #pragma unroll
         for (int u = 0; u < XS_V; u++) {
             const int idx = t + 512 * u;
@@ -523,6 +526,7 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
         }
     };
     auto put = [&](int b) {   // the fetched chunk -> LDS buffer b
+        if (t < 32) wv_l[b][t] = pw;
 #pragma unroll
         for (int u = 0; u < XS_V; u++) {
             const int idx = t + 512 * u;
@@ -568,8 +572,7 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
         // common zeta = 4 (W is then split into 16-bit halves), in FP64 otherwise.
 #pragma unroll
         for (int j = 0; j < 2; j++) {
-            const int sp = s0 + 16 * j + r16;
-            const double wsv = sp < M ? wvec[sp] : 0.0;
+            const double wsv = wv_l[b][16 * j + r16];
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 T h, lo;
