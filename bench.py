@@ -290,16 +290,29 @@ def run_serving(path, prec, n_pool, duration, fracs, n_slots, max_batch_atoms):
     return out
 
 
-def gemm1_traffic():
-    """HBM bytes per GEMM 1 launch from the committed counter summary (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration it records), or None."""
-    path = os.path.join(ROOT, "profiles", "k_gemm_f64_gemm1_si4096_traffic.txt")
+def kernel_traffic(name):
+    """HBM bytes per launch from a committed counter summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration it records), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("hbm_bytes_per_launch"):
             return float(line.split()[1])
     return None
+
+
+def fp64_gp_choice(pot):
+    """The FP64 GP form the autotuner chose for coordinate 0: ("fused", parts) or ("gemms", None)."""
+    n = C.c_int()
+    b, g1, g2, sp = (np.zeros(8, np.int32) for _ in range(4))
+    m1, m2 = np.zeros(8), np.zeros(8)
+    L.check(L.lib().gapx_model_tuning(pot.handle, 0, 8, L.ptr(b), L.ptr(g1), L.ptr(g2), L.ptr(sp), L.ptr(m1),
+                                      L.ptr(m2), C.byref(n)), "gapx_model_tuning")
+    k = n.value - 1
+    if k >= 0 and g1[k] == -1:
+        return "fused", int(sp[k])
+    return "gemms", None
 
 
 def stage_counters():
@@ -399,10 +412,13 @@ def main():
         dev.hip.hipMemcpy(dst, src, C.c_size_t(nb), 3)
         dev.hip.hipDeviceSynchronize()
         log("copy probe: %d bytes device-to-device" % nb)
+    gp_choice = ("gemms", None)
     for prec in form_list:
         pot = Potential(param_filename=path, precision=prec)
         log("timing", prec)
         dt, res, stages = run_form(pot, at, dev, bufs, args.steps, args.warmup)
+        if prec == "fp64":
+            gp_choice = fp64_gp_choice(pot)
         forms[prec] = dict(atoms_per_s=N * args.steps / dt, ms_per_step=1e3 * dt / args.steps,
                            stage_ms={k: round(v, 5) for k, v in stages.items()})
         results[prec] = res
@@ -495,14 +511,23 @@ def main():
                                         (0.5, 0.8, 0.95) if k == 0 else (), 2, 16384)
     st = forms["fp64"]["stage_ms"]
     nc, M, d = N, model.coords[0].n_sparse, model.coords[0].descriptor.dim
-    gemm_flops = 2.0 * nc * M * d
-    achieved = gemm_flops / (st["gemm1"] * 1e-3)
-    roofline = dict(bound="mfma", kernel="k_gemm_f64 GEMM 1 (K = X Xs^T, fused pow/alpha epilogue; FP64 MFMA "
-                    "16x16x4)", achieved=achieved / 1e12, peak=MFMA_FP64_PEAK / 1e12, unit="TFLOP/s",
-                    frac=achieved / MFMA_FP64_PEAK, traffic=gemm1_traffic(), flops_per_launch=gemm_flops,
-                    avg_launch_ms=st["gemm1"],
-                    traffic_source="HBM bytes per launch: FETCH_SIZE + WRITE_SIZE of this kernel scaled by a "
-                                   "copy-kernel calibration, profiles/k_gemm_f64_gemm1_si4096_traffic.txt")
+    if gp_choice[0] == "fused":   # K = X Xs^T, W and G = W Xs in one kernel: both products
+        gp_flops = 4.0 * nc * M * d
+        roofline = dict(bound="mfma", kernel="k_gp_fused_f64 (K = X Xs^T, energy and W = w zeta K^(zeta-1), "
+                        "G = W Xs in one pass, W in registers; FP64 MFMA 16x16x4; %d sparse-point parts)"
+                        % gp_choice[1], traffic=kernel_traffic("k_gp_fused_f64_si4096_traffic.txt"),
+                        traffic_source="HBM bytes per launch: FETCH_SIZE + WRITE_SIZE of this kernel scaled by a "
+                                       "copy-kernel calibration, profiles/k_gp_fused_f64_si4096_traffic.txt")
+    else:
+        gp_flops = 2.0 * nc * M * d
+        roofline = dict(bound="mfma", kernel="k_gemm_f64 GEMM 1 (K = X Xs^T, fused pow/alpha epilogue; FP64 MFMA "
+                        "16x16x4)", traffic=kernel_traffic("k_gemm_f64_gemm1_si4096_traffic.txt"),
+                        traffic_source="HBM bytes per launch: FETCH_SIZE + WRITE_SIZE of this kernel scaled by a "
+                                       "copy-kernel calibration, profiles/k_gemm_f64_gemm1_si4096_traffic.txt")
+    achieved = gp_flops / (st["gemm1"] * 1e-3)
+    roofline.update(achieved=achieved / 1e12, peak=MFMA_FP64_PEAK / 1e12, unit="TFLOP/s",
+                    frac=achieved / MFMA_FP64_PEAK, flops_per_launch=gp_flops, avg_launch_ms=st["gemm1"],
+                    time_source="HIP events around the kernel on the library's stream, this run")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

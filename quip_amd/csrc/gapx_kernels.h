@@ -167,8 +167,9 @@ void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, i
                            const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
                            int *ntile_out, hipStream_t s);
 void launch_gp_fused_reduce(int nc, int d, int parts, const float *Gpart, double wscale, double *G, hipStream_t s);
+// chunk_order (transpose only): sparse points permuted within each 32 as the fused kernel reads them
 void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
-                         void *hi, void *lo, hipStream_t s);
+                         void *hi, void *lo, hipStream_t s, bool chunk_order = false);
 
 void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_part, double *esite, hipStream_t s);
 // mask (nullable): only atoms with mask[i] != 0 are the first atom of a pair descriptor.

@@ -51,14 +51,20 @@ struct Half<2> {   // fp16
 // split rows x cols FP64 (row-major, ld src_ld) into hi/lo with leading dimension
 // ld_out; transpose=true writes the transpose (cols x rows).  Padding is zero.
 template <int PREC>
+// chunk_order (transposed copies): within every 32 output positions the sparse points
+// are in the fused kernel's order [0-3, 16-19 | 4-7, 20-23 | 8-11, 24-27 | 12-15, 28-31]
 __global__ void k_split(int rows, int cols, int ld_out, int rows_pad, const double *src, int src_ld, bool transpose,
-                        typename Half<PREC>::T *hi, typename Half<PREC>::T *lo, double scale) {
+                        typename Half<PREC>::T *hi, typename Half<PREC>::T *lo, double scale, bool chunk_order) {
     typedef Half<PREC> H;
     const size_t tot = transpose ? (size_t)cols * ld_out : (size_t)rows_pad * ld_out;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
         int r, c;
         if (!transpose) { r = (int)(idx / ld_out); c = (int)(idx % ld_out); }
         else { c = (int)(idx / ld_out); r = (int)(idx % ld_out); }
+        if (chunk_order) {
+            const int p = r & 31, slot = p >> 2;
+            r = (r & ~31) + 4 * ((slot >> 1) + 4 * (slot & 1)) + (p & 3);
+        }
         double x = (r < rows && c < cols) ? src[(size_t)r * src_ld + c] * scale : 0.0;
         typename H::T h = H::from(x);
         typename H::T l = H::from(x - H::to(h));
@@ -68,16 +74,16 @@ __global__ void k_split(int rows, int cols, int ld_out, int rows_pad, const doub
 }
 
 void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
-                         void *hi, void *lo, hipStream_t s) {
+                         void *hi, void *lo, hipStream_t s, bool chunk_order) {
     const int rows_pad = (rows + 63) / 64 * 64;
     const size_t tot = transpose ? (size_t)cols * ld_out : (size_t)rows_pad * ld_out;
     const int grid = (int)std::min<size_t>((tot + 255) / 256, 65536);
     if (prec == 1)
         hipLaunchKernelGGL(k_split<1>, dim3(grid), dim3(256), 0, s, rows, cols, ld_out, rows_pad, src, src_ld, transpose,
-                           (__bf16 *)hi, (__bf16 *)lo, 1.0);
+                           (__bf16 *)hi, (__bf16 *)lo, 1.0, chunk_order);
     else
         hipLaunchKernelGGL(k_split<2>, dim3(grid), dim3(256), 0, s, rows, cols, ld_out, rows_pad, src, src_ld, transpose,
-                           (_Float16 *)hi, (_Float16 *)lo, 1.0);
+                           (_Float16 *)hi, (_Float16 *)lo, 1.0, chunk_order);
 }
 
 // C[row, col] = sum_k A[row, k] B[col, k] with A, B hi/lo split, k-contiguous,
@@ -423,21 +429,175 @@ static bool use_lds_tiles(int rows, int cols, int lds_mode = -1) {
     return (long)cdivm(rows, kLT) * cdivm(cols, kLT) >= 512;
 }
 
+static __device__ __attribute__((aligned(16))) unsigned int kZero16[4];   // a zero 16-byte vector (never written)
+
+// 16 bytes per lane from global memory straight into LDS: lane l's bytes land at
+// lds + 16 l (lds uniform over the wavefront)
+__device__ __forceinline__ void dma16(const void *g, void *lds) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
+
+// LDS rows without padding; 16-byte vectors XOR-swizzled so that the rows one fragment
+// read touches fall on distinct 16-byte bank groups in every group of 8 lanes (a 16-byte
+// read moves 128 bytes per cycle):
+//   Xs rows (DP/8 vectors):  vector c of row r at c ^ (r & XSM)
+//   Xs^T rows (4 vectors of 8 sparse points in the chunk order, 64 bytes): vector c of
+//   row q at c ^ ((q >> 1) & 3), so rows q..q+7 cover the 8 groups of 128 bytes
+template <int DK>
+struct FusedLds {
+    static constexpr int DP = 32 * DK, XSV = DP / 8, XSM = XSV >= 16 ? 15 : XSV - 1;
+    static __device__ __forceinline__ int xs_at(int r, int c) { return r * DP + ((c ^ (r & XSM)) << 3); }
+    static __device__ __forceinline__ int xt_at(int q, int c) { return q * 32 + ((c ^ ((q >> 1) & 3)) << 3); }
+};
+
+// Stage chunk s0 (32 sparse points) into one LDS buffer: the Xs rows and the
+// (chunk-ordered) Xs^T rows go from global memory straight to LDS (global_load_lds, 16
+// bytes per lane): wave instruction u fills LDS vectors 64u .. 64u+63 of a matrix, each
+// lane loading the global vector that the swizzle puts in its slot; vectors outside the
+// matrix come from a zero vector.  The chunk's weights w_s pass through a register (*pw).
+template <typename T, int DK>
+__device__ __forceinline__ void fused_stage(int s0, int M, int d, int Mpad, int ldS, const T *Shi, const T *Slo,
+                                            const T *Thi, const T *Tlo, const double *wvec, T *__restrict__ xs_h,
+                                            T *__restrict__ xs_lo, T *__restrict__ xt_h, T *__restrict__ xt_lo,
+                                            double *pw) {
+    typedef FusedLds<DK> L;
+    constexpr int XS_N = 32 * L::XSV, XT_N = L::DP * 4;   // 16-byte vectors per matrix
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const T *zero = (const T *)kZero16;
+    if (t < 32) *pw = s0 + t < M ? wvec[s0 + t] : 0.0;
+#pragma unroll
+    for (int u0 = 0; u0 < XS_N / 64; u0 += 8) {
+        const int u = u0 + wv;
+        if (XS_N / 64 % 8 == 0 || u < XS_N / 64) {
+            const int vi = 64 * u + lane, r = vi / L::XSV, c = (vi % L::XSV) ^ (r & L::XSM);
+            const bool ok = 8 * c < ldS;
+            const size_t go = (size_t)(s0 + r) * ldS + 8 * c;
+            dma16(ok ? Shi + go : zero, xs_h + 512 * u);
+            dma16(ok ? Slo + go : zero, xs_lo + 512 * u);
+        }
+    }
+#pragma unroll
+    for (int u0 = 0; u0 < XT_N / 64; u0 += 8) {
+        const int u = u0 + wv;
+        if (XT_N / 64 % 8 == 0 || u < XT_N / 64) {
+            const int vi = 64 * u + lane, q = vi >> 2, c = (vi & 3) ^ ((q >> 1) & 3);
+            const bool ok = q < d;
+            const size_t go = (size_t)q * Mpad + s0 + 8 * c;
+            dma16(ok ? Thi + go : zero, xt_h + 512 * u);
+            dma16(ok ? Tlo + go : zero, xt_lo + 512 * u);
+        }
+    }
+}
+
+// One chunk on the current buffers (read) while the next chunk streams into the other
+// buffers (written by fused_stage's loads).  Distinct restrict-qualified buffers tell
+// the compiler that the reads here do not wait for those loads.
+template <int PREC, int DK, int ZI>
+__device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, int Mpad, int ldS,
+                                            const typename Half<PREC>::T *Shi, const typename Half<PREC>::T *Slo,
+                                            const typename Half<PREC>::T *Thi, const typename Half<PREC>::T *Tlo,
+                                            const double *wvec, double zeta, int izeta, double wscale,
+                                            const typename Half<PREC>::T *__restrict__ xs_h,
+                                            const typename Half<PREC>::T *__restrict__ xs_lo,
+                                            const typename Half<PREC>::T *__restrict__ xt_h,
+                                            const typename Half<PREC>::T *__restrict__ xt_lo,
+                                            const double *__restrict__ wcur, typename Half<PREC>::T *__restrict__ nxs_h,
+                                            typename Half<PREC>::T *__restrict__ nxs_lo,
+                                            typename Half<PREC>::T *__restrict__ nxt_h,
+                                            typename Half<PREC>::T *__restrict__ nxt_lo, double *__restrict__ wnext,
+                                            const typename Half<PREC>::V (&xh)[DK],
+                                            const typename Half<PREC>::V (&xl)[DK], float4_t (&gacc)[2 * DK],
+                                            double &rs) {
+    typedef Half<PREC> H;
+    typedef typename H::T T;
+    typedef typename H::V V;
+    typedef FusedLds<DK> L;
+    const int t = threadIdx.x, lane = t & 63;
+    const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
+    double pw = 0.0;
+    if (next) fused_stage<T, DK>(s0n, M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, nxs_h, nxs_lo, nxt_h, nxt_lo, &pw);
+    // K^T tiles: 32 sparse points (two tiles) x 16 centres
+    float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int ks = 0; ks < DK; ks++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int o = L::xs_at(16 * j + r16, 4 * ks + (kg >> 3));
+            const V ah = *(const V *)(xs_h + o), al = *(const V *)(xs_lo + o);
+            kacc[j] = H::mfma(ah, xl[ks], kacc[j]);
+            kacc[j] = H::mfma(al, xh[ks], kacc[j]);
+            kacc[j] = H::mfma(ah, xh[ks], kacc[j]);
+        }
+    // epilogue: element v of tile j is centre r16 at sparse point s0 + 16 j + 4 (lane/16) + v,
+    // and element 4 j + v of this lane's A fragment of W.  The energy sum is FP64; K^zeta
+    // and W are formed from the FP32 K in FP32 for the common zeta = 4 (W is then split
+    // into 16-bit halves), in FP64 otherwise.
+    V wh, wl;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double wsv = wcur[16 * j + rq + v];
+            T h, lo;
+            if constexpr (ZI == 4) {
+                const float kf = kacc[j][v];
+                const float k2 = kf * kf;
+                rs += wsv * (double)(k2 * k2);
+                const float wk = (float)(wsv * 4.0 * wscale) * (k2 * kf);
+                h = (T)wk;
+                lo = (T)(wk - (float)h);
+            } else {
+                const double kv = (double)kacc[j][v];
+                double kz1, kz;
+                if (izeta > 0) {
+                    kz1 = 1.0;
+                    for (int q = 1; q < izeta; q++) kz1 *= kv;
+                    kz = kz1 * kv;
+                } else {
+                    kz1 = pow(kv, zeta - 1.0);
+                    kz = pow(kv, zeta);
+                }
+                rs += wsv * kz;
+                const double wk = wsv * zeta * kz1 * wscale;
+                h = H::from(wk);
+                lo = H::from(wk - H::to(h));
+            }
+            wh[4 * j + v] = h;
+            wl[4 * j + v] = lo;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * DK; q++) {
+        const int o = L::xt_at(16 * q + r16, kg >> 3);
+        const V bh = *(const V *)(xt_h + o), bl = *(const V *)(xt_lo + o);
+        gacc[q] = H::mfma(wl, bh, gacc[q]);
+        gacc[q] = H::mfma(wh, bl, gacc[q]);
+        gacc[q] = H::mfma(wh, bh, gacc[q]);
+    }
+    // the next buffers were last read before the previous barrier
+    if (next && t < 32) wnext[t] = pw;
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's loads into the next buffers have landed
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Fused GP kernel for descriptors of up to 32*DK components (split forms).
 //
 // One block = 8 wavefronts = 128 centres; wave w owns centres 16w..16w+15 and
-// keeps their X rows (split hi/lo, A-operand layout) in registers.  The block
-// walks the sparse points of its part in chunks of 32.  Per chunk the chunk's
-// rows of Xs (k = descriptor index) and of Xs^T (k = sparse index) are staged
-// in LDS; each wave computes its 16 x 32 tile of K = X Xs^T (three MFMA
-// passes, FP32 accumulation), applies the epilogue (energy w K^zeta summed in
-// FP64, W = w zeta K^(zeta-1)), splits W into hi/lo through a per-wave LDS tile
-// and accumulates G += W Xs (16 x 32DK, in registers).  W never reaches HBM.
-// The next chunk is fetched into registers while the current one is used and
-// written to the other of two LDS buffers after it, one barrier per chunk.
-// With several parts (gridDim.y) the sparse points are divided between blocks
-// and the FP32 partial G tiles are summed in a fixed order afterwards.
+// keeps their X rows (split hi/lo) in registers.  The block walks the sparse points
+// of its part in chunks of 32.  Per chunk the chunk's rows of Xs (k = descriptor
+// index) and of Xs^T (k = sparse index) are staged in LDS.  Each wave computes the
+// transposed tile K^T = Xs X^T (32 sparse points x 16 centres; three MFMA passes, FP32
+// accumulation): in the accumulator layout lane l holds centre l%16 at sparse points
+// 16 j + 4 (l/16) + v (tile j, element v), which is one 8-element A fragment of the
+// next product when its k runs over the sparse points in the order
+// [0-3, 16-19 | 4-7, 20-23 | 8-11, 24-27 | 12-15, 28-31] -- the order of the Xs^T copy
+// the kernel reads.  So the epilogue (energy w K^zeta summed in FP64, W = w zeta
+// K^(zeta-1) split into hi/lo) leaves W in registers ready for G += W Xs (16 x 32DK, in
+// registers): W never leaves the lane.  The next chunk streams from global memory into
+// the other of two LDS buffer sets while the current one is used, one barrier per chunk.
+// With several parts (gridDim.y) the sparse points are divided between blocks and the
+// FP32 partial G tiles are summed in a fixed order afterwards.
 // ---------------------------------------------------------------------------
 template <int PREC, int DK, int ZI>
 __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, int Mpad, int ldS, int chunks_per_part,
@@ -453,29 +613,16 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     typedef typename H::T T;
     typedef typename H::V V;
     constexpr int DP = 32 * DK;     // descriptor components handled (zero padded)
-    // LDS rows without padding; 16-byte vectors XOR-swizzled so that the rows one
-    // fragment read touches fall on distinct 16-byte bank groups in every group of 8
-    // lanes (a 16-byte read moves 128 bytes per cycle):
-    //   Xs rows (XSV vectors):                     vector c of row r at c ^ (r & XSM)
-    //   Xs^T and W rows (4 vectors of 8 sparse points, 64 bytes): vector c of row q at
-    //   c ^ ((q >> 1) & 3), so rows q..q+7 cover the 8 groups of 128 bytes
-    // Two chunk buffers: the next chunk is written while the current one is read, so
-    // each chunk needs one barrier.
-    constexpr int XSV = DP / 8;                     // 8, 16 or 32
-    constexpr int XSM = XSV >= 16 ? 15 : XSV - 1;
-    __shared__ __attribute__((aligned(16))) T xs_l[2][2][32 * DP];   // [buffer][hi, lo]
-    __shared__ __attribute__((aligned(16))) T xt_l[2][2][DP * 32];
-    __shared__ __attribute__((aligned(16))) T w_l[8][2][16 * 32];
-    __shared__ double wv_l[2][32];   // the chunk's w_s (0 past M), staged with it
-    auto xs_at = [](int r, int c) { return r * DP + ((c ^ (r & XSM)) << 3); };
-    auto xt_at = [](int q, int c) { return q * 32 + ((c ^ ((q >> 1) & 3)) << 3); };
+    __shared__ __attribute__((aligned(16))) T xs0[2][32 * DP], xs1[2][32 * DP];   // [hi, lo] of two buffers
+    __shared__ __attribute__((aligned(16))) T xt0[2][DP * 32], xt1[2][DP * 32];
+    __shared__ double wv0[32], wv1[32];   // the chunk's w_s (0 past M)
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
+    const int r16 = lane & 15, g4 = lane >> 4, kg = g4 * 8, rq = 4 * g4;
     const int row0 = blockIdx.x * 128 + wv * 16;
     const int part = blockIdx.y, nparts = gridDim.y;
     const int c_beg = part * chunks_per_part, c_end = min(Mpad / 32, c_beg + chunks_per_part);
 
-    V xh[DK], xl[DK];
+    V xh[DK], xl[DK];   // B operand of K^T: X[row0 + r16][32 ks + kg .. + 7]
     {
         const int row = row0 + r16;
 #pragma unroll
@@ -493,148 +640,29 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     float4_t gacc[2 * DK];
 #pragma unroll
     for (int q = 0; q < 2 * DK; q++) gacc[q] = (float4_t){0.f, 0.f, 0.f, 0.f};
-    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    double rs = 0.0;   // energy of centre row0 + r16 over this lane's sparse points
 
-    // Staging of a chunk: thread t owns 16-byte vectors of the Xs rows and of the
-    // Xs^T rows (hi and lo each); the next chunk is fetched into registers while
-    // the current one is used.
-    constexpr int XS_N = 32 * (DP / 8), XT_N = DP * 4;
-    constexpr int XS_V = (XS_N + 511) / 512, XT_V = (XT_N + 511) / 512;
-    V ps_h[XS_V], ps_l[XS_V], pt_h[XT_V], pt_l[XT_V];
-    double pw = 0.0;
-    auto fetch = [&](int s0) {
-        if (t < 32) pw = s0 + t < M ? wvec[s0 + t] : 0.0;
-#pragma unroll
-        for (int u = 0; u < XS_V; u++) {
-            const int idx = t + 512 * u;
-            const int r = idx / (DP / 8), c8 = (idx % (DP / 8)) * 8;
-            ps_h[u] = (V){}; ps_l[u] = (V){};
-            if (idx < XS_N && c8 < ldS) {
-                ps_h[u] = *(const V *)(Shi + (size_t)(s0 + r) * ldS + c8);
-                ps_l[u] = *(const V *)(Slo + (size_t)(s0 + r) * ldS + c8);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < XT_V; u++) {
-            const int idx = t + 512 * u;
-            const int q = idx >> 2, c8 = (idx & 3) * 8;
-            pt_h[u] = (V){}; pt_l[u] = (V){};
-            if (idx < XT_N && q < d) {
-                pt_h[u] = *(const V *)(Thi + (size_t)q * Mpad + s0 + c8);
-                pt_l[u] = *(const V *)(Tlo + (size_t)q * Mpad + s0 + c8);
-            }
-        }
-    };
-    auto put = [&](int b) {   // the fetched chunk -> LDS buffer b
-        if (t < 32) wv_l[b][t] = pw;
-#pragma unroll
-        for (int u = 0; u < XS_V; u++) {
-            const int idx = t + 512 * u;
-            if (idx < XS_N) {
-                const int o = xs_at(idx / XSV, idx % XSV);
-                *(V *)(xs_l[b][0] + o) = ps_h[u];
-                *(V *)(xs_l[b][1] + o) = ps_l[u];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < XT_V; u++) {
-            const int idx = t + 512 * u;
-            if (idx < XT_N) {
-                const int o = xt_at(idx >> 2, idx & 3);
-                *(V *)(xt_l[b][0] + o) = pt_h[u];
-                *(V *)(xt_l[b][1] + o) = pt_l[u];
-            }
-        }
-    };
     if (c_beg < c_end) {
-        fetch(32 * c_beg);
-        put(0);
+        double pw = 0.0;
+        fused_stage<T, DK>(32 * c_beg, M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, xs0[0], xs0[1], xt0[0], xt0[1],
+                              &pw);
+        if (t < 32) wv0[t] = pw;
     }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int ch = c_beg; ch < c_end; ch++) {
-        const int b = (ch - c_beg) & 1;
-        const int s0 = 32 * ch;
-        if (ch + 1 < c_end) fetch(s0 + 32);
-        // K tile: 16 centres x 32 sparse points
-        float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-        for (int ks = 0; ks < DK; ks++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int o = xs_at(16 * j + r16, 4 * ks + (kg >> 3));
-                const V bh = *(const V *)(xs_l[b][0] + o), bl = *(const V *)(xs_l[b][1] + o);
-                kacc[j] = H::mfma(xl[ks], bh, kacc[j]);
-                kacc[j] = H::mfma(xh[ks], bl, kacc[j]);
-                kacc[j] = H::mfma(xh[ks], bh, kacc[j]);
-            }
-        // epilogue; accumulator element v of lane l is centre rq + v, sparse point 16 j + r16.
-        // The energy sum is FP64; K^zeta and W are formed from the FP32 K in FP32 for the
-        // common zeta = 4 (W is then split into 16-bit halves), in FP64 otherwise.
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const double wsv = wv_l[b][16 * j + r16];
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                T h, lo;
-                if constexpr (ZI == 4) {
-                    const float kf = kacc[j][v];
-                    const float k2 = kf * kf;
-                    rs[v] += wsv * (double)(k2 * k2);
-                    const float wk = (float)(wsv * 4.0 * wscale) * (k2 * kf);
-                    h = (T)wk;
-                    lo = (T)(wk - (float)h);
-                } else {
-                    const double kv = (double)kacc[j][v];
-                    double kz1, kz;
-                    if (izeta > 0) {
-                        kz1 = 1.0;
-                        for (int q = 1; q < izeta; q++) kz1 *= kv;
-                        kz = kz1 * kv;
-                    } else {
-                        kz1 = pow(kv, zeta - 1.0);
-                        kz = pow(kv, zeta);
-                    }
-                    rs[v] += wsv * kz;
-                    const double wk = wsv * zeta * kz1 * wscale;
-                    h = H::from(wk);
-                    lo = H::from(wk - H::to(h));
-                }
-                const int row = rq + v, e = 16 * j + r16;
-                const int o = xt_at(row, e >> 3) + (e & 7);
-                w_l[wv][0][o] = h;
-                w_l[wv][1][o] = lo;
-            }
-        }
-        // the W tile is this wave's own: a wavefront barrier makes it visible in A-operand layout
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int ow = xt_at(r16, kg >> 3);
-        const V wh = *(const V *)(w_l[wv][0] + ow), wl = *(const V *)(w_l[wv][1] + ow);
-#pragma unroll
-        for (int q = 0; q < 2 * DK; q++) {
-            const int o = xt_at(16 * q + r16, kg >> 3);
-            const V bh = *(const V *)(xt_l[b][0] + o), bl = *(const V *)(xt_l[b][1] + o);
-            gacc[q] = H::mfma(wl, bh, gacc[q]);
-            gacc[q] = H::mfma(wh, bl, gacc[q]);
-            gacc[q] = H::mfma(wh, bh, gacc[q]);
-        }
-        if (ch + 1 < c_end) put(b ^ 1);   // the other buffer was last read before the previous barrier
-        __syncthreads();
+    for (int ch = c_beg; ch < c_end; ch += 2) {   // buffer set 0, then set 1
+        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, zeta,
+                                  izeta, wscale, xs0[0], xs0[1], xt0[0], xt0[1], wv0, xs1[0], xs1[1], xt1[0], xt1[1],
+                                  wv1, xh, xl, gacc, rs);
+        if (ch + 1 >= c_end) break;
+        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, zeta,
+                                  izeta, wscale, xs1[0], xs1[1], xt1[0], xt1[1], wv1, xs0[0], xs0[1], xt0[0], xt0[1],
+                                  wv0, xh, xl, gacc, rs);
     }
-    // energy partials of this part: lanes sharing lane>>4 hold the same centres
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        double x = rs[v];
-        for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        rs[v] = x;
-    }
-    if (r16 == 0)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = row0 + rq + v;
-            if (row < nc) e_part[(size_t)row * nparts + part] = rs[v];
-        }
+    // energy partials of this part: the four lane groups hold different sparse points
+    rs += __shfl_xor(rs, 16, 64);
+    rs += __shfl_xor(rs, 32, 64);
+    if (g4 == 0 && row0 + r16 < nc) e_part[(size_t)(row0 + r16) * nparts + part] = rs;
 #pragma unroll
     for (int q = 0; q < 2 * DK; q++) {
         const int col = 16 * q + r16;
