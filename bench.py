@@ -332,17 +332,20 @@ def stage_counters():
 SINGLE_KERNEL_STAGES = ("soap_forward", "soap_adjoint", "gemm1")
 
 
-def stage_roofline(forms, model, nc_batch):
+def stage_roofline(forms, model, n_si, nc_batch):
     """Achieved HBM GB/s and FP64-VALU fraction per descriptor stage.  Stages that are one
     kernel (forward, adjoint, the GP kernel) divide the committed per-launch bytes and
     flops by the stage time measured in this run with HIP events on the library's stream;
     the neighbour-list and gather stages span several kernels, so their dominant kernel
-    is priced at its kernel-trace time and the live stage time is given beside it."""
+    is priced at its kernel-trace time and the live stage time is given beside it.  The
+    GP kernels also get their MFMA fraction (algorithmic 4 nc M d; the split form runs
+    three passes)."""
     cnt = stage_counters()
     if not cnt:
         return None
     live = {"si4096_fp64": forms.get("fp64", {}).get("stage_ms", {}),
             "batch512_bf16x3": forms.get("batch512_bf16x3", {}).get("stage_ms", {})}
+    centres = {"si4096_fp64": n_si, "batch512_bf16x3": nc_batch}
     out = {}
     for (wl, stage), (kernel, nbytes, flops, avg_us) in sorted(cnt.items()):
         st_ms = live.get(wl, {}).get(stage)
@@ -354,12 +357,16 @@ def stage_roofline(forms, model, nc_batch):
                  kernel_ms=round(t * 1e3, 4), time_source=src, stage_ms_live=st_ms,
                  hbm_GBps=round(nbytes / t / 1e9, 1), hbm_frac=round(nbytes / t / HBM_PEAK, 4),
                  fp64_valu_TFps=round(flops / t / 1e12, 3), fp64_valu_frac=round(flops / t / VALU_FP64_PEAK, 4))
-        if stage == "gemm1" and wl == "batch512_bf16x3":   # split GP kernel: MFMA is its roof
+        if stage == "gemm1":   # the GP kernel: MFMA is its roof
             c = model.coords[0]
-            alg = 4.0 * nc_batch * c.n_sparse * c.descriptor.dim
-            e.update(bound="mfma", algorithmic_flops=alg, mfma_flops_executed=3 * alg,
-                     mfma_frac_algorithmic=round(alg / t / MFMA_BF16_PEAK, 4),
-                     mfma_frac_executed=round(3 * alg / t / MFMA_BF16_PEAK, 4))
+            alg = 4.0 * centres[wl] * c.n_sparse * c.descriptor.dim
+            if "split" in kernel:
+                e.update(bound="mfma", algorithmic_flops=alg, mfma_flops_executed=3 * alg,
+                         mfma_frac_algorithmic=round(alg / t / MFMA_BF16_PEAK, 4),
+                         mfma_frac_executed=round(3 * alg / t / MFMA_BF16_PEAK, 4), mfma_peak="bf16 2.5 PFLOP/s")
+            else:
+                e.update(bound="mfma", algorithmic_flops=alg, mfma_frac=round(alg / t / MFMA_FP64_PEAK, 4),
+                         mfma_peak="FP64 78.6 TFLOP/s")
         out.setdefault(wl, {})[stage] = e
     out["source"] = ("bytes and flops per launch: profiles/descriptor_stages_si4096_batch512.txt (rocprofv3 "
                      "FETCH_SIZE, WRITE_SIZE, SQ_INSTS_VALU_*_F64); peaks HBM 8 TB/s, FP64 VALU 78.6 TFLOP/s")
@@ -545,7 +552,7 @@ def main():
         if a in forms and b in forms:
             same[key] = forms[a]["atoms_per_s"] / forms[b]["atoms_per_s"]
     nc_batch = sum(len(a) for a in frames)
-    stages_roof = stage_roofline(forms, model, nc_batch)
+    stages_roof = stage_roofline(forms, model, N, nc_batch)
     dev.free()
     fp = forms["fp64"]
     mixed = [k for k in forms if k.startswith("batch512_") and "fp64" not in k]

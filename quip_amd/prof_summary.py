@@ -6,6 +6,7 @@
                                             [--probe-bytes B]
     python -m quip_amd.prof_summary stages   <kernel_trace.csv> <fetch.csv> <write.csv> <sq.csv>
                                             [--match A|B|...] [--probe-bytes B]
+    python -m quip_amd.prof_summary steps    <run_kernel_trace.csv> --start S [--contains T] [--which K]
 
 kernels:  calls, total and average duration per (kernel, grid size), the
           --stats view split by launch shape.
@@ -13,6 +14,10 @@ counters: average counter value per dispatch per (kernel, grid size, counter).
 stages:   per kernel shape the achieved HBM GB/s and FP64 VALU TFLOP/s against the
           MI355X peaks, from a kernel trace, FETCH_SIZE and WRITE_SIZE runs and a run
           of SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, SQ_INSTS_VALU, SQ_INSTS_LDS, SQ_WAVES.
+steps:    the kernels of one evaluation in launch order: a step begins at each launch of
+          the kernel named S; of the complete steps that launch a kernel named T, the
+          K-th from the end (default 2, a late timed step) is listed with its start,
+          duration and grid, then the sum of durations and first start to next step.
 traffic:  HBM bytes per launch of one kernel shape from a FETCH_SIZE run and a
           WRITE_SIZE run (KiB per dispatch), scaled by what the same counters
           read for the device-to-device copy kernel (__amd_rocclr_copyBuffer)
@@ -151,15 +156,40 @@ def stages(trace_csv, fetch_csv, write_csv, sq_csv, match, probe_bytes):
     return "\n".join(out)
 
 
+def steps(trace_csv, start, contains, which):
+    rows = list(csv.DictReader(open(trace_csv)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    idx = [i for i, r in enumerate(rows) if start in r["Kernel_Name"]]
+    spans = [(a, b) for a, b in zip(idx, idx[1:])
+             if not contains or any(contains in r["Kernel_Name"] for r in rows[a:b])]
+    if len(spans) < which:
+        return "fewer than %d complete steps" % which
+    a, b = spans[-which]
+    t0 = int(rows[a]["Start_Timestamp"])
+    out, tot = [], 0
+    for r in rows[a:b]:
+        s0, e0 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        tot += e0 - s0
+        g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        out.append("  %-44s %10d  start %9.1f us  %8.1f us" % (short(r["Kernel_Name"])[:44], g, (s0 - t0) / 1e3,
+                                                                (e0 - s0) / 1e3))
+    out.append("  sum of kernel durations %.1f us; first start to next step %.1f us (%d steps seen)"
+               % (tot / 1e3, (int(rows[b]["Start_Timestamp"]) - t0) / 1e3, len(spans)))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["kernels", "counters", "traffic", "stages"])
+    ap.add_argument("what", choices=["kernels", "counters", "traffic", "stages", "steps"])
     ap.add_argument("files", nargs="+")
     ap.add_argument("--match")
     ap.add_argument("--top", type=int, default=40)
     ap.add_argument("--kernel")
     ap.add_argument("--grid")
     ap.add_argument("--probe-bytes", type=int, default=256 << 20)
+    ap.add_argument("--start")
+    ap.add_argument("--contains")
+    ap.add_argument("--which", type=int, default=2)
     a = ap.parse_args()
     if a.what == "kernels":
         print(kernels(a.files[0], a.match, a.top))
@@ -167,6 +197,8 @@ def main():
         print(counters(a.files[0], a.match))
     elif a.what == "traffic":
         print(traffic(a.files[0], a.files[1], a.kernel, a.grid, a.probe_bytes))
+    elif a.what == "steps":
+        print(steps(a.files[0], a.start, a.contains, a.which))
     else:
         print(stages(a.files[0], a.files[1], a.files[2], a.files[3], a.match, a.probe_bytes))
 
