@@ -95,7 +95,13 @@ int gapx_calc(gapx_model *model, int n_atoms, const double *pos, const int *Z, c
 
 /* As gapx_calc with positions and every output in device memory (the inputs
  * are already resident in HBM); Z, lattice and pbc stay host arrays.  Work is
- * queued on the model's stream; gapx_sync waits for it. */
+ * queued on the model's stream and the call returns without waiting for it.
+ * After the first call on a model the neighbour list is built speculatively, into
+ * tables sized from earlier builds, with no host synchronisation; a call whose lists
+ * overflow them is detected on the device and redone by gapx_sync.  So the outputs
+ * are final only after gapx_sync (or the next calc entry point that cannot run
+ * speculatively: options, a skin, the host-buffer entry points), which waits for
+ * every queued call of the model and redoes, in order, those that overflowed. */
 int gapx_calc_device(gapx_model *model, int n_atoms, const double *d_pos, const int *Z, const double *lattice,
                      const int *pbc, double nl_cutoff, unsigned want, double *d_e, double *d_local_e, double *d_f,
                      double *d_virial, double *d_local_virial);
@@ -167,7 +173,7 @@ int gapx_calc_batch(gapx_model *model, int n_struct, const int *atom_off, const 
                     const double *lattices, const int *pbcs, double nl_cutoff, unsigned want, double *energies,
                     double *local_e, double *f, double *virials, double *local_virial);
 /* As gapx_calc_batch with positions and every output in device memory; queued on
- * the model's stream like gapx_calc_device. */
+ * the model's stream like gapx_calc_device, and final after gapx_sync as there. */
 int gapx_calc_batch_device(gapx_model *model, int n_struct, const int *atom_off, const double *d_pos, const int *Z,
                            const double *lattices, const int *pbcs, double nl_cutoff, unsigned want, double *d_e,
                            double *d_local_e, double *d_f, double *d_virials, double *d_local_virial);
