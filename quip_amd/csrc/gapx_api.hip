@@ -579,7 +579,7 @@ extern "C" int gapx_model_free(gapx_model *m) {
 void gapx::reset_arena_views(Engine &eg) {
     Workspace &w = eg.ws;
     for (DBuf *b : {&w.pos, &w.Z, &w.sid, &w.atom_off, &w.geo, &w.cell_base, &w.ginv, &w.cell_of, &w.map_shift,
-                    &w.cell_cnt, &w.cell_start, &w.cell_fill, &w.cell_atoms, &w.cell_unsorted, &w.scan_part,
+                    &w.cell_cnt, &w.cell_start, &w.cell_fill, &w.cell_atoms, &w.cell_unsorted, &w.scan_part, &w.cpos, &w.cms,
                     &w.scan_part2, &w.n1_cnt, &w.n1_off, &w.n1_j, &w.n1_shift, &w.n1_r, &w.n1_rev, &w.n2_cnt,
                     &w.n2_off, &w.n2_fill, &w.n2_i, &w.n2_k, &w.n2_is, &w.n2_ks, &w.nb_off, &w.nb_j, &w.nb_rev,
                     &w.nb_r, &w.nb_d, &w.pairF, &w.pairE, &w.esite, &w.X, &w.norm, &w.cbuf, &w.W, &w.Wf, &w.e_part,
@@ -989,6 +989,8 @@ static int nl_prepare(gapx_model *m, Engine &eg, int S, const int *atom_off, con
     CHK(w.cell_fill.ensure((size_t)NC * 4 + 4));
     CHK(w.cell_start.ensure((size_t)(NC + 1) * 4));
     CHK(w.cell_atoms.ensure((size_t)N * 4 + 4));
+    CHK(w.cpos.ensure((size_t)N * sizeof(CellAtom) + sizeof(CellAtom)));
+    CHK(w.cms.ensure((size_t)N * sizeof(CellShift) + sizeof(CellShift)));
     CHK(w.scan_part.ensure((size_t)scan_partials_size(std::max(NC, N) + 1) * 4 + 64));
     CHK(w.scan_part2.ensure((size_t)scan_partials_size(N + 1) * 4 + 64));
     CHK(w.n1_cnt.ensure((size_t)N * 4 + 4));
@@ -1056,7 +1058,8 @@ static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
     if (small) {
         launch_nl_front_small(N, NC, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(),
                               w.cell_cnt.as<int>(), w.cell_start.as<int>(), w.cell_atoms.as<int>(),
-                              w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags, s);
+                              w.cpos.as<CellAtom>(), w.cms.as<CellShift>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(),
+                              flags, s);
     } else {
         {
             int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(),
@@ -1074,16 +1077,19 @@ static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
         launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
                             w.cell_unsorted.as<int>(), s);
         launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
-                            w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), s);
+                            w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), d_pos, w.map_shift.as<int>(),
+                            w.cpos.as<CellAtom>(), w.cms.as<CellShift>(), s);
     }
     if (pl.one_pass) {
         launch_nl_pairs(2, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), w.cpos.as<CellAtom>(),
+                        w.cms.as<CellShift>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
                         w.t1_j.as<int>(), w.t1_shift.as<int>(), w.t1_r.as<double>(), w.n2_cnt.as<int>(), nullptr,
                         nullptr, w.t2_i.as<int>(), w.t2_k.as<int>(), pl.cap1, pl.cap2, flags, s);
     } else {
         launch_nl_pairs(0, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), w.cpos.as<CellAtom>(),
+                        w.cms.as<CellShift>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
                         nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, 0, 0,
                         nullptr, s);
     }
@@ -1123,7 +1129,8 @@ static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bo
                           w.n2_k.as<int>(), s);
     } else {
         launch_nl_pairs(1, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), dgeo, dsid, dcb, nullptr, w.n1_off.as<int>(),
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), w.cpos.as<CellAtom>(),
+                        w.cms.as<CellShift>(), dgeo, dsid, dcb, nullptr, w.n1_off.as<int>(),
                         w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), nullptr, w.n2_off.as<int>(),
                         w.n2_fill.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), 0, 0, nullptr, s);
     }

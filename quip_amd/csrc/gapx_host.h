@@ -158,6 +158,7 @@ struct Coord {
 struct Workspace {
     DBuf pos, Z, sid, atom_off, geo, cell_base, ginv, cell_of, map_shift, cell_cnt, cell_start, cell_fill, cell_atoms,
         cell_unsorted, scan_part, scan_part2;
+    DBuf cpos, cms;   // cell-ordered positions and map shifts (CellAtom, CellShift) for the candidate search
     DBuf n1_cnt, n1_off, n1_j, n1_shift, n1_r, n1_rev, n2_cnt, n2_off, n2_fill, n2_i, n2_k, n2_is, n2_ks;
     DBuf nb_off, nb_j, nb_rev, nb_r, nb_d, pairF, pairE, esite;
     DBuf X, norm, cbuf, W, Wf, e_part, G, Gparts, split_ws;

@@ -48,21 +48,27 @@ void launch_scan2_guard(const int *in1, const int *in2, int n, int *out1, int *o
 void launch_nl_bin(int N, const double *pos, const NLGeom *geo, const int *sid, const int *cell_base, int *cell_of,
                    int *map_shift, int *cell_cnt, hipStream_t s);
 void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc, int *atoms, hipStream_t s);
+// The cell-ordered copies the candidate search reads: slot q of the cell lists gets its atom's
+// position (cpos, x y z and a pad) and map shift with the atom index (cms: ms0 ms1 ms2 atom).
+struct CellAtom { double p[4]; };
+struct CellShift { int v[4]; };
 void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
-                         int *atoms, hipStream_t s);
+                         int *atoms, const double *pos, const int *map_shift, CellAtom *cpos, CellShift *cms,
+                         hipStream_t s);
 // bin, scan, fill and rank in one block for N <= kNlFrontAtoms atoms in NC <= kNlFrontCells
 // cells (also zeroes n2_cnt, n2_fill and the four flags)
 constexpr int kNlFrontAtoms = 8192, kNlFrontCells = 8192;
 void launch_nl_front_small(int N, int NC, const double *pos, const NLGeom *geo, const int *sid,
                            const int *cell_base, int *cell_of, int *map_shift, int *cell_cnt, int *cell_start,
-                           int *cell_atoms, int *n2_cnt, int *n2_fill, int *flags, hipStream_t s);
+                           int *cell_atoms, CellAtom *cpos, CellShift *cms, int *n2_cnt, int *n2_fill, int *flags,
+                           hipStream_t s);
 // mode 0: count; 1: fill the CSR tables; 2: one pass into per-atom slots of cap1 / cap2
 // (counts in n1_cnt / n2_cnt, *ovf = 1 when a slot range overflows)
 void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, const int *cell_of,
-                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom *geo,
-                     const int *sid, const int *cell_base, int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift,
-                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, int cap1,
-                     int cap2, int *ovf, hipStream_t s);
+                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const CellAtom *cpos,
+                     const CellShift *cms, const NLGeom *geo, const int *sid, const int *cell_base, int *n1_cnt,
+                     const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt, const int *n2_off,
+                     int *n2_fill, int *n2_i, int *n2_k, int cap1, int cap2, int *ovf, hipStream_t s);
 void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *n2_off, const int *t1_j,
                        const int *t1_shift, const double *t1_r, const int *t2_i, const int *t2_k, int *n1_j,
                        int *n1_shift, double *n1_r, int *n2_i, int *n2_k, hipStream_t s);

@@ -443,19 +443,27 @@ void launch_nl_cell_fill(int N, const int *cell_of, const int *start, int *fillc
 // Each cell's list in descending atom index: the order of QUIP's head-inserted
 // linked lists (Connection.f95:1580-1584).  Atom n's slot is the number of atoms
 // of its cell with a larger index.
+__device__ __forceinline__ void put_cell_atom(int slot, int n, const double *pos, const int *map_shift,
+                                              CellAtom *cpos, CellShift *cms) {
+    cpos[slot] = CellAtom{{pos[3 * n], pos[3 * n + 1], pos[3 * n + 2], 0.0}};
+    cms[slot] = CellShift{{map_shift[3 * n], map_shift[3 * n + 1], map_shift[3 * n + 2], n}};
+}
 __global__ void k_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
-                               int *atoms) {
+                               int *atoms, const double *pos, const int *map_shift, CellAtom *cpos, CellShift *cms) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int c = cell_of[n], st = start[c], m = cnt[c];
     int rank = 0;
     for (int q = 0; q < m; q++) rank += unsorted[st + q] > n ? 1 : 0;
     atoms[st + rank] = n;
+    put_cell_atom(st + rank, n, pos, map_shift, cpos, cms);
 }
 void launch_nl_cell_rank(int N, const int *cell_of, const int *start, const int *cnt, const int *unsorted,
-                         int *atoms, hipStream_t s) {
+                         int *atoms, const double *pos, const int *map_shift, CellAtom *cpos, CellShift *cms,
+                         hipStream_t s) {
     if (N > 0)
-        hipLaunchKernelGGL(k_nl_cell_rank, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, start, cnt, unsorted, atoms);
+        hipLaunchKernelGGL(k_nl_cell_rank, dim3(cdiv(N, 256)), dim3(256), 0, s, N, cell_of, start, cnt, unsorted, atoms,
+                           pos, map_shift, cpos, cms);
 }
 
 // The front of the neighbour list for small problems in one block of 1024 threads (each
@@ -470,8 +478,8 @@ __global__ __launch_bounds__(kFrontThreads) void k_nl_front_small(int N, int NC,
                                                                   const int *__restrict__ sid,
                                                                   const int *__restrict__ cell_base, int *cell_of,
                                                                   int *map_shift, int *cell_cnt, int *cell_start,
-                                                                  int *cell_atoms, int *n2_cnt, int *n2_fill,
-                                                                  int *flags) {
+                                                                  int *cell_atoms, CellAtom *cpos, CellShift *cms,
+                                                                  int *n2_cnt, int *n2_fill, int *flags) {
     __shared__ int scnt[kNlFrontCells], sfill[kNlFrontCells];
     __shared__ int scell[kNlFrontAtoms], suns[kNlFrontAtoms];
     __shared__ int wsum[kFrontThreads / 64];
@@ -532,13 +540,15 @@ __global__ __launch_bounds__(kFrontThreads) void k_nl_front_small(int N, int NC,
         int rank = 0;
         for (int q = 0; q < m; q++) rank += suns[st + q] > n ? 1 : 0;
         cell_atoms[st + rank] = n;
+        put_cell_atom(st + rank, n, pos, map_shift, cpos, cms);
     }
 }
 void launch_nl_front_small(int N, int NC, const double *pos, const NLGeom *geo, const int *sid,
                            const int *cell_base, int *cell_of, int *map_shift, int *cell_cnt, int *cell_start,
-                           int *cell_atoms, int *n2_cnt, int *n2_fill, int *flags, hipStream_t s) {
+                           int *cell_atoms, CellAtom *cpos, CellShift *cms, int *n2_cnt, int *n2_fill, int *flags,
+                           hipStream_t s) {
     hipLaunchKernelGGL(k_nl_front_small, dim3(1), dim3(kFrontThreads), 0, s, N, NC, pos, geo, sid, cell_base, cell_of,
-                       map_shift, cell_cnt, cell_start, cell_atoms, n2_cnt, n2_fill, flags);
+                       map_shift, cell_cnt, cell_start, cell_atoms, cpos, cms, n2_cnt, n2_fill, flags);
 }
 
 // Wave-parallel form of the same search: one wavefront per atom1, lanes over
@@ -573,6 +583,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
                                                     const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
                                                     const int *__restrict__ cell_atoms,
+                                                    const CellAtom *__restrict__ cpos,
+                                                    const CellShift *__restrict__ cms,
                                                     const NLGeom *__restrict__ geo, const int *__restrict__ sid,
                                                     const int *__restrict__ cell_base, int *n1_cnt,
                                                     const int *n1_off, int *n1_j, int *n1_shift, double *n1_r,
@@ -659,16 +671,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                     else hi_o = mid - 1;
                 }
                 const int q = c - s_pref[wv][lo_o];
-                atom2 = cell_atoms[s_start[wv][lo_o] + q];
+                const int slot = s_start[wv][lo_o] + q;   // the cell-ordered copies: no dependent load
+                const CellShift cs2 = cms[slot];
+                const CellAtom ca2 = cpos[slot];
+                atom2 = cs2.v[3];
                 const int pk = s_sh[wv][lo_o];
                 const int i4 = (int)(signed char)(pk & 255), j4 = (int)(signed char)((pk >> 8) & 255),
                           k4 = (int)(signed char)((pk >> 16) & 255);
                 const bool same_cell = (pk >> 24) & 1;
                 if (!(atom2 == atom1 && i4 == 0 && j4 == 0 && k4 == 0 && same_cell)) {
-                    sh0 = i4 - ms1x + map_shift[3 * atom2];
-                    sh1 = j4 - ms1y + map_shift[3 * atom2 + 1];
-                    sh2 = k4 - ms1z + map_shift[3 * atom2 + 2];
-                    double d0 = pos[3 * atom2] - p1x, d1 = pos[3 * atom2 + 1] - p1y, d2 = pos[3 * atom2 + 2] - p1z;
+                    sh0 = i4 - ms1x + cs2.v[0];
+                    sh1 = j4 - ms1y + cs2.v[1];
+                    sh2 = k4 - ms1z + cs2.v[2];
+                    double d0 = ca2.p[0] - p1x, d1 = ca2.p[1] - p1y, d2 = ca2.p[2] - p1z;
                     d0 = d0 + g.lat[0] * (double)sh0; d1 = d1 + g.lat[1] * (double)sh0; d2 = d2 + g.lat[2] * (double)sh0;
                     d0 = d0 + g.lat[3] * (double)sh1; d1 = d1 + g.lat[4] * (double)sh1; d2 = d2 + g.lat[5] * (double)sh1;
                     d0 = d0 + g.lat[6] * (double)sh2; d1 = d1 + g.lat[7] * (double)sh2; d2 = d2 + g.lat[8] * (double)sh2;
@@ -721,15 +736,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 }
 
 void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, const int *cell_of,
-                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const NLGeom *geo,
-                     const int *sid, const int *cell_base, int *n1_cnt, const int *n1_off, int *n1_j, int *n1_shift,
-                     double *n1_r, int *n2_cnt, const int *n2_off, int *n2_fill, int *n2_i, int *n2_k, int cap1,
-                     int cap2, int *ovf, hipStream_t s) {
+                     const int *cell_start, const int *cell_cnt, const int *cell_atoms, const CellAtom *cpos,
+                     const CellShift *cms, const NLGeom *geo, const int *sid, const int *cell_base, int *n1_cnt,
+                     const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt, const int *n2_off,
+                     int *n2_fill, int *n2_i, int *n2_k, int cap1, int cap2, int *ovf, hipStream_t s) {
     if (N <= 0) return;
 #define NLP(M)                                                                                                  \
     hipLaunchKernelGGL(k_nl_pairs_w<M>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of, cell_start, \
-                       cell_cnt, cell_atoms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r, n2_cnt,   \
-                       n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
+                       cell_cnt, cell_atoms, cpos, cms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r, \
+                       n2_cnt, n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
     if (mode == 1) NLP(1);
     else if (mode == 2) NLP(2);
     else NLP(0);

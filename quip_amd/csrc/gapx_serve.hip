@@ -104,7 +104,7 @@ static size_t engine_bytes(const Engine &eg) {
                           &w.nb_r, &w.nb_d, &w.pairF, &w.pairE, &w.esite, &w.X, &w.norm, &w.cbuf, &w.W, &w.Wf,
                           &w.e_part, &w.G, &w.Gparts, &w.split_ws, &w.f, &w.le, &w.lv, &w.e, &w.vir, &w.minmax,
                           &w.last_pos, &w.skin_flags, &w.cmask, &w.t1_j, &w.t1_shift, &w.t1_r, &w.t2_i, &w.t2_k,
-                          &w.nl_flags})
+                          &w.nl_flags, &w.spart, &w.cpos, &w.cms})
         b += x->cap;
     for (const DBuf &x : w.centres) b += x.cap;
     return b;
@@ -129,7 +129,7 @@ static int admit_device(gapx_server *srv, size_t footprint, size_t held) {
 static size_t footprint_bytes(const gapx_model *m, int N) {
     const double rc = m->cutoff;
     const double nn = 0.1 * 4.18879 * rc * rc * rc + 8;
-    double per_atom = 160.0 + 80.0;                   // NL per-atom arrays, outputs
+    double per_atom = 208.0 + 80.0;                   // NL per-atom arrays (cell-ordered copies included), outputs
     per_atom += nn * (4 + 4 + 8 + 24 + 24 + 8);        // full pair view + pair forces / energies
     per_atom += 0.5 * nn * (4 + 12 + 8 + 4 + 16);      // neighbour1 / neighbour2 tables
     for (auto &cp : m->coords) {
