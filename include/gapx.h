@@ -285,14 +285,21 @@ int gapx_model_set_autotune(gapx_model *model, int on);
 /* Split-form autotuner (on with gapx_model_set_autotune).  No counterpart in the reference.
  * For each gpCoordinate, split precision and centre-count bucket the first evaluation
  * times the fused GP kernel (descriptors up to 256 components) with 1, 2, 4 and 8
- * sparse-point parts, or the split GEMMs (larger descriptors) with 64 x 64 and 128 x 128
- * LDS tiles, and keeps the fastest.  gapx_model_tuning_split lists the choices:
- * fused_parts (0 where the GEMMs are used), gemm1_lds / gemm2_lds (-1 where the fused
+ * sparse-point parts and with the stream-K schedule (every block of a CU-filling grid
+ * takes an equal run of (centre block, sparse chunk) units), or the split GEMMs (larger
+ * descriptors) with 64 x 64 and 128 x 128 LDS tiles, and keeps the fastest.
+ * gapx_model_tuning_split lists the choices: fused_parts (-1 the stream-K schedule,
+ * 0 where the GEMMs are used), gemm1_lds / gemm2_lds (-1 where the fused
  * kernel is used) and the measured milliseconds of one launch (fused kernel with its
  * partial sums, or GEMM 1; GEMM 2). */
 int gapx_model_tuning_split(gapx_model *model, int coord, int max_entries, int *precision, int *bucket,
                             int *fused_parts, int *gemm1_lds, int *gemm2_lds, double *fused_or_gemm1_ms,
                             double *gemm2_ms, int *n_entries);
+/* Fixes the schedule of the fused split GP kernel over the autotuner and the default
+ * rule: -1 the stream-K schedule, 1..8 that many sparse-point parts (rounded so that
+ * every part has chunks), 0 back to the tuner / default rule; other values are
+ * GAPX_E_ARG.  No counterpart in the reference (a tuning override, as set_autotune). */
+int gapx_model_set_fused_schedule(gapx_model *model, int parts);
 
 /* Replay of the launch sequence after the neighbour count as a HIP graph (off by
  * default).  The sequence is captured on the first call and re-captured when a

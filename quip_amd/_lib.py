@@ -75,6 +75,7 @@ _SIGS = [
     ("gapx_descriptors", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_double, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
     ("gapx_model_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
+    ("gapx_model_set_fused_schedule", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_model_set_graphs", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_model_set_nl_reuse", C.c_int, [C.c_void_p, C.c_int]),
     ("gapx_last_nl_rebuilt", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

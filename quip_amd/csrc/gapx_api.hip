@@ -647,6 +647,12 @@ extern "C" int gapx_model_set_autotune(gapx_model *m, int on) {
     m->autotune = on != 0;
     return GAPX_OK;
 }
+extern "C" int gapx_model_set_fused_schedule(gapx_model *m, int parts) {
+    if (!m) return fail(GAPX_E_ARG, "NULL model");
+    if (parts < -1 || parts > 8) return fail(GAPX_E_ARG, "fused schedule %d: -1 (stream-K), 0 or 1..8 parts", parts);
+    m->fused_schedule = parts;
+    return GAPX_OK;
+}
 extern "C" int gapx_model_tuning(gapx_model *m, int coord, int max_entries, int *bucket, int *gemm1_tile,
                                  int *gemm2_tile, int *gemm2_split, double *gemm1_ms, double *gemm2_ms,
                                  int *n_entries) {
@@ -1363,9 +1369,17 @@ static int gemm_tune(gapx_model *m, Engine &eg, Coord &c, int nc, GemmTune &out,
 
 // Split-form autotuner, as gemm_tune for the FP64 tiles: per coordinate, precision and
 // centre-count bucket the first evaluation times the fused kernel (with its partial-G sum)
-// for 1, 2, 4 and 8 sparse-point parts, or the split GEMMs with 64 x 64 and 128 x 128
-// tiles, on the call's own operands, and keeps the fastest (one warm-up, best of three).
-static int fused_part_candidates(const Coord &c, int nc, std::vector<int> &out) {
+// for 1, 2, 4 and 8 sparse-point parts and the stream-K schedule (-1), or the split GEMMs
+// with 64 x 64 and 128 x 128 tiles, on the call's own operands, and keeps the fastest
+// (one warm-up, best of three).
+// Bytes of the partial-G buffer the fused kernel needs with `parts` (0: the default rule)
+static size_t fused_scratch_bytes(int prec, const Coord &c, int nc, int parts) {
+    const int dpad = (c.dim + 31) / 32 * 32, Mpad = (c.M + 31) / 32 * 32;
+    if (parts < 0) return gp_fused_sk_scratch_bytes(prec, c.dim, dpad, nc, Mpad);
+    const int v = parts > 0 ? gp_fused_parts_valid(parts, Mpad) : gp_fused_parts(nc, Mpad);
+    return v > 1 ? (size_t)v * nc * c.dim * 4 : 0;
+}
+static int fused_part_candidates(int prec, const Coord &c, int nc, std::vector<int> &out) {
     out.clear();
     const int Mpad = (c.M + 31) / 32 * 32;
     for (int p = 1; p <= 8; p *= 2) {
@@ -1374,6 +1388,7 @@ static int fused_part_candidates(const Coord &c, int nc, std::vector<int> &out) 
             std::find(out.begin(), out.end(), v) == out.end())
             out.push_back(v);
     }
+    if (fused_scratch_bytes(prec, c, nc, -1) <= kTuneScratchCap) out.push_back(-1);
     return GAPX_OK;
 }
 
@@ -1419,7 +1434,7 @@ static int mixed_tune(gapx_model *m, Engine &eg, Coord &c, int nc, bool fused, G
     int st = GAPX_OK;
     if (fused) {
         std::vector<int> cand;
-        fused_part_candidates(c, nc, cand);
+        fused_part_candidates(m->precision, c, nc, cand);
         for (int parts : cand) {
             float t = 0;
             int nt = 0;
@@ -1428,7 +1443,11 @@ static int mixed_tune(gapx_model *m, Engine &eg, Coord &c, int nc, bool fused, G
                                       c.d_Xs_hi.p, c.d_Xs_lo.p, c.d_XsTc_hi.p, c.d_XsTc_lo.p, c.d_w.as<double>(),
                                       c.zeta, c.wscale, w.G.as<double>(), w.Gparts.as<float>(),
                                       w.e_part.as<double>(), &nt, s);
-                launch_gp_fused_reduce(nc, c.dim, nt, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
+                if (nt < 0)
+                    launch_gp_sk_finish(m->precision, nc, c.dim, c.dpad, c.Mpad, true, w.Gparts.as<float>(), c.wscale,
+                                        w.G.as<double>(), w.e_part.as<double>(), nullptr, nullptr, s);
+                else
+                    launch_gp_fused_reduce(nc, c.dim, nt, w.Gparts.as<float>(), c.wscale, w.G.as<double>(), s);
             }, &t);
             if (st != GAPX_OK) break;
             if (t < best.g1_ms) { best.g1_ms = t; best.fparts = parts; }
@@ -1709,6 +1728,7 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     const size_t NCOORD = m->coords.size();
     std::vector<GemmTune> tunes(NCOORD);
     std::vector<char> fused(NCOORD, 0);
+    std::vector<size_t> gp_bytes(NCOORD, 0);   // partial-G bytes sized per coordinate (split fused kernel)
     size_t mX = 8, mN = 8, mC = 8, mW = 8, mE = 8, mGp = 8, mWf = 8, mSw = 8, mV = 8;
     bool has_d2b_var = false;
     for (size_t k = 0; k < NCOORD; k++) {
@@ -1738,13 +1758,15 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
             mW = std::max(mW, (size_t)nc * c.M * 8);
             mGp = std::max(mGp, (size_t)8 * nc * c.dim * 8);
         } else if (fused[k]) {
-            size_t parts = gp_fused_parts(nc, Mpad);
+            size_t b = fused_scratch_bytes(m->precision, c, nc, 0);
             if (m->autotune && !noalloc) {   // the tuner's candidates
                 std::vector<int> cand;
-                fused_part_candidates(c, nc, cand);
-                for (int p : cand) parts = std::max(parts, (size_t)p);
+                fused_part_candidates(m->precision, c, nc, cand);
+                for (int p : cand) b = std::max(b, fused_scratch_bytes(m->precision, c, nc, p));
             }
-            mGp = std::max(mGp, parts * nc * c.dim * 4);
+            if (m->fused_schedule) b = std::max(b, fused_scratch_bytes(m->precision, c, nc, m->fused_schedule));
+            gp_bytes[k] = b;
+            mGp = std::max(mGp, b);
         } else {
             mWf = std::max(mWf, (size_t)nc * Mpad * 4);
             mSw = std::max(mSw, (size_t)(nc + 63) / 64 * 64 * dpad * 4);
@@ -1777,6 +1799,11 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                 CHK(ensure_split(m, c, m->precision, s));
             }
             CHK(mixed_tune(m, eg, c, nc, fused[k], tunes[k], noalloc));
+            if (fused[k] && m->fused_schedule) tunes[k].fparts = m->fused_schedule;
+            // a tuning cached by an allocating call may need more partial-G space than
+            // the allocation-free call sized: the default rule then
+            if (fused[k] && fused_scratch_bytes(m->precision, c, nc, tunes[k].fparts) > gp_bytes[k])
+                tunes[k].fparts = 0;
         }
     }
     double *o_e = (want & GAPX_WANT_E) && d_e ? d_e : w.e.as<double>();
@@ -1867,7 +1894,11 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
             if (f64fused)
                 launch_gp_finish_f64(nc, c.dim, ntile, w.Gparts.as<double>(), w.G.as<double>(), ntile, centres,
                                      w.e_part.as<double>(), w.esite.as<double>(), s);
-            else if (split_fused && grad)
+            else if (split_fused && ntile < 0) {   // stream-K schedule: shared row blocks and every energy
+                launch_gp_sk_finish(m->precision, nc, c.dim, c.dpad, c.Mpad, grad, w.Gparts.as<float>(), c.wscale,
+                                    w.G.as<double>(), w.e_part.as<double>(), centres, w.esite.as<double>(), s);
+                ntile = 1;
+            } else if (split_fused && grad)
                 launch_gp_finish_f32(nc, c.dim, ntile, w.Gparts.as<float>(), 1.0 / c.wscale, w.G.as<double>(), ntile,
                                      centres, w.e_part.as<double>(), w.esite.as<double>(), s);
             else

@@ -357,6 +357,7 @@ struct ModelState {
     std::vector<std::unique_ptr<Coord>> coords;
     DBuf d_e0;
     bool autotune = true;
+    int fused_schedule = 0;   // fused split GP kernel: -1 stream-K, 1..8 parts, 0 the tuner / default rule
     bool graphs = false;
     bool nl_reuse = true;   // keep a neighbour list within its skin (calc_connect with cutoff_skin)
     std::unique_ptr<Engine> eng;   // the engine of the plain entry points

@@ -167,7 +167,14 @@ void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, int 
 bool gp_fused_supported(int dpad);
 int gp_fused_parts(int nc, int Mpad);              // the default number of sparse-point parts
 int gp_fused_parts_valid(int parts, int Mpad);     // a requested number, as the launch rounds it
-// parts_req > 0 asks for that many parts (rounded so that every part has chunks), 0 the default
+// parts_req > 0 asks for that many parts (rounded so that every part has chunks), 0 the
+// default, -1 the stream-K schedule (*ntile_out = -1): whole row blocks write G and
+// e_part[i] directly, the rest go to slots in Gpart (gp_fused_sk_scratch_bytes) that
+// launch_gp_sk_finish sums; it also writes every centre's energy to e_part[i] (one
+// column) and adds it to esite[centres[i]] (centres may be null: e_part only)
+size_t gp_fused_sk_scratch_bytes(int prec, int d, int dpad, int nc, int Mpad);
+void launch_gp_sk_finish(int prec, int nc, int d, int dpad, int Mpad, bool want_g, const float *Gpart, double wscale,
+                         double *G, double *e_part, const int *centres, double *esite, hipStream_t s);
 void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, int parts_req, const double *X,
                            const void *Xs_hi, const void *Xs_lo, const void *XsT_hi, const void *XsT_lo,
                            const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,

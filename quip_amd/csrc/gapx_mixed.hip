@@ -598,17 +598,27 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, in
 // the other of two LDS buffer sets while the current one is used, one barrier per chunk.
 // With several parts (gridDim.y) the sparse points are divided between blocks and the
 // FP32 partial G tiles are summed in a fixed order afterwards.
+//
+// Stream-K schedule (sk_units > 0): the work is sk_units = (row blocks) x (chunks)
+// units in row-block-major order, and block b of the grid (one or a few per CU) takes
+// the contiguous units [b U / B, (b+1) U / B): every block gets the same number of
+// chunks to within one, so the last round of blocks is never a partial one.  A block's
+// range covers whole row blocks in its middle and possibly part of one at either end;
+// a whole row block writes G and its energies directly, a part writes FP32 partials
+// to one of the block's two slots (its first row block slot 0, its last slot 1), which
+// k_gp_sk_finish sums in block order.
 // ---------------------------------------------------------------------------
 template <int PREC, int DK, int ZI>
 __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, int Mpad, int ldS, int chunks_per_part,
-                                                        const double *__restrict__ X,
+                                                        long sk_units, const double *__restrict__ X,
                                                         const typename Half<PREC>::T *__restrict__ Shi,
                                                         const typename Half<PREC>::T *__restrict__ Slo,
                                                         const typename Half<PREC>::T *__restrict__ Thi,
                                                         const typename Half<PREC>::T *__restrict__ Tlo,
                                                         const double *__restrict__ wvec, double zeta, int izeta,
                                                         double wscale, double inv_wscale, double *__restrict__ G,
-                                                        float *__restrict__ Gpart, double *__restrict__ e_part) {
+                                                        float *__restrict__ Gpart, double *__restrict__ e_part,
+                                                        double *__restrict__ e_slot) {
     typedef Half<PREC> H;
     typedef typename H::T T;
     typedef typename H::V V;
@@ -618,9 +628,22 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __shared__ double wv0[32], wv1[32];   // the chunk's w_s (0 past M)
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int r16 = lane & 15, g4 = lane >> 4, kg = g4 * 8, rq = 4 * g4;
-    const int row0 = blockIdx.x * 128 + wv * 16;
     const int part = blockIdx.y, nparts = gridDim.y;
-    const int c_beg = part * chunks_per_part, c_end = min(Mpad / 32, c_beg + chunks_per_part);
+    const int C = Mpad / 32;
+    const bool sk = sk_units > 0;
+    long u, u_end;   // this block's units (row block rb, chunk c) = rb C + c
+    if (sk) {
+        u = (long)blockIdx.x * sk_units / gridDim.x;
+        u_end = (long)(blockIdx.x + 1) * sk_units / gridDim.x;
+    } else {
+        u = (long)blockIdx.x * C + min(C, part * chunks_per_part);
+        u_end = (long)blockIdx.x * C + min(C, (part + 1) * chunks_per_part);
+    }
+    int seg = 0;
+    do {
+    const int rb = sk ? (int)(u / C) : (int)blockIdx.x;
+    const int c_beg = (int)(u - (long)rb * C), c_end = (int)min((long)C, u_end - (long)rb * C);
+    const int row0 = rb * 128 + wv * 16;
 
     V xh[DK], xl[DK];   // B operand of K^T: X[row0 + r16][32 ks + kg .. + 7]
     {
@@ -662,7 +685,15 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     // energy partials of this part: the four lane groups hold different sparse points
     rs += __shfl_xor(rs, 16, 64);
     rs += __shfl_xor(rs, 32, 64);
-    if (g4 == 0 && row0 + r16 < nc) e_part[(size_t)(row0 + r16) * nparts + part] = rs;
+    // outputs: G and the energies directly (one part, or a whole row block of the
+    // stream-K schedule), else the partials of this part or of this block's slot
+    const bool whole = sk ? (c_beg == 0 && c_end == C) : nparts == 1;
+    const size_t slot = sk ? (size_t)blockIdx.x * 2 + (seg ? 1 : 0) : 0;
+    if (g4 == 0 && row0 + r16 < nc) {
+        if (!sk) e_part[(size_t)(row0 + r16) * nparts + part] = rs;
+        else if (whole) e_part[row0 + r16] = rs;
+        else e_slot[slot * 128 + wv * 16 + r16] = rs;
+    }
 #pragma unroll
     for (int q = 0; q < 2 * DK; q++) {
         const int col = 16 * q + r16;
@@ -671,10 +702,79 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
         for (int v = 0; v < 4; v++) {
             const int row = row0 + rq + v;
             if (row >= nc) continue;
-            if (nparts == 1) G[(size_t)row * d + col] = (double)gacc[q][v] * inv_wscale;
+            if (whole) G[(size_t)row * d + col] = (double)gacc[q][v] * inv_wscale;
+            else if (sk) Gpart[(slot * 128 + wv * 16 + rq + v) * d + col] = gacc[q][v];
             else Gpart[((size_t)part * nc + row) * d + col] = gacc[q][v];
         }
     }
+    u = (long)rb * C + c_end;
+    seg++;
+    } while (sk && u < u_end);
+}
+
+// The block of the stream-K grid (B blocks over U units) whose range holds unit x:
+// the largest b with floor(b U / B) <= x.
+__device__ __forceinline__ int sk_block_of(long x, int B, long U) { return (int)(((x + 1) * B - 1) / U); }
+
+// Stream-K fix-up.  Blocks [0, gblocks): G of the row blocks that several blocks of the
+// grid share, each such row block once (by the grid boundary k that falls first inside
+// it), kSkQ blocks per boundary streaming a quarter of the row block's elements each.
+// A shared row block's slots are contiguous 128 x d FP32 tiles and its G rows are
+// contiguous, so elements go by flat index, four at a time.  Its first block b0 reached
+// it after an earlier row block (slot 1) unless its range starts at the row block
+// (slot 0); every later block starts in it (slot 0).  The slots are summed in block
+// order.  Blocks from gblocks on: the energy of every centre (whole row blocks from
+// e_part, shared ones from the slots), written back to e_part[i] and added to its site.
+constexpr int kSkQ = 4;
+__global__ __launch_bounds__(256) void k_gp_sk_finish(int nc, int d, int C, int B, long U, int gblocks,
+                                                      const float *__restrict__ Gslot,
+                                                      const double *__restrict__ e_slot, double inv_wscale,
+                                                      double *__restrict__ G, double *__restrict__ e_part,
+                                                      const int *__restrict__ centres, double *__restrict__ esite) {
+    if ((int)blockIdx.x < gblocks) {
+        const int k = blockIdx.x / kSkQ + 1, q = blockIdx.x % kSkQ;   // boundary between blocks k - 1 and k
+        const long x = (long)k * U / B;
+        if (x % C == 0) return;
+        const int rb = (int)(x / C);
+        const int b0 = sk_block_of((long)rb * C, B, U), b1 = sk_block_of((long)rb * C + C - 1, B, U);
+        if (b0 != k - 1) return;   // an earlier boundary of this row block does it
+        const int b0_slot = 2 * b0 + ((long)b0 * U / B < (long)rb * C ? 1 : 0);
+        const size_t n = (size_t)min(128, nc - rb * 128) * d;   // elements of the row block's G rows
+        const size_t tile = (size_t)128 * d;                    // floats per slot
+        double *g = G + (size_t)rb * tile;
+        for (size_t i4 = (size_t)q * 256 + threadIdx.x; i4 < n / 4; i4 += 256 * kSkQ) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            for (int b = b0; b <= b1; b++) {
+                const size_t sl = b == b0 ? b0_slot : 2 * (size_t)b;
+                const float4 v = ((const float4 *)(Gslot + sl * tile))[i4];
+                a0 += (double)v.x;
+                a1 += (double)v.y;
+                a2 += (double)v.z;
+                a3 += (double)v.w;
+            }
+            ((double2 *)g)[2 * i4] = make_double2(a0 * inv_wscale, a1 * inv_wscale);
+            ((double2 *)g)[2 * i4 + 1] = make_double2(a2 * inv_wscale, a3 * inv_wscale);
+        }
+        if (q == 0 && threadIdx.x < n % 4) {   // the last row block's tail
+            const size_t e = n / 4 * 4 + threadIdx.x;
+            double a = 0.0;
+            for (int b = b0; b <= b1; b++) a += (double)Gslot[(b == b0 ? b0_slot : 2 * (size_t)b) * tile + e];
+            g[e] = a * inv_wscale;
+        }
+        return;
+    }
+    const int i = (blockIdx.x - gblocks) * 256 + threadIdx.x;
+    if (i >= nc) return;
+    const int rb = i / 128, r = i - rb * 128;
+    const int b0 = sk_block_of((long)rb * C, B, U), b1 = sk_block_of((long)rb * C + C - 1, B, U);
+    double s = 0.0;
+    if (b0 == b1) s = e_part[i];
+    else {
+        const int b0_slot = 2 * b0 + ((long)b0 * U / B < (long)rb * C ? 1 : 0);
+        for (int b = b0; b <= b1; b++) s += e_slot[(size_t)(b == b0 ? b0_slot : 2 * b) * 128 + r];
+        e_part[i] = s;
+    }
+    if (centres) esite[centres[i]] += s;
 }
 
 // G = inv_wscale * sum over parts (fixed order) of the FP32 partials
@@ -736,22 +836,61 @@ int gp_fused_parts(int nc, int Mpad) {
     return cdivm(chunks, per);
 }
 
+// Blocks of one kernel instance that fit a CU at once, and the CU count, per device
+// (the stream-K grid is their product)
+template <typename K>
+static int sk_grid_blocks(K kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 512, 0) != hipSuccess) return 0;
+    return std::max(1, cus) * std::max(1, per);
+}
+
+static int fused_dk(int dpad) { return dpad <= 64 ? 2 : (dpad <= 128 ? 4 : 8); }
+
+// the stream-K grid of a launch: B blocks over U = row blocks x chunks units (B <= U)
+static void sk_plan(int prec, int dpad, int nc, int Mpad, int *B, long *U) {
+    static int cache[64][2][3] = {};   // [device][prec][dk]
+    int dev = 0;
+    hipGetDevice(&dev);
+    const int dk = fused_dk(dpad), di = dk == 2 ? 0 : dk == 4 ? 1 : 2, pi = prec == 1 ? 0 : 1;
+    int &b = cache[dev & 63][pi][di];
+    if (b == 0) {
+#define SKB(P, D) b = sk_grid_blocks(k_gp_fused_split<P, D, 4>)
+        if (prec == 1) { if (dk == 2) SKB(1, 2); else if (dk == 4) SKB(1, 4); else SKB(1, 8); }
+        else { if (dk == 2) SKB(2, 2); else if (dk == 4) SKB(2, 4); else SKB(2, 8); }
+#undef SKB
+        if (b <= 0) b = 256;
+    }
+    *U = (long)cdivm(nc, 128) * (Mpad / 32);
+    *B = (int)std::max(1L, std::min((long)b, *U));
+}
+
+size_t gp_fused_sk_scratch_bytes(int prec, int d, int dpad, int nc, int Mpad) {
+    int B = 0;
+    long U = 0;
+    sk_plan(prec, dpad, nc, Mpad, &B, &U);
+    return (size_t)B * 256 * d * 4 + (size_t)B * 256 * 8;
+}
+
 template <int PREC, int DK>
-static void gp_fused_go(int nc, int M, int d, int dpad, int Mpad, int parts, const double *X, const void *Shi,
-                        const void *Slo, const void *Thi, const void *Tlo, const double *wvec, double zeta,
-                        double wscale, double *G, float *Gpart, double *e_part, hipStream_t s) {
+static void gp_fused_go(int nc, int M, int d, int dpad, int Mpad, int parts, long sk_units, int sk_blocks,
+                        const double *X, const void *Shi, const void *Slo, const void *Thi, const void *Tlo,
+                        const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
+                        double *e_slot, hipStream_t s) {
     typedef typename Half<PREC>::T T;
     const int per = cdivm(Mpad / 32, parts);
-    dim3 grid(cdivm(nc, 128), parts);
+    dim3 grid = sk_units > 0 ? dim3(sk_blocks) : dim3(cdivm(nc, 128), parts);
     const int iz = int_zeta_m(zeta);
     if (iz == 4)
-        hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 4>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per, X,
-                           (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta, iz, wscale,
-                           1.0 / wscale, G, Gpart, e_part);
+        hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 4>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per,
+                           sk_units, X, (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta,
+                           iz, wscale, 1.0 / wscale, G, Gpart, e_part, e_slot);
     else
-        hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 0>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per, X,
-                           (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta, iz, wscale,
-                           1.0 / wscale, G, Gpart, e_part);
+        hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 0>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per,
+                           sk_units, X, (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta,
+                           iz, wscale, 1.0 / wscale, G, Gpart, e_part, e_slot);
 }
 
 int gp_fused_parts_valid(int parts, int Mpad) {
@@ -765,17 +904,35 @@ void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, i
                            const void *Xs_hi, const void *Xs_lo, const void *XsT_hi, const void *XsT_lo,
                            const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
                            int *ntile_out, hipStream_t s) {
-    const int parts = parts_req > 0 ? gp_fused_parts_valid(parts_req, Mpad) : gp_fused_parts(nc, Mpad);
-    *ntile_out = parts;
+    const bool sk = parts_req < 0;
+    const int parts = sk ? 1 : parts_req > 0 ? gp_fused_parts_valid(parts_req, Mpad) : gp_fused_parts(nc, Mpad);
+    *ntile_out = sk ? -1 : parts;
     if (nc <= 0) return;
-    const int dk = dpad <= 64 ? 2 : (dpad <= 128 ? 4 : 8);
-#define GO(P, D) gp_fused_go<P, D>(nc, M, d, dpad, Mpad, parts, X, Xs_hi, Xs_lo, XsT_hi, XsT_lo, wvec, zeta, wscale,                                    G, Gpart, e_part, s)
+    int B = 0;
+    long U = 0;
+    if (sk) sk_plan(prec, dpad, nc, Mpad, &B, &U);
+    double *e_slot = sk ? (double *)(Gpart + (size_t)B * 256 * d) : nullptr;
+    const int dk = fused_dk(dpad);
+#define GO(P, D) gp_fused_go<P, D>(nc, M, d, dpad, Mpad, parts, sk ? U : 0, B, X, Xs_hi, Xs_lo, XsT_hi, XsT_lo, wvec, \
+                                   zeta, wscale, G, Gpart, e_part, e_slot, s)
     if (prec == 1) {
         if (dk == 2) GO(1, 2); else if (dk == 4) GO(1, 4); else GO(1, 8);
     } else {
         if (dk == 2) GO(2, 2); else if (dk == 4) GO(2, 4); else GO(2, 8);
     }
 #undef GO
+}
+
+void launch_gp_sk_finish(int prec, int nc, int d, int dpad, int Mpad, bool want_g, const float *Gpart, double wscale,
+                         double *G, double *e_part, const int *centres, double *esite, hipStream_t s) {
+    if (nc <= 0) return;
+    int B = 0;
+    long U = 0;
+    sk_plan(prec, dpad, nc, Mpad, &B, &U);
+    const double *e_slot = (const double *)(Gpart + (size_t)B * 256 * d);
+    const int gb = want_g ? (B - 1) * kSkQ : 0;
+    hipLaunchKernelGGL(k_gp_sk_finish, dim3(gb + cdivm(nc, 256)), dim3(256), 0, s, nc, d, Mpad / 32, B, U, gb, Gpart,
+                       e_slot, 1.0 / wscale, G, e_part, centres, esite);
 }
 
 void launch_gp_fused_reduce(int nc, int d, int parts, const float *Gpart, double wscale, double *G, hipStream_t s) {

@@ -177,7 +177,7 @@ def test_split_tuning_is_listed(path):
                                             L.ptr(g2), L.ptr(ms1), L.ptr(ms2), C.byref(n)), "tuning_split")
     assert n.value == 1
     assert prec[0] == 1 and bucket[0] == 9                      # 512 centres: 2^8 < 512 <= 2^9
-    assert fparts[0] in (1, 2, 4, 8) and g1[0] == -1 and g2[0] == -1 and ms1[0] > 0
+    assert fparts[0] in (-1, 1, 2, 4, 8) and g1[0] == -1 and g2[0] == -1 and ms1[0] > 0
     r64 = Potential(param_filename=path).calc(at.copy(), force=True)
     assert abs(r["energy"] - r64["energy"]) / len(at) < 1e-3
     assert np.sqrt(np.mean((r["force"] - r64["force"]) ** 2)) < 1e-2
@@ -240,3 +240,54 @@ def test_forms_in_turn_on_shared_buffers(path):
         assert outs[p]["e"][0] == refs[p]["energy"], p
         assert np.array_equal(outs[p]["f"].reshape(-1, 3), refs[p]["force"]), p
     assert outs["fp64"]["e"][0] != outs["bf16x3"]["e"][0]
+
+
+@pytest.fixture(scope="module")
+def path300(tmp_path_factory):
+    """A SOAP model with 300 sparse points: ten chunks of 32 per row block of the fused
+    split kernel, so that the stream-K schedule splits row blocks in many ways."""
+    train = [configs.perturb(configs.diamond(5.43, (1, 1, 1)), 0.1, configs.rng_for(0, 1200 + k)) for k in range(40)]
+    m = configs.soap_model(_desc_fn, train, 300, 0, [configs.soap_string()], "K", alpha_std=0.1)
+    p = str(tmp_path_factory.mktemp("k") / "k.xml")
+    write_gap_xml(p, m)
+    return p
+
+
+def _run_schedule(path, frames, prec, schedule):
+    pot = Potential(param_filename=path, precision=prec)
+    L.check(L.lib().gapx_model_set_fused_schedule(pot.handle, schedule), "set_fused_schedule")
+    return [pot.calc_batch(frames) for _ in range(2)]
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("n_frames", [1, 70])
+def test_stream_k_schedule_matches_parts(path300, prec, n_frames):
+    """The fused split GP kernel under the stream-K schedule against one sparse-point
+    part (G written directly): one 512-atom cell (four row blocks over ten chunks: every
+    row block shared by several blocks of the grid) and 70 of them (35,840 centres, more
+    units per block than a row block has: whole and shared row blocks).  Energies agree
+    to the rounding of FP64 partial sums, forces and virials to that of the FP32 G
+    partials; two calls of each schedule are bitwise equal; the results stay within the
+    split form's budget of the FP64 ones."""
+    frames = [configs.perturb(configs.diamond(5.43, (4, 4, 4)), 0.1, configs.rng_for(3, 50 + k))
+              for k in range(n_frames)]
+    sk = _run_schedule(path300, frames, prec, -1)
+    p1 = _run_schedule(path300, frames, prec, 1)
+    for runs in (sk, p1):
+        for a, b in zip(*runs):
+            assert a["energy"] == b["energy"] and np.array_equal(a["force"], b["force"])
+    dE = max(abs(a["energy"] - b["energy"]) / len(f) for a, b, f in zip(sk[0], p1[0], frames))
+    dF = max(np.abs(a["force"] - b["force"]).max() for a, b in zip(sk[0], p1[0]))
+    dV = max(np.abs(a["virial"] - b["virial"]).max() for a, b in zip(sk[0], p1[0]))
+    print(prec, n_frames, "stream-K vs one part: dE/atom", dE, "max dF", dF, "max dV", dV)
+    assert dE < 1e-12 and dF < 1e-5 and dV < 1e-3
+    r64 = Potential(param_filename=path300).calc(frames[0].copy(), force=True)
+    assert abs(sk[0][0]["energy"] - r64["energy"]) / len(frames[0]) < 1e-3
+    assert np.sqrt(np.mean((sk[0][0]["force"] - r64["force"]) ** 2)) < 1e-2
+
+
+def test_fused_schedule_argument_checked(path300):
+    pot = Potential(param_filename=path300, precision="bf16x3")
+    assert L.lib().gapx_model_set_fused_schedule(pot.handle, -2) != 0
+    assert L.lib().gapx_model_set_fused_schedule(pot.handle, 9) != 0
+    assert L.lib().gapx_model_set_fused_schedule(pot.handle, 0) == 0
