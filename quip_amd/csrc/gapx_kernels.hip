@@ -535,12 +535,28 @@ __global__ __launch_bounds__(kFrontThreads) void k_nl_front_small(int N, int NC,
     __syncthreads();
     for (int n = t; n < N; n += kFrontThreads) suns[atomicAdd(&sfill[scell[n]], 1)] = n;
     __syncthreads();
-    for (int n = t; n < N; n += kFrontThreads) {
-        const int c = scell[n], m = scnt[c], st = sfill[c] - m;   // the cursor ended at the cell's end
-        int rank = 0;
-        for (int q = 0; q < m; q++) rank += suns[st + q] > n ? 1 : 0;
-        cell_atoms[st + rank] = n;
-        put_cell_atom(st + rank, n, pos, map_shift, cpos, cms);
+    for (int n0 = t; n0 < N; n0 += 4 * kFrontThreads) {   // four atoms' position loads in flight per thread
+        CellAtom ca[4];
+        CellShift cs[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int n = n0 + u * kFrontThreads;
+            if (n < N) {
+                ca[u] = CellAtom{{pos[3 * n], pos[3 * n + 1], pos[3 * n + 2], 0.0}};
+                cs[u] = CellShift{{map_shift[3 * n], map_shift[3 * n + 1], map_shift[3 * n + 2], n}};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int n = n0 + u * kFrontThreads;
+            if (n >= N) break;
+            const int c = scell[n], m = scnt[c], st = sfill[c] - m;   // the cursor ended at the cell's end
+            int rank = 0;
+            for (int q = 0; q < m; q++) rank += suns[st + q] > n ? 1 : 0;
+            cell_atoms[st + rank] = n;
+            cpos[st + rank] = ca[u];
+            cms[st + rank] = cs[u];
+        }
     }
 }
 void launch_nl_front_small(int N, int NC, const double *pos, const NLGeom *geo, const int *sid,
