@@ -1763,6 +1763,11 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                 std::vector<int> cand;
                 fused_part_candidates(m->precision, c, nc, cand);
                 for (int p : cand) b = std::max(b, fused_scratch_bytes(m->precision, c, nc, p));
+            } else if (m->autotune) {   // allocation-free: a tuning cached by an allocating call, if its slots are modest
+                GemmTune cached;
+                CHK(mixed_tune(m, eg, c, nc, true, cached, true));
+                const size_t cb = fused_scratch_bytes(m->precision, c, nc, cached.fparts);
+                if (cb <= kNoallocTuneCap) b = std::max(b, cb);
             }
             if (m->fused_schedule) b = std::max(b, fused_scratch_bytes(m->precision, c, nc, m->fused_schedule));
             gp_bytes[k] = b;
@@ -1800,8 +1805,8 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
             }
             CHK(mixed_tune(m, eg, c, nc, fused[k], tunes[k], noalloc));
             if (fused[k] && m->fused_schedule) tunes[k].fparts = m->fused_schedule;
-            // a tuning cached by an allocating call may need more partial-G space than
-            // the allocation-free call sized: the default rule then
+            // a tuning cached by an allocating call may need more partial-G space than the
+            // allocation-free call sized (over kNoallocTuneCap): the default rule then
             if (fused[k] && fused_scratch_bytes(m->precision, c, nc, tunes[k].fparts) > gp_bytes[k])
                 tunes[k].fparts = 0;
         }

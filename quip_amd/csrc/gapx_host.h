@@ -118,6 +118,7 @@ struct GemmTune {
     float f64_ms = 0;
 };
 constexpr size_t kTuneScratchCap = (size_t)256 << 20;   // largest partial-G buffer a tuning candidate may use
+constexpr size_t kNoallocTuneCap = (size_t)32 << 20;    // largest one an allocation-free call sizes for a cached tuning
 
 struct Coord {
     int kind = 0;          // 1 soap, 2 distance_2b

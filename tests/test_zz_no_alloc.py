@@ -1,6 +1,7 @@
 """The allocation-free entry point (gapx_calc_device_noalloc) against the entry point
 it is a variant of (gapx_calc_device): the same results, bit for bit, on two calls in
-a row, the second with a smaller structure, in the FP64 and bf16x3 forms; a request
+a row, the second with a smaller structure, in the FP64 and bf16x3 forms (bf16x3 also
+under the stream-K schedule of its fused GP kernel); a request
 beyond its device block is refused with GAPX_E_CAPACITY."""
 import ctypes as C
 
@@ -68,10 +69,14 @@ def _call(fn, pot, at, dev):
     return st
 
 
-@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3", "bf16x3-sk"])
 def test_noalloc_equals_device_entry(model_path, precision):
-    pot = Potential(param_filename=model_path, precision=precision)
+    """bf16x3: the allocating call tunes the fused split kernel and the allocation-free
+    call sizes its slots for that tuning; bf16x3-sk: both under the stream-K schedule."""
+    pot = Potential(param_filename=model_path, precision=precision.split("-")[0])
     lib = L.lib()
+    if precision.endswith("-sk"):
+        L.check(lib.gapx_model_set_fused_schedule(pot.handle, -1), "set_fused_schedule")
     big = configs.perturb(configs.diamond(5.43, (4, 4, 4)), 0.1, np.random.default_rng(1))     # 512 atoms
     small = configs.perturb(configs.diamond(5.43, (2, 2, 2)), 0.1, np.random.default_rng(2))   # 64 atoms
     for at in (big, small):
