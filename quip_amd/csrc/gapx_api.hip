@@ -1049,6 +1049,13 @@ static int ensure_nl_tables(Workspace &w, int T1cap) {
     return GAPX_OK;
 }
 
+// The one-block front serves small problems whose geometry is on the host (the device
+// geometry kernels write flags, which the front would clear).  Its search reads the
+// candidates through the cells' atom lists; larger problems also write cell-ordered
+// copies of positions and shifts, which their search reads without a dependent load
+// (the copies cost the one-block front more than its search gains).
+static bool nl_small(const NLPlan &pl) { return !pl.dev_geo && pl.N <= kNlFrontAtoms && pl.NC <= kNlFrontCells; }
+
 // Binning, the candidate search and the count scans (flags: [0] overflow, [1] largest
 // neighbour1 count, [2] largest neighbour2 count, [3] neighbour1 total).
 static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
@@ -1058,14 +1065,13 @@ static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
     const NLGeom *dgeo = w.geo.as<NLGeom>();
     const int *dsid = w.sid.as<int>(), *dcb = w.cell_base.as<int>();
     int *flags = w.nl_flags.as<int>();
-    // the one-block front: small problems whose geometry is on the host (the device
-    // geometry kernels write flags, which the front would clear)
-    const bool small = !pl.dev_geo && N <= kNlFrontAtoms && NC <= kNlFrontCells;
+    const bool small = nl_small(pl);
+    CellAtom *cpos = small ? nullptr : w.cpos.as<CellAtom>();
+    CellShift *cms = small ? nullptr : w.cms.as<CellShift>();
     if (small) {
         launch_nl_front_small(N, NC, d_pos, dgeo, dsid, dcb, w.cell_of.as<int>(), w.map_shift.as<int>(),
-                              w.cell_cnt.as<int>(), w.cell_start.as<int>(), w.cell_atoms.as<int>(),
-                              w.cpos.as<CellAtom>(), w.cms.as<CellShift>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(),
-                              flags, s);
+                              w.cell_cnt.as<int>(), w.cell_start.as<int>(), w.cell_atoms.as<int>(), nullptr,
+                              nullptr, w.n2_cnt.as<int>(), w.n2_fill.as<int>(), flags, s);
     } else {
         {
             int *zp[5] = {w.cell_cnt.as<int>(), w.cell_fill.as<int>(), w.n2_cnt.as<int>(), w.n2_fill.as<int>(),
@@ -1083,19 +1089,19 @@ static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
         launch_nl_cell_fill(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_fill.as<int>(),
                             w.cell_unsorted.as<int>(), s);
         launch_nl_cell_rank(N, w.cell_of.as<int>(), w.cell_start.as<int>(), w.cell_cnt.as<int>(),
-                            w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), d_pos, w.map_shift.as<int>(),
-                            w.cpos.as<CellAtom>(), w.cms.as<CellShift>(), s);
+                            w.cell_unsorted.as<int>(), w.cell_atoms.as<int>(), d_pos, w.map_shift.as<int>(), cpos,
+                            cms, s);
     }
     if (pl.one_pass) {
         launch_nl_pairs(2, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), w.cpos.as<CellAtom>(),
-                        w.cms.as<CellShift>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), cpos, cms, dgeo, dsid, dcb,
+                        w.n1_cnt.as<int>(), nullptr,
                         w.t1_j.as<int>(), w.t1_shift.as<int>(), w.t1_r.as<double>(), w.n2_cnt.as<int>(), nullptr,
                         nullptr, w.t2_i.as<int>(), w.t2_k.as<int>(), pl.cap1, pl.cap2, flags, s);
     } else {
         launch_nl_pairs(0, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), w.cpos.as<CellAtom>(),
-                        w.cms.as<CellShift>(), dgeo, dsid, dcb, w.n1_cnt.as<int>(), nullptr,
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), cpos, cms, dgeo, dsid, dcb,
+                        w.n1_cnt.as<int>(), nullptr,
                         nullptr, nullptr, nullptr, w.n2_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, 0, 0,
                         nullptr, s);
     }
@@ -1135,8 +1141,8 @@ static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bo
                           w.n2_k.as<int>(), s);
     } else {
         launch_nl_pairs(1, N, d_pos, w.map_shift.as<int>(), w.cell_of.as<int>(), w.cell_start.as<int>(),
-                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), w.cpos.as<CellAtom>(),
-                        w.cms.as<CellShift>(), dgeo, dsid, dcb, nullptr, w.n1_off.as<int>(),
+                        w.cell_cnt.as<int>(), w.cell_atoms.as<int>(), nl_small(pl) ? nullptr : w.cpos.as<CellAtom>(),
+                        nl_small(pl) ? nullptr : w.cms.as<CellShift>(), dgeo, dsid, dcb, nullptr, w.n1_off.as<int>(),
                         w.n1_j.as<int>(), w.n1_shift.as<int>(), w.n1_r.as<double>(), nullptr, w.n2_off.as<int>(),
                         w.n2_fill.as<int>(), w.n2_i.as<int>(), w.n2_k.as<int>(), 0, 0, nullptr, s);
     }

@@ -535,28 +535,12 @@ __global__ __launch_bounds__(kFrontThreads) void k_nl_front_small(int N, int NC,
     __syncthreads();
     for (int n = t; n < N; n += kFrontThreads) suns[atomicAdd(&sfill[scell[n]], 1)] = n;
     __syncthreads();
-    for (int n0 = t; n0 < N; n0 += 4 * kFrontThreads) {   // four atoms' position loads in flight per thread
-        CellAtom ca[4];
-        CellShift cs[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int n = n0 + u * kFrontThreads;
-            if (n < N) {
-                ca[u] = CellAtom{{pos[3 * n], pos[3 * n + 1], pos[3 * n + 2], 0.0}};
-                cs[u] = CellShift{{map_shift[3 * n], map_shift[3 * n + 1], map_shift[3 * n + 2], n}};
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int n = n0 + u * kFrontThreads;
-            if (n >= N) break;
-            const int c = scell[n], m = scnt[c], st = sfill[c] - m;   // the cursor ended at the cell's end
-            int rank = 0;
-            for (int q = 0; q < m; q++) rank += suns[st + q] > n ? 1 : 0;
-            cell_atoms[st + rank] = n;
-            cpos[st + rank] = ca[u];
-            cms[st + rank] = cs[u];
-        }
+    for (int n = t; n < N; n += kFrontThreads) {
+        const int c = scell[n], m = scnt[c], st = sfill[c] - m;   // the cursor ended at the cell's end
+        int rank = 0;
+        for (int q = 0; q < m; q++) rank += suns[st + q] > n ? 1 : 0;
+        cell_atoms[st + rank] = n;
+        if (cpos) put_cell_atom(st + rank, n, pos, map_shift, cpos, cms);
     }
 }
 void launch_nl_front_small(int N, int NC, const double *pos, const NLGeom *geo, const int *sid,
@@ -594,7 +578,7 @@ __device__ __forceinline__ int wave_excl_scan_i(int v, int *total) {
 // and raises *ovf when a count exceeds its capacity.
 // Latency-bound (most of its time waits on the candidates' global loads): the occupancy
 // hint gives five wavefronts per SIMD (96 VGPRs, no spills) instead of four.
-template <int MODE>
+template <int MODE, bool CC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_nl_pairs_w(int N, const double *__restrict__ pos,
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
                                                     const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
@@ -687,9 +671,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                     else hi_o = mid - 1;
                 }
                 const int q = c - s_pref[wv][lo_o];
-                const int slot = s_start[wv][lo_o] + q;   // the cell-ordered copies: no dependent load
-                const CellShift cs2 = cms[slot];
-                const CellAtom ca2 = cpos[slot];
+                const int slot = s_start[wv][lo_o] + q;
+                CellShift cs2;
+                CellAtom ca2;
+                if constexpr (CC) {   // the cell-ordered copies: no dependent load
+                    cs2 = cms[slot];
+                    ca2 = cpos[slot];
+                } else {              // through the cell's atom list
+                    const int a2 = cell_atoms[slot];
+                    cs2 = CellShift{{map_shift[3 * a2], map_shift[3 * a2 + 1], map_shift[3 * a2 + 2], a2}};
+                    ca2 = CellAtom{{pos[3 * a2], pos[3 * a2 + 1], pos[3 * a2 + 2], 0.0}};
+                }
                 atom2 = cs2.v[3];
                 const int pk = s_sh[wv][lo_o];
                 const int i4 = (int)(signed char)(pk & 255), j4 = (int)(signed char)((pk >> 8) & 255),
@@ -757,13 +749,15 @@ void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, c
                      const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt, const int *n2_off,
                      int *n2_fill, int *n2_i, int *n2_k, int cap1, int cap2, int *ovf, hipStream_t s) {
     if (N <= 0) return;
-#define NLP(M)                                                                                                  \
-    hipLaunchKernelGGL(k_nl_pairs_w<M>, dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of, cell_start, \
-                       cell_cnt, cell_atoms, cpos, cms, geo, sid, cell_base, n1_cnt, n1_off, n1_j, n1_shift, n1_r, \
-                       n2_cnt, n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
-    if (mode == 1) NLP(1);
-    else if (mode == 2) NLP(2);
-    else NLP(0);
+#define NLP(M, CC)                                                                                              \
+    hipLaunchKernelGGL((k_nl_pairs_w<M, CC>), dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,      \
+                       cell_start, cell_cnt, cell_atoms, cpos, cms, geo, sid, cell_base, n1_cnt, n1_off, n1_j,   \
+                       n1_shift, n1_r, n2_cnt, n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
+    if (cpos) {
+        if (mode == 1) NLP(1, true); else if (mode == 2) NLP(2, true); else NLP(0, true);
+    } else {
+        if (mode == 1) NLP(1, false); else if (mode == 2) NLP(2, false); else NLP(0, false);
+    }
 #undef NLP
 }
 
