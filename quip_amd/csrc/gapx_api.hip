@@ -1119,7 +1119,10 @@ static void nl_launch_front(Engine &eg, const NLPlan &pl, const double *d_pos) {
 // order and the full per-atom view.  slots_only (a speculative build, whose tables no
 // later call reuses): the neighbour2 sort and the full view read the one-pass slots
 // directly and the compacted neighbour1/neighbour2 copies are not made.
-static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bool compact, bool slots_only = false) {
+// slots_only with pairF: the full view's kernel also clears the step's pair forces,
+// pair energies (pairE, if any) and site energies
+static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bool compact, bool slots_only = false,
+                           double *pairF = nullptr, double *pairE = nullptr, double *esite = nullptr) {
     Workspace &w = eg.ws;
     hipStream_t s = eg.stream;
     const int N = pl.N;
@@ -1131,7 +1134,7 @@ static void nl_launch_back(Engine &eg, const NLPlan &pl, const double *d_pos, bo
         launch_nl_full(N, d_pos, dgeo, dsid, w.n1_off.as<int>(), w.t1_j.as<int>(), w.t1_shift.as<int>(),
                        w.t1_r.as<double>(), w.n2_off.as<int>(), w.n2_is.as<int>(), w.n2_ks.as<int>(),
                        w.n1_rev.as<int>(), w.nb_off.as<int>(), w.nb_j.as<int>(), w.nb_rev.as<int>(),
-                       w.nb_r.as<double>(), w.nb_d.as<double>(), pl.cap1, s);
+                       w.nb_r.as<double>(), w.nb_d.as<double>(), pl.cap1, s, pairF, pairE, esite);
         return;
     }
     if (compact) {
@@ -1833,10 +1836,8 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     auto launch_all = [&](bool marks) {
         if (spec) {   // the neighbour list joins the sequence; pair arrays are cleared to its count
             nl_launch_front(eg, pl, d_pos);
-            nl_launch_back(eg, pl, d_pos, true, true);
+            nl_launch_back(eg, pl, d_pos, true, true, w.pairF.as<double>(), pairE, w.esite.as<double>());
             if (marks) tm.stage(0, t_nl0, tm.mark(s));
-            launch_zero_step(w.pairF.as<double>(), pairE, w.esite.as<double>(), N, w.nb_off.as<int>() + N, 0,
-                             (size_t)P, s);
         } else {
             launch_zero_step(w.pairF.as<double>(), pairE, w.esite.as<double>(), N, nullptr, P, (size_t)P, s);
         }

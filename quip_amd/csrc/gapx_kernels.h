@@ -79,7 +79,8 @@ void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_
 void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d, int cap1,
-                    hipStream_t s);
+                    hipStream_t s, double *pairF = nullptr,
+                    double *pairE = nullptr, double *esite = nullptr);
 // skin reuse: max displacement since the last build (bits of a double, atomicMax into a
 // zeroed word) and the stored neighbour1 distances recomputed as calc_dists does
 void launch_nl_skin_check(int N, const double *pos, const double *last, const NLGeom *geo, const int *sid,

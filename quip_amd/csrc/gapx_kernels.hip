@@ -871,13 +871,16 @@ void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_
 // SLOT: the neighbour1 data (j, shift, r) are read from the per-atom slots of a one-pass
 // search (atom a's entry k at a * cap1 + k) instead of the compacted tables; offsets,
 // neighbour2 and the mirror slots n1_rev keep the compacted numbering.
+// With pairF non-null the step's accumulators are cleared on the way: pairF (and pairE,
+// if given) over every pair written, esite over every atom.
 template <bool SLOT>
 __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict__ pos,
                                                  const NLGeom *__restrict__ geo, const int *__restrict__ sid,
                                                  const int *n1_off, const int *n1_j, const int *n1_shift,
                                                  const double *n1_r, const int *n2_off, const int *n2_i,
                                                  const int *n2_k, const int *n1_rev, int *nb_off, int *nb_j,
-                                                 int *nb_rev, double *nb_r, double *nb_d, int cap1) {
+                                                 int *nb_rev, double *nb_r, double *nb_d, int cap1, double *pairF,
+                                                 double *pairE, double *esite) {
 #pragma clang fp contract(off)
     auto at1 = [&](int a, int k) -> size_t { return SLOT ? (size_t)a * cap1 + k : (size_t)n1_off[a] + k; };
     const int lane = threadIdx.x & 31;
@@ -886,6 +889,7 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
     const int base = n1_off[i] + n2_off[i];
     if (lane == 0) nb_off[i] = base;
     if (i == N) return;
+    if (pairF && lane == 0) esite[i] = 0.0;
     const NLGeom &g = geo[sid[i]];
     const int n2c = n2_off[i + 1] - n2_off[i];
     const int n1c = n1_off[i + 1] - n1_off[i];
@@ -922,18 +926,24 @@ __global__ __launch_bounds__(256) void k_nl_full(int N, const double *__restrict
         nb_rev[p] = rev;
         nb_r[p] = n1_r[e1];
         nb_d[3 * p] = d0; nb_d[3 * p + 1] = d1; nb_d[3 * p + 2] = d2;
+        if (pairF) {
+            pairF[3 * p] = 0.0; pairF[3 * p + 1] = 0.0; pairF[3 * p + 2] = 0.0;
+            if (pairE) pairE[p] = 0.0;
+        }
     }
 }
 void launch_nl_full(int N, const double *pos, const NLGeom *geo, const int *sid, const int *n1_off, const int *n1_j,
                     const int *n1_shift, const double *n1_r, const int *n2_off, const int *n2_i, const int *n2_k,
                     const int *n1_rev, int *nb_off, int *nb_j, int *nb_rev, double *nb_r, double *nb_d, int cap1,
-                    hipStream_t s) {
+                    hipStream_t s, double *pairF, double *pairE, double *esite) {
     if (cap1 > 0)
         hipLaunchKernelGGL(k_nl_full<true>, dim3(cdiv(N + 1, 8)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j,
-                           n1_shift, n1_r, n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d, cap1);
+                           n1_shift, n1_r, n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d, cap1, pairF,
+                           pairE, esite);
     else
         hipLaunchKernelGGL(k_nl_full<false>, dim3(cdiv(N + 1, 8)), dim3(256), 0, s, N, pos, geo, sid, n1_off, n1_j,
-                           n1_shift, n1_r, n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d, 0);
+                           n1_shift, n1_r, n2_off, n2_i, n2_k, n1_rev, nb_off, nb_j, nb_rev, nb_r, nb_d, 0, pairF,
+                           pairE, esite);
 }
 
 // Skin reuse (calc_connect with cutoff_skin, Connection.f95:1085-1127).
