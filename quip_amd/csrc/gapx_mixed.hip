@@ -501,10 +501,12 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, in
                                             const typename Half<PREC>::T *__restrict__ xs_lo,
                                             const typename Half<PREC>::T *__restrict__ xt_h,
                                             const typename Half<PREC>::T *__restrict__ xt_lo,
-                                            const double *__restrict__ wcur, typename Half<PREC>::T *__restrict__ nxs_h,
+                                            const double *__restrict__ wcur, const float *__restrict__ wfcur,
+                                            typename Half<PREC>::T *__restrict__ nxs_h,
                                             typename Half<PREC>::T *__restrict__ nxs_lo,
                                             typename Half<PREC>::T *__restrict__ nxt_h,
                                             typename Half<PREC>::T *__restrict__ nxt_lo, double *__restrict__ wnext,
+                                            float *__restrict__ wfnext,
                                             const typename Half<PREC>::V (&xh)[DK],
                                             const typename Half<PREC>::V (&xl)[DK], float4_t (&gacc)[2 * DK],
                                             double &rs) {
@@ -543,7 +545,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, in
                 const float kf = kacc[j][v];
                 const float k2 = kf * kf;
                 rs += wsv * (double)(k2 * k2);
-                const float wk = (float)(wsv * 4.0 * wscale) * (k2 * kf);
+                const float wk = wfcur[16 * j + rq + v] * (k2 * kf);   // (float)(w_s 4 wscale), staged
                 h = (T)wk;
                 lo = (T)(wk - (float)h);
             } else {
@@ -575,7 +577,10 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, in
         gacc[q] = H::mfma(wh, bh, gacc[q]);
     }
     // the next buffers were last read before the previous barrier
-    if (next && t < 32) wnext[t] = pw;
+    if (next && t < 32) {
+        wnext[t] = pw;
+        wfnext[t] = (float)(pw * 4.0 * wscale);
+    }
     __builtin_amdgcn_s_waitcnt(0);   // this wave's loads into the next buffers have landed
     __syncthreads();
 }
@@ -626,6 +631,7 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __shared__ __attribute__((aligned(16))) T xs0[2][32 * DP], xs1[2][32 * DP];   // [hi, lo] of two buffers
     __shared__ __attribute__((aligned(16))) T xt0[2][DP * 32], xt1[2][DP * 32];
     __shared__ double wv0[32], wv1[32];   // the chunk's w_s (0 past M)
+    __shared__ float wf0[32], wf1[32];    // and (float)(w_s 4 wscale), the zeta = 4 epilogue's factor
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int r16 = lane & 15, g4 = lane >> 4, kg = g4 * 8, rq = 4 * g4;
     const int part = blockIdx.y, nparts = gridDim.y;
@@ -669,18 +675,21 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
         double pw = 0.0;
         fused_stage<T, DK>(32 * c_beg, M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, xs0[0], xs0[1], xt0[0], xt0[1],
                               &pw);
-        if (t < 32) wv0[t] = pw;
+        if (t < 32) {
+            wv0[t] = pw;
+            wf0[t] = (float)(pw * 4.0 * wscale);
+        }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int ch = c_beg; ch < c_end; ch += 2) {   // buffer set 0, then set 1
         fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, zeta,
-                                  izeta, wscale, xs0[0], xs0[1], xt0[0], xt0[1], wv0, xs1[0], xs1[1], xt1[0], xt1[1],
-                                  wv1, xh, xl, gacc, rs);
+                                  izeta, wscale, xs0[0], xs0[1], xt0[0], xt0[1], wv0, wf0, xs1[0], xs1[1], xt1[0],
+                                  xt1[1], wv1, wf1, xh, xl, gacc, rs);
         if (ch + 1 >= c_end) break;
         fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, zeta,
-                                  izeta, wscale, xs1[0], xs1[1], xt1[0], xt1[1], wv1, xs0[0], xs0[1], xt0[0], xt0[1],
-                                  wv0, xh, xl, gacc, rs);
+                                  izeta, wscale, xs1[0], xs1[1], xt1[0], xt1[1], wv1, wf1, xs0[0], xs0[1], xt0[0],
+                                  xt0[1], wv0, wf0, xh, xl, gacc, rs);
     }
     // energy partials of this part: the four lane groups hold different sparse points
     rs += __shfl_xor(rs, 16, 64);
