@@ -1691,6 +1691,7 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         const size_t need_max = (16 + 48 * (size_t)S + 15) & ~(size_t)15;
         if (!spec_ok || eg.pending.size() >= 64 || eg.checks_used + need_max > eg.checks.cap) {
             CHK(resolve_pending_list(m, eg, nullptr));
+            eg.checks.coherent = true;   // written by the step's last kernel in eager calls
             if (spec_ok) CHK(eg.checks.ensure(std::max<size_t>(need_max * 64, 4096)));
         }
     }
@@ -1833,6 +1834,11 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     double *pairE = has_d2b ? w.pairE.as<double>() : nullptr;
     double *d_epc = opt ? opt->d_epc : nullptr;
     if (S == 1) CHK(w.spart.ensure((size_t)(N / 8 + 2) * 80));   // the gather's block partials
+    // Eager speculative calls hand the build flags to their read-back slot from the step's
+    // last kernel (the slot is this call's: pending calls take consecutive slots); a graph
+    // replay keeps a copy after the graph, whose kernel arguments are fixed at capture.
+    const bool use_graph = m->graphs && !tm.on && !ext && !opt && !noalloc;
+    int *flag_slot = spec && !use_graph ? (int *)(eg.checks.as<char>() + eg.checks_used) : nullptr;
     auto launch_all = [&](bool marks) {
         if (spec) {   // the neighbour list joins the sequence; pair arrays are cleared to its count
             nl_launch_front(eg, pl, d_pos);
@@ -1958,8 +1964,8 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         launch_gather(N, w.Z.as<int>(), m->d_e0.as<double>(), m->E_scale, w.nb_off.as<int>(), w.nb_rev.as<int>(),
                       w.nb_d.as<double>(), w.pairF.as<double>(), pairE, w.esite.as<double>(), o_f, o_le,
                       o_lv, cmask, S == 1 ? w.spart.as<double>() : nullptr, s);
-        if (S == 1) launch_struct_finish(N, w.spart.as<double>(), o_e, o_v, s);
-        else launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s);
+        if (S == 1) launch_struct_finish(N, w.spart.as<double>(), o_e, o_v, s, w.nl_flags.as<int>(), flag_slot);
+        else launch_struct_reduce(S, w.atom_off.as<int>(), o_le, o_lv, o_e, o_v, s, w.nl_flags.as<int>(), flag_slot);
         if (want_var)
             launch_gather_var(N, w.nb_off.as<int>(), w.nb_rev.as<int>(), w.pairFv.as<double>(),
                               has_d2b_var ? w.pairVE.as<double>() : nullptr, w.varsite.as<double>(), opt->d_gvg,
@@ -1969,7 +1975,7 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
     eg.var_valid = want_var;
     eg.var_P = P;
 
-    if (m->graphs && !tm.on && !ext && !opt && !noalloc) {
+    if (use_graph) {
         // replay the captured sequence when nothing it depends on has changed
         std::vector<long long> key = {S, N, P, m->precision, (long long)want, (long long)(uintptr_t)o_e,
                                       (long long)(uintptr_t)o_v, (long long)(uintptr_t)o_f, (long long)(uintptr_t)o_le,
@@ -2021,7 +2027,7 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         pc.slot = eg.checks_used;
         pc.extents = pl.dev_geo;
         char *slot = eg.checks.as<char>() + pc.slot;
-        HIPCHK(hipMemcpyAsync(slot, w.nl_flags.p, 16, hipMemcpyDeviceToHost, s));
+        if (!flag_slot) HIPCHK(hipMemcpyAsync(slot, w.nl_flags.p, 16, hipMemcpyDeviceToHost, s));
         if (pl.dev_geo) HIPCHK(hipMemcpyAsync(slot + 16, w.minmax.p, 48 * (size_t)S, hipMemcpyDeviceToHost, s));
         eg.checks_used += (need + 15) & ~(size_t)15;
         eg.pending.push_back(std::move(pc));

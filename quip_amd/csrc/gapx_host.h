@@ -196,11 +196,12 @@ struct ExtNL {
 };
 
 // Pinned host staging (hipHostMalloc), grow-only; `pageable` staging (ordinary host
-// memory, for the allocation-free entry point) when set before the first use.
+// memory, for the allocation-free entry point) when set before the first use;
+// `coherent` (fine-grained) when kernels write it directly.
 struct HBuf {
     void *p = nullptr;
     size_t cap = 0;
-    bool pageable = false;
+    bool pageable = false, coherent = false;
     std::vector<char> host;
     int ensure(size_t bytes) {
         if (bytes <= cap) return GAPX_OK;
@@ -214,7 +215,7 @@ struct HBuf {
         p = nullptr;
         cap = 0;
         size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
-        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, want, coherent ? hipHostMallocCoherent : hipHostMallocDefault);
         if (e != hipSuccess) return fail(GAPX_E_DEVICE, "hipHostMalloc(%zu): %s", want, hipGetErrorString(e));
         cap = want;
         return GAPX_OK;
@@ -337,7 +338,7 @@ struct Engine {
         bool extents = false;
     };
     std::vector<Pending> pending;
-    HBuf checks;                     // pinned read-back slots of the pending calls
+    HBuf checks;                     // pinned (fine-grained) read-back slots of the pending calls
     size_t checks_used = 0;
     ~Engine() {
         if (gexec) hipGraphExecDestroy(gexec);

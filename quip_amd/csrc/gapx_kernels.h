@@ -228,9 +228,13 @@ void launch_gather(int N, const int *Z, const double *e0, double E_scale, const 
                    const double *nb_d, const double *pairF, const double *pairE, const double *esite, double *f,
                    double *local_e, double *local_virial, const unsigned char *mask, double *block_part,
                    hipStream_t s);
-// the energy and virial of one structure of N atoms from the gather's block partials
-void launch_struct_finish(int N, const double *part, double *e_out, double *virial_out, hipStream_t s);
+// the energy and virial of one structure of N atoms from the gather's block partials;
+// with flag_dst (pinned host memory, written by the device without a copy launch) the
+// four neighbour-list build flags at flag_src are copied there as well
+void launch_struct_finish(int N, const double *part, double *e_out, double *virial_out, hipStream_t s,
+                          const int *flag_src = nullptr, int *flag_dst = nullptr);
 void launch_struct_reduce(int S, const int *atom_off, const double *local_e, const double *local_virial,
-                          double *e_out, double *virial_out, hipStream_t s);
+                          double *e_out, double *virial_out, hipStream_t s, const int *flag_src = nullptr,
+                          int *flag_dst = nullptr);
 
 }  // namespace gapx

@@ -2123,12 +2123,16 @@ void launch_gather(int N, const int *Z, const double *e0, double E_scale, const 
 // thread t sums the structure's atoms t, t+512, ... in that order (four atoms' loads
 // issued together), a fixed shuffle tree joins the lanes and threads 0..9 the 8 waves,
 // so a structure's totals do not depend on the other structures of a batch.
+// With flag_dst (pinned host memory) the first block also copies the four neighbour-list
+// build flags there: the step's last kernel hands them to the host without a copy launch.
 constexpr int kReduceThreads = 512;
 __global__ __launch_bounds__(kReduceThreads) void k_struct_reduce(const int *atom_off, const double *local_e,
                                                                   const double *local_virial, double *e_out,
-                                                                  double *virial_out) {
+                                                                  double *virial_out, const int *flag_src,
+                                                                  int *flag_dst) {
     __shared__ double sred[10][kReduceThreads / 64];
     const int st = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (flag_dst && st == 0 && t < 4) flag_dst[t] = flag_src[t];
     const int a0 = atom_off[st], a1 = atom_off[st + 1];
     double acc[10];
 #pragma unroll
@@ -2165,9 +2169,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_struct_reduce(const int *ato
 // One structure: the totals from the gather's block partials (10 per block, blocks of 8
 // atoms in order): thread t sums partials t, t+512, ... in turn, then the fixed tree.
 __global__ __launch_bounds__(kReduceThreads) void k_struct_finish(int nb, const double *part, double *e_out,
-                                                                  double *virial_out) {
+                                                                  double *virial_out, const int *flag_src,
+                                                                  int *flag_dst) {
     __shared__ double sred[10][kReduceThreads / 64];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (flag_dst && t < 4) flag_dst[t] = flag_src[t];
     double acc[10];
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.0;
@@ -2198,15 +2204,17 @@ __global__ __launch_bounds__(kReduceThreads) void k_struct_finish(int nb, const 
         else virial_out[t - 1] = x;
     }
 }
-void launch_struct_finish(int N, const double *part, double *e_out, double *virial_out, hipStream_t s) {
+void launch_struct_finish(int N, const double *part, double *e_out, double *virial_out, hipStream_t s,
+                          const int *flag_src, int *flag_dst) {
     if (N > 0)
-        hipLaunchKernelGGL(k_struct_finish, dim3(1), dim3(kReduceThreads), 0, s, cdiv(N, 8), part, e_out, virial_out);
+        hipLaunchKernelGGL(k_struct_finish, dim3(1), dim3(kReduceThreads), 0, s, cdiv(N, 8), part, e_out, virial_out,
+                           flag_src, flag_dst);
 }
 void launch_struct_reduce(int S, const int *atom_off, const double *local_e, const double *local_virial,
-                          double *e_out, double *virial_out, hipStream_t s) {
+                          double *e_out, double *virial_out, hipStream_t s, const int *flag_src, int *flag_dst) {
     if (S > 0)
         hipLaunchKernelGGL(k_struct_reduce, dim3(S), dim3(kReduceThreads), 0, s, atom_off, local_e, local_virial,
-                           e_out, virial_out);
+                           e_out, virial_out, flag_src, flag_dst);
 }
 
 }  // namespace gapx
