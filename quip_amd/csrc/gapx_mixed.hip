@@ -14,6 +14,7 @@
 #include "gapx_kernels.h"
 
 #include <algorithm>
+#include <atomic>
 
 namespace gapx {
 
@@ -860,17 +861,18 @@ static int fused_dk(int dpad) { return dpad <= 64 ? 2 : (dpad <= 128 ? 4 : 8); }
 
 // the stream-K grid of a launch: B blocks over U = row blocks x chunks units (B <= U)
 static void sk_plan(int prec, int dpad, int nc, int Mpad, int *B, long *U) {
-    static int cache[64][2][3] = {};   // [device][prec][dk]
+    static std::atomic<int> cache[64][2][3];   // [device][prec][dk]; engines on several threads share it
     int dev = 0;
     hipGetDevice(&dev);
     const int dk = fused_dk(dpad), di = dk == 2 ? 0 : dk == 4 ? 1 : 2, pi = prec == 1 ? 0 : 1;
-    int &b = cache[dev & 63][pi][di];
+    int b = cache[dev & 63][pi][di].load(std::memory_order_relaxed);
     if (b == 0) {
 #define SKB(P, D) b = sk_grid_blocks(k_gp_fused_split<P, D, 4>)
         if (prec == 1) { if (dk == 2) SKB(1, 2); else if (dk == 4) SKB(1, 4); else SKB(1, 8); }
         else { if (dk == 2) SKB(2, 2); else if (dk == 4) SKB(2, 4); else SKB(2, 8); }
 #undef SKB
         if (b <= 0) b = 256;
+        cache[dev & 63][pi][di].store(b, std::memory_order_relaxed);
     }
     *U = (long)cdivm(nc, 128) * (Mpad / 32);
     *B = (int)std::max(1L, std::min((long)b, *U));
