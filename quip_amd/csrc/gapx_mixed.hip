@@ -430,8 +430,6 @@ static bool use_lds_tiles(int rows, int cols, int lds_mode = -1) {
     return (long)cdivm(rows, kLT) * cdivm(cols, kLT) >= 512;
 }
 
-static __device__ __attribute__((aligned(16))) unsigned int kZero16[4];   // a zero 16-byte vector (never written)
-
 // 16 bytes per lane from global memory straight into LDS: lane l's bytes land at
 // lds + 16 l (lds uniform over the wavefront)
 __device__ __forceinline__ void dma16(const void *g, void *lds) {
@@ -444,6 +442,23 @@ __device__ __forceinline__ void dma16(const void *g, void *lds) {
 //   Xs rows (DP/8 vectors):  vector c of row r at c ^ (r & XSM)
 //   Xs^T rows (4 vectors of 8 sparse points in the chunk order, 64 bytes): vector c of
 //   row q at c ^ ((q >> 1) & 3), so rows q..q+7 cover the 8 groups of 128 bytes
+// One k step of the fused split kernel's products is its own scheduling region: the
+// four fragment reads of the next step first, then the six MFMAs of this one, so that
+// the reads are in flight under the MFMAs.  Left to itself the compiler issues each
+// step's reads right before its MFMAs and waits for them there (both waves of a SIMD
+// leave each chunk barrier together, so neither covers the other's wait): batch512
+// 822 -> 796 us.
+// Pinned for the BF16 zeta = 4 kernels only: the others need more registers for their
+// epilogue, and with the reads held ahead they spill or lose a wavefront per SIMD.
+#define GP_SCHED_STEP()                                           \
+    do {                                                          \
+        if constexpr (PREC == 1 && ZI == 4) {                     \
+            __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);    \
+            __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);    \
+            __builtin_amdgcn_sched_barrier(0);                    \
+        }                                                         \
+    } while (0)
+
 template <int DK>
 struct FusedLds {
     static constexpr int DP = 32 * DK, XSV = DP / 8, XSM = XSV >= 16 ? 15 : XSV - 1;
@@ -451,41 +466,69 @@ struct FusedLds {
     static __device__ __forceinline__ int xt_at(int q, int c) { return q * 32 + ((c ^ ((q >> 1) & 3)) << 3); }
 };
 
+// Where this lane's 16-byte vectors of a chunk come from: byte offsets from the chunk's
+// first Xs row and from column s0 of the Xs^T rows, the same for every chunk.  Wave
+// instruction u fills LDS vectors 64u .. 64u+63 of a matrix; lane l loads the global
+// vector that the swizzle puts in slot 64u + l.  Vectors past the matrix need no zeros:
+// Xs columns past the padded width ldS meet X components that are zero in registers
+// (they take column ldS - 8 .. ldS - 1 of the same row), and Xs^T rows past d give G
+// columns that are not written (they take row d - 1).
+template <int DK>
+struct StageOff {
+    typedef FusedLds<DK> L;
+    static constexpr int XS_U = 32 * L::XSV / 64, XT_U = L::DP * 4 / 64;   // wave instructions per matrix
+    static constexpr int NXS = (XS_U + 7) / 8, NXT = (XT_U + 7) / 8;       // per wave
+    unsigned xs[NXS], xt[NXT];
+};
+
+template <typename T, int DK>
+__device__ __forceinline__ StageOff<DK> stage_offsets(int d, int Mpad, int ldS, int wv) {
+    typedef FusedLds<DK> L;
+    typedef StageOff<DK> O;
+    O so;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < O::NXS; i++) {
+        const int vi = 64 * (8 * i + wv) + lane, r = vi / L::XSV, c = (vi % L::XSV) ^ (r & L::XSM);
+        so.xs[i] = (unsigned)((r * ldS + min(8 * c, ldS - 8)) * (int)sizeof(T));
+    }
+#pragma unroll
+    for (int i = 0; i < O::NXT; i++) {
+        const int vi = 64 * (8 * i + wv) + lane, q = vi >> 2, c = (vi & 3) ^ ((q >> 1) & 3);
+        so.xt[i] = (unsigned)(((size_t)min(q, d - 1) * Mpad + 8 * c) * sizeof(T));
+    }
+    return so;
+}
+
 // Stage chunk s0 (32 sparse points) into one LDS buffer: the Xs rows and the
 // (chunk-ordered) Xs^T rows go from global memory straight to LDS (global_load_lds, 16
-// bytes per lane): wave instruction u fills LDS vectors 64u .. 64u+63 of a matrix, each
-// lane loading the global vector that the swizzle puts in its slot; vectors outside the
-// matrix come from a zero vector.  The chunk's weights w_s pass through a register (*pw).
+// bytes per lane, a wave-uniform chunk base plus the lane's offset).  The chunk's
+// weights w_s pass through a register (*pw).
 template <typename T, int DK>
-__device__ __forceinline__ void fused_stage(int s0, int M, int d, int Mpad, int ldS, const T *Shi, const T *Slo,
-                                            const T *Thi, const T *Tlo, const double *wvec, T *__restrict__ xs_h,
-                                            T *__restrict__ xs_lo, T *__restrict__ xt_h, T *__restrict__ xt_lo,
-                                            double *pw) {
-    typedef FusedLds<DK> L;
-    constexpr int XS_N = 32 * L::XSV, XT_N = L::DP * 4;   // 16-byte vectors per matrix
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const T *zero = (const T *)kZero16;
+__device__ __forceinline__ void fused_stage(int s0, int M, int Mpad, int ldS, const T *Shi, const T *Slo,
+                                            const T *Thi, const T *Tlo, const double *wvec, const StageOff<DK> &so,
+                                            int wv, T *__restrict__ xs_h, T *__restrict__ xs_lo,
+                                            T *__restrict__ xt_h, T *__restrict__ xt_lo, double *pw) {
+    typedef StageOff<DK> O;
+    (void)Mpad;
+    const int t = threadIdx.x;
     if (t < 32) *pw = s0 + t < M ? wvec[s0 + t] : 0.0;
+    const char *sh = (const char *)(Shi + (size_t)s0 * ldS), *sl = (const char *)(Slo + (size_t)s0 * ldS);
+    const char *th = (const char *)(Thi + s0), *tl = (const char *)(Tlo + s0);
 #pragma unroll
-    for (int u0 = 0; u0 < XS_N / 64; u0 += 8) {
-        const int u = u0 + wv;
-        if (XS_N / 64 % 8 == 0 || u < XS_N / 64) {
-            const int vi = 64 * u + lane, r = vi / L::XSV, c = (vi % L::XSV) ^ (r & L::XSM);
-            const bool ok = 8 * c < ldS;
-            const size_t go = (size_t)(s0 + r) * ldS + 8 * c;
-            dma16(ok ? Shi + go : zero, xs_h + 512 * u);
-            dma16(ok ? Slo + go : zero, xs_lo + 512 * u);
+    for (int i = 0; i < O::NXS; i++) {
+        const int u = 8 * i + wv;
+        if (O::XS_U % 8 == 0 || u < O::XS_U) {
+            dma16(sh + so.xs[i], xs_h + 512 * u);
+            dma16(sl + so.xs[i], xs_lo + 512 * u);
         }
     }
 #pragma unroll
-    for (int u0 = 0; u0 < XT_N / 64; u0 += 8) {
-        const int u = u0 + wv;
-        if (XT_N / 64 % 8 == 0 || u < XT_N / 64) {
-            const int vi = 64 * u + lane, q = vi >> 2, c = (vi & 3) ^ ((q >> 1) & 3);
-            const bool ok = q < d;
-            const size_t go = (size_t)q * Mpad + s0 + 8 * c;
-            dma16(ok ? Thi + go : zero, xt_h + 512 * u);
-            dma16(ok ? Tlo + go : zero, xt_lo + 512 * u);
+    for (int i = 0; i < O::NXT; i++) {
+        const int u = 8 * i + wv;
+        if (O::XT_U % 8 == 0 || u < O::XT_U) {
+            dma16(th + so.xt[i], xt_h + 512 * u);
+            dma16(tl + so.xt[i], xt_lo + 512 * u);
         }
     }
 }
@@ -494,7 +537,8 @@ __device__ __forceinline__ void fused_stage(int s0, int M, int d, int Mpad, int 
 // buffers (written by fused_stage's loads).  Distinct restrict-qualified buffers tell
 // the compiler that the reads here do not wait for those loads.
 template <int PREC, int DK, int ZI>
-__device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, int Mpad, int ldS,
+__device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad, int ldS, const StageOff<DK> &so,
+                                            int wv,
                                             const typename Half<PREC>::T *Shi, const typename Half<PREC>::T *Slo,
                                             const typename Half<PREC>::T *Thi, const typename Half<PREC>::T *Tlo,
                                             const double *wvec, double zeta, int izeta, double wscale,
@@ -518,19 +562,55 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, in
     const int t = threadIdx.x, lane = t & 63;
     const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
     double pw = 0.0;
-    if (next) fused_stage<T, DK>(s0n, M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, nxs_h, nxs_lo, nxt_h, nxt_lo, &pw);
-    // K^T tiles: 32 sparse points (two tiles) x 16 centres
+    if (next) fused_stage<T, DK>(s0n, M, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, so, wv, nxs_h, nxs_lo, nxt_h, nxt_lo, &pw);
+    // K^T tiles: 32 sparse points (two tiles) x 16 centres; the fragments of k step
+    // ks + 1 are read before the MFMAs of step ks.
     float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
+    V ah[2], al[2];
 #pragma unroll
-    for (int ks = 0; ks < DK; ks++)
+    for (int j = 0; j < 2; j++) {
+        const int o = L::xs_at(16 * j + r16, kg >> 3);
+        ah[j] = *(const V *)(xs_h + o);
+        al[j] = *(const V *)(xs_lo + o);
+    }
+#pragma unroll
+    for (int ks = 0; ks < DK; ks++) {
+        V nh[2], nl[2];
+        if (ks + 1 < DK) {
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int o = L::xs_at(16 * j + r16, 4 * (ks + 1) + (kg >> 3));
+                nh[j] = *(const V *)(xs_h + o);
+                nl[j] = *(const V *)(xs_lo + o);
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 2; j++) {
-            const int o = L::xs_at(16 * j + r16, 4 * ks + (kg >> 3));
-            const V ah = *(const V *)(xs_h + o), al = *(const V *)(xs_lo + o);
-            kacc[j] = H::mfma(ah, xl[ks], kacc[j]);
-            kacc[j] = H::mfma(al, xh[ks], kacc[j]);
-            kacc[j] = H::mfma(ah, xh[ks], kacc[j]);
+            kacc[j] = H::mfma(ah[j], xl[ks], kacc[j]);
+            kacc[j] = H::mfma(al[j], xh[ks], kacc[j]);
+            kacc[j] = H::mfma(ah[j], xh[ks], kacc[j]);
         }
+        if (ks + 1 < DK) {
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                ah[j] = nh[j];
+                al[j] = nl[j];
+            }
+        }
+        GP_SCHED_STEP();
+    }
+    // the first two column tiles of G += W Xs, read before the FP32 epilogue (after
+    // the FP64 one, whose powers need the registers)
+    V bh[2], bl[2];
+    auto read_b0 = [&]() {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int o = L::xt_at(16 * i + r16, kg >> 3);
+            bh[i] = *(const V *)(xt_h + o);
+            bl[i] = *(const V *)(xt_lo + o);
+        }
+    };
+    if constexpr (ZI == 4) read_b0();
     // epilogue: element v of tile j is centre r16 at sparse point s0 + 16 j + 4 (lane/16) + v,
     // and element 4 j + v of this lane's A fragment of W.  The energy sum is FP64; K^zeta
     // and W are formed from the FP32 K in FP32 for the common zeta = 4 (W is then split
@@ -569,13 +649,33 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int d, in
             wl[4 * j + v] = lo;
         }
     }
+    if constexpr (ZI != 4) read_b0();
+    // column tiles in pairs, the next pair's fragments read before this pair's MFMAs
 #pragma unroll
-    for (int q = 0; q < 2 * DK; q++) {
-        const int o = L::xt_at(16 * q + r16, kg >> 3);
-        const V bh = *(const V *)(xt_h + o), bl = *(const V *)(xt_lo + o);
-        gacc[q] = H::mfma(wl, bh, gacc[q]);
-        gacc[q] = H::mfma(wh, bl, gacc[q]);
-        gacc[q] = H::mfma(wh, bh, gacc[q]);
+    for (int q0 = 0; q0 < 2 * DK; q0 += 2) {
+        V nh[2], nl[2];
+        if (q0 + 2 < 2 * DK) {
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int o = L::xt_at(16 * (q0 + 2 + i) + r16, kg >> 3);
+                nh[i] = *(const V *)(xt_h + o);
+                nl[i] = *(const V *)(xt_lo + o);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            gacc[q0 + i] = H::mfma(wl, bh[i], gacc[q0 + i]);
+            gacc[q0 + i] = H::mfma(wh, bl[i], gacc[q0 + i]);
+            gacc[q0 + i] = H::mfma(wh, bh[i], gacc[q0 + i]);
+        }
+        if (q0 + 2 < 2 * DK) {
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                bh[i] = nh[i];
+                bl[i] = nl[i];
+            }
+        }
+        GP_SCHED_STEP();
     }
     // the next buffers were last read before the previous barrier
     if (next && t < 32) {
@@ -633,11 +733,12 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __shared__ __attribute__((aligned(16))) T xt0[2][DP * 32], xt1[2][DP * 32];
     __shared__ double wv0[32], wv1[32];   // the chunk's w_s (0 past M)
     __shared__ float wf0[32], wf1[32];    // and (float)(w_s 4 wscale), the zeta = 4 epilogue's factor
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const int r16 = lane & 15, g4 = lane >> 4, kg = g4 * 8, rq = 4 * g4;
     const int part = blockIdx.y, nparts = gridDim.y;
     const int C = Mpad / 32;
     const bool sk = sk_units > 0;
+    const StageOff<DK> so = stage_offsets<T, DK>(d, Mpad, ldS, wv);
     long u, u_end;   // this block's units (row block rb, chunk c) = rb C + c
     if (sk) {
         u = (long)blockIdx.x * sk_units / gridDim.x;
@@ -674,8 +775,8 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
 
     if (c_beg < c_end) {
         double pw = 0.0;
-        fused_stage<T, DK>(32 * c_beg, M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, xs0[0], xs0[1], xt0[0], xt0[1],
-                              &pw);
+        fused_stage<T, DK>(32 * c_beg, M, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, so, wv, xs0[0], xs0[1], xt0[0],
+                           xt0[1], &pw);
         if (t < 32) {
             wv0[t] = pw;
             wf0[t] = (float)(pw * 4.0 * wscale);
@@ -684,11 +785,11 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int ch = c_beg; ch < c_end; ch += 2) {   // buffer set 0, then set 1
-        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, zeta,
+        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, Mpad, ldS, so, wv, Shi, Slo, Thi, Tlo, wvec, zeta,
                                   izeta, wscale, xs0[0], xs0[1], xt0[0], xt0[1], wv0, wf0, xs1[0], xs1[1], xt1[0],
                                   xt1[1], wv1, wf1, xh, xl, gacc, rs);
         if (ch + 1 >= c_end) break;
-        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, d, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, zeta,
+        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, Mpad, ldS, so, wv, Shi, Slo, Thi, Tlo, wvec, zeta,
                                   izeta, wscale, xs1[0], xs1[1], xt1[0], xt1[1], wv1, wf1, xs0[0], xs0[1], xt0[0],
                                   xt0[1], wv0, wf0, xh, xl, gacc, rs);
     }
