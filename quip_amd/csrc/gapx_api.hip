@@ -1759,7 +1759,7 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
         if (c.kind != 1 || nc == 0) continue;
         const size_t KLM = (size_t)c.S * c.n * (c.L + 1) * (c.L + 1);
         const int dpad = (c.dim + 31) / 32 * 32, Mpad = (c.M + 31) / 32 * 32;
-        fused[k] = m->precision != GAPX_PREC_FP64 && gp_fused_supported(dpad);
+        fused[k] = m->precision != GAPX_PREC_FP64 && gp_fused_supported(dpad, Mpad);
         mX = std::max(mX, (size_t)nc * c.dim * 8);
         mN = std::max(mN, (size_t)nc * 8);
         mC = std::max(mC, (size_t)nc * KLM * 8);

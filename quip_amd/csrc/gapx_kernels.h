@@ -165,7 +165,7 @@ void launch_gemm2_split(int prec, int nc, int M, int d, int dpad, int Mpad, int 
 // one launch computes the energy partials e_part[i*parts + p] and, with parts > 1,
 // FP32 partial G tiles (parts x nc x d in Gpart) that launch_gp_fused_reduce sums;
 // with parts == 1 it writes G directly.  *ntile_out = parts.
-bool gp_fused_supported(int dpad);
+bool gp_fused_supported(int dpad, int Mpad);
 int gp_fused_parts(int nc, int Mpad);              // the default number of sparse-point parts
 int gp_fused_parts_valid(int parts, int Mpad);     // a requested number, as the launch rounds it
 // parts_req > 0 asks for that many parts (rounded so that every part has chunks), 0 the

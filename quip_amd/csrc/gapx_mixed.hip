@@ -430,11 +430,6 @@ static bool use_lds_tiles(int rows, int cols, int lds_mode = -1) {
     return (long)cdivm(rows, kLT) * cdivm(cols, kLT) >= 512;
 }
 
-// 16 bytes per lane from global memory straight into LDS: lane l's bytes land at
-// lds + 16 l (lds uniform over the wavefront)
-__device__ __forceinline__ void dma16(const void *g, void *lds) {
-    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
-}
 
 // LDS rows without padding; 16-byte vectors XOR-swizzled so that the rows one fragment
 // read touches fall on distinct 16-byte bank groups in every group of 8 lanes (a 16-byte
@@ -501,34 +496,50 @@ __device__ __forceinline__ StageOff<DK> stage_offsets(int d, int Mpad, int ldS, 
 }
 
 // Stage chunk s0 (32 sparse points) into one LDS buffer: the Xs rows and the
-// (chunk-ordered) Xs^T rows go from global memory straight to LDS (global_load_lds, 16
-// bytes per lane, a wave-uniform chunk base plus the lane's offset).  The chunk's
-// weights w_s pass through a register (*pw).
+// (chunk-ordered) Xs^T rows go from global memory straight to LDS (buffer loads with
+// the lds flag, 16 bytes per lane: the lane's offset plus a wave-uniform chunk offset).
+// The chunk's weights w_s pass through a register (*pw).
+// A raw buffer resource over a whole matrix (stride 0, 32-bit data format word of the
+// gfx9 descriptor); its loads are bounded by nbytes.  The chunk loads go through buffer
+// instructions rather than global_load_lds: those are FLAT instructions, which may count
+// on lgkmcnt too, and while one is outstanding every wait for an LDS read has to drain
+// the counter (lgkmcnt(0)); buffer loads count on vmcnt only, so the fragment reads of
+// the next k step stay in flight (lgkmcnt(4)..(7)) under the current step's MFMAs.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mat_rsrc(const void *p, unsigned nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)nbytes, 0x00020000);
+}
+// 16 bytes per lane from the buffer at voffset + soffset straight into LDS: lane l's
+// bytes land at lds + 16 l (lds uniform over the wavefront)
+__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, void *lds) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)lds, 16, (int)voff,
+                                             (int)soff, 0, 0);
+}
+
 template <typename T, int DK>
-__device__ __forceinline__ void fused_stage(int s0, int M, int Mpad, int ldS, const T *Shi, const T *Slo,
-                                            const T *Thi, const T *Tlo, const double *wvec, const StageOff<DK> &so,
+__device__ __forceinline__ void fused_stage(int s0, int M, int Mpad, int ldS, __amdgpu_buffer_rsrc_t rsh,
+                                            __amdgpu_buffer_rsrc_t rsl, __amdgpu_buffer_rsrc_t rth,
+                                            __amdgpu_buffer_rsrc_t rtl, const double *wvec, const StageOff<DK> &so,
                                             int wv, T *__restrict__ xs_h, T *__restrict__ xs_lo,
                                             T *__restrict__ xt_h, T *__restrict__ xt_lo, double *pw) {
     typedef StageOff<DK> O;
     (void)Mpad;
     const int t = threadIdx.x;
     if (t < 32) *pw = s0 + t < M ? wvec[s0 + t] : 0.0;
-    const char *sh = (const char *)(Shi + (size_t)s0 * ldS), *sl = (const char *)(Slo + (size_t)s0 * ldS);
-    const char *th = (const char *)(Thi + s0), *tl = (const char *)(Tlo + s0);
+    const unsigned ss = (unsigned)(s0 * ldS * (int)sizeof(T)), st = (unsigned)(s0 * (int)sizeof(T));
 #pragma unroll
     for (int i = 0; i < O::NXS; i++) {
         const int u = 8 * i + wv;
         if (O::XS_U % 8 == 0 || u < O::XS_U) {
-            dma16(sh + so.xs[i], xs_h + 512 * u);
-            dma16(sl + so.xs[i], xs_lo + 512 * u);
+            dma16(rsh, so.xs[i], ss, xs_h + 512 * u);
+            dma16(rsl, so.xs[i], ss, xs_lo + 512 * u);
         }
     }
 #pragma unroll
     for (int i = 0; i < O::NXT; i++) {
         const int u = 8 * i + wv;
         if (O::XT_U % 8 == 0 || u < O::XT_U) {
-            dma16(th + so.xt[i], xt_h + 512 * u);
-            dma16(tl + so.xt[i], xt_lo + 512 * u);
+            dma16(rth, so.xt[i], st, xt_h + 512 * u);
+            dma16(rtl, so.xt[i], st, xt_lo + 512 * u);
         }
     }
 }
@@ -539,8 +550,8 @@ __device__ __forceinline__ void fused_stage(int s0, int M, int Mpad, int ldS, co
 template <int PREC, int DK, int ZI>
 __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad, int ldS, const StageOff<DK> &so,
                                             int wv,
-                                            const typename Half<PREC>::T *Shi, const typename Half<PREC>::T *Slo,
-                                            const typename Half<PREC>::T *Thi, const typename Half<PREC>::T *Tlo,
+                                            __amdgpu_buffer_rsrc_t Shi, __amdgpu_buffer_rsrc_t Slo,
+                                            __amdgpu_buffer_rsrc_t Thi, __amdgpu_buffer_rsrc_t Tlo,
                                             const double *wvec, double zeta, int izeta, double wscale,
                                             const typename Half<PREC>::T *__restrict__ xs_h,
                                             const typename Half<PREC>::T *__restrict__ xs_lo,
@@ -565,6 +576,10 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
     if (next) fused_stage<T, DK>(s0n, M, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, so, wv, nxs_h, nxs_lo, nxt_h, nxt_lo, &pw);
     // K^T tiles: 32 sparse points (two tiles) x 16 centres; the fragments of k step
     // ks + 1 are read before the MFMAs of step ks.
+    // Wave priority is raised over the two MFMA phases and dropped over the epilogue,
+    // so that of the two waves of a SIMD one runs its MFMAs while the other runs its
+    // epilogue's VALU work (batch512 796 -> 780 us).
+    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(1);
     float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
     V ah[2], al[2];
 #pragma unroll
@@ -611,6 +626,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
         }
     };
     if constexpr (ZI == 4) read_b0();
+    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(0);
     // epilogue: element v of tile j is centre r16 at sparse point s0 + 16 j + 4 (lane/16) + v,
     // and element 4 j + v of this lane's A fragment of W.  The energy sum is FP64; K^zeta
     // and W are formed from the FP32 K in FP32 for the common zeta = 4 (W is then split
@@ -650,6 +666,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
         }
     }
     if constexpr (ZI != 4) read_b0();
+    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(1);
     // column tiles in pairs, the next pair's fragments read before this pair's MFMAs
 #pragma unroll
     for (int q0 = 0; q0 < 2 * DK; q0 += 2) {
@@ -677,6 +694,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
         }
         GP_SCHED_STEP();
     }
+    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(0);
     // the next buffers were last read before the previous barrier
     if (next && t < 32) {
         wnext[t] = pw;
@@ -739,6 +757,9 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     const int C = Mpad / 32;
     const bool sk = sk_units > 0;
     const StageOff<DK> so = stage_offsets<T, DK>(d, Mpad, ldS, wv);
+    const unsigned nbytes = (unsigned)((size_t)Mpad * ldS * sizeof(T));   // within both copies' allocations
+    const __amdgpu_buffer_rsrc_t rsh = mat_rsrc(Shi, nbytes), rsl = mat_rsrc(Slo, nbytes);
+    const __amdgpu_buffer_rsrc_t rth = mat_rsrc(Thi, nbytes), rtl = mat_rsrc(Tlo, nbytes);
     long u, u_end;   // this block's units (row block rb, chunk c) = rb C + c
     if (sk) {
         u = (long)blockIdx.x * sk_units / gridDim.x;
@@ -775,7 +796,7 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
 
     if (c_beg < c_end) {
         double pw = 0.0;
-        fused_stage<T, DK>(32 * c_beg, M, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, so, wv, xs0[0], xs0[1], xt0[0],
+        fused_stage<T, DK>(32 * c_beg, M, Mpad, ldS, rsh, rsl, rth, rtl, wvec, so, wv, xs0[0], xs0[1], xt0[0],
                            xt0[1], &pw);
         if (t < 32) {
             wv0[t] = pw;
@@ -785,11 +806,11 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int ch = c_beg; ch < c_end; ch += 2) {   // buffer set 0, then set 1
-        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, Mpad, ldS, so, wv, Shi, Slo, Thi, Tlo, wvec, zeta,
+        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, Mpad, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
                                   izeta, wscale, xs0[0], xs0[1], xt0[0], xt0[1], wv0, wf0, xs1[0], xs1[1], xt1[0],
                                   xt1[1], wv1, wf1, xh, xl, gacc, rs);
         if (ch + 1 >= c_end) break;
-        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, Mpad, ldS, so, wv, Shi, Slo, Thi, Tlo, wvec, zeta,
+        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, Mpad, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
                                   izeta, wscale, xs1[0], xs1[1], xt1[0], xt1[1], wv1, wf1, xs0[0], xs0[1], xt0[0],
                                   xt0[1], wv0, wf0, xh, xl, gacc, rs);
     }
@@ -938,7 +959,9 @@ void launch_gemm1_split(int prec, int nc, int M, int d, int dpad, int lds_mode, 
 #undef G1L
 }
 
-bool gp_fused_supported(int dpad) { return dpad <= 256; }
+// d <= 256, and the sparse copies' byte size within the 31-bit bound of the buffer
+// resources the kernel reads them through (Mpad < 4M at d = 256)
+bool gp_fused_supported(int dpad, int Mpad) { return dpad <= 256 && (size_t)Mpad * dpad * 2 < (1ull << 31); }
 
 int gp_fused_parts(int nc, int Mpad) {
     const int row_blocks = cdivm(nc, 128), chunks = Mpad / 32;
