@@ -443,11 +443,12 @@ static bool use_lds_tiles(int rows, int cols, int lds_mode = -1) {
 // step's reads right before its MFMAs and waits for them there (both waves of a SIMD
 // leave each chunk barrier together, so neither covers the other's wait): batch512
 // 822 -> 796 us.
-// Pinned for the BF16 zeta = 4 kernels only: the others need more registers for their
-// epilogue, and with the reads held ahead they spill or lose a wavefront per SIMD.
+// Pinned for the zeta = 4 kernels of descriptors up to 256 components, where it was
+// measured: the FP64 epilogue of other zeta needs the registers (with the reads held
+// ahead it spills), and at 64 components FP16 would lose a wavefront per SIMD.
 #define GP_SCHED_STEP()                                           \
     do {                                                          \
-        if constexpr (PREC == 1 && ZI == 4) {                     \
+        if constexpr (ZI == 4 && DK == 8) {                       \
             __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);    \
             __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);    \
             __builtin_amdgcn_sched_barrier(0);                    \
@@ -579,7 +580,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
     // Wave priority is raised over the two MFMA phases and dropped over the epilogue,
     // so that of the two waves of a SIMD one runs its MFMAs while the other runs its
     // epilogue's VALU work (batch512 796 -> 780 us).
-    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(1);
+    if constexpr (ZI == 4 && DK == 8) __builtin_amdgcn_s_setprio(1);
     float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
     V ah[2], al[2];
 #pragma unroll
@@ -626,7 +627,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
         }
     };
     if constexpr (ZI == 4) read_b0();
-    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(0);
+    if constexpr (ZI == 4 && DK == 8) __builtin_amdgcn_s_setprio(0);
     // epilogue: element v of tile j is centre r16 at sparse point s0 + 16 j + 4 (lane/16) + v,
     // and element 4 j + v of this lane's A fragment of W.  The energy sum is FP64; K^zeta
     // and W are formed from the FP32 K in FP32 for the common zeta = 4 (W is then split
@@ -666,7 +667,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
         }
     }
     if constexpr (ZI != 4) read_b0();
-    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(1);
+    if constexpr (ZI == 4 && DK == 8) __builtin_amdgcn_s_setprio(1);
     // column tiles in pairs, the next pair's fragments read before this pair's MFMAs
 #pragma unroll
     for (int q0 = 0; q0 < 2 * DK; q0 += 2) {
@@ -694,7 +695,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
         }
         GP_SCHED_STEP();
     }
-    if constexpr (PREC == 1 && ZI == 4) __builtin_amdgcn_s_setprio(0);
+    if constexpr (ZI == 4 && DK == 8) __builtin_amdgcn_s_setprio(0);
     // the next buffers were last read before the previous barrier
     if (next && t < 32) {
         wnext[t] = pw;

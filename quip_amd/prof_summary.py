@@ -79,6 +79,29 @@ def counters(path, match=None):
     return "\n".join(out)
 
 
+def busy(path, match=None):
+    """MFMA pipe occupancy and per-wave instruction counts of each (kernel, grid) from a
+    run with SQ_VALU_MFMA_BUSY_CYCLES, SQ_BUSY_CYCLES, SQ_INSTS_MFMA, SQ_WAVES,
+    SQ_WAIT_INST_ANY, SQ_WAVE_CYCLES and SQ_INSTS_LDS.  The MFMA busy counter sums over
+    the 1024 SIMDs and the busy counter over the 32 shader engines, so busy =
+    MFMA_BUSY / (32 BUSY)."""
+    per = defaultdict(dict)
+    for (name, grid, cn), (_, v) in _counter_avgs(path).items():
+        per[(name, grid)][cn] = v
+    out = []
+    for (name, grid), a in sorted(per.items()):
+        if match and match not in name:
+            continue
+        w = a["SQ_WAVES"]
+        out.append("## %s, grid %s (%d waves)" % (name, grid, int(w)))
+        out.append("  MFMA busy %.1f%%, MFMA / wave %.0f, LDS / wave %.0f, waiting %.0f%% of wave cycles"
+                   % (100 * a["SQ_VALU_MFMA_BUSY_CYCLES"] / (32 * a["SQ_BUSY_CYCLES"]), a["SQ_INSTS_MFMA"] / w,
+                      a["SQ_INSTS_LDS"] / w, 100 * a["SQ_WAIT_INST_ANY"] / a["SQ_WAVE_CYCLES"]))
+        out.append("  " + ", ".join("%s %d" % (k, round(v)) for k, v in sorted(a.items())))
+        out.append("")
+    return "\n".join(out)
+
+
 def traffic(fetch_csv, write_csv, kernel, grid, probe_bytes):
     f, w = _counter_avgs(fetch_csv), _counter_avgs(write_csv)
 
@@ -180,7 +203,7 @@ def steps(trace_csv, start, contains, which):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["kernels", "counters", "traffic", "stages", "steps"])
+    ap.add_argument("what", choices=["kernels", "counters", "busy", "traffic", "stages", "steps"])
     ap.add_argument("files", nargs="+")
     ap.add_argument("--match")
     ap.add_argument("--top", type=int, default=40)
@@ -195,6 +218,8 @@ def main():
         print(kernels(a.files[0], a.match, a.top))
     elif a.what == "counters":
         print(counters(a.files[0], a.match))
+    elif a.what == "busy":
+        print(busy(a.files[0], a.match))
     elif a.what == "traffic":
         print(traffic(a.files[0], a.files[1], a.kernel, a.grid, a.probe_bytes))
     elif a.what == "steps":
