@@ -172,6 +172,32 @@ def test_skin_and_nonperiodic(models):
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("nl", [(6, 6), (8, 4), (5, 4)])
+def test_split_padded_descriptor_widths(nl, prec, tmp_path):
+    """The fused split GP kernel on descriptors whose 32-padded width is below the width the
+    kernel is built for (d 148 -> 160 of 256, 181 -> 192 of 256, 76 -> 96 of 128): its staging
+    lanes past the padded width read in-bounds values that must meet zero X components, and
+    its rows past d give G columns that are not written.  70 sparse points make three chunks,
+    the last one partial.  The split error on these models is about 1e-7; a padding lane
+    that reached a result would be far above the bounds checked here."""
+    n, lm = nl
+    train = [configs.perturb(configs.diamond(5.43, (1, 1, 1)), 0.1, configs.rng_for(3, 800 + k)) for k in range(10)]
+    m = configs.soap_model(_desc_fn, train, 70, 3, [configs.soap_string(n_max=n, l_max=lm)], "GAP_pad%d_%d" % nl,
+                           alpha_std=1e-2)
+    path = str(tmp_path / "m.xml")
+    write_gap_xml(path, m)
+    at = configs.perturb(configs.diamond(5.43, (4, 4, 4)), 0.1, configs.rng_for(3, 900))
+    print("case", nl, prec, len(at), flush=True)
+    r = Potential(param_filename=path, precision=prec).calc(at.copy(), force=True, virial=True, local_energy=True)
+    ro = O.gap_calc(m, at)
+    de = abs(r["energy"] - ro["energy"]) / len(at)
+    frmse = np.sqrt(np.mean((r["force"] - ro["forces"]) ** 2))
+    print(nl, prec, "dE/atom", de, "F rmse", frmse)
+    assert de < 1e-5 and frmse < 1e-4
+    assert np.abs(r["local_energy"] - ro["local_energy"]).max() < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
 def test_split_precision_within_budget(models, prec):
     m, path = models["si4096"]
     at = configs.si4096()
