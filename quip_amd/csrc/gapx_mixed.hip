@@ -517,13 +517,12 @@ __device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, unsigned voff, u
 }
 
 template <typename T, int DK>
-__device__ __forceinline__ void fused_stage(int s0, int M, int Mpad, int ldS, __amdgpu_buffer_rsrc_t rsh,
+__device__ __forceinline__ void fused_stage(int s0, int M, int ldS, __amdgpu_buffer_rsrc_t rsh,
                                             __amdgpu_buffer_rsrc_t rsl, __amdgpu_buffer_rsrc_t rth,
                                             __amdgpu_buffer_rsrc_t rtl, const double *wvec, const StageOff<DK> &so,
                                             int wv, T *__restrict__ xs_h, T *__restrict__ xs_lo,
                                             T *__restrict__ xt_h, T *__restrict__ xt_lo, double *pw) {
     typedef StageOff<DK> O;
-    (void)Mpad;
     const int t = threadIdx.x;
     if (t < 32) *pw = s0 + t < M ? wvec[s0 + t] : 0.0;
     const unsigned ss = (unsigned)(s0 * ldS * (int)sizeof(T)), st = (unsigned)(s0 * (int)sizeof(T));
@@ -549,7 +548,7 @@ __device__ __forceinline__ void fused_stage(int s0, int M, int Mpad, int ldS, __
 // buffers (written by fused_stage's loads).  Distinct restrict-qualified buffers tell
 // the compiler that the reads here do not wait for those loads.
 template <int PREC, int DK, int ZI>
-__device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad, int ldS, const StageOff<DK> &so,
+__device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, const StageOff<DK> &so,
                                             int wv,
                                             __amdgpu_buffer_rsrc_t Shi, __amdgpu_buffer_rsrc_t Slo,
                                             __amdgpu_buffer_rsrc_t Thi, __amdgpu_buffer_rsrc_t Tlo,
@@ -574,7 +573,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int Mpad,
     const int t = threadIdx.x, lane = t & 63;
     const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
     double pw = 0.0;
-    if (next) fused_stage<T, DK>(s0n, M, Mpad, ldS, Shi, Slo, Thi, Tlo, wvec, so, wv, nxs_h, nxs_lo, nxt_h, nxt_lo, &pw);
+    if (next) fused_stage<T, DK>(s0n, M, ldS, Shi, Slo, Thi, Tlo, wvec, so, wv, nxs_h, nxs_lo, nxt_h, nxt_lo, &pw);
     // K^T tiles: 32 sparse points (two tiles) x 16 centres; the fragments of k step
     // ks + 1 are read before the MFMAs of step ks.
     // Wave priority is raised over the two MFMA phases and dropped over the epilogue,
@@ -797,7 +796,7 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
 
     if (c_beg < c_end) {
         double pw = 0.0;
-        fused_stage<T, DK>(32 * c_beg, M, Mpad, ldS, rsh, rsl, rth, rtl, wvec, so, wv, xs0[0], xs0[1], xt0[0],
+        fused_stage<T, DK>(32 * c_beg, M, ldS, rsh, rsl, rth, rtl, wvec, so, wv, xs0[0], xs0[1], xt0[0],
                            xt0[1], &pw);
         if (t < 32) {
             wv0[t] = pw;
@@ -807,11 +806,11 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int ch = c_beg; ch < c_end; ch += 2) {   // buffer set 0, then set 1
-        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, Mpad, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
+        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
                                   izeta, wscale, xs0[0], xs0[1], xt0[0], xt0[1], wv0, wf0, xs1[0], xs1[1], xt1[0],
                                   xt1[1], wv1, wf1, xh, xl, gacc, rs);
         if (ch + 1 >= c_end) break;
-        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, Mpad, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
+        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
                                   izeta, wscale, xs1[0], xs1[1], xt1[0], xt1[1], wv1, wf1, xs0[0], xs0[1], xt0[0],
                                   xt0[1], wv0, wf0, xh, xl, gacc, rs);
     }
