@@ -172,14 +172,15 @@ def test_skin_and_nonperiodic(models):
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
-@pytest.mark.parametrize("nl", [(6, 6), (8, 4), (5, 4)])
+@pytest.mark.parametrize("nl", [(6, 6), (8, 4), (5, 4), (3, 3), (4, 4)])
 def test_split_padded_descriptor_widths(nl, prec, tmp_path):
     """The fused split GP kernel on descriptors whose 32-padded width is below the width the
-    kernel is built for (d 148 -> 160 of 256, 181 -> 192 of 256, 76 -> 96 of 128): its staging
-    lanes past the padded width read in-bounds values that must meet zero X components, and
-    its rows past d give G columns that are not written.  70 sparse points make three chunks,
-    the last one partial.  The split error on these models is about 1e-7; a padding lane
-    that reached a result would be far above the bounds checked here."""
+    kernel is built for (d 148 -> 160 of 256, 181 -> 192 of 256, 76 -> 96 of 128, 25 -> 32 of
+    64), and at d 51 -> 64 of 64: its staging lanes past the padded width read in-bounds values
+    that must meet zero X components, and its rows past d give G columns that are not
+    written.  70 sparse points make three chunks, the last one partial.  The split error on
+    these models is about 1e-7; a padding lane that reached a result would be far above the
+    bounds checked here."""
     n, lm = nl
     train = [configs.perturb(configs.diamond(5.43, (1, 1, 1)), 0.1, configs.rng_for(3, 800 + k)) for k in range(10)]
     m = configs.soap_model(_desc_fn, train, 70, 3, [configs.soap_string(n_max=n, l_max=lm)], "GAP_pad%d_%d" % nl,
