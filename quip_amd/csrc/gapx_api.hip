@@ -1449,9 +1449,8 @@ static int mixed_tune(gapx_model *m, Engine &eg, Coord &c, int nc, bool fused, G
             int nt = 0;
             st = timed([&] {
                 launch_gp_fused_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, parts, w.X.as<double>(),
-                                      c.d_Xs_hi.p, c.d_Xs_lo.p, c.d_XsTc_hi.p, c.d_XsTc_lo.p, c.d_w.as<double>(),
-                                      c.zeta, c.wscale, w.G.as<double>(), w.Gparts.as<float>(),
-                                      w.e_part.as<double>(), &nt, s);
+                                      c.d_Xs_hi.p, c.d_Xs_lo.p, c.d_w.as<double>(), c.zeta, c.wscale,
+                                      w.G.as<double>(), w.Gparts.as<float>(), w.e_part.as<double>(), &nt, s);
                 if (nt < 0)
                     launch_gp_sk_finish(m->precision, nc, c.dim, c.dpad, c.Mpad, true, w.Gparts.as<float>(), c.wscale,
                                         w.G.as<double>(), w.e_part.as<double>(), nullptr, nullptr, s);
@@ -1897,8 +1896,7 @@ int gapx::eval_batch(gapx_model *m, Engine &eg, int S, const int *atom_off, cons
                                  c.zeta, w.W.as<double>(), w.e_part.as<double>(), &ntile, s);
             } else if (fused[k]) {   // split forms: K -> W -> G in one kernel
                 launch_gp_fused_split(m->precision, nc, c.M, c.dim, c.dpad, c.Mpad, tu.fparts, w.X.as<double>(),
-                                      c.d_Xs_hi.p,
-                                      c.d_Xs_lo.p, c.d_XsTc_hi.p, c.d_XsTc_lo.p, c.d_w.as<double>(), c.zeta, c.wscale,
+                                      c.d_Xs_hi.p, c.d_Xs_lo.p, c.d_w.as<double>(), c.zeta, c.wscale,
                                       w.G.as<double>(), w.Gparts.as<float>(), w.e_part.as<double>(), &ntile, s);
             } else {
                 launch_gemm1_split(m->precision, nc, c.M, c.dim, c.dpad, tu.g1lds, w.X.as<double>(), c.d_Xs_hi.p,
@@ -2508,10 +2506,6 @@ int gapx::ensure_split(gapx_model *m, Coord &c, int prec, hipStream_t s) {
     CHK(c.d_XsT_lo.ensure((size_t)c.Mpad * c.dpad * 2));
     launch_split_matrix(prec, c.M, c.dim, c.dpad, c.d_Xs.as<double>(), c.dim, false, c.d_Xs_hi.p, c.d_Xs_lo.p, s);
     launch_split_matrix(prec, c.M, c.dim, c.Mpad, c.d_Xs.as<double>(), c.dim, true, c.d_XsT_hi.p, c.d_XsT_lo.p, s);
-    CHK(c.d_XsTc_hi.ensure((size_t)c.Mpad * c.dpad * 2));
-    CHK(c.d_XsTc_lo.ensure((size_t)c.Mpad * c.dpad * 2));
-    launch_split_matrix(prec, c.M, c.dim, c.Mpad, c.d_Xs.as<double>(), c.dim, true, c.d_XsTc_hi.p, c.d_XsTc_lo.p, s,
-                        true);
     HIPCHK(hipGetLastError());
     c.split_prec = prec;
     return GAPX_OK;

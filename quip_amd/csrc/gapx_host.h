@@ -145,7 +145,6 @@ struct Coord {
     int dpad = 0, Mpad = 0;
     double wscale = 1.0;
     DBuf d_Xs_hi, d_Xs_lo, d_XsT_hi, d_XsT_lo;
-    DBuf d_XsTc_hi, d_XsTc_lo;   // Xs^T with the fused kernel's sparse-point order in each 32
     // FP64 GEMM tile choices per centre-count bucket (ceil log2 nc), from the autotuner
     std::mutex tune_mu;
     std::map<int, GemmTune> tune;

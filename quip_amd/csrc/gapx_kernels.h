@@ -177,13 +177,11 @@ size_t gp_fused_sk_scratch_bytes(int prec, int d, int dpad, int nc, int Mpad);
 void launch_gp_sk_finish(int prec, int nc, int d, int dpad, int Mpad, bool want_g, const float *Gpart, double wscale,
                          double *G, double *e_part, const int *centres, double *esite, hipStream_t s);
 void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, int parts_req, const double *X,
-                           const void *Xs_hi, const void *Xs_lo, const void *XsT_hi, const void *XsT_lo,
-                           const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
-                           int *ntile_out, hipStream_t s);
+                           const void *Xs_hi, const void *Xs_lo, const double *wvec, double zeta, double wscale,
+                           double *G, float *Gpart, double *e_part, int *ntile_out, hipStream_t s);
 void launch_gp_fused_reduce(int nc, int d, int parts, const float *Gpart, double wscale, double *G, hipStream_t s);
-// chunk_order (transpose only): sparse points permuted within each 32 as the fused kernel reads them
 void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
-                         void *hi, void *lo, hipStream_t s, bool chunk_order = false);
+                         void *hi, void *lo, hipStream_t s);
 
 void launch_energy_rows(int nc, int ntile, const int *centres, const double *e_part, double *esite, hipStream_t s);
 // mask (nullable): only atoms with mask[i] != 0 are the first atom of a pair descriptor.

@@ -52,20 +52,14 @@ struct Half<2> {   // fp16
 // split rows x cols FP64 (row-major, ld src_ld) into hi/lo with leading dimension
 // ld_out; transpose=true writes the transpose (cols x rows).  Padding is zero.
 template <int PREC>
-// chunk_order (transposed copies): within every 32 output positions the sparse points
-// are in the fused kernel's order [0-3, 16-19 | 4-7, 20-23 | 8-11, 24-27 | 12-15, 28-31]
 __global__ void k_split(int rows, int cols, int ld_out, int rows_pad, const double *src, int src_ld, bool transpose,
-                        typename Half<PREC>::T *hi, typename Half<PREC>::T *lo, double scale, bool chunk_order) {
+                        typename Half<PREC>::T *hi, typename Half<PREC>::T *lo, double scale) {
     typedef Half<PREC> H;
     const size_t tot = transpose ? (size_t)cols * ld_out : (size_t)rows_pad * ld_out;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
         int r, c;
         if (!transpose) { r = (int)(idx / ld_out); c = (int)(idx % ld_out); }
         else { c = (int)(idx / ld_out); r = (int)(idx % ld_out); }
-        if (chunk_order) {
-            const int p = r & 31, slot = p >> 2;
-            r = (r & ~31) + 4 * ((slot >> 1) + 4 * (slot & 1)) + (p & 3);
-        }
         double x = (r < rows && c < cols) ? src[(size_t)r * src_ld + c] * scale : 0.0;
         typename H::T h = H::from(x);
         typename H::T l = H::from(x - H::to(h));
@@ -75,16 +69,16 @@ __global__ void k_split(int rows, int cols, int ld_out, int rows_pad, const doub
 }
 
 void launch_split_matrix(int prec, int rows, int cols, int ld_out, const double *src, int src_ld, bool transpose,
-                         void *hi, void *lo, hipStream_t s, bool chunk_order) {
+                         void *hi, void *lo, hipStream_t s) {
     const int rows_pad = (rows + 63) / 64 * 64;
     const size_t tot = transpose ? (size_t)cols * ld_out : (size_t)rows_pad * ld_out;
     const int grid = (int)std::min<size_t>((tot + 255) / 256, 65536);
     if (prec == 1)
         hipLaunchKernelGGL(k_split<1>, dim3(grid), dim3(256), 0, s, rows, cols, ld_out, rows_pad, src, src_ld, transpose,
-                           (__bf16 *)hi, (__bf16 *)lo, 1.0, chunk_order);
+                           (__bf16 *)hi, (__bf16 *)lo, 1.0);
     else
         hipLaunchKernelGGL(k_split<2>, dim3(grid), dim3(256), 0, s, rows, cols, ld_out, rows_pad, src, src_ld, transpose,
-                           (_Float16 *)hi, (_Float16 *)lo, 1.0, chunk_order);
+                           (_Float16 *)hi, (_Float16 *)lo, 1.0);
 }
 
 // C[row, col] = sum_k A[row, k] B[col, k] with A, B hi/lo split, k-contiguous,
@@ -431,25 +425,24 @@ static bool use_lds_tiles(int rows, int cols, int lds_mode = -1) {
 }
 
 
-// LDS rows without padding; 16-byte vectors XOR-swizzled so that the rows one fragment
-// read touches fall on distinct 16-byte bank groups in every group of 8 lanes (a 16-byte
-// read moves 128 bytes per cycle):
-//   Xs rows (DP/8 vectors):  vector c of row r at c ^ (r & XSM)
-//   Xs^T rows (4 vectors of 8 sparse points in the chunk order, 64 bytes): vector c of
-//   row q at c ^ ((q >> 1) & 3), so rows q..q+7 cover the 8 groups of 128 bytes
+// LDS rows of the fused split kernel: no padding; 16-byte vectors XOR-swizzled, vector c
+// of row r at c ^ (r & XSM), so that the rows one fragment read touches fall on distinct
+// 16-byte bank groups in every group of 8 lanes (a 16-byte read moves 128 bytes per
+// cycle); the transposing 8-byte reads of four rows x four column words per lane group
+// fall on distinct groups too.
+//
 // One k step of the fused split kernel's products is its own scheduling region: the
-// four fragment reads of the next step first, then the six MFMAs of this one, so that
-// the reads are in flight under the MFMAs.  Left to itself the compiler issues each
-// step's reads right before its MFMAs and waits for them there (both waves of a SIMD
-// leave each chunk barrier together, so neither covers the other's wait): batch512
-// 822 -> 796 us.
+// fragment reads of the next step first (NR of them), then the six MFMAs of this one,
+// so that the reads are in flight under the MFMAs.  Left to itself the compiler issues
+// each step's reads right before its MFMAs and waits for them there (both waves of a
+// SIMD leave each barrier together, so neither covers the other's wait).
 // Pinned for the zeta = 4 kernels of descriptors up to 256 components, where it was
 // measured: the FP64 epilogue of other zeta needs the registers (with the reads held
 // ahead it spills), and at 64 components FP16 would lose a wavefront per SIMD.
-#define GP_SCHED_STEP()                                           \
+#define GP_SCHED_STEP(NR)                                         \
     do {                                                          \
         if constexpr (ZI == 4 && DK == 8) {                       \
-            __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);    \
+            __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);   \
             __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);    \
             __builtin_amdgcn_sched_barrier(0);                    \
         }                                                         \
@@ -459,26 +452,24 @@ template <int DK>
 struct FusedLds {
     static constexpr int DP = 32 * DK, XSV = DP / 8, XSM = XSV >= 16 ? 15 : XSV - 1;
     static __device__ __forceinline__ int xs_at(int r, int c) { return r * DP + ((c ^ (r & XSM)) << 3); }
-    static __device__ __forceinline__ int xt_at(int q, int c) { return q * 32 + ((c ^ ((q >> 1) & 3)) << 3); }
 };
 
-// Where this lane's 16-byte vectors of a chunk come from: byte offsets from the chunk's
-// first Xs row and from column s0 of the Xs^T rows, the same for every chunk.  Wave
-// instruction u fills LDS vectors 64u .. 64u+63 of a matrix; lane l loads the global
-// vector that the swizzle puts in slot 64u + l.  Vectors past the matrix need no zeros:
-// Xs columns past the padded width ldS meet X components that are zero in registers
-// (they take column ldS - 8 .. ldS - 1 of the same row), and Xs^T rows past d give G
-// columns that are not written (they take row d - 1).
+// Where this lane's 16-byte vectors of a staged pair of chunks (64 Xs rows) come from:
+// byte offsets from the pair's first row, the same for every pair.  Wave instruction u
+// fills LDS vectors 64u .. 64u+63 of a matrix; lane l loads the global vector that the
+// swizzle puts in slot 64u + l.  Instructions u < XSV/2 fill the first chunk's 32 rows.
+// Columns past the padded width ldS need no zeros: they meet X components that are zero
+// in registers (they take columns ldS - 8 .. ldS - 1 of the same row).
 template <int DK>
 struct StageOff {
     typedef FusedLds<DK> L;
-    static constexpr int XS_U = 32 * L::XSV / 64, XT_U = L::DP * 4 / 64;   // wave instructions per matrix
-    static constexpr int NXS = (XS_U + 7) / 8, NXT = (XT_U + 7) / 8;       // per wave
-    unsigned xs[NXS], xt[NXT];
+    static constexpr int XS_U = L::XSV;              // wave instructions per matrix (64 rows)
+    static constexpr int NXS = (XS_U + 7) / 8;       // per wave
+    unsigned xs[NXS];
 };
 
 template <typename T, int DK>
-__device__ __forceinline__ StageOff<DK> stage_offsets(int d, int Mpad, int ldS, int wv) {
+__device__ __forceinline__ StageOff<DK> stage_offsets(int ldS, int wv) {
     typedef FusedLds<DK> L;
     typedef StageOff<DK> O;
     O so;
@@ -488,18 +479,9 @@ __device__ __forceinline__ StageOff<DK> stage_offsets(int d, int Mpad, int ldS, 
         const int vi = 64 * (8 * i + wv) + lane, r = vi / L::XSV, c = (vi % L::XSV) ^ (r & L::XSM);
         so.xs[i] = (unsigned)((r * ldS + min(8 * c, ldS - 8)) * (int)sizeof(T));
     }
-#pragma unroll
-    for (int i = 0; i < O::NXT; i++) {
-        const int vi = 64 * (8 * i + wv) + lane, q = vi >> 2, c = (vi & 3) ^ ((q >> 1) & 3);
-        so.xt[i] = (unsigned)(((size_t)min(q, d - 1) * Mpad + 8 * c) * sizeof(T));
-    }
     return so;
 }
 
-// Stage chunk s0 (32 sparse points) into one LDS buffer: the Xs rows and the
-// (chunk-ordered) Xs^T rows go from global memory straight to LDS (buffer loads with
-// the lds flag, 16 bytes per lane: the lane's offset plus a wave-uniform chunk offset).
-// The chunk's weights w_s pass through a register (*pw).
 // A raw buffer resource over a whole matrix (stride 0, 32-bit data format word of the
 // gfx9 descriptor); its loads are bounded by nbytes.  The chunk loads go through buffer
 // instructions rather than global_load_lds: those are FLAT instructions, which may count
@@ -516,75 +498,67 @@ __device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, unsigned voff, u
                                              (int)soff, 0, 0);
 }
 
+// Stage nch (1 or 2) chunks from sparse point s0 into one LDS buffer: the Xs rows go from
+// global memory straight to LDS (buffer loads with the lds flag, 16 bytes per lane: the
+// lane's offset plus a wave-uniform offset).  Their weights w_s pass through a register
+// (*pw; 0 past M and past the staged chunks).
 template <typename T, int DK>
-__device__ __forceinline__ void fused_stage(int s0, int M, int ldS, __amdgpu_buffer_rsrc_t rsh,
-                                            __amdgpu_buffer_rsrc_t rsl, __amdgpu_buffer_rsrc_t rth,
-                                            __amdgpu_buffer_rsrc_t rtl, const double *wvec, const StageOff<DK> &so,
-                                            int wv, T *__restrict__ xs_h, T *__restrict__ xs_lo,
-                                            T *__restrict__ xt_h, T *__restrict__ xt_lo, double *pw) {
+__device__ __forceinline__ void fused_stage(int s0, int nch, int M, int ldS, __amdgpu_buffer_rsrc_t rsh,
+                                            __amdgpu_buffer_rsrc_t rsl, const double *wvec,
+                                            const StageOff<DK> &so, int wv, T *__restrict__ xs_h,
+                                            T *__restrict__ xs_lo, double *pw) {
     typedef StageOff<DK> O;
     const int t = threadIdx.x;
-    if (t < 32) *pw = s0 + t < M ? wvec[s0 + t] : 0.0;
-    const unsigned ss = (unsigned)(s0 * ldS * (int)sizeof(T)), st = (unsigned)(s0 * (int)sizeof(T));
+    if (t < 64) *pw = (t < 32 * nch && s0 + t < M) ? wvec[s0 + t] : 0.0;
+    const unsigned ss = (unsigned)(s0 * ldS * (int)sizeof(T));
 #pragma unroll
     for (int i = 0; i < O::NXS; i++) {
         const int u = 8 * i + wv;
-        if (O::XS_U % 8 == 0 || u < O::XS_U) {
+        if ((O::XS_U % 8 == 0 || u < O::XS_U) && (nch == 2 || u < O::XS_U / 2)) {
             dma16(rsh, so.xs[i], ss, xs_h + 512 * u);
             dma16(rsl, so.xs[i], ss, xs_lo + 512 * u);
         }
     }
-#pragma unroll
-    for (int i = 0; i < O::NXT; i++) {
-        const int u = 8 * i + wv;
-        if (O::XT_U % 8 == 0 || u < O::XT_U) {
-            dma16(rth, so.xt[i], st, xt_h + 512 * u);
-            dma16(rtl, so.xt[i], st, xt_lo + 512 * u);
-        }
-    }
 }
 
-// One chunk on the current buffers (read) while the next chunk streams into the other
-// buffers (written by fused_stage's loads).  Distinct restrict-qualified buffers tell
-// the compiler that the reads here do not wait for those loads.
+typedef short short4_t __attribute__((ext_vector_type(4)));
+
+// Four 16-bit elements through the transposing LDS read: within each group of 16 lanes,
+// lane r receives element r % 4 of the 8-byte words that lanes r/4, 4 + r/4, 8 + r/4 and
+// 12 + r/4 of its group address.  So when lane s of the group addresses row R + s/4 at
+// column C + 4 (s % 4) of a row-major matrix, lane r gets rows R .. R+3 of column C + r.
+__device__ __forceinline__ short4_t lds_read_tr4(const void *p) {
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) short4_t *)((__attribute__((address_space(3))) char *)p));
+}
+
+// One chunk of 32 sparse points (rows 32h .. 32h+31 of the current buffer).
 template <int PREC, int DK, int ZI>
-__device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, const StageOff<DK> &so,
-                                            int wv,
-                                            __amdgpu_buffer_rsrc_t Shi, __amdgpu_buffer_rsrc_t Slo,
-                                            __amdgpu_buffer_rsrc_t Thi, __amdgpu_buffer_rsrc_t Tlo,
-                                            const double *wvec, double zeta, int izeta, double wscale,
-                                            const typename Half<PREC>::T *__restrict__ xs_h,
-                                            const typename Half<PREC>::T *__restrict__ xs_lo,
-                                            const typename Half<PREC>::T *__restrict__ xt_h,
-                                            const typename Half<PREC>::T *__restrict__ xt_lo,
-                                            const double *__restrict__ wcur, const float *__restrict__ wfcur,
-                                            typename Half<PREC>::T *__restrict__ nxs_h,
-                                            typename Half<PREC>::T *__restrict__ nxs_lo,
-                                            typename Half<PREC>::T *__restrict__ nxt_h,
-                                            typename Half<PREC>::T *__restrict__ nxt_lo, double *__restrict__ wnext,
-                                            float *__restrict__ wfnext,
-                                            const typename Half<PREC>::V (&xh)[DK],
-                                            const typename Half<PREC>::V (&xl)[DK], float4_t (&gacc)[2 * DK],
-                                            double &rs) {
+__device__ __forceinline__ void fused_sub(int h, const typename Half<PREC>::T *__restrict__ xs_h,
+                                          const typename Half<PREC>::T *__restrict__ xs_lo,
+                                          const double *__restrict__ wcur, const float *__restrict__ wfcur,
+                                          double zeta, int izeta, double wscale,
+                                          const typename Half<PREC>::V (&xh)[DK],
+                                          const typename Half<PREC>::V (&xl)[DK], float4_t (&gacc)[2 * DK],
+                                          double &rs) {
     typedef Half<PREC> H;
     typedef typename H::T T;
     typedef typename H::V V;
     typedef FusedLds<DK> L;
-    const int t = threadIdx.x, lane = t & 63;
-    const int r16 = lane & 15, kg = (lane >> 4) * 8, rq = 4 * (lane >> 4);
-    double pw = 0.0;
-    if (next) fused_stage<T, DK>(s0n, M, ldS, Shi, Slo, Thi, Tlo, wvec, so, wv, nxs_h, nxs_lo, nxt_h, nxt_lo, &pw);
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g4 = lane >> 4, kg = g4 * 8, rq = 4 * g4;
+    const int r0 = 32 * h;
     // K^T tiles: 32 sparse points (two tiles) x 16 centres; the fragments of k step
     // ks + 1 are read before the MFMAs of step ks.
     // Wave priority is raised over the two MFMA phases and dropped over the epilogue,
     // so that of the two waves of a SIMD one runs its MFMAs while the other runs its
-    // epilogue's VALU work (batch512 796 -> 780 us).
+    // epilogue's VALU work.
     if constexpr (ZI == 4 && DK == 8) __builtin_amdgcn_s_setprio(1);
     float4_t kacc[2] = {(float4_t){0.f, 0.f, 0.f, 0.f}, (float4_t){0.f, 0.f, 0.f, 0.f}};
     V ah[2], al[2];
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const int o = L::xs_at(16 * j + r16, kg >> 3);
+        const int o = L::xs_at(r0 + 16 * j + r16, kg >> 3);
         ah[j] = *(const V *)(xs_h + o);
         al[j] = *(const V *)(xs_lo + o);
     }
@@ -594,7 +568,7 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, 
         if (ks + 1 < DK) {
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                const int o = L::xs_at(16 * j + r16, 4 * (ks + 1) + (kg >> 3));
+                const int o = L::xs_at(r0 + 16 * j + r16, 4 * (ks + 1) + (kg >> 3));
                 nh[j] = *(const V *)(xs_h + o);
                 nl[j] = *(const V *)(xs_lo + o);
             }
@@ -612,39 +586,49 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, 
                 al[j] = nl[j];
             }
         }
-        GP_SCHED_STEP();
+        GP_SCHED_STEP(4);
     }
-    // the first two column tiles of G += W Xs, read before the FP32 epilogue (after
-    // the FP64 one, whose powers need the registers)
+    // B fragment of column tile q of G += W Xs, from the Xs rows: elements 0-3 are sparse
+    // points 4 (lane/16) .. +3 and elements 4-7 points 16 + 4 (lane/16) .. +3 of this chunk
+    // (the K^T accumulator order), each four by one transposing read: lane s of a group
+    // addresses row (group base) + s/4 at column 16q + 4 (s % 4), i.e. 16-byte vector
+    // 2q + (s % 4)/2 of the row, half (s % 4) % 2.
+    const int tr_c = (r16 & 3) >> 1, tr_e = (r16 & 1) * 4, tr_r = r0 + 4 * g4 + (r16 >> 2);
+    auto frag = [&](const T *base, int q) -> V {
+        const short4_t x = lds_read_tr4(base + L::xs_at(tr_r, 2 * q + tr_c) + tr_e);
+        const short4_t y = lds_read_tr4(base + L::xs_at(tr_r + 16, 2 * q + tr_c) + tr_e);
+        return __builtin_bit_cast(V, __builtin_shufflevector(x, y, 0, 1, 2, 3, 4, 5, 6, 7));
+    };
+    // the first two column tiles, read before the FP32 epilogue (after the FP64 one,
+    // whose powers need the registers)
     V bh[2], bl[2];
     auto read_b0 = [&]() {
 #pragma unroll
         for (int i = 0; i < 2; i++) {
-            const int o = L::xt_at(16 * i + r16, kg >> 3);
-            bh[i] = *(const V *)(xt_h + o);
-            bl[i] = *(const V *)(xt_lo + o);
+            bh[i] = frag(xs_h, i);
+            bl[i] = frag(xs_lo, i);
         }
     };
     if constexpr (ZI == 4) read_b0();
     if constexpr (ZI == 4 && DK == 8) __builtin_amdgcn_s_setprio(0);
-    // epilogue: element v of tile j is centre r16 at sparse point s0 + 16 j + 4 (lane/16) + v,
-    // and element 4 j + v of this lane's A fragment of W.  The energy sum is FP64; K^zeta
-    // and W are formed from the FP32 K in FP32 for the common zeta = 4 (W is then split
-    // into 16-bit halves), in FP64 otherwise.
+    // epilogue: element v of tile j is centre r16 at sparse point 16 j + 4 (lane/16) + v of
+    // the chunk, and element 4 j + v of this lane's A fragment of W.  The energy sum is
+    // FP64; K^zeta and W are formed from the FP32 K in FP32 for the common zeta = 4 (W is
+    // then split into 16-bit halves), in FP64 otherwise.
     V wh, wl;
 #pragma unroll
     for (int j = 0; j < 2; j++) {
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const double wsv = wcur[16 * j + rq + v];
-            T h, lo;
+            const double wsv = wcur[r0 + 16 * j + rq + v];
+            T hv, lo;
             if constexpr (ZI == 4) {
                 const float kf = kacc[j][v];
                 const float k2 = kf * kf;
                 rs += wsv * (double)(k2 * k2);
-                const float wk = wfcur[16 * j + rq + v] * (k2 * kf);   // (float)(w_s 4 wscale), staged
-                h = (T)wk;
-                lo = (T)(wk - (float)h);
+                const float wk = wfcur[r0 + 16 * j + rq + v] * (k2 * kf);   // (float)(w_s 4 wscale), staged
+                hv = (T)wk;
+                lo = (T)(wk - (float)hv);
             } else {
                 const double kv = (double)kacc[j][v];
                 double kz1, kz;
@@ -658,10 +642,10 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, 
                 }
                 rs += wsv * kz;
                 const double wk = wsv * zeta * kz1 * wscale;
-                h = H::from(wk);
-                lo = H::from(wk - H::to(h));
+                hv = H::from(wk);
+                lo = H::from(wk - H::to(hv));
             }
-            wh[4 * j + v] = h;
+            wh[4 * j + v] = hv;
             wl[4 * j + v] = lo;
         }
     }
@@ -674,9 +658,8 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, 
         if (q0 + 2 < 2 * DK) {
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                const int o = L::xt_at(16 * (q0 + 2 + i) + r16, kg >> 3);
-                nh[i] = *(const V *)(xt_h + o);
-                nl[i] = *(const V *)(xt_lo + o);
+                nh[i] = frag(xs_h, q0 + 2 + i);
+                nl[i] = frag(xs_lo, q0 + 2 + i);
             }
         }
 #pragma unroll
@@ -692,11 +675,34 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, 
                 bl[i] = nl[i];
             }
         }
-        GP_SCHED_STEP();
+        GP_SCHED_STEP(8);
     }
     if constexpr (ZI == 4 && DK == 8) __builtin_amdgcn_s_setprio(0);
+}
+
+// One pair of chunks on the current buffer (read) while the next pair streams into the
+// other buffer (written by fused_stage's loads).  Distinct restrict-qualified buffers
+// tell the compiler that the reads here do not wait for those loads.
+template <int PREC, int DK, int ZI>
+__device__ __forceinline__ void fused_pair(int ncur, int nnext, int s0n, int M, int ldS, const StageOff<DK> &so,
+                                           int wv, __amdgpu_buffer_rsrc_t rsh, __amdgpu_buffer_rsrc_t rsl,
+                                           const double *wvec, double zeta, int izeta, double wscale,
+                                           const typename Half<PREC>::T *__restrict__ xs_h,
+                                           const typename Half<PREC>::T *__restrict__ xs_lo,
+                                           const double *__restrict__ wcur, const float *__restrict__ wfcur,
+                                           typename Half<PREC>::T *__restrict__ nxs_h,
+                                           typename Half<PREC>::T *__restrict__ nxs_lo, double *__restrict__ wnext,
+                                           float *__restrict__ wfnext, const typename Half<PREC>::V (&xh)[DK],
+                                           const typename Half<PREC>::V (&xl)[DK], float4_t (&gacc)[2 * DK],
+                                           double &rs) {
+    typedef typename Half<PREC>::T T;
+    const int t = threadIdx.x;
+    double pw = 0.0;
+    if (nnext > 0) fused_stage<T, DK>(s0n, nnext, M, ldS, rsh, rsl, wvec, so, wv, nxs_h, nxs_lo, &pw);
+    fused_sub<PREC, DK, ZI>(0, xs_h, xs_lo, wcur, wfcur, zeta, izeta, wscale, xh, xl, gacc, rs);
+    if (ncur > 1) fused_sub<PREC, DK, ZI>(1, xs_h, xs_lo, wcur, wfcur, zeta, izeta, wscale, xh, xl, gacc, rs);
     // the next buffers were last read before the previous barrier
-    if (next && t < 32) {
+    if (nnext > 0 && t < 64) {
         wnext[t] = pw;
         wfnext[t] = (float)(pw * 4.0 * wscale);
     }
@@ -709,17 +715,19 @@ __device__ __forceinline__ void fused_chunk(bool next, int s0n, int M, int ldS, 
 //
 // One block = 8 wavefronts = 128 centres; wave w owns centres 16w..16w+15 and
 // keeps their X rows (split hi/lo) in registers.  The block walks the sparse points
-// of its part in chunks of 32.  Per chunk the chunk's rows of Xs (k = descriptor
-// index) and of Xs^T (k = sparse index) are staged in LDS.  Each wave computes the
-// transposed tile K^T = Xs X^T (32 sparse points x 16 centres; three MFMA passes, FP32
+// of its part in chunks of 32, staged two at a time: per pair the chunks' Xs rows (k =
+// descriptor index) are staged in LDS.  Each wave computes the transposed tile K^T =
+// Xs X^T of a chunk (32 sparse points x 16 centres; three MFMA passes, FP32
 // accumulation): in the accumulator layout lane l holds centre l%16 at sparse points
 // 16 j + 4 (l/16) + v (tile j, element v), which is one 8-element A fragment of the
 // next product when its k runs over the sparse points in the order
-// [0-3, 16-19 | 4-7, 20-23 | 8-11, 24-27 | 12-15, 28-31] -- the order of the Xs^T copy
-// the kernel reads.  So the epilogue (energy w K^zeta summed in FP64, W = w zeta
-// K^(zeta-1) split into hi/lo) leaves W in registers ready for G += W Xs (16 x 32DK, in
-// registers): W never leaves the lane.  The next chunk streams from global memory into
-// the other of two LDS buffer sets while the current one is used, one barrier per chunk.
+// [0-3, 16-19 | 4-7, 20-23 | 8-11, 24-27 | 12-15, 28-31].  So the epilogue (energy
+// w K^zeta summed in FP64, W = w zeta K^(zeta-1) split into hi/lo) leaves W in
+// registers ready for G += W Xs (16 x 32DK, in registers): W never leaves the lane.
+// That product's B fragments (eight sparse points of one descriptor column, in the
+// same order) come from the same Xs rows through transposing LDS reads, so no
+// transposed copy of Xs is staged.  The next pair streams from global memory into the
+// other of two LDS buffers while the current one is used, one barrier per pair.
 // With several parts (gridDim.y) the sparse points are divided between blocks and the
 // FP32 partial G tiles are summed in a fixed order afterwards.
 //
@@ -737,8 +745,6 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
                                                         long sk_units, const double *__restrict__ X,
                                                         const typename Half<PREC>::T *__restrict__ Shi,
                                                         const typename Half<PREC>::T *__restrict__ Slo,
-                                                        const typename Half<PREC>::T *__restrict__ Thi,
-                                                        const typename Half<PREC>::T *__restrict__ Tlo,
                                                         const double *__restrict__ wvec, double zeta, int izeta,
                                                         double wscale, double inv_wscale, double *__restrict__ G,
                                                         float *__restrict__ Gpart, double *__restrict__ e_part,
@@ -747,19 +753,17 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
     typedef typename H::T T;
     typedef typename H::V V;
     constexpr int DP = 32 * DK;     // descriptor components handled (zero padded)
-    __shared__ __attribute__((aligned(16))) T xs0[2][32 * DP], xs1[2][32 * DP];   // [hi, lo] of two buffers
-    __shared__ __attribute__((aligned(16))) T xt0[2][DP * 32], xt1[2][DP * 32];
-    __shared__ double wv0[32], wv1[32];   // the chunk's w_s (0 past M)
-    __shared__ float wf0[32], wf1[32];    // and (float)(w_s 4 wscale), the zeta = 4 epilogue's factor
+    __shared__ __attribute__((aligned(16))) T xs0[2][64 * DP], xs1[2][64 * DP];   // [hi, lo] of two buffers
+    __shared__ double wv0[64], wv1[64];   // the pair's w_s (0 past M)
+    __shared__ float wf0[64], wf1[64];    // and (float)(w_s 4 wscale), the zeta = 4 epilogue's factor
     const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const int r16 = lane & 15, g4 = lane >> 4, kg = g4 * 8, rq = 4 * g4;
     const int part = blockIdx.y, nparts = gridDim.y;
     const int C = Mpad / 32;
     const bool sk = sk_units > 0;
-    const StageOff<DK> so = stage_offsets<T, DK>(d, Mpad, ldS, wv);
-    const unsigned nbytes = (unsigned)((size_t)Mpad * ldS * sizeof(T));   // within both copies' allocations
+    const StageOff<DK> so = stage_offsets<T, DK>(ldS, wv);
+    const unsigned nbytes = (unsigned)((size_t)Mpad * ldS * sizeof(T));   // within the copy's allocation
     const __amdgpu_buffer_rsrc_t rsh = mat_rsrc(Shi, nbytes), rsl = mat_rsrc(Slo, nbytes);
-    const __amdgpu_buffer_rsrc_t rth = mat_rsrc(Thi, nbytes), rtl = mat_rsrc(Tlo, nbytes);
     long u, u_end;   // this block's units (row block rb, chunk c) = rb C + c
     if (sk) {
         u = (long)blockIdx.x * sk_units / gridDim.x;
@@ -796,23 +800,22 @@ __global__ __launch_bounds__(512) void k_gp_fused_split(int nc, int M, int d, in
 
     if (c_beg < c_end) {
         double pw = 0.0;
-        fused_stage<T, DK>(32 * c_beg, M, ldS, rsh, rsl, rth, rtl, wvec, so, wv, xs0[0], xs0[1], xt0[0],
-                           xt0[1], &pw);
-        if (t < 32) {
+        fused_stage<T, DK>(32 * c_beg, min(2, c_end - c_beg), M, ldS, rsh, rsl, wvec, so, wv, xs0[0], xs0[1], &pw);
+        if (t < 64) {
             wv0[t] = pw;
             wf0[t] = (float)(pw * 4.0 * wscale);
         }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int ch = c_beg; ch < c_end; ch += 2) {   // buffer set 0, then set 1
-        fused_chunk<PREC, DK, ZI>(ch + 1 < c_end, 32 * (ch + 1), M, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
-                                  izeta, wscale, xs0[0], xs0[1], xt0[0], xt0[1], wv0, wf0, xs1[0], xs1[1], xt1[0],
-                                  xt1[1], wv1, wf1, xh, xl, gacc, rs);
-        if (ch + 1 >= c_end) break;
-        fused_chunk<PREC, DK, ZI>(ch + 2 < c_end, 32 * (ch + 2), M, ldS, so, wv, rsh, rsl, rth, rtl, wvec, zeta,
-                                  izeta, wscale, xs1[0], xs1[1], xt1[0], xt1[1], wv1, wf1, xs0[0], xs0[1], xt0[0],
-                                  xt0[1], wv0, wf0, xh, xl, gacc, rs);
+    // pairs of chunks: [ch, ch + 2) on buffer 0, then [ch + 2, ch + 4) on buffer 1
+    for (int ch = c_beg; ch < c_end; ch += 4) {
+        const int n0 = min(2, c_end - ch), n1 = max(0, min(2, c_end - ch - 2)), n2 = max(0, min(2, c_end - ch - 4));
+        fused_pair<PREC, DK, ZI>(n0, n1, 32 * (ch + 2), M, ldS, so, wv, rsh, rsl, wvec, zeta, izeta, wscale, xs0[0],
+                                 xs0[1], wv0, wf0, xs1[0], xs1[1], wv1, wf1, xh, xl, gacc, rs);
+        if (n1 == 0) break;
+        fused_pair<PREC, DK, ZI>(n1, n2, 32 * (ch + 4), M, ldS, so, wv, rsh, rsl, wvec, zeta, izeta, wscale, xs1[0],
+                                 xs1[1], wv1, wf1, xs0[0], xs0[1], wv0, wf0, xh, xl, gacc, rs);
     }
     // energy partials of this part: the four lane groups hold different sparse points
     rs += __shfl_xor(rs, 16, 64);
@@ -1011,7 +1014,7 @@ size_t gp_fused_sk_scratch_bytes(int prec, int d, int dpad, int nc, int Mpad) {
 
 template <int PREC, int DK>
 static void gp_fused_go(int nc, int M, int d, int dpad, int Mpad, int parts, long sk_units, int sk_blocks,
-                        const double *X, const void *Shi, const void *Slo, const void *Thi, const void *Tlo,
+                        const double *X, const void *Shi, const void *Slo,
                         const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
                         double *e_slot, hipStream_t s) {
     typedef typename Half<PREC>::T T;
@@ -1020,11 +1023,11 @@ static void gp_fused_go(int nc, int M, int d, int dpad, int Mpad, int parts, lon
     const int iz = int_zeta_m(zeta);
     if (iz == 4)
         hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 4>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per,
-                           sk_units, X, (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta,
+                           sk_units, X, (const T *)Shi, (const T *)Slo, wvec, zeta,
                            iz, wscale, 1.0 / wscale, G, Gpart, e_part, e_slot);
     else
         hipLaunchKernelGGL((k_gp_fused_split<PREC, DK, 0>), grid, dim3(512), 0, s, nc, M, d, Mpad, dpad, per,
-                           sk_units, X, (const T *)Shi, (const T *)Slo, (const T *)Thi, (const T *)Tlo, wvec, zeta,
+                           sk_units, X, (const T *)Shi, (const T *)Slo, wvec, zeta,
                            iz, wscale, 1.0 / wscale, G, Gpart, e_part, e_slot);
 }
 
@@ -1036,9 +1039,8 @@ int gp_fused_parts_valid(int parts, int Mpad) {
 }
 
 void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, int parts_req, const double *X,
-                           const void *Xs_hi, const void *Xs_lo, const void *XsT_hi, const void *XsT_lo,
-                           const double *wvec, double zeta, double wscale, double *G, float *Gpart, double *e_part,
-                           int *ntile_out, hipStream_t s) {
+                           const void *Xs_hi, const void *Xs_lo, const double *wvec, double zeta, double wscale,
+                           double *G, float *Gpart, double *e_part, int *ntile_out, hipStream_t s) {
     const bool sk = parts_req < 0;
     const int parts = sk ? 1 : parts_req > 0 ? gp_fused_parts_valid(parts_req, Mpad) : gp_fused_parts(nc, Mpad);
     *ntile_out = sk ? -1 : parts;
@@ -1048,8 +1050,8 @@ void launch_gp_fused_split(int prec, int nc, int M, int d, int dpad, int Mpad, i
     if (sk) sk_plan(prec, dpad, nc, Mpad, &B, &U);
     double *e_slot = sk ? (double *)(Gpart + (size_t)B * 256 * d) : nullptr;
     const int dk = fused_dk(dpad);
-#define GO(P, D) gp_fused_go<P, D>(nc, M, d, dpad, Mpad, parts, sk ? U : 0, B, X, Xs_hi, Xs_lo, XsT_hi, XsT_lo, wvec, \
-                                   zeta, wscale, G, Gpart, e_part, e_slot, s)
+#define GO(P, D) gp_fused_go<P, D>(nc, M, d, dpad, Mpad, parts, sk ? U : 0, B, X, Xs_hi, Xs_lo, wvec, zeta, wscale, \
+                                   G, Gpart, e_part, e_slot, s)
     if (prec == 1) {
         if (dk == 2) GO(1, 2); else if (dk == 4) GO(1, 4); else GO(1, 8);
     } else {
