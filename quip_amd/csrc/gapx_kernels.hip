@@ -551,22 +551,25 @@ void launch_nl_front_small(int N, int NC, const double *pos, const NLGeom *geo, 
                        map_shift, cell_cnt, cell_start, cell_atoms, cpos, cms, n2_cnt, n2_fill, flags);
 }
 
-// Wave-parallel form of the same search: one wavefront per atom1, lanes over
-// the neighbour-cell offsets in the reference's loop order (k2 slowest, i2
-// fastest).  Each lane scans one cell list; an exclusive wave scan of the lane
-// counts places every lane's pairs, so neighbour1 keeps the append order.
-__device__ __forceinline__ int wave_excl_scan_i(int v, int *total) {
-    const int lane = threadIdx.x & 63;
+// Wave-parallel form of the same search: a group of lanes per atom1, lanes over the
+// neighbour-cell offsets in the reference's loop order (k2 slowest, i2 fastest).  Each
+// lane scans one cell list; an exclusive scan of the lane counts places every lane's
+// pairs, so neighbour1 keeps the append order.
+// Exclusive scan over the W lanes of each W-lane group of the wavefront (W = 64 or 32);
+// *total is the group's sum.
+template <int W>
+__device__ __forceinline__ int group_excl_scan_i(int v, int *total) {
+    const int lane = threadIdx.x & (W - 1);
     int x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        int y = __shfl_up(x, off, 64);
+    for (int off = 1; off < W; off <<= 1) {
+        int y = __shfl_up(x, off, W);
         if (lane >= off) x += y;
     }
-    *total = __shfl(x, 63, 64);
+    *total = __shfl(x, W - 1, W);
     return x - v;
 }
 
-// One wavefront per atom1.  Candidates are flattened: the offset cells are
+// LPA lanes per atom1 (64: one atom per wavefront; 32: two, each half on its own).  Candidates are flattened: the offset cells are
 // taken in the reference's loop order (k2 slowest, i2 fastest), and in each
 // cell's descending list the atoms with index >= atom1 form a prefix (found by
 // binary search).  An exclusive wave scan of the prefix lengths numbers the
@@ -577,8 +580,12 @@ __device__ __forceinline__ int wave_excl_scan_i(int v, int *total) {
 // (neighbour1 of atom a at a*cap1, neighbour2 of atom a at a*cap2), counts both,
 // and raises *ovf when a count exceeds its capacity.
 // Latency-bound (most of its time waits on the candidates' global loads): the occupancy
-// hint gives five wavefronts per SIMD (96 VGPRs, no spills) instead of four.
-template <int MODE, bool CC>
+// hint gives five wavefronts per SIMD (96 VGPRs, no spills) instead of four.  With two
+// atoms per wavefront (large problems) the wavefronts that wait in turn are halved; the
+// halves' loops run their own trip counts (each lane reads only its half's ballot bits
+// and shuffles stay within the half), and a half past N loads through a valid index and
+// runs no iteration.
+template <int MODE, bool CC, int LPA>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_nl_pairs_w(int N, const double *__restrict__ pos,
                                                     const int *__restrict__ map_shift, const int *__restrict__ cell_of,
                                                     const int *__restrict__ cell_start, const int *__restrict__ cell_cnt,
@@ -592,12 +599,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                     int *n2_k, int cap1, int cap2, int *ovf) {
 #pragma clang fp contract(off)
     constexpr bool FILL = MODE == 1;
-    __shared__ int s_pref[4][65];
-    __shared__ int s_start[4][64];
-    __shared__ int s_sh[4][64];   // i4, j4, k4 packed (each in [-128, 127]) and the same-cell flag
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int atom1 = blockIdx.x * 4 + wv;
-    if (atom1 >= N) return;   // whole wavefront leaves together
+    constexpr int APW = 64 / LPA;   // atoms per wavefront
+    __shared__ int s_pref[4 * APW][LPA + 1];
+    __shared__ int s_start[4 * APW][LPA];
+    __shared__ int s_sh[4 * APW][LPA];   // i4, j4, k4 packed (each in [-128, 127]) and the same-cell flag
+    const int lane = threadIdx.x & (LPA - 1), wv = threadIdx.x >> 6, row = (threadIdx.x >> 6) * APW + ((threadIdx.x & 63) / LPA);
+    const int atom1_r = blockIdx.x * 4 * APW + row;
+    if (blockIdx.x * 4 * APW + wv * APW >= N) return;   // the wavefront's first atom: whole wavefront leaves together
+    const bool live = atom1_r < N;
+    const int atom1 = live ? atom1_r : N - 1;   // a half past N reads a valid atom and runs no iteration
+    const int gshift = LPA * ((threadIdx.x & 63) / LPA);
+    const unsigned long long gmask = LPA == 64 ? ~0ull : ((1ull << LPA) - 1ull);
     const int st = sid[atom1];
     const NLGeom &g = geo[st];
     const int cb = cell_base[st];
@@ -613,7 +625,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         }
         span[a] = hi - lo[a] + 1;
     }
-    const int noff = span[0] * span[1] * span[2];
+    const int noff = live ? span[0] * span[1] * span[2] : 0;
     const double p1x = pos[3 * atom1], p1y = pos[3 * atom1 + 1], p1z = pos[3 * atom1 + 2];
     const int ms1x = map_shift[3 * atom1], ms1y = map_shift[3 * atom1 + 1], ms1z = map_shift[3 * atom1 + 2];
     const double rc = g.cutoff;
@@ -621,7 +633,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int base1 = MODE == 1 ? n1_off[atom1] : (MODE == 2 ? atom1 * cap1 : 0);
     int base = base1;
     int total_all = 0;
-    for (int o0 = 0; o0 < noff; o0 += 64) {
+    for (int o0 = 0; o0 < noff; o0 += LPA) {
         // this lane's offset cell: list start, shift, and the length of its atom >= atom1 prefix
         const int o = o0 + lane;
         int cnt = 0, start = 0, packed = 0;
@@ -649,29 +661,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             }
         }
         int tot;
-        const int pre = wave_excl_scan_i(cnt, &tot);
-        s_pref[wv][lane] = pre;
-        s_start[wv][lane] = start;
-        s_sh[wv][lane] = packed;
-        if (lane == 0) s_pref[wv][64] = tot;
+        const int pre = group_excl_scan_i<LPA>(cnt, &tot);
+        s_pref[row][lane] = pre;
+        s_start[row][lane] = start;
+        s_sh[row][lane] = packed;
+        if (lane == 0) s_pref[row][LPA] = tot;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int c0 = 0; c0 < tot; c0 += 64) {
+        for (int c0 = 0; c0 < tot; c0 += LPA) {
             const int c = c0 + lane;
             bool acc = false;
             int atom2 = 0, sh0 = 0, sh1 = 0, sh2 = 0;
             double d = 0.0;
             if (c < tot) {
                 // offset cell of candidate c: the last lane whose prefix start is <= c
-                int lo_o = 0, hi_o = 63;
-                while (lo_o < hi_o) {   // prefix starts are non-decreasing over the 64 lanes
+                int lo_o = 0, hi_o = LPA - 1;
+                while (lo_o < hi_o) {   // prefix starts are non-decreasing over the group's lanes
                     const int mid = (lo_o + hi_o + 1) >> 1;
-                    if (s_pref[wv][mid] <= c) lo_o = mid;
+                    if (s_pref[row][mid] <= c) lo_o = mid;
                     else hi_o = mid - 1;
                 }
-                const int q = c - s_pref[wv][lo_o];
-                const int slot = s_start[wv][lo_o] + q;
+                const int q = c - s_pref[row][lo_o];
+                const int slot = s_start[row][lo_o] + q;
                 CellShift cs2;
                 CellAtom ca2;
                 if constexpr (CC) {   // the cell-ordered copies: no dependent load
@@ -683,7 +695,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                     ca2 = CellAtom{{pos[3 * a2], pos[3 * a2 + 1], pos[3 * a2 + 2], 0.0}};
                 }
                 atom2 = cs2.v[3];
-                const int pk = s_sh[wv][lo_o];
+                const int pk = s_sh[row][lo_o];
                 const int i4 = (int)(signed char)(pk & 255), j4 = (int)(signed char)((pk >> 8) & 255),
                           k4 = (int)(signed char)((pk >> 16) & 255);
                 const bool same_cell = (pk >> 24) & 1;
@@ -704,7 +716,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                     }
                 }
             }
-            const unsigned long long mask = __ballot(acc);
+            const unsigned long long mask = (__ballot(acc) >> gshift) & gmask;   // this group's lanes
             if (FILL && acc) {
                 const int e = base + __popcll(mask & below);
                 n1_j[e] = atom2;
@@ -739,25 +751,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         }
         __builtin_amdgcn_wave_barrier();   // this group's LDS rows are read before the next group writes them
     }
-    if (MODE != 1 && lane == 0) n1_cnt[atom1] = total_all;
-    if (MODE == 2 && lane == 0 && total_all > cap1) *ovf = 1;
+    if (MODE != 1 && lane == 0 && live) n1_cnt[atom1] = total_all;
+    if (MODE == 2 && lane == 0 && live && total_all > cap1) *ovf = 1;
 }
 
+constexpr int kNlTwoPerWave = 16384;
 void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, const int *cell_of,
                      const int *cell_start, const int *cell_cnt, const int *cell_atoms, const CellAtom *cpos,
                      const CellShift *cms, const NLGeom *geo, const int *sid, const int *cell_base, int *n1_cnt,
                      const int *n1_off, int *n1_j, int *n1_shift, double *n1_r, int *n2_cnt, const int *n2_off,
                      int *n2_fill, int *n2_i, int *n2_k, int cap1, int cap2, int *ovf, hipStream_t s) {
     if (N <= 0) return;
-#define NLP(M, CC)                                                                                              \
-    hipLaunchKernelGGL((k_nl_pairs_w<M, CC>), dim3(cdiv(N, 4)), dim3(256), 0, s, N, pos, map_shift, cell_of,      \
-                       cell_start, cell_cnt, cell_atoms, cpos, cms, geo, sid, cell_base, n1_cnt, n1_off, n1_j,   \
-                       n1_shift, n1_r, n2_cnt, n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
+    // two atoms per wavefront once the one-atom grid would run more than twice the
+    // wavefronts the GPU holds at five per SIMD (about 10K atoms on MI355X)
+    const bool two = N >= kNlTwoPerWave;
+#define NLP(M, CC, W)                                                                                           \
+    hipLaunchKernelGGL((k_nl_pairs_w<M, CC, W>), dim3(cdiv(N, 4 * (64 / W))), dim3(256), 0, s, N, pos, map_shift, \
+                       cell_of, cell_start, cell_cnt, cell_atoms, cpos, cms, geo, sid, cell_base, n1_cnt, n1_off, \
+                       n1_j, n1_shift, n1_r, n2_cnt, n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
+#define NLPW(M, CC) \
+    if (two) NLP(M, CC, 32); else NLP(M, CC, 64)
     if (cpos) {
-        if (mode == 1) NLP(1, true); else if (mode == 2) NLP(2, true); else NLP(0, true);
+        if (mode == 1) NLPW(1, true); else if (mode == 2) NLPW(2, true); else NLPW(0, true);
     } else {
-        if (mode == 1) NLP(1, false); else if (mode == 2) NLP(2, false); else NLP(0, false);
+        if (mode == 1) NLPW(1, false); else if (mode == 2) NLPW(2, false); else NLPW(0, false);
     }
+#undef NLPW
 #undef NLP
 }
 
