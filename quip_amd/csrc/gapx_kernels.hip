@@ -755,7 +755,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     if (MODE == 2 && lane == 0 && live && total_all > cap1) *ovf = 1;
 }
 
-constexpr int kNlTwoPerWave = 16384;
+constexpr int kNlTwoPerWave = 16384, kNlFourPerWave = 65536;
 void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, const int *cell_of,
                      const int *cell_start, const int *cell_cnt, const int *cell_atoms, const CellAtom *cpos,
                      const CellShift *cms, const NLGeom *geo, const int *sid, const int *cell_base, int *n1_cnt,
@@ -763,14 +763,15 @@ void launch_nl_pairs(int mode, int N, const double *pos, const int *map_shift, c
                      int *n2_fill, int *n2_i, int *n2_k, int cap1, int cap2, int *ovf, hipStream_t s) {
     if (N <= 0) return;
     // two atoms per wavefront once the one-atom grid would run more than twice the
-    // wavefronts the GPU holds at five per SIMD (about 10K atoms on MI355X)
-    const bool two = N >= kNlTwoPerWave;
+    // wavefronts the GPU holds at five per SIMD (about 10K atoms on MI355X), four from
+    // 64K atoms (batch512: 239 us with one, 178 with two, 170 with four)
+    const int lpa = N >= kNlFourPerWave ? 16 : N >= kNlTwoPerWave ? 32 : 64;
 #define NLP(M, CC, W)                                                                                           \
     hipLaunchKernelGGL((k_nl_pairs_w<M, CC, W>), dim3(cdiv(N, 4 * (64 / W))), dim3(256), 0, s, N, pos, map_shift, \
                        cell_of, cell_start, cell_cnt, cell_atoms, cpos, cms, geo, sid, cell_base, n1_cnt, n1_off, \
                        n1_j, n1_shift, n1_r, n2_cnt, n2_off, n2_fill, n2_i, n2_k, cap1, cap2, ovf)
 #define NLPW(M, CC) \
-    if (two) NLP(M, CC, 32); else NLP(M, CC, 64)
+    if (lpa == 16) NLP(M, CC, 16); else if (lpa == 32) NLP(M, CC, 32); else NLP(M, CC, 64)
     if (cpos) {
         if (mode == 1) NLPW(1, true); else if (mode == 2) NLPW(2, true); else NLPW(0, true);
     } else {
