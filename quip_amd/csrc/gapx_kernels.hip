@@ -818,49 +818,58 @@ void launch_nl_compact(int N, int cap1, int cap2, const int *n1_off, const int *
 
 // neighbour2(j) in the reference's append order: the global loop visits atom1
 // by (cell linear index, position in its descending cell list), then by its
-// neighbour1 entry index.  A group of 32 lanes per atom ranks its entries by that
-// key (all keys distinct; up to 32 entries are staged in LDS first) and writes them
+// neighbour1 entry index.  A group of G lanes per atom (32, or 16 for large problems,
+// where halving the wavefronts pays) ranks its entries by that key (all keys distinct;
+// up to 32 entries are staged in LDS first) and writes them
 // in order; it also records, for every neighbour1 entry, the slot of its mirror
 // in neighbour2.
 // SLOT: the unsorted entries are read from the per-atom slots (atom j's at j * cap2)
 // of a one-pass search, with no compacted copy.
-template <bool SLOT>
+template <bool SLOT, int G>
 __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k,
                                                     const int *cell_of, const int *n1_off, int *n2_is, int *n2_ks,
                                                     int *n1_rev, int cap2) {
-    __shared__ int s_i[8][32], s_k[8][32], s_c[8][32];   // the group's entries when they fit one row
-    const int lane = threadIdx.x & 31, wv = threadIdx.x >> 5;
-    const int j = blockIdx.x * 8 + wv;
+    constexpr int GPB = 256 / G, EPL = 32 / G;   // atoms per block; staged entries per lane
+    __shared__ int s_i[GPB][32], s_k[GPB][32], s_c[GPB][32];   // the group's entries when they fit one row
+    const int lane = threadIdx.x & (G - 1), wv = threadIdx.x / G;
+    const int j = blockIdx.x * GPB + wv;
     if (j >= N) return;   // the whole group leaves together
     const int b = n2_off[j], n = n2_off[j + 1] - b;
     const size_t bs = SLOT ? (size_t)j * cap2 : (size_t)b;   // where the unsorted entries sit
     if (n <= 32) {
-        int vi = 0, vk = 0, vc = 0;
-        if (lane < n) {
-            vi = n2_i[bs + lane];
-            vk = n2_k[bs + lane];
-            vc = cell_of[vi];
-            s_i[wv][lane] = vi;
-            s_k[wv][lane] = vk;
-            s_c[wv][lane] = vc;
+        int vi[EPL], vk[EPL], vc[EPL];
+#pragma unroll
+        for (int u = 0; u < EPL; u++) {
+            const int x = lane + u * G;
+            vi[u] = 0; vk[u] = 0; vc[u] = 0;
+            if (x < n) {
+                vi[u] = n2_i[bs + x];
+                vk[u] = n2_k[bs + x];
+                vc[u] = cell_of[vi[u]];
+                s_i[wv][x] = vi[u];
+                s_k[wv][x] = vk[u];
+                s_c[wv][x] = vc[u];
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane < n) {
+#pragma unroll
+        for (int u = 0; u < EPL; u++) {
+            if (lane + u * G >= n) continue;
             int rank = 0;
             for (int y = 0; y < n; y++) {
                 const int yi = s_i[wv][y], yk = s_k[wv][y], yc = s_c[wv][y];
-                const bool before = (yc < vc) || (yc == vc && (yi > vi || (yi == vi && yk < vk)));
+                const bool before = (yc < vc[u]) || (yc == vc[u] && (yi > vi[u] || (yi == vi[u] && yk < vk[u])));
                 rank += before ? 1 : 0;
             }
-            n2_is[b + rank] = vi;
-            n2_ks[b + rank] = vk;
-            n1_rev[n1_off[vi] + vk] = rank;
+            n2_is[b + rank] = vi[u];
+            n2_ks[b + rank] = vk[u];
+            n1_rev[n1_off[vi[u]] + vk[u]] = rank;
         }
         return;
     }
-    for (int x = lane; x < n; x += 32) {
+    for (int x = lane; x < n; x += G) {
         const int vi = n2_i[bs + x], vk = n2_k[bs + x], vc = cell_of[vi];
         int rank = 0;
         for (int y = 0; y < n; y++) {
@@ -876,12 +885,17 @@ __global__ __launch_bounds__(256) void k_nl_n2_sort(int N, const int *n2_off, co
 void launch_nl_n2_sort(int N, const int *n2_off, const int *n2_i, const int *n2_k, const int *cell_of,
                        const int *n1_off, int *n2_is, int *n2_ks, int *n1_rev, int cap2, hipStream_t s) {
     if (N <= 0) return;
-    if (cap2 > 0)
-        hipLaunchKernelGGL(k_nl_n2_sort<true>, dim3(cdiv(N, 8)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of,
-                           n1_off, n2_is, n2_ks, n1_rev, cap2);
-    else
-        hipLaunchKernelGGL(k_nl_n2_sort<false>, dim3(cdiv(N, 8)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, cell_of,
-                           n1_off, n2_is, n2_ks, n1_rev, 0);
+    // 16 lanes per atom (two staged entries each) from 64K atoms: half the wavefronts
+    const bool g16 = N >= kNlFourPerWave;
+#define N2S(SL, GG)                                                                                             \
+    hipLaunchKernelGGL((k_nl_n2_sort<SL, GG>), dim3(cdiv(N, 256 / GG)), dim3(256), 0, s, N, n2_off, n2_i, n2_k, \
+                       cell_of, n1_off, n2_is, n2_ks, n1_rev, SL ? cap2 : 0)
+    if (cap2 > 0) {
+        if (g16) N2S(true, 16); else N2S(true, 32);
+    } else {
+        if (g16) N2S(false, 16); else N2S(false, 32);
+    }
+#undef N2S
 }
 
 // Full per-atom neighbour view in connection_neighbour order (neighbour2 first,
