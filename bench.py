@@ -11,7 +11,8 @@ Reported beside the FP64 plain form: the bf16x3 and fp16x3 split-precision
 forms (throughput and error against FP64), a per-stage breakdown from HIP
 events on the library's stream, the roofline of the dominant kernel, the HBM
 GB/s and FP64-VALU fraction of each descriptor stage (per-launch bytes and flops
-from the committed counter profile over this run's stage times), the mixed/FP64
+from the committed counter profile over this run's stage times), the batching
+form on batch512 and on si4096x8 (eight si4096-sized structures), the mixed/FP64
 ratio on each workload measured in both forms, and the oracle's FP64 CPU
 throughput on this host's cores (si4096, with bounded samples of si64, sic32768
 and batch512, and the host's lscpu figures) -- a reported baseline.
@@ -390,6 +391,9 @@ def main():
     ap.add_argument("--batch", default="fp64,bf16x3",
                     help="precision forms of the 512-structure batching form ('' to skip)")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--big-batch", default="fp64,bf16x3",
+                    help="forms to time on si4096x8 (eight si4096-sized structures in one batch)")
+    ap.add_argument("--big-batch-steps", type=int, default=5)
     ap.add_argument("--sic", default="fp64,bf16x3,fp16x3",
                     help="precision forms timed on the 32768-atom SiC structure ('' to skip)")
     ap.add_argument("--sic-steps", type=int, default=5)
@@ -482,6 +486,30 @@ def main():
             forms["batch512_" + prec]["max_dE_per_atom_vs_fp64"] = float(
                 np.max(np.abs(r["energy"] - rb["energy"]) / rb["natoms"]))
             forms["batch512_" + prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - rb["force"]) ** 2)))
+    big_list = [f for f in args.big_batch.split(",") if f]
+    if big_list:   # the batching form on structures of the headline's size: a like-for-like ratio
+        if "fp64" not in big_list:
+            big_list.insert(0, "fp64")
+        big = configs.si4096_batch()
+        gres = {}
+        for prec in big_list:
+            pot = Potential(param_filename=path, precision=prec)
+            log("timing si4096x8 batch", prec)
+            dt, NG, r, stages = run_batch(pot, big, dev, args.big_batch_steps, 2)
+            gres[prec] = r
+            forms["si4096x8_" + prec] = dict(
+                atoms_per_s=NG * args.big_batch_steps / dt, ms_per_step=1e3 * dt / args.big_batch_steps,
+                structures=len(big), atoms=NG, stage_ms={k: round(v, 4) for k, v in stages.items()})
+            del pot
+        rb = gres["fp64"]
+        n0 = len(big[0])   # the first structure is si4096: the batched FP64 forces against the plain form's
+        forms["si4096x8_fp64"]["max_dF_first_vs_plain"] = float(
+            np.abs(rb["force"][:n0] - results["fp64"]["force"]).max())
+        for prec in big_list[1:]:
+            r = gres[prec]
+            forms["si4096x8_" + prec]["max_dE_per_atom_vs_fp64"] = float(
+                np.max(np.abs(r["energy"] - rb["energy"]) / rb["natoms"]))
+            forms["si4096x8_" + prec]["force_rmse_vs_fp64"] = float(np.sqrt(np.mean((r["force"] - rb["force"]) ** 2)))
     serving = None
     serve_list = [f for f in args.serve.split(",") if f]
     sic_list = [f for f in args.sic.split(",") if f]
@@ -555,6 +583,7 @@ def main():
     same = {}
     for key, a, b in (("si4096_eager", "bf16x3", "fp64"), ("si4096_graph", "bf16x3_graph", "fp64_graph"),
                       ("batch512_eager", "batch512_bf16x3", "batch512_fp64"),
+                      ("si4096x8_eager", "si4096x8_bf16x3", "si4096x8_fp64"),
                       ("sic32768_eager", "sic32768_bf16x3", "sic32768_fp64")):
         if a in forms and b in forms:
             same[key] = forms[a]["atoms_per_s"] / forms[b]["atoms_per_s"]

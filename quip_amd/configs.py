@@ -70,6 +70,17 @@ def si4096():
     return remove_random(at, 41, rng)
 
 
+def si4096_batch(n_struct=8):
+    """n_struct structures made as si4096 is (8x8x8 diamond Si, sigma 0.15, 41 vacancies), one seed
+    each; the first is si4096 itself."""
+    out = []
+    for k in range(n_struct):
+        rng = rng_for(1, 1 + k)
+        at = perturb(diamond(5.43, (8, 8, 8)), 0.15, rng)
+        out.append(remove_random(at, 41, rng))
+    return out
+
+
 def sic32768():
     rng = rng_for(2, 1)
     at = perturb(diamond(4.36, (16, 16, 16), species=(14, 6)), 0.1, rng)
