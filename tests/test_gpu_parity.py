@@ -139,6 +139,30 @@ def test_fp64_matches_oracle(models, name):
     _compare(r, ro, len(at))
 
 
+def test_batch_of_si4096_sized_structures(models):
+    """The batching form on two si4096-sized structures (one seed each, the first si4096):
+    FP64 against the oracle structure by structure and against the plain form's calls;
+    bf16x3 within the split forms' bound of FP64."""
+    m, path = models["si4096"]
+    frames = configs.si4096_batch(2)
+    print("case si4096x2", [len(a) for a in frames], flush=True)
+    rb = Potential(param_filename=path).calc_batch([a.copy() for a in frames])
+    rs = Potential(param_filename=path, precision="bf16x3").calc_batch([a.copy() for a in frames])
+    plain = Potential(param_filename=path)
+    for a, r, s in zip(frames, rb, rs):
+        N = len(a)
+        ro = O.gap_calc(m, a)
+        assert abs(r["energy"] - ro["energy"]) / N < E_TOL
+        assert np.abs(r["force"] - ro["forces"]).max() < F_TOL
+        rp = plain.calc(a.copy(), force=True, virial=True)
+        assert abs(r["energy"] - rp["energy"]) / N < 1e-13
+        assert np.abs(r["force"] - rp["force"]).max() < 1e-12
+        assert np.allclose(np.asarray(r["virial"]).reshape(3, 3), np.asarray(rp["virial"]).reshape(3, 3),
+                           rtol=1e-11, atol=1e-10)
+        assert abs(s["energy"] - r["energy"]) / N < 1e-5
+        assert np.sqrt(np.mean((s["force"] - r["force"]) ** 2)) < 1e-4
+
+
 @pytest.mark.parametrize("nl", [(4, 4), (5, 4), (6, 4), (8, 4), (6, 6), (10, 6), (12, 6), (8, 8), (10, 8)])
 def test_descriptor_variants_match_oracle(nl, tmp_path):
     """Every (n_max, l_max) the specialised kernels cover, on a cell with an odd number of
